@@ -1,0 +1,61 @@
+"""ctypes binding of libgemini_fk.so.  There is no fallback: if the HIP library
+is missing or cannot be loaded, every entry point raises."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgemini_fk.so")
+_lib = None
+
+D = C.POINTER(C.c_double)
+F = C.POINTER(C.c_float)
+I = C.POINTER(C.c_int)
+LL = C.POINTER(C.c_longlong)
+VP = C.c_void_p
+
+# every symbol declared in include/gemini_fk.h
+EXPORTS = ["gm_ctx_create", "gm_ctx_destroy", "gm_set_option", "gm_last_error", "gm_model_set",
+           "gm_nodes_set", "gm_minors_batch", "gm_gfk_batch", "gm_last_timing", "gm_last_system_stats"]
+
+
+class GeminiFKError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"gemini_fk error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GeminiFKError(-1, f"{LIB_PATH} not built; run __graft_entry__.build() or python -m gemini_unified_amd.build")
+        L = C.CDLL(LIB_PATH)
+        L.gm_ctx_create.argtypes = [C.c_int, C.POINTER(VP)]
+        L.gm_ctx_destroy.argtypes = [VP]
+        L.gm_set_option.argtypes = [VP, C.c_int, C.c_int]
+        L.gm_last_error.argtypes = [VP, C.c_char_p, C.c_int]
+        L.gm_model_set.argtypes = [VP, C.c_int] + [D] * 9 + [C.c_int, I, D, I, C.c_int, C.c_int, C.c_int, C.c_int,
+                                                              C.c_double, C.c_double, C.c_double, C.c_double]
+        L.gm_nodes_set.argtypes = [VP, C.c_int, D, I, I]
+        L.gm_minors_batch.argtypes = [VP, C.c_int, D, I, I, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                                      C.c_int, D, D, D, D, D, D, D, D, I, LL]
+        L.gm_gfk_batch.argtypes = [VP, C.c_int, D, I, I, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                                   C.c_int, D, F, F, I, LL]
+        L.gm_last_timing.argtypes = [VP, D]
+        L.gm_last_system_stats.argtypes = [VP, C.c_int, LL]
+        _lib = L
+    return _lib
+
+
+def last_error(h) -> str:
+    buf = C.create_string_buffer(512)
+    lib().gm_last_error(h, buf, 512)
+    return buf.value.decode(errors="replace")
+
+
+def check(h, rc):
+    if rc != 0:
+        raise GeminiFKError(rc, last_error(h))
+    return rc

@@ -1,0 +1,344 @@
+// C ABI of the Green-FK engine: context, model upload, batch orchestration.
+#include <hip/hip_runtime.h>
+#include <cstdio>
+#include <cstring>
+#include <cstdarg>
+#include <cmath>
+#include <string>
+#include <vector>
+#include "gm_common.h"
+#include "../../include/gemini_fk.h"
+
+namespace gm_exact { void launch_stage(int, const GmModel*, const GmBatch&, int, const double*, int, GmKnot*, hipStream_t); }
+namespace gm_fast { void launch_stage(int, const GmModel*, const GmBatch&, int, const double*, int, GmKnot*, hipStream_t); }
+
+// per-system constants of the algorithmic work model
+static const int kN[5] = {5, 2, 2, 2, 4};
+static const int kNNZ[5] = {19, 4, 3, 4, 14};
+static const int kESYS[5] = {219, 148, 156, 196, 206};
+
+struct gm_ctx {
+  int device = 0;
+  int variant = 1;
+  hipStream_t stream = nullptr;
+  std::string err;
+  GmModel hmodel;
+  bool has_model = false, has_nodes = false;
+  GmModel* dmodel = nullptr;
+  // growable device scratch
+  void* dbuf = nullptr;
+  size_t dbuf_size = 0;
+  hipEvent_t ev[7];
+  double timing[6] = {0};
+  unsigned long long last_hist[48] = {0};
+};
+
+static std::string g_create_err;
+
+static int fail(gm_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf; else g_create_err = buf;
+  return code;
+}
+#define HIPCHK(c, x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return fail((c), GM_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); } while (0)
+
+extern "C" int gm_ctx_create(int device, gm_ctx** out) {
+  if (!out) return GM_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) return fail(nullptr, GM_ERR_HIP, "no HIP device available (%s)", hipGetErrorString(e));
+  if (device < 0 || device >= n) return fail(nullptr, GM_ERR_ARG, "device %d out of range", device);
+  gm_ctx* c = new gm_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&c->dmodel, sizeof(GmModel)) != hipSuccess) {
+    delete c;
+    return fail(nullptr, GM_ERR_HIP, "context creation failed on device %d", device);
+  }
+  for (int i = 0; i < 7; ++i) hipEventCreate(&c->ev[i]);
+  memset(&c->hmodel, 0, sizeof(GmModel));
+  *out = c;
+  return GM_OK;
+}
+
+extern "C" int gm_ctx_destroy(gm_ctx* c) {
+  if (!c) return GM_OK;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->dbuf) hipFree(c->dbuf);
+  if (c->dmodel) hipFree(c->dmodel);
+  for (int i = 0; i < 7; ++i) hipEventDestroy(c->ev[i]);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return GM_OK;
+}
+
+extern "C" int gm_set_option(gm_ctx* c, int option, int value) {
+  if (!c) return GM_ERR_ARG;
+  if (option == GM_OPT_VARIANT) {
+    if (value != 0 && value != 1) return fail(c, GM_ERR_ARG, "variant must be 0 or 1");
+    c->variant = value;
+    return GM_OK;
+  }
+  return fail(c, GM_ERR_ARG, "unknown option %d", option);
+}
+
+extern "C" int gm_last_error(gm_ctx* c, char* buf, int len) {
+  if (!buf || len <= 0) return GM_ERR_ARG;
+  const std::string& s = c ? c->err : g_create_err;
+  snprintf(buf, (size_t)len, "%s", s.c_str());
+  return GM_OK;
+}
+
+extern "C" int gm_last_timing(gm_ctx* c, double* ms) {
+  if (!c || !ms) return GM_ERR_ARG;
+  for (int i = 0; i < 6; ++i) ms[i] = c->timing[i];
+  return GM_OK;
+}
+
+extern "C" int gm_model_set(gm_ctx* c, int nk, const double* rk, const double* ro, const double* vpv, const double* vph,
+                            const double* vsv, const double* vsh, const double* eta, const double* qkinv,
+                            const double* qminv, int nlay, const int* iktop, const double* rb, const int* iflso,
+                            int nic, int noc, int fsflag, int equidis, double rearth, double ric, double roc,
+                            double fref) {
+  if (!c) return GM_ERR_ARG;
+  if (nk < 2 || nk > GM_MAXK) return fail(c, GM_ERR_ARG, "nk=%d outside [2,%d]", nk, GM_MAXK);
+  if (nlay < 2 || nlay > GM_MAXL) return fail(c, GM_ERR_ARG, "nlay=%d outside [2,%d]", nlay, GM_MAXL);
+  GmModel& m = c->hmodel;
+  int nnod = m.nnod;
+  double rn[GM_MAXN + 1]; int stt[GM_MAXN + 1], lay[GM_MAXN + 1];
+  memcpy(rn, m.rnod, sizeof rn); memcpy(stt, m.state, sizeof stt); memcpy(lay, m.layer, sizeof lay);
+  memset(&m, 0, sizeof m);
+  m.nnod = nnod; memcpy(m.rnod, rn, sizeof rn); memcpy(m.state, stt, sizeof stt); memcpy(m.layer, lay, sizeof lay);
+  m.nk = nk; m.nlay = nlay; m.nic = nic; m.noc = noc; m.fsflag = fsflag; m.equidis = equidis;
+  m.rearth = rearth; m.ric = ric; m.roc = roc; m.fref = fref;
+  for (int i = 0; i < nk; ++i) {
+    m.rk[i + 1] = rk[i]; m.ro[i + 1] = ro[i]; m.vpv[i + 1] = vpv[i]; m.vph[i + 1] = vph[i]; m.vsv[i + 1] = vsv[i];
+    m.vsh[i + 1] = vsh[i]; m.eta[i + 1] = eta[i]; m.qkinv[i + 1] = qkinv[i]; m.qminv[i + 1] = qminv[i];
+  }
+  for (int i = 0; i < nlay; ++i) { m.iktop[i + 1] = iktop[i]; m.rb[i + 1] = rb[i]; m.iflso[i + 1] = iflso[i]; }
+  if (m.iktop[nlay] != nk) return fail(c, GM_ERR_ARG, "iktop(nlay)=%d differs from nk=%d", m.iktop[nlay], nk);
+  m.ikbot[1] = 0;
+  for (int nl = 2; nl <= nlay; ++nl) m.ikbot[nl] = m.iktop[nl - 1] + 1;
+  for (int nl = 1; nl <= nlay; ++nl) m.drk[nl] = -1.0;
+  if (equidis == 1)
+    for (int nl = 2; nl <= nlay; ++nl) m.drk[nl] = (m.rk[m.iktop[nl]] - m.rk[m.ikbot[nl]]) / (m.iktop[nl] - m.ikbot[nl]);
+  m.hascore = (noc > 0 && m.iflso[noc] == 1) ? 1 : 0;
+  m.hasocean = m.iflso[nlay] == 1 ? 1 : 0;
+  m.ruc = m.hasocean ? m.rb[nlay - 1] : rearth;
+  m.rhs = m.rb[1];
+  c->has_model = true;
+  return GM_OK;
+}
+
+extern "C" int gm_nodes_set(gm_ctx* c, int nnod, const double* rnod, const int* state, const int* layer) {
+  if (!c) return GM_ERR_ARG;
+  if (nnod < 1 || nnod > GM_MAXN) return fail(c, GM_ERR_ARG, "nnod=%d outside [1,%d]", nnod, GM_MAXN);
+  GmModel& m = c->hmodel;
+  m.nnod = nnod;
+  for (int i = 0; i < nnod; ++i) { m.rnod[i + 1] = rnod[i]; m.state[i + 1] = state[i]; m.layer[i + 1] = layer[i]; }
+  c->has_nodes = true;
+  return GM_OK;
+}
+
+static void derive_stats(const unsigned long long* hist, long long* st, int only_sys = -1) {
+  static const int seq[8] = {0, 2, 4, 6, 8, 12, 16, 24};
+  static const int nx[8] = {0, 3, 2, 4, 4, 4, 8, 8};
+  long long MV[8] = {0}, SE[8] = {0}, NC[8] = {0};
+  for (int k = 1; k <= 7; ++k) {
+    MV[k] = MV[k - 1] + seq[k] + 1;
+    SE[k] = SE[k - 1] + nx[k];
+    NC[k] = NC[k - 1] + (k >= 3 ? k * (k - 1) / 2 : 0);
+  }
+  for (int s = 0; s < 5; ++s) {
+    if (only_sys >= 0 && s != only_sys) continue;
+    const unsigned long long* h = hist + s * 8;
+    for (int k = 3; k <= 8; ++k) {           // k = 8 encodes rejected steps (7 tries)
+      long long cnt = (long long)(k <= 7 ? h[k - 3] : h[5]);
+      int kk = k <= 7 ? k : 7;
+      if (!cnt) continue;
+      st[3] += cnt * kk;
+      st[4] += cnt * MV[kk];
+      st[5] += cnt * SE[kk];
+      st[6] += cnt * NC[kk];
+      st[7] += cnt * (SE[kk] * kESYS[s] + MV[kk] * 8LL * kNNZ[s] + 4LL * kN[s] * MV[kk] + 8LL * kN[s] * NC[kk]);
+      if (k <= 7) st[1] += cnt; else st[2] += cnt;
+    }
+    st[0] += (long long)h[6];
+    st[9] += (long long)h[7];
+  }
+  if (only_sys < 0) st[8] += (long long)hist[5 * 8];
+}
+
+namespace {
+struct Plan {
+  int ninst = 0, nfreq = 0;
+  std::vector<int> q, l;
+  std::vector<double> omre;
+};
+}
+
+// common driver of gm_minors_batch / gm_gfk_batch
+static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const int* nl, double sigma, double eps,
+               int attmode, int jsl, int jsr, int jre, int istyp, bool full, int flags,
+               double* u_tor, double* d_tor, double* dettor, double* u_sph, double* d_sph, double* ulg, double* dlg,
+               double* detsph, double* gf, float* gfre, float* gfim, int* status, long long* stats) {
+  if (!c) return GM_ERR_ARG;
+  if (!c->has_model || !c->has_nodes) return fail(c, GM_ERR_ARG, "model and nodes must be set first");
+  if (nfreq < 0 || (nfreq > 0 && (!freqs || !lmin || !nl))) return fail(c, GM_ERR_ARG, "invalid frequency arguments");
+  if (attmode < 0 || attmode > 3) return fail(c, GM_ERR_ARG, "invalid attenuation mode %d", attmode);
+  if (!(eps > 0.0)) return fail(c, GM_ERR_ARG, "eps must be positive");
+  GmModel& m = c->hmodel;
+  const int nnod = m.nnod;
+  if (jsl < 1 || jsl > nnod || jsr < 1 || jsr > nnod) return fail(c, GM_ERR_NODEIDX, "Invalid index for source node");
+  if (full) {
+    if (istyp != 0 && istyp != 1) return fail(c, GM_ERR_ARG, "istyp must be 0 (FORCE) or 1 (MOMENT)");
+    if (m.state[jre] != 0) return fail(c, GM_ERR_UNSUPPORTED, "receiver node in a fluid layer is not supported");
+    for (int k = 1; k <= nnod; ++k)
+      if (m.state[k] != 0 || !(m.rnod[k] > m.roc))
+        return fail(c, GM_ERR_UNSUPPORTED, "node %d lies in the core or a fluid layer; only mantle nodes are supported", k);
+  }
+  if (m.fsflag == 1) return fail(c, GM_ERR_UNSUPPORTED, "full-space models are not supported");
+  // (omega, l) plan
+  Plan P;
+  P.nfreq = nfreq;
+  P.omre.resize(nfreq);
+  for (int f = 0; f < nfreq; ++f) {
+    if (nl[f] < 0 || lmin[f] < 0) return fail(c, GM_ERR_ARG, "negative degree range at frequency %d", f);
+    if (!(freqs[f] > 0.0)) return fail(c, GM_ERR_ARG, "frequency %d must be positive", f);
+    P.omre[f] = 2.0 * (double)3.1415927410125732 * freqs[f];
+    for (int i = 0; i < nl[f]; ++i) { P.q.push_back(f); P.l.push_back(lmin[f] + i); }
+  }
+  P.ninst = (int)P.q.size();
+  const size_t n = P.ninst;
+  const int nsp = istyp == 1 ? 10 : 5;
+  HIPCHK(c, hipSetDevice(c->device));
+  // device buffer layout
+  size_t off = 0;
+  auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~size_t(255); return o; };
+  size_t o_model = take(sizeof(GmModel));
+  size_t o_freqs = take(sizeof(double) * (nfreq + 1));
+  size_t o_omre = take(sizeof(double) * (nfreq + 1));
+  size_t o_tables = take(sizeof(GmKnot) * (size_t)(m.nk + 1) * (nfreq + 1));
+  size_t o_q = take(sizeof(int) * (n + 1)), o_l = take(sizeof(int) * (n + 1)), o_status = take(sizeof(int) * (n + 1));
+  size_t o_rs = take(sizeof(double) * 2 * (n + 1));
+  size_t o_ut = take(16 * 2 * nnod * n + 16), o_dt = take(16 * 2 * nnod * n + 16), o_dett = take(16 * nnod * n + 16);
+  size_t o_us = take(16 * 6 * nnod * n + 16), o_ds = take(16 * 6 * nnod * n + 16);
+  size_t o_ulg = take(8 * nnod * n + 8), o_dlg = take(8 * nnod * n + 8), o_dets = take(16 * nnod * n + 16);
+  size_t o_ujs = full ? take(16 * 16 * nnod * n + 16) : 0;
+  bool dev = (flags & GM_FLAG_DEVICE_PTRS) != 0;
+  size_t o_gf = (full && (gf && !dev)) ? take(16 * nsp * nnod * n + 16) : 0;
+  size_t o_gfre = (full && (gfre && !dev)) ? take(4 * nsp * nnod * n + 4) : 0;
+  size_t o_gfim = (full && (gfim && !dev)) ? take(4 * nsp * nnod * n + 4) : 0;
+  size_t o_hist = take(sizeof(unsigned long long) * 6 * 8);
+  if (off > c->dbuf_size) {
+    if (c->dbuf) { hipFree(c->dbuf); c->dbuf = nullptr; c->dbuf_size = 0; }
+    if (hipMalloc(&c->dbuf, off) != hipSuccess) return fail(c, GM_ERR_NOMEM, "device allocation of %zu bytes failed", off);
+    c->dbuf_size = off;
+  }
+  char* base = (char*)c->dbuf;
+  hipStream_t s = c->stream;
+  GmModel* dM = (GmModel*)(base + o_model);
+  HIPCHK(c, hipMemcpyAsync(dM, &m, sizeof(GmModel), hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(base + o_freqs, freqs, sizeof(double) * nfreq, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(base + o_omre, P.omre.data(), sizeof(double) * nfreq, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(base + o_q, P.q.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(base + o_l, P.l.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+  std::vector<unsigned long long> zero(48, 0ULL);
+  HIPCHK(c, hipMemcpyAsync(base + o_hist, zero.data(), sizeof(unsigned long long) * 48, hipMemcpyHostToDevice, s));
+  GmBatch B;
+  memset(&B, 0, sizeof B);
+  B.n = (int)n; B.nnod = nnod; B.jsl = jsl; B.jsr = jsr; B.jre = jre; B.istyp = istyp;
+  B.omim = -sigma; B.eps = eps;
+  B.q = (const int*)(base + o_q); B.l = (const int*)(base + o_l); B.omre = (const double*)(base + o_omre);
+  B.nkp1 = m.nk + 1; B.tables = (const GmKnot*)(base + o_tables);
+  B.status = (int*)(base + o_status); B.rs = (double*)(base + o_rs);
+  B.ut = (double*)(base + o_ut); B.dt = (double*)(base + o_dt); B.dett = (double*)(base + o_dett);
+  B.us = (double*)(base + o_us); B.ds = (double*)(base + o_ds); B.ulg = (double*)(base + o_ulg);
+  B.dlg = (double*)(base + o_dlg); B.dets = (double*)(base + o_dets);
+  B.ujs = full ? (double*)(base + o_ujs) : nullptr;
+  if (full) {
+    B.gf = gf ? (dev ? gf : (double*)(base + o_gf)) : nullptr;
+    B.gfre = gfre ? (dev ? gfre : (float*)(base + o_gfre)) : nullptr;
+    B.gfim = gfim ? (dev ? gfim : (float*)(base + o_gfim)) : nullptr;
+    if ((B.gfre == nullptr) != (B.gfim == nullptr)) return fail(c, GM_ERR_ARG, "gfre and gfim must both be given");
+  }
+  B.hist = (unsigned long long*)(base + o_hist);
+  auto launch = c->variant == 0 ? gm_exact::launch_stage : gm_fast::launch_stage;
+  GmKnot* dT = (GmKnot*)(base + o_tables);
+  const double* dF = (const double*)(base + o_freqs);
+  HIPCHK(c, hipEventRecord(c->ev[0], s));
+  launch(0, dM, B, nfreq, dF, attmode, dT, s);
+  HIPCHK(c, hipEventRecord(c->ev[1], s));
+  launch(1, dM, B, nfreq, dF, attmode, dT, s);
+  HIPCHK(c, hipEventRecord(c->ev[2], s));
+  launch(2, dM, B, nfreq, dF, attmode, dT, s);
+  HIPCHK(c, hipEventRecord(c->ev[3], s));
+  launch(3, dM, B, nfreq, dF, attmode, dT, s);
+  HIPCHK(c, hipEventRecord(c->ev[4], s));
+  if (full) { launch(4, dM, B, nfreq, dF, attmode, dT, s); launch(5, dM, B, nfreq, dF, attmode, dT, s); }
+  HIPCHK(c, hipEventRecord(c->ev[5], s));
+  HIPCHK(c, hipGetLastError());
+  // results
+  auto cp = [&](void* dst, size_t o, size_t bytes) -> hipError_t {
+    if (!dst) return hipSuccess;
+    if (dev) return hipMemcpyAsync(dst, base + o, bytes, hipMemcpyDeviceToDevice, s);
+    return hipMemcpyAsync(dst, base + o, bytes, hipMemcpyDeviceToHost, s);
+  };
+  HIPCHK(c, cp(u_tor, o_ut, 16 * 2 * nnod * n));
+  HIPCHK(c, cp(d_tor, o_dt, 16 * 2 * nnod * n));
+  HIPCHK(c, cp(dettor, o_dett, 16 * nnod * n));
+  HIPCHK(c, cp(u_sph, o_us, 16 * 6 * nnod * n));
+  HIPCHK(c, cp(d_sph, o_ds, 16 * 6 * nnod * n));
+  HIPCHK(c, cp(ulg, o_ulg, 8 * nnod * n));
+  HIPCHK(c, cp(dlg, o_dlg, 8 * nnod * n));
+  HIPCHK(c, cp(detsph, o_dets, 16 * nnod * n));
+  if (full && !dev) {
+    if (gf) HIPCHK(c, hipMemcpyAsync(gf, base + o_gf, 16 * nsp * nnod * n, hipMemcpyDeviceToHost, s));
+    if (gfre) HIPCHK(c, hipMemcpyAsync(gfre, base + o_gfre, 4 * nsp * nnod * n, hipMemcpyDeviceToHost, s));
+    if (gfim) HIPCHK(c, hipMemcpyAsync(gfim, base + o_gfim, 4 * nsp * nnod * n, hipMemcpyDeviceToHost, s));
+  }
+  std::vector<int> hstatus(n + 1);
+  HIPCHK(c, hipMemcpyAsync(hstatus.data(), base + o_status, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+  unsigned long long hist[48];
+  HIPCHK(c, hipMemcpyAsync(hist, base + o_hist, sizeof hist, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  float ms;
+  for (int i = 0; i < 5; ++i) { hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]); c->timing[i] = ms; }
+  hipEventElapsedTime(&ms, c->ev[0], c->ev[5]);
+  c->timing[5] = ms;
+  memcpy(c->last_hist, hist, sizeof hist);
+  if (status) memcpy(status, hstatus.data(), sizeof(int) * n);
+  if (stats) derive_stats(hist, stats);
+  int first = 0;
+  for (size_t i = 0; i < n; ++i) if (hstatus[i]) { first = hstatus[i]; break; }
+  if (first) return fail(c, first, "at least one (omega,l) instance failed with error %d", first);
+  return GM_OK;
+}
+
+extern "C" int gm_minors_batch(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const int* nl, double sigma,
+                               double eps, int attmode, int jsl, int jsr, int flags, double* u_tor, double* d_tor,
+                               double* dettor, double* u_sph, double* d_sph, double* ulg, double* dlg,
+                               double* detsph, int* status, long long* stats) {
+  return run(c, nfreq, freqs, lmin, nl, sigma, eps, attmode, jsl, jsr, jsr, 1, false, flags, u_tor, d_tor, dettor,
+             u_sph, d_sph, ulg, dlg, detsph, nullptr, nullptr, nullptr, status, stats);
+}
+
+extern "C" int gm_gfk_batch(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const int* nl, double sigma,
+                            double eps, int attmode, int jre, int istyp, int flags, double* gf, float* gfre,
+                            float* gfim, int* status, long long* stats) {
+  return run(c, nfreq, freqs, lmin, nl, sigma, eps, attmode, jre, jre, jre, istyp, true, flags, nullptr, nullptr,
+             nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, gf, gfre, gfim, status, stats);
+}
+
+extern "C" int gm_last_system_stats(gm_ctx* c, int sys, long long* stats) {
+  if (!c || !stats || sys < 0 || sys > 4) return GM_ERR_ARG;
+  for (int i = 0; i < GM_NSTAT; ++i) stats[i] = 0;
+  derive_stats(c->last_hist, stats, sys);
+  return GM_OK;
+}

@@ -1,0 +1,73 @@
+// Shared host/device definitions of the Green-FK engine for MI355X (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define GM_MAXK 512      // maximum earth-model knots
+#define GM_MAXL 32       // maximum layers
+#define GM_MAXN 128      // maximum external radial nodes
+
+// error codes returned per (omega, l) instance and by the C ABI
+enum {
+  GM_OK = 0, GM_ERR_MINSTEP = 1, GM_ERR_MAXSTEP = 2, GM_ERR_NODEIDX = 3, GM_ERR_CF = 4,
+  GM_ERR_RANGE = 5, GM_ERR_SPLINE = 6, GM_ERR_STATE = 7, GM_ERR_UNSUPPORTED = 8, GM_ERR_ARG = 9,
+  GM_ERR_HIP = 10, GM_ERR_NOMEM = 11
+};
+
+// work counters kept by every kernel (same meaning as the oracle's)
+enum {
+  GM_ST_PROP = 0, GM_ST_ACC = 1, GM_ST_REJ = 2, GM_ST_MMP = 3, GM_ST_MATVEC = 4, GM_ST_SYSMAT = 5,
+  GM_ST_NEVILLE = 6, GM_ST_FLOPS = 7, GM_ST_ZERODET = 8, GM_ST_SUBINT = 9, GM_NSTAT = 10
+};
+
+// systems of the radial problem
+enum { GM_SYS_MINORS = 0, GM_SYS_TOROIDAL = 1, GM_SYS_FLUID = 2, GM_SYS_RADIAL = 3, GM_SYS_SOLIDT = 4 };
+
+// Earth model + external nodes, 1-based arrays as in the reference
+struct GmModel {
+  int nk, nlay, nic, noc, fsflag, equidis, nnod, hascore;
+  int hasocean, pad0, pad1, pad2;
+  double rearth, ric, roc, fref, ruc, rhs;
+  double rk[GM_MAXK + 1];
+  double ro[GM_MAXK + 1], vpv[GM_MAXK + 1], vph[GM_MAXK + 1], vsv[GM_MAXK + 1], vsh[GM_MAXK + 1];
+  double eta[GM_MAXK + 1], qkinv[GM_MAXK + 1], qminv[GM_MAXK + 1];
+  double rb[GM_MAXL + 1], drk[GM_MAXL + 1];
+  int iktop[GM_MAXL + 1], ikbot[GM_MAXL + 1], iflso[GM_MAXL + 1], pad3;
+  double rnod[GM_MAXN + 1];
+  int state[GM_MAXN + 1], layer[GM_MAXN + 1];
+};
+
+// spline table entry for one knot and one frequency: density, its spline
+// coefficient, 7 complex moduli (A,C,F,L,N,kappa,mu) and their coefficients
+struct GmKnot {
+  double ro, ro2;
+  double er[7], ei[7];
+  double e2r[7], e2i[7];
+};
+
+// one batch of (omega, l) instances and its device buffers
+struct GmBatch {
+  int n;                           // number of (omega, l) instances
+  int nnod, jsl, jsr, jre, istyp;
+  double omim, eps;
+  const int* __restrict__ q;       // frequency slot per instance
+  const int* __restrict__ l;       // harmonic degree per instance
+  const double* __restrict__ omre; // per frequency slot
+  int nkp1;                        // table stride (nk + 1)
+  const GmKnot* __restrict__ tables;
+  int* __restrict__ status;
+  double* __restrict__ rs;         // [2][n]: toroidal, spheroidal start radius
+  double* __restrict__ ut;         // toroidal up   [nnod][2][n] complex (re,im interleaved)
+  double* __restrict__ dt;         // toroidal down [nnod][2][n]
+  double* __restrict__ dett;       // [nnod][n] complex
+  double* __restrict__ us;         // spheroidal up   [nnod][6][n]
+  double* __restrict__ ds;         // spheroidal down [nnod][6][n]
+  double* __restrict__ ulg;        // [nnod][n]
+  double* __restrict__ dlg;        // [nnod][n]
+  double* __restrict__ dets;       // [nnod][n]
+  double* __restrict__ ujs;        // spheroidal UJGF [nnod][4 jumps][4 comps][n] complex
+  double* __restrict__ gf;         // FK column, complex [nnod][nsp][n] (may be null)
+  float* __restrict__ gfre;        // FP32 real part [nnod][nsp][n] (may be null)
+  float* __restrict__ gfim;
+  unsigned long long* __restrict__ hist;   // [5 systems][8]
+};

@@ -1,0 +1,8 @@
+// Throughput variant: identical algorithm and step control; the compiler may
+// fuse multiply-adds (results differ from the exact variant at round-off level).
+#include "gm_common.h"
+#pragma clang fp contract(fast)
+#define GM_NS gm_fast
+#include "gm_device.inc"
+#include "gm_kernels.inc"
+#include "gm_launch.inc"

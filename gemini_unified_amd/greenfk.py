@@ -1,0 +1,176 @@
+"""Host-side mirror of the reference's Green-FK driver interface.
+
+``GreenFKEngine`` owns one C-ABI context (one HIP device, one stream) and
+replaces, for batches of (omega, l):
+
+* ``computeToroidalMinors`` / ``computeSpheroidalMinors``
+  (f90/toroidalMinors.f90:71, f90/spheroidalMinors.f90:72) -> :meth:`minors`,
+* the degree-loop body of ``computeGreenFKSpectraForSynthetics``
+  (f90/computeGreenFKSpectraForSynthetics.f90:410-494) -> :meth:`gfk`,
+* the whole frequency loop writing ``gfre/gfim(nkf, nsp, nnod)``
+  -> :meth:`compute_green_fk_spectra_for_synthetics`.
+
+Errors follow the reference's level-2 convention: a failing call raises
+``GeminiFKError`` with the code and message (the reference prints and calls
+MPI_Abort).  All numerics run in libgemini_fk.so on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import D, F, I, LL, check
+from .earthmodel import ATTMODES, ExternalRadialNodes, FrequencyGrid, NodeEarthmodel
+
+STAT_NAMES = ["propagations", "accepted_steps", "rejected_steps", "mmp_sequences", "matvecs",
+              "sysmat_evals", "neville_cells", "flops", "zero_det", "subintervals"]
+VARIANTS = {"exact": 0, "fast": 1}
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(D)
+
+
+class GreenFKEngine:
+    def __init__(self, nem: NodeEarthmodel, nodes: ExternalRadialNodes, device: int = 0, variant: str = "fast"):
+        L = _lib.lib()
+        h = C.c_void_p()
+        rc = L.gm_ctx_create(int(device), C.byref(h))
+        if rc != 0:
+            raise _lib.GeminiFKError(rc, _lib.last_error(None))
+        self.h = h
+        self.nem, self.nodes = nem, nodes
+        self.set_variant(variant)
+        k = [np.ascontiguousarray(x, dtype=np.float64) for x in
+             (nem.rk, nem.ro, nem.vpv, nem.vph, nem.vsv, nem.vsh, nem.eta, nem.qkinv, nem.qminv)]
+        iktop = np.ascontiguousarray(nem.iktop, dtype=np.int32)
+        rb = np.ascontiguousarray(nem.rb, dtype=np.float64)
+        iflso = np.ascontiguousarray(nem.iflso, dtype=np.int32)
+        check(h, L.gm_model_set(h, nem.nk, *[x.ctypes.data_as(D) for x in k], nem.nlay, iktop.ctypes.data_as(I),
+                                rb.ctypes.data_as(D), iflso.ctypes.data_as(I), nem.nic, nem.noc, nem.fsflag,
+                                nem.equidis, nem.rearth, nem.ric, nem.roc, nem.fref))
+        rn = np.ascontiguousarray(nodes.rnod, dtype=np.float64)
+        st = np.ascontiguousarray(nodes.state, dtype=np.int32)
+        ly = np.ascontiguousarray(nodes.layer, dtype=np.int32)
+        check(h, L.gm_nodes_set(h, nodes.nnod, rn.ctypes.data_as(D), st.ctypes.data_as(I), ly.ctypes.data_as(I)))
+        self.stats = np.zeros(10, dtype=np.int64)
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.lib().gm_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_variant(self, variant: str):
+        check(self.h, _lib.lib().gm_set_option(self.h, 1, VARIANTS[variant]))
+        self.variant = variant
+
+    def timing(self):
+        ms = np.zeros(6)
+        _lib.lib().gm_last_timing(self.h, ms.ctypes.data_as(D))
+        return ms
+
+    def system_stats(self, sys: int):
+        st = np.zeros(10, np.int64)
+        check(self.h, _lib.lib().gm_last_system_stats(self.h, int(sys), st.ctypes.data_as(LL)))
+        return st
+
+    def run_minors_resident(self, freqs, lmin, nl, sigma, eps, jsl, jsr, attmode="ATTENUATION_AND_DISPERSION"):
+        """Minors of all (f, l) pairs left resident in the engine's HBM buffers (no copy-out)."""
+        freqs, lmin, nl, nk = self._plan(freqs, lmin, nl)
+        st = np.zeros(10, np.int64)
+        am = ATTMODES[attmode] if isinstance(attmode, str) else int(attmode)
+        rc = _lib.lib().gm_minors_batch(self.h, freqs.size, freqs.ctypes.data_as(D), lmin.ctypes.data_as(I),
+                                        nl.ctypes.data_as(I), float(sigma), float(eps), am, int(jsl), int(jsr), 0,
+                                        *([None] * 8), None, st.ctypes.data_as(LL))
+        check(self.h, rc)
+        return st
+
+    def run_gfk_resident(self, freqs, lmin, nl, sigma, eps, jre, istyp=1, attmode="ATTENUATION_AND_DISPERSION"):
+        """Green-FK columns of all (f, l) pairs left resident in HBM (no copy-out)."""
+        freqs, lmin, nl, nk = self._plan(freqs, lmin, nl)
+        st = np.zeros(10, np.int64)
+        am = ATTMODES[attmode] if isinstance(attmode, str) else int(attmode)
+        rc = _lib.lib().gm_gfk_batch(self.h, freqs.size, freqs.ctypes.data_as(D), lmin.ctypes.data_as(I),
+                                     nl.ctypes.data_as(I), float(sigma), float(eps), am, int(jre), int(istyp), 0,
+                                     None, None, None, None, st.ctypes.data_as(LL))
+        check(self.h, rc)
+        return st
+
+    @staticmethod
+    def _plan(freqs, lmin, nl):
+        freqs = np.ascontiguousarray(np.atleast_1d(freqs), dtype=np.float64)
+        nf = freqs.size
+        lmin = np.ascontiguousarray(np.broadcast_to(lmin, (nf,)), dtype=np.int32)
+        nl = np.ascontiguousarray(np.broadcast_to(nl, (nf,)), dtype=np.int32)
+        return freqs, lmin, nl, int(nl.sum())
+
+    def minors(self, freqs, lmin, nl, sigma, eps, jsl, jsr, attmode="ATTENUATION_AND_DISPERSION", raise_on_error=True):
+        """Toroidal + spheroidal minors of all (f, l) pairs; arrays indexed [node, comp, k]."""
+        freqs, lmin, nl, nk = self._plan(freqs, lmin, nl)
+        nn = self.nodes.nnod
+        out = dict(u_tor=np.zeros((nn, 2, nk), np.complex128), d_tor=np.zeros((nn, 2, nk), np.complex128),
+                   det_tor=np.zeros((nn, nk), np.complex128), u_sph=np.zeros((nn, 6, nk), np.complex128),
+                   d_sph=np.zeros((nn, 6, nk), np.complex128), ulg=np.zeros((nn, nk)), dlg=np.zeros((nn, nk)),
+                   det_sph=np.zeros((nn, nk), np.complex128), status=np.zeros(nk, np.int32))
+        st = np.zeros(10, np.int64)
+        am = ATTMODES[attmode] if isinstance(attmode, str) else int(attmode)
+        rc = _lib.lib().gm_minors_batch(self.h, freqs.size, freqs.ctypes.data_as(D), lmin.ctypes.data_as(I),
+                                        nl.ctypes.data_as(I), float(sigma), float(eps), am, int(jsl), int(jsr), 0,
+                                        *[out[k].ctypes.data_as(D) for k in
+                                          ("u_tor", "d_tor", "det_tor", "u_sph", "d_sph", "ulg", "dlg", "det_sph")],
+                                        out["status"].ctypes.data_as(I), st.ctypes.data_as(LL))
+        self.stats += st
+        out["stats"] = st
+        if raise_on_error:
+            check(self.h, rc)
+        return out
+
+    def gfk(self, freqs, lmin, nl, sigma, eps, jre, istyp=1, attmode="ATTENUATION_AND_DISPERSION",
+            want_fp32=True, raise_on_error=True):
+        """Green-FK columns; gf[node, isp, k] complex128 (and FP32 re/im as written to the file)."""
+        freqs, lmin, nl, nk = self._plan(freqs, lmin, nl)
+        nn = self.nodes.nnod
+        nsp = 10 if istyp == 1 else 5
+        gf = np.zeros((nn, nsp, nk), np.complex128)
+        gfre = np.zeros((nn, nsp, nk), np.float32) if want_fp32 else None
+        gfim = np.zeros((nn, nsp, nk), np.float32) if want_fp32 else None
+        status = np.zeros(nk, np.int32)
+        st = np.zeros(10, np.int64)
+        am = ATTMODES[attmode] if isinstance(attmode, str) else int(attmode)
+        rc = _lib.lib().gm_gfk_batch(self.h, freqs.size, freqs.ctypes.data_as(D), lmin.ctypes.data_as(I),
+                                     nl.ctypes.data_as(I), float(sigma), float(eps), am, int(jre), int(istyp), 0,
+                                     gf.ctypes.data_as(D), None if gfre is None else gfre.ctypes.data_as(F),
+                                     None if gfim is None else gfim.ctypes.data_as(F), status.ctypes.data_as(I),
+                                     st.ctypes.data_as(LL))
+        self.stats += st
+        if raise_on_error:
+            check(self.h, rc)
+        return dict(gf=gf, gfre=gfre, gfim=gfim, status=status, stats=st)
+
+    def compute_green_fk_spectra_for_synthetics(self, grid: FrequencyGrid, jre: int, eps: float, istyp: int = 1,
+                                                attmode="ATTENUATION_AND_DISPERSION", freq_block: int = 64):
+        """The frequency loop of computeGreenFKSpectraForSynthetics: returns gfre, gfim with the
+        file layout (nkf, nsp, nnod) in Fortran order, i.e. C arrays [nnod][nsp][nkf]."""
+        nn = self.nodes.nnod
+        nsp = 10 if istyp == 1 else 5
+        gfre = np.zeros((nn, nsp, grid.nkf), np.float32)
+        gfim = np.zeros((nn, nsp, grid.nkf), np.float32)
+        freqs = grid.freqs
+        off = 0
+        for b in range(0, freqs.size, freq_block):
+            fb = freqs[b:b + freq_block]
+            nwn = grid.nwn[b:b + freq_block]
+            r = self.gfk(fb, 0, nwn, grid.sigma, eps, jre, istyp, attmode)
+            n = int(nwn.sum())
+            gfre[:, :, off:off + n] = r["gfre"]
+            gfim[:, :, off:off + n] = r["gfim"]
+            off += n
+        return gfre, gfim

@@ -1,0 +1,123 @@
+/*
+ * gemini_fk.h — C ABI of the MI355X Green-FK engine (libgemini_fk.so).
+ *
+ * Drop-in boundary for the per-(omega, l) radial problem of GEMINI_UNIFIED and
+ * its frequency loop.  Plain pointers and sizes only; every function returns an
+ * int status (0 ok, > 0 error code, see GM_ERR_*), mirroring the reference's
+ * error_message level-2 convention (f90/errorMessage.f90:45-258): the caller
+ * checks the status and aborts or reports, the callee never terminates the
+ * process.  gm_last_error() returns the message of the last failure.
+ *
+ * Array conventions follow the reference's Fortran column-major layouts so an
+ * iso_c_binding caller can pass its own allocations with c_loc:
+ *   complex values are interleaved (re, im) doubles ("double complex");
+ *   per-(omega, l) outputs are indexed [node][component][k] in C order, i.e.
+ *   Fortran (k, component, node), k = running index over all (frequency, l)
+ *   pairs of the call, l fastest — the layout of gfre/gfim in
+ *   f90/computeGreenFKSpectraForSynthetics.f90:473-489 and of the GFK file
+ *   datasets GreenFKSpectraReal/Imag (nkfmax, nsp, nnod).
+ * Node and layer indices are 1-based as in the reference.
+ *
+ * Threading: one gm_ctx per host thread; all work runs asynchronously on the
+ * context's HIP stream and is synchronised before a call returns.
+ */
+#ifndef GEMINI_FK_H
+#define GEMINI_FK_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gm_ctx gm_ctx;
+
+/* error codes */
+#define GM_OK 0
+#define GM_ERR_MINSTEP 1      /* "Steps size smaller than minimum" (episode/integrationStep.f90:177-180) */
+#define GM_ERR_MAXSTEP 2      /* "Maximum number of steps reached" (episode/integrationStep.f90:186) */
+#define GM_ERR_NODEIDX 3      /* invalid source node index (f90/toroidalMinors.f90:90-97) */
+#define GM_ERR_CF 4           /* continued fraction did not converge (f90/initialValues.f90:515,572) */
+#define GM_ERR_RANGE 5        /* node/discontinuity bookkeeping (episode/propagateOde.f90:190-200,314-320) */
+#define GM_ERR_SPLINE 6       /* tridiagonal system could not be solved (f90/cubicSpline.f90:104) */
+#define GM_ERR_STATE 7        /* solid-fluid state sequence (f90/spheroidalMinors.f90:268-271) */
+#define GM_ERR_UNSUPPORTED 8  /* configuration outside the implemented branches */
+#define GM_ERR_ARG 9          /* invalid argument */
+#define GM_ERR_HIP 10         /* HIP runtime failure */
+#define GM_ERR_NOMEM 11       /* device allocation failed */
+
+/* attenuation modes (f90/complexElasticConstants.f90:118-131) */
+#define GM_ELASTIC 0
+#define GM_ATTENUATION_ONLY 1
+#define GM_DISPERSION_ONLY 2
+#define GM_ATTENUATION_AND_DISPERSION 3
+
+/* flags of the batch calls */
+#define GM_FLAG_DEVICE_PTRS 1  /* output pointers are device (HBM) pointers */
+
+/* options (gm_set_option) */
+#define GM_OPT_VARIANT 1       /* 0: exact (no FMA contraction, bitwise operation order), 1: fast (FMA) */
+
+/* work counters returned in stats[10] (same meaning as the CPU oracle's) */
+#define GM_NSTAT 10            /* propagations, accepted steps, rejected steps, MMP sequences,
+                                  matvecs, sysmat evaluations, Neville cells, algorithmic flops,
+                                  flagged zero determinants, integration sub-intervals */
+
+/* Context on HIP device `device`.  Replaces the module-global state the
+ * reference keeps between calls (f90/splineEarthmodelCoefficients.f90:28-41). */
+int gm_ctx_create(int device, gm_ctx** out);
+int gm_ctx_destroy(gm_ctx* ctx);
+int gm_set_option(gm_ctx* ctx, int option, int value);
+/* copies the last error message of ctx (or of a failed gm_ctx_create when ctx is NULL) */
+int gm_last_error(gm_ctx* ctx, char* buf, int len);
+
+/* Node earth model, SI units, knots bottom-up; iktop is 1-based.
+ * Replaces createNodeEarthmodel's result (f90/nodeEarthmodel.f90:98-257). */
+int gm_model_set(gm_ctx* ctx, int nk, const double* rk, const double* ro, const double* vpv, const double* vph,
+                 const double* vsv, const double* vsh, const double* eta, const double* qkinv, const double* qminv,
+                 int nlay, const int* iktop, const double* rb, const int* iflso, int nic, int noc, int fsflag,
+                 int equidis, double rearth, double ric, double roc, double fref);
+
+/* External radial nodes (ascending radii, state 1 = fluid, 1-based layer).
+ * Replaces the external_radial_nodes object (f90/externalRadialNodes.f90:62-72). */
+int gm_nodes_set(gm_ctx* ctx, int nnod, const double* rnod, const int* state, const int* layer);
+
+/* Toroidal and spheroidal minors for all (f_q, l), l = lmin[q] .. lmin[q]+nl[q]-1,
+ * with omega = 2*pi32*f_q - i*sigma and dll1 = (sqrt(l(l+1))/R*R)^2.
+ * Replaces computeToroidalMinors (f90/toroidalMinors.f90:71) and
+ * computeSpheroidalMinors (f90/spheroidalMinors.f90:72) called in the degree loop
+ * (f90/computeGreenFKSpectraForSynthetics.f90:410-429), including the
+ * per-frequency computeSplineEarthmodelCoefficients (:391).
+ * Outputs (any may be NULL), k over all instances:
+ *   u_tor, d_tor  [nnod][2][k] complex        dettor [nnod][k] complex
+ *   u_sph, d_sph  [nnod][6][k] complex        ulg, dlg [nnod][k]   detsph [nnod][k] complex
+ *   status [k] (0 or GM_ERR_*), stats[GM_NSTAT] (accumulated). */
+int gm_minors_batch(gm_ctx* ctx, int nfreq, const double* freqs, const int* lmin, const int* nl, double sigma,
+                    double eps, int attmode, int jsl, int jsr, int flags, double* u_tor, double* d_tor,
+                    double* dettor, double* u_sph, double* d_sph, double* ulg, double* dlg, double* detsph,
+                    int* status, long long* stats);
+
+/* Full Green-FK columns: minors, unit-jump Green functions with the spheroidal
+ * adjoints, reciprocity, source terms and packing, for a receiver at node jre
+ * and a MOMENT (istyp = 1) or FORCE (istyp = 0) source type.
+ * Replaces the degree loop body of computeGreenFKSpectraForSynthetics
+ * (f90/computeGreenFKSpectraForSynthetics.f90:410-494) together with
+ * computeUnitJumpGreenFunctions (f90/unitJumpGreenFunctions.f90:62),
+ * spheroidalReciprocity/toroidalReciprocity (f90/reciprocity.f90:82,169) and
+ * oneNodeSourceTerms (f90/sourceTerms.f90:59).
+ * gf: complex FP64 [nnod][nsp][k]; gfre/gfim: FP32 [nnod][nsp][k] (the file's
+ * values); nsp = 10 (moment) or 5 (force).  Any output may be NULL. */
+int gm_gfk_batch(gm_ctx* ctx, int nfreq, const double* freqs, const int* lmin, const int* nl, double sigma,
+                 double eps, int attmode, int jre, int istyp, int flags, double* gf, float* gfre, float* gfim,
+                 int* status, long long* stats);
+
+/* Kernel time of the last batch call in milliseconds (HIP events on the
+ * context stream), per stage: 0 tables, 1 setup, 2 toroidal, 3 spheroidal,
+ * 4 adjoint+assembly, 5 total. */
+int gm_last_timing(gm_ctx* ctx, double* ms6);
+
+/* Work counters of the last batch call restricted to one ODE system
+ * (0 minors, 1 toroidal, 2 fluid, 3 radial, 4 adjoint), for per-kernel rooflines. */
+int gm_last_system_stats(gm_ctx* ctx, int sys, long long* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

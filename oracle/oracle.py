@@ -1,0 +1,222 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product path
+(gemini_unified_amd) never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+D = C.POINTER(C.c_double)
+I = C.POINTER(C.c_int)
+L = C.POINTER(C.c_int64)
+
+
+def build(force: bool = False) -> str:
+    src = os.path.join(_HERE, "gemini_oracle.cpp")
+    if force or not os.path.exists(_LIB) or os.path.getmtime(_LIB) < os.path.getmtime(src):
+        subprocess.check_call(["make", "-s", "-B" if force else "-s", "-C", _HERE, "liboracle.so"])
+    return _LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        _lib = C.CDLL(_LIB)
+        _lib.orc_create.restype = C.c_void_p
+        _lib.orc_create.argtypes = [C.c_int] + [D] * 9 + [C.c_int, I, D, I, C.c_int, C.c_int, C.c_int, C.c_int,
+                                                          C.c_double, C.c_double, C.c_double, C.c_double,
+                                                          C.c_int, D, I, I]
+        _lib.orc_destroy.argtypes = [C.c_void_p]
+        _lib.orc_set_frequency.argtypes = [C.c_void_p, C.c_double, C.c_int]
+        _lib.orc_get_spline.argtypes = [C.c_void_p, D, D, D, D]
+        _lib.orc_eval_elcon.argtypes = [C.c_void_p, C.c_double, C.c_int, D, D, D]
+        _lib.orc_layer_index.argtypes = [C.c_void_p, C.c_double, I, I]
+        _lib.orc_start_radius.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, D]
+        _lib.orc_initial_values.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                                            C.c_int, D, D]
+        _lib.orc_sphancf.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, D]
+        _lib.orc_zetl.argtypes = [C.c_double, C.c_double, C.c_double, D]
+        _lib.orc_toroidal_minors.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                             C.c_double, D, D, D, L]
+        _lib.orc_spheroidal_minors.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                               C.c_double, D, D, D, D, D, L]
+        _lib.orc_gfk_column.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                        C.c_double, D, D, D, I, L]
+        _lib.orc_gfk_degrees.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                         C.c_int, C.c_int, C.c_double, C.c_int, D, I, L]
+        _lib.orc_propagate_const.argtypes = [C.c_int, D, C.c_double, C.c_double, C.c_double, C.c_double, D, D, L]
+        _lib.orc_num_threads.restype = C.c_int
+        _lib.orc_reciprocity_sph.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, D, D]
+        _lib.orc_vti.argtypes = [C.c_double] * 9 + [D]
+        _lib.orc_minors_degrees.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                            C.c_int, C.c_int, C.c_double, C.c_int, I, L]
+    return _lib
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(D)
+
+
+def _i(a):
+    return np.ascontiguousarray(a, dtype=np.int32).ctypes.data_as(I)
+
+
+def _c(a):
+    """complex128 array -> double* view"""
+    assert a.dtype == np.complex128 and a.flags.c_contiguous
+    return a.ctypes.data_as(D)
+
+
+STAT_NAMES = ["propagations", "accepted_steps", "rejected_steps", "mmp_sequences", "matvecs",
+              "sysmat_evals", "neville_cells", "flops", "zero_det", "subintervals"]
+
+
+class Oracle:
+    """One model + node set; mirrors the reference module state (spline tables per frequency)."""
+
+    def __init__(self, nem, nodes):
+        self.nem, self.nodes = nem, nodes
+        self._keep = [np.ascontiguousarray(x, dtype=np.float64) for x in
+                      (nem.rk, nem.ro, nem.vpv, nem.vph, nem.vsv, nem.vsh, nem.eta, nem.qkinv, nem.qminv, nem.rb,
+                       nodes.rnod)]
+        self._ikeep = [np.ascontiguousarray(x, dtype=np.int32) for x in
+                       (nem.iktop, nem.iflso, nodes.state, nodes.layer)]
+        k, ik = self._keep, self._ikeep
+        self.h = lib().orc_create(nem.nk, *(x.ctypes.data_as(D) for x in k[:9]), nem.nlay,
+                                  ik[0].ctypes.data_as(I), k[9].ctypes.data_as(D), ik[1].ctypes.data_as(I),
+                                  nem.nic, nem.noc, nem.fsflag, nem.equidis, nem.rearth, nem.ric, nem.roc,
+                                  nem.fref, nodes.nnod, k[10].ctypes.data_as(D), ik[2].ctypes.data_as(I),
+                                  ik[3].ctypes.data_as(I))
+        self.stats = np.zeros(10, dtype=np.int64)
+
+    def __del__(self):
+        try:
+            lib().orc_destroy(self.h)
+        except Exception:
+            pass
+
+    def set_frequency(self, f, attmode=3):
+        rc = lib().orc_set_frequency(self.h, float(f), int(attmode))
+        if rc:
+            raise RuntimeError(f"oracle spline failed rc={rc}")
+
+    def spline(self):
+        nk = self.nem.nk
+        ro, ro2 = np.zeros(nk), np.zeros(nk)
+        el, el2 = np.zeros((7, nk), np.complex128), np.zeros((7, nk), np.complex128)
+        lib().orc_get_spline(self.h, _d(ro) if False else ro.ctypes.data_as(D), ro2.ctypes.data_as(D),
+                             _c(el), _c(el2))
+        return ro, ro2, el, el2
+
+    def eval_elcon(self, r, nl):
+        ro = C.c_double()
+        z = np.zeros(7, np.complex128)
+        q = np.zeros(2)
+        lib().orc_eval_elcon(self.h, float(r), int(nl), C.byref(ro), _c(z), q.ctypes.data_as(D))
+        return ro.value, z, q
+
+    def start_radius(self, omre, dll1, tor):
+        r = C.c_double()
+        lib().orc_start_radius(self.h, float(omre), float(dll1), int(bool(tor)), C.byref(r))
+        return r.value
+
+    def initial_values(self, omre, omim, dll1, r, nl, kind):
+        y = np.zeros(5, np.complex128)
+        h = C.c_double()
+        rc = lib().orc_initial_values(self.h, omre, omim, dll1, r, nl, kind, _c(y), C.byref(h))
+        return rc, y, h.value
+
+    def toroidal_minors(self, omre, omim, dll1, jsl, jsr, eps):
+        n = self.nodes.nnod
+        u, d, det = (np.zeros((n, 2), np.complex128), np.zeros((n, 2), np.complex128), np.zeros(n, np.complex128))
+        rc = lib().orc_toroidal_minors(self.h, omre, omim, dll1, jsl, jsr, eps, _c(u), _c(d), _c(det),
+                                       self.stats.ctypes.data_as(L))
+        return rc, u, d, det
+
+    def spheroidal_minors(self, omre, omim, dll1, jsl, jsr, eps):
+        n = self.nodes.nnod
+        u, d = np.zeros((n, 6), np.complex128), np.zeros((n, 6), np.complex128)
+        ulg, dlg, det = np.zeros(n), np.zeros(n), np.zeros(n, np.complex128)
+        rc = lib().orc_spheroidal_minors(self.h, omre, omim, dll1, jsl, jsr, eps, _c(u), _c(d),
+                                         ulg.ctypes.data_as(D), dlg.ctypes.data_as(D), _c(det),
+                                         self.stats.ctypes.data_as(L))
+        return rc, u, d, ulg, dlg, det
+
+    def gfk_column(self, omre, omim, dll1, jre, istyp, eps):
+        n = self.nodes.nnod
+        nsp = 10 if istyp == 1 else 5
+        gf = np.zeros((n, nsp), np.complex128)
+        ujs = np.zeros((n, 4, 4), np.complex128)
+        ujt = np.zeros((n, 2, 2), np.complex128)
+        flag = C.c_int()
+        rc = lib().orc_gfk_column(self.h, omre, omim, dll1, jre, istyp, eps, _c(gf), _c(ujs), _c(ujt),
+                                  C.byref(flag), self.stats.ctypes.data_as(L))
+        return rc, gf, ujs, ujt, flag.value
+
+    def minors_degrees(self, omre, omim, lmin, nl, jsl, jsr, eps, nthreads=1):
+        """Toroidal + spheroidal minors for l = lmin..lmin+nl-1 (results discarded); returns status."""
+        status = np.zeros(nl, np.int32)
+        lib().orc_minors_degrees(self.h, omre, omim, self.nem.rearth, lmin, nl, jsl, jsr, eps, nthreads,
+                                 status.ctypes.data_as(I), self.stats.ctypes.data_as(L))
+        return status
+
+    def gfk_degrees(self, omre, omim, lmin, nl, jre, istyp, eps, nthreads=1):
+        """Columns for l = lmin..lmin+nl-1 at the current frequency; gf[n, isp, il]."""
+        n = self.nodes.nnod
+        nsp = 10 if istyp == 1 else 5
+        gf = np.zeros((n, nsp, nl), np.complex128)
+        status = np.zeros(nl, np.int32)
+        rc = lib().orc_gfk_degrees(self.h, omre, omim, self.nem.rearth, lmin, nl, jre, istyp, eps, nthreads,
+                                   _c(gf), status.ctypes.data_as(I), self.stats.ctypes.data_as(L))
+        return rc, gf, status
+
+
+def propagate_const(A, x1, x2, htry, eps, y0):
+    A = np.ascontiguousarray(A, dtype=np.complex128)
+    y0 = np.ascontiguousarray(y0, dtype=np.complex128)
+    y1 = np.zeros_like(y0)
+    st = np.zeros(10, np.int64)
+    rc = lib().orc_propagate_const(A.shape[0], _c(A), x1, x2, htry, eps, _c(y0), _c(y1), st.ctypes.data_as(L))
+    return rc, y1, st
+
+
+def sphancf(zx, el, kind):
+    out = np.zeros(2)
+    rc = lib().orc_sphancf(zx.real, zx.imag, el, kind, out.ctypes.data_as(D))
+    return rc, complex(out[0], out[1])
+
+
+def zetl(zx2, el):
+    out = np.zeros(2)
+    rc = lib().orc_zetl(zx2.real, zx2.imag, el, out.ctypes.data_as(D))
+    return rc, complex(out[0], out[1])
+
+
+def num_threads():
+    return lib().orc_num_threads()
+
+
+def reciprocity_sph(elp1, re, r, state_node, bgin):
+    bgin = np.asfortranarray(bgin, dtype=np.complex128)
+    flat = np.ascontiguousarray(bgin.T.reshape(-1))        # column-major order
+    out = np.zeros(16, np.complex128)
+    lib().orc_reciprocity_sph(elp1, re, r, state_node, flat.ctypes.data_as(D), out.ctypes.data_as(D))
+    return out.reshape(4, 4).T                               # back to [comp, jump]
+
+
+def vti(fratio, ro, alfv, alfh, betv, beth, eta, qkinv, qminv):
+    z = np.zeros(7, np.complex128)
+    lib().orc_vti(fratio, ro, alfv, alfh, betv, beth, eta, qkinv, qminv, z.ctypes.data_as(D))
+    return z

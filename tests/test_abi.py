@@ -1,0 +1,32 @@
+"""The C-ABI library loads and exports every symbol include/gemini_fk.h declares."""
+import ctypes
+import os
+import re
+
+from gemini_unified_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "gemini_fk.h")).read()
+    return sorted(set(re.findall(r"^int (gm_\w+)\(", txt, flags=re.M)))
+
+
+def test_header_matches_binding_list():
+    assert declared_symbols() == sorted(_lib.EXPORTS)
+
+
+def test_library_exports_all_symbols():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+
+
+def test_no_device_is_a_clean_error():
+    """gm_ctx_create on a host without a usable device index returns an error code."""
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    rc = L.gm_ctx_create(10_000, ctypes.byref(h))
+    assert rc != 0 and not h.value
+    assert len(_lib.last_error(None)) > 0

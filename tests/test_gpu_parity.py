@@ -1,0 +1,98 @@
+"""GPU parity of the Green-FK path against the CPU oracle (C1 model: seeded
+global model, source node 30 km deep, receiver node at the surface)."""
+import numpy as np
+import pytest
+
+import oracle
+from gemini_unified_amd import GreenFKEngine, MC_PI, dll1_of
+
+pytestmark = pytest.mark.gpu
+
+EPS = 1e-6
+TLEN = 2048.0
+SIGMA = 5.0 / TLEN
+# frequencies (Hz) x degree ranges exercised; together they cover l = 0, l = 1,
+# propagating and evanescent degrees, low and high frequency
+CASES = [(2 / TLEN, 0, 40), (40 / TLEN, 0, 40), (40 / TLEN, 280, 40), (102 / TLEN, 0, 24), (102 / TLEN, 560, 41)]
+
+
+def rel_l2(a, b):
+    """per-column relative L2 error over all but the last axis; NaN == NaN"""
+    both = np.isnan(a) & np.isnan(b)
+    a = np.where(both, 0, a)
+    b = np.where(both, 0, b)
+    k = a.shape[-1]
+    num = np.linalg.norm((a - b).reshape(-1, k), axis=0)
+    den = np.linalg.norm(b.reshape(-1, k), axis=0)
+    return np.where(den > 0, num / np.where(den > 0, den, 1), num)
+
+
+@pytest.fixture(scope="module")
+def engines(c1_setup):
+    nem, nodes, jre = c1_setup
+    ex = GreenFKEngine(nem, nodes, variant="exact")
+    fa = GreenFKEngine(nem, nodes, variant="fast")
+    yield ex, fa
+    ex.close()
+    fa.close()
+
+
+@pytest.fixture(scope="module")
+def orc(c1_setup):
+    nem, nodes, jre = c1_setup
+    return oracle.Oracle(nem, nodes)
+
+
+@pytest.mark.parametrize("f,lmin,nl", CASES)
+def test_minors_match_oracle(engines, orc, c1_setup, f, lmin, nl):
+    nem, nodes, jre = c1_setup
+    ex, fa = engines
+    g = ex.minors([f], lmin, nl, SIGMA, EPS, jre, jre)
+    assert (g["status"] == 0).all()
+    orc.set_frequency(f, 3)
+    om = 2.0 * MC_PI * f
+    worst = 0.0
+    for il in range(nl):
+        l = lmin + il
+        dll1 = dll1_of(l, nem.rearth)
+        rc, u, d, det = orc.toroidal_minors(om, -SIGMA, dll1, jre, jre, EPS)
+        assert rc == 0
+        np.testing.assert_allclose(g["u_tor"][:, :, il], u, rtol=1e-10, atol=1e-300)
+        np.testing.assert_allclose(g["d_tor"][:, :, il], d, rtol=1e-10, atol=1e-300)
+        rc, us, ds, ulg, dlg, dets = orc.spheroidal_minors(om, -SIGMA, dll1, jre, jre, EPS)
+        assert rc == 0
+        # minor 6 is -m1/dll1: infinite / NaN for l = 0 in the reference as well
+        for a, b in ((g["u_sph"][:, :, il], us), (g["d_sph"][:, :, il], ds)):
+            both = ~np.isfinite(a) & ~np.isfinite(b)
+            a = np.where(both, 0, a)
+            b = np.where(both, 0, b)
+            den = np.abs(b).max()
+            if den > 0:
+                worst = max(worst, np.abs(a - b).max() / den)
+        np.testing.assert_allclose(g["ulg"][:, il], ulg, rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(g["dlg"][:, il], dlg, rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(g["det_sph"][:, il], dets, rtol=1e-8, atol=1e-300)
+    assert worst < 1e-10, worst
+
+
+@pytest.mark.parametrize("f,lmin,nl", CASES)
+def test_gfk_columns_match_oracle(engines, orc, c1_setup, f, lmin, nl):
+    nem, nodes, jre = c1_setup
+    ex, fa = engines
+    orc.set_frequency(f, 3)
+    orc.stats[:] = 0
+    rc, ref, status = orc.gfk_degrees(2.0 * MC_PI * f, -SIGMA, lmin, nl, jre, 1, EPS)
+    assert rc == 0
+    r = ex.gfk([f], lmin, nl, SIGMA, EPS, jre, 1)
+    err = rel_l2(r["gf"], ref)
+    print(f"exact variant f={f:.5f} l={lmin}..{lmin + nl - 1}: max rel-L2 {err.max():.2e}")
+    assert err.max() < 1e-8
+    # the FP32 file values are the rounded FP64 values
+    np.testing.assert_array_equal(r["gfre"], ref.real.astype(np.float32))
+    # identical step sequences => identical work counters
+    np.testing.assert_array_equal(r["stats"], orc.stats)
+    rf = fa.gfk([f], lmin, nl, SIGMA, EPS, jre, 1)
+    errf = rel_l2(rf["gf"], ref)
+    print(f"fast variant  f={f:.5f}: max rel-L2 {errf.max():.2e}, p99 {np.quantile(errf, 0.99):.2e}")
+    assert np.quantile(errf, 0.99) < 1e-8
+    assert errf.max() < 1e-5

@@ -1,0 +1,357 @@
+"""Pins of the CPU oracle against closed forms, library routines and invariants.
+
+Nothing in the reference pins the hot path numerically (its tests are ad-hoc
+executables without golden data for it), so each oracle function is checked
+against what the mathematics fixes, chosen so that a plausible transcription
+mistake (dropped term, wrong sign or index, transposed operand) fails a test.
+"""
+import numpy as np
+import pytest
+import scipy.linalg as sl
+import scipy.special as sp
+from scipy.interpolate import CubicSpline
+
+import oracle
+from gemini_unified_amd import ExternalRadialNodes, MC_PI, NodeEarthmodel, dll1_of, global_model
+from gemini_unified_amd.earthmodel import _layered_json
+
+RNG = np.random.default_rng(12345)
+
+
+# ---------------------------------------------------------------------------
+# Bulirsch-Stoer engine
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("nvar", [2, 4, 5])
+def test_engine_matches_matrix_exponential(nvar):
+    A = (RNG.normal(size=(nvar, nvar)) + 1j * RNG.normal(size=(nvar, nvar))) * 0.4
+    y0 = RNG.normal(size=nvar) + 1j * RNG.normal(size=nvar)
+    rc, y1, st = oracle.propagate_const(A, 0.0, 2.5, 0.05, 1e-11, y0)
+    assert rc == 0
+    ex = sl.expm(A * 2.5) @ y0
+    assert np.linalg.norm(y1 - ex) / np.linalg.norm(ex) < 1e-9
+    assert st[1] > 5          # several accepted steps were needed
+    # downward integration is the inverse map
+    rc, y2, _ = oracle.propagate_const(A, 2.5, 0.0, 0.05, 1e-11, y1)
+    assert np.linalg.norm(y2 - y0) / np.linalg.norm(y0) < 1e-8
+
+
+def test_extrapolation_is_exact_for_nilpotent_system():
+    """y' = A y with A^3 = 0: every modified-midpoint sequence is a polynomial in
+    the substep; Neville extrapolation in h removes the error terms, so the
+    first error test (after 3 sequences) passes at round-off and the solution is
+    (I + A x + A^2 x^2/2) y0."""
+    A = np.triu(RNG.normal(size=(3, 3)) + 1j * RNG.normal(size=(3, 3)), 1)
+    A2 = A @ A
+    assert np.abs(A2 @ A).max() == 0
+    y0 = RNG.normal(size=3) + 1j * RNG.normal(size=3)
+    x = 1.7
+    rc, y1, st = oracle.propagate_const(A, 0.0, x, 10.0, 1e-12, y0)
+    assert rc == 0
+    ex = y0 + x * (A @ y0) + 0.5 * x * x * (A2 @ y0)
+    assert np.linalg.norm(y1 - ex) / np.linalg.norm(ex) < 1e-13
+    assert st[1] == 1 and st[2] == 0 and st[3] == 3     # one step, accepted after 3 sequences
+
+
+def test_step_control_constants():
+    """A step that needs more than 6 sequences shrinks the next step by the FP32
+    value of 0.8; fewer than 6 grows it by 1.25; a failure halves it.  Observed
+    through the step counts of a stiff-ish oscillator."""
+    w = 40.0
+    A = np.array([[0, 1], [-w * w, 0]], dtype=np.complex128)
+    y0 = np.array([1.0, 0.0], np.complex128)
+    rc, y1, st = oracle.propagate_const(A, 0.0, 1.0, 1.0, 1e-8, y0)
+    assert rc == 0
+    np.testing.assert_allclose(y1, [np.cos(w), -w * np.sin(w)], rtol=0, atol=1e-5 * w)
+    assert st[2] >= 1          # the oversized first step was rejected
+
+
+# ---------------------------------------------------------------------------
+# model: complex moduli and splines
+# ---------------------------------------------------------------------------
+def test_vti_isotropic_closed_form():
+    ro, vp, vs, qk, qm, fr = 3300.0, 8000.0, 4500.0, 1 / 57823.0, 1 / 143.0, 0.031
+    z = oracle.vti(fr, ro, vp, vp, vs, vs, 1.0, qk, qm)
+    pi32 = float(np.float32(np.pi))
+    mu0, kap0 = ro * vs ** 2, ro * (vp ** 2 - 4.0 / 3.0 * vs ** 2)
+    phk, phm = np.arctan(qk), np.arctan(qm)
+    zk = kap0 * np.exp(1j * phk) * fr ** (2 * phk / pi32)
+    zm = mu0 * np.exp(1j * phm) * fr ** (2 * phm / pi32)
+    want = [zk + 4 / 3 * zm, zk + 4 / 3 * zm, zk - 2 / 3 * zm, zm, zm, zk, zm]
+    np.testing.assert_allclose(z, want, rtol=1e-13)
+    # fluid: shear moduli vanish, A = C = F = kappa
+    z = oracle.vti(fr, ro, vp, vp, 0.0, 0.0, 1.0, qk, 0.0)
+    zk = ro * vp ** 2 * np.exp(1j * phk) * fr ** (2 * phk / pi32)
+    np.testing.assert_allclose(z, [zk, zk, zk, 0, 0, zk, 0], rtol=1e-13, atol=0)
+
+
+def test_vti_anisotropic_relations():
+    ro, vpv, vph, vsv, vsh, eta = 3380.0, 7900.0, 8150.0, 4400.0, 4600.0, 0.93
+    z = oracle.vti(1.0, ro, vpv, vph, vsv, vsh, eta, 0.0, 0.0)      # elastic: real moduli
+    A, Cc, F, L, N = ro * vph ** 2, ro * vpv ** 2, eta * (ro * vph ** 2 - 2 * ro * vsv ** 2), ro * vsv ** 2, ro * vsh ** 2
+    np.testing.assert_allclose(z[:5].real, [A, Cc, F, L, N], rtol=1e-13)
+    assert np.abs(z.imag).max() == 0
+
+
+def test_natural_spline_matches_scipy():
+    nem = global_model(1, 200)
+    nodes = ExternalRadialNodes.from_blocks(nem, [1], [30000.0])
+    orc = oracle.Oracle(nem, nodes)
+    orc.set_frequency(0.03, 3)
+    ro, ro2, el, el2 = orc.spline()
+    for nl in range(2, nem.nlay + 1):
+        j1 = (nem.iktop[nl - 2] + 1) if nl > 1 else 1
+        j2 = nem.iktop[nl - 1]
+        sl_ = slice(j1 - 1, j2)
+        x = nem.rk[sl_]
+        for y, y2 in [(ro[sl_], ro2[sl_])] + [(el[k, sl_].real, el2[k, sl_].real) for k in range(7)] + \
+                     [(el[k, sl_].imag, el2[k, sl_].imag) for k in range(7)]:
+            if np.all(y == 0):
+                assert np.all(y2 == 0)
+                continue
+            want = CubicSpline(x, y, bc_type="natural")(x, 2)
+            scale = np.abs(want).max() + 1e-300
+            assert np.abs(y2 - want).max() / scale < 1e-9
+
+
+def test_spline_evaluation_interpolates_knots_and_is_c2():
+    nem = global_model(1, 200)
+    nodes = ExternalRadialNodes.from_blocks(nem, [1], [30000.0])
+    orc = oracle.Oracle(nem, nodes)
+    orc.set_frequency(0.03, 3)
+    ro, ro2, el, el2 = orc.spline()
+    nl = 4
+    j1, j2 = nem.iktop[nl - 2] + 1, nem.iktop[nl - 1]
+    for j in range(j1 + 1, j2 - 1):
+        r = nem.rk[j - 1]
+        rho, z, _ = orc.eval_elcon(r, nl)
+        np.testing.assert_allclose(rho, ro[j - 1], rtol=1e-13)
+        np.testing.assert_allclose(z, el[:, j - 1], rtol=1e-12)
+    # against scipy's spline between knots
+    x = nem.rk[j1 - 1:j2]
+    cs = CubicSpline(x, el[4, j1 - 1:j2].real, bc_type="natural")
+    for r in np.linspace(x[0] + 1, x[-1] - 1, 37):
+        _, z, _ = orc.eval_elcon(r, nl)
+        assert abs(z[4].real - cs(r)) / abs(cs(r)) < 1e-11
+
+
+# ---------------------------------------------------------------------------
+# start values: continued fractions for spherical Bessel / Hankel ratios
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("x,el", [(0.7, 3.0), (2.5 + 0.3j, 10.0), (15.0 - 0.2j, 40.0), (30.0 + 1j, 5.0)])
+def test_zetl_bessel_ratio(x, el):
+    rc, z = oracle.zetl(complex(x) ** 2, el)
+    assert rc == 0
+    n = int(el)
+    want = x * sp.spherical_jn(n + 1, x) / sp.spherical_jn(n, x)
+    assert abs(z - want) / abs(want) < 1e-5
+
+
+@pytest.mark.parametrize("x,el,kind", [(3.0 + 0.1j, 2.0, 1), (25.0 - 0.5j, 7.0, 1), (12.0 + 0.3j, 20.0, 1),
+                                       (60.0 - 1j, 3.0, 1)])
+def test_sphancf_hankel_ratio(x, el, kind):
+    # kind 1 only: kind 2 serves the full-space start values, a branch outside this build
+    rc, z = oracle.sphancf(complex(x), el, kind)
+    assert rc == 0
+    n = int(el)
+    s = 1 if kind == 1 else -1
+    h = lambda m: sp.spherical_jn(m, x) + s * 1j * sp.spherical_yn(m, x)
+    want = x * h(n + 1) / h(n)
+    assert abs(z - want) / abs(want) < 1e-5
+
+
+# ---------------------------------------------------------------------------
+# minors: invariants of the radial systems
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def mantle_nodes():
+    nem = global_model(1, 200)
+    nodes = ExternalRadialNodes.from_blocks(nem, [10], [100000.0])
+    orc = oracle.Oracle(nem, nodes)
+    return nem, nodes, orc
+
+
+@pytest.mark.parametrize("f,l", [(0.02, 5), (0.02, 80), (0.045, 250)])
+def test_toroidal_wronskian(mantle_nodes, f, l):
+    """trace A_tor = -2/r, so r^2 (u1 d2 - d1 u2) is the same at every node."""
+    nem, nodes, orc = mantle_nodes
+    orc.set_frequency(f, 3)
+    om = 2 * MC_PI * f
+    rc, u, d, det = orc.toroidal_minors(om, -0.0025, dll1_of(l, nem.rearth), 1, nodes.nnod, 1e-10)
+    assert rc == 0
+    w = nodes.rnod ** 2 * det
+    ok = np.abs(det) > 0
+    assert ok.sum() >= 3
+    assert np.abs(w[ok] / w[ok][-1] - 1).max() < 1e-7
+
+
+@pytest.mark.parametrize("f,l", [(0.02, 5), (0.02, 80), (0.045, 250)])
+def test_spheroidal_determinant_and_pluecker(mantle_nodes, f, l):
+    """trace of the adjoint system is +4/r, so r^4 det is constant; the minors of
+    two solutions satisfy the Pluecker relation m1 m6 - m2 m5 + m3 m4 = 0."""
+    nem, nodes, orc = mantle_nodes
+    orc.set_frequency(f, 3)
+    om = 2 * MC_PI * f
+    rc, u, d, ulg, dlg, det = orc.spheroidal_minors(om, -0.0025, dll1_of(l, nem.rearth), 1, nodes.nnod, 1e-10)
+    assert rc == 0
+    w = nodes.rnod ** 4 * det
+    ok = np.abs(det) > 0
+    assert ok.sum() >= 3
+    assert np.abs(w[ok] / w[ok][-1] - 1).max() < 1e-7
+    for m in (u, d):
+        pl = m[:, 0] * m[:, 5] - m[:, 1] * m[:, 4] + m[:, 2] * m[:, 3]
+        scale = np.abs(m).max(axis=1) ** 2 + 1e-300
+        assert np.abs(pl / scale).max() < 1e-9
+
+
+def homogeneous_model(vp=8000.0, vs=4500.0, rho=3300.0, n=60):
+    R = 6371000.0
+    r2 = np.linspace(100000.0, R, n)
+    sph = dict(r=np.array([100000.0]), fluid=False, vp=[vp], vs=[vs], density=[rho], qk=[57823.0], qm=[300.0])
+    lay = dict(r=r2, fluid=False, vp=np.full(n, vp), vs=np.full(n, vs), density=np.full(n, rho),
+               qk=np.full(n, 57823.0), qm=np.full(n, 300.0))
+    return NodeEarthmodel.from_json(_layered_json("homogeneous", R, [sph, lay]))
+
+
+@pytest.mark.parametrize("f,l", [(0.02, 30), (0.035, 120)])
+def test_homogeneous_sphere_known_answer(f, l):
+    """In a homogeneous sphere the start-value routine gives the exact regular
+    solution at any radius; integrated minors / toroidal vectors must stay
+    parallel to it (pins every entry of both system matrices).  The known answer
+    itself carries the 1e-6 stopping tolerance of the continued fractions, so the
+    bound is 1e-4; a wrong matrix entry shows as an O(1) deviation."""
+    nem = homogeneous_model()
+    nodes = ExternalRadialNodes.split_at_discos(nem, np.array([2.5e6, 3.5e6, 4.5e6, 5.5e6, nem.rearth]), nem.rearth)
+    orc = oracle.Oracle(nem, nodes)
+    orc.set_frequency(f, 3)
+    om, omim, dll1 = 2 * MC_PI * f, -0.0024, dll1_of(l, nem.rearth)
+    rst = orc.start_radius(om, dll1, False)
+    assert nem.rb[0] < rst < nodes.rnod[0]
+    rc, u, d, ulg, dlg, det = orc.spheroidal_minors(om, omim, dll1, 1, nodes.nnod, 1e-10)
+    assert rc == 0
+    rc2, ut, dt, dett = orc.toroidal_minors(om, omim, dll1, 1, nodes.nnod, 1e-10)
+    assert rc2 == 0
+    for k in range(nodes.nnod - 1):
+        r = nodes.rnod[k]
+        rc, ya, _ = orc.initial_values(om, omim, dll1, r, 2, 0)
+        assert rc == 0
+        v = u[k, :5]
+        c = np.vdot(v, ya) / np.vdot(v, v)
+        assert np.linalg.norm(ya - c * v) / np.linalg.norm(ya) < 1e-4
+        rc, yt, _ = orc.initial_values(om, omim, dll1, r, 2, 2)
+        v = ut[k]
+        c = np.vdot(v, yt[:2]) / np.vdot(v, v)
+        assert np.linalg.norm(yt[:2] - c * v) / np.linalg.norm(yt[:2]) < 1e-4
+
+
+# ---------------------------------------------------------------------------
+# unit-jump Green functions, reciprocity
+# ---------------------------------------------------------------------------
+def test_ujgf_jump_conditions(c1_setup):
+    """Across the source node the Green functions jump by the unit jump vector:
+    toroidal e*D - d*U = s_j; spheroidal M~d(f+) - M~u(f-) = s_j with the
+    adjoint start values f+ = -meps_u s_j/det, f- = +meps_d s_j/det."""
+    nem, nodes, jre = c1_setup
+    orc = oracle.Oracle(nem, nodes)
+    f = 50 / 2048.0
+    orc.set_frequency(f, 3)
+    om, dll1 = 2 * MC_PI * f, dll1_of(37, nem.rearth)
+    rc, gf, ujs, ujt, flag = orc.gfk_column(om, -5 / 2048.0, dll1, jre, 1, 1e-8)
+    assert rc == 0 and flag == 0
+    rc, ut, dt, dett = orc.toroidal_minors(om, -5 / 2048.0, dll1, jre, jre, 1e-8)
+    rc, us, ds, ulg, dlg, dets = orc.spheroidal_minors(om, -5 / 2048.0, dll1, jre, jre, 1e-8)
+    jn = jre - 1
+    # toroidal: value above (oracle) minus value below (d * U)
+    for j in range(2):
+        s = np.eye(2)[:, j]
+        d = (s[1] * dt[jn, 0] - s[0] * dt[jn, 1]) / dett[jn]
+        jump = ujt[jn][:, j] if ujt.ndim == 3 else None
+        below = d * ut[jn]
+        np.testing.assert_allclose(ujt[jn, j, :] - below, s, atol=1e-9 * np.abs(below).max() + 1e-12)
+
+    def mtil(m):
+        M = np.zeros((4, 4), complex)
+        idx = {(0, 1): 0, (0, 2): 1, (0, 3): 2, (1, 2): 3, (1, 3): 4, (2, 3): 5}
+        for (a, b), k in idx.items():
+            M[a, b] = m[k]
+            M[b, a] = -m[k]
+        return M
+
+    def meps(m):
+        E = np.zeros((4, 4), complex)
+        idx = {(0, 1): 0, (0, 2): 1, (0, 3): 2, (1, 2): 3, (1, 3): 4, (2, 3): 5}
+        for (a, b), k in idx.items():
+            vf = -1.0 + 2.0 * ((a + b + 2) % 2)
+            E[a, b] = vf * m[5 - k]
+            E[b, a] = -E[a, b]
+        return E
+
+    for j in range(4):
+        s = np.eye(4)[:, j]
+        fminus = meps(ds[jn]) @ s / dets[jn]
+        below = mtil(us[jn]) @ fminus
+        above = ujs[jn, j, :]
+        scale = np.abs(above).max() + np.abs(below).max()
+        np.testing.assert_allclose(above - below, s, atol=1e-7 * scale + 1e-12)
+
+
+def test_reciprocity_general_relation():
+    """G(a,l)(re,rs) = s_a (rs/re)^2 sum_i G(i, c_a)(rs,re) T(i,l) / (elp1 for a>2),
+    c = (2,1,4,3), s = (+,-,+,-), T(2,1)=-1, T(1,2)=1, T(4,3)=-elp1, T(3,4)=elp1."""
+    elp1, re, r = 137.0 * 138.0, 6.37e6, 6.1e6
+    G = RNG.normal(size=(4, 4)) + 1j * RNG.normal(size=(4, 4))
+    out = oracle.reciprocity_sph(elp1, re, r, 0, G)
+    T = np.zeros((4, 4))
+    T[1, 0], T[0, 1], T[3, 2], T[2, 3] = -1.0, 1.0, -elp1, elp1
+    c, s = [1, 0, 3, 2], [1, -1, 1, -1]
+    f = (r / re) ** 2
+    want = np.zeros((4, 4), complex)
+    for a in range(4):
+        for l in range(4):
+            want[a, l] = s[a] * f * sum(G[i, c[a]] * T[i, l] for i in range(4)) / (elp1 if a >= 2 else 1.0)
+    np.testing.assert_allclose(out, want, rtol=1e-13, atol=1e-13 * np.abs(want).max())
+
+
+def test_source_below_start_radius_gives_zero(c1_setup):
+    """rstart >= receiver radius: no motion is excited, all minors vanish."""
+    nem, nodes, jre = c1_setup
+    orc = oracle.Oracle(nem, nodes)
+    f = 2 / 2048.0
+    orc.set_frequency(f, 3)
+    om, dll1 = 2 * MC_PI * f, dll1_of(5000, nem.rearth)
+    assert orc.start_radius(om, dll1, False) >= nodes.rnod[0]          # source node 30 km deep
+    rc, u, d, ulg, dlg, det = orc.spheroidal_minors(om, -5 / 2048.0, dll1, 1, 1, 1e-6)
+    assert rc == 0 and np.all(u == 0) and np.all(det == 0)
+
+
+def test_start_radius_decay_criterion(c1_setup):
+    """The start radius lies where the evanescent decay integral of sqrt(F(r)),
+    F = l(l+1)/r^2 - (omega/v)^2, accumulated in steps of (R - r_hs)/400 below
+    the turning point, first reaches ln(1e5)."""
+    nem, nodes, jre = c1_setup
+    orc = oracle.Oracle(nem, nodes)
+    f = 30 / 2048.0
+    orc.set_frequency(f, 3)
+    om, l = 2 * MC_PI * f, 100
+    dll1 = dll1_of(l, nem.rearth)
+    rs = orc.start_radius(om, dll1, False)
+
+    def F(r):
+        nl, _ = nem.layer_index(r)
+        rho, z, _ = orc.eval_elcon(r, nl)
+        v = np.sqrt((z[5] if nem.iflso[nl - 1] else z[6]).real / rho)
+        return dll1 / r ** 2 - (om / v) ** 2
+
+    dr = (nem.rearth - nem.rb[0]) / 400.0
+    rt = None
+    r = nem.rb[0]
+    while r < nem.rearth:
+        if F(r) > 0 and F(min(r + dr, nem.rearth)) <= 0:
+            rt = r + dr - dr * F(r + dr) / (F(r + dr) - F(r))
+            break
+        r = min(r + dr, nem.rearth)
+    assert rt is not None
+    q, r = 0.0, rt
+    while q < np.log(1e5):
+        r -= dr
+        q += dr * np.sqrt(F(r))
+    assert abs(r - rs) < 1e-6 * nem.rearth
