@@ -6,11 +6,18 @@
 #include <cmath>
 #include <string>
 #include <vector>
+#include <algorithm>
 #include "gm_common.h"
 #include "../../include/gemini_fk.h"
 
-namespace gm_exact { void launch_stage(int, const GmModel*, const GmBatch&, int, const double*, int, GmKnot*, hipStream_t); }
-namespace gm_fast { void launch_stage(int, const GmModel*, const GmBatch&, int, const double*, int, GmKnot*, hipStream_t); }
+namespace gm_exact {
+void launch_stage(int, const GmModel*, const GmBatch&, int, const double*, int, GmKnot*, hipStream_t, int, size_t, int);
+int persist_blocks_per_cu(size_t, int);
+}
+namespace gm_fast {
+void launch_stage(int, const GmModel*, const GmBatch&, int, const double*, int, GmKnot*, hipStream_t, int, size_t, int);
+int persist_blocks_per_cu(size_t, int);
+}
 
 // per-system constants of the algorithmic work model
 static const int kN[5] = {5, 2, 2, 2, 4};
@@ -20,6 +27,10 @@ static const int kESYS[5] = {219, 148, 156, 196, 206};
 struct gm_ctx {
   int device = 0;
   int variant = 1;
+  int persistent = 1;      // 1: substep-scheduled persistent kernels, 0: one lane per job
+  int unit = 1024;         // instances per work unit of the persistent kernels
+  int waves = 1;           // waves per SIMD the persistent kernels are compiled for (1 or 2)
+  int ncu = 0;
   hipStream_t stream = nullptr;
   std::string err;
   GmModel hmodel;
@@ -61,6 +72,8 @@ extern "C" int gm_ctx_create(int device, gm_ctx** out) {
     return fail(nullptr, GM_ERR_HIP, "context creation failed on device %d", device);
   }
   for (int i = 0; i < 7; ++i) hipEventCreate(&c->ev[i]);
+  hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device);
+  if (c->ncu <= 0) c->ncu = 1;
   memset(&c->hmodel, 0, sizeof(GmModel));
   *out = c;
   return GM_OK;
@@ -83,6 +96,21 @@ extern "C" int gm_set_option(gm_ctx* c, int option, int value) {
   if (option == GM_OPT_VARIANT) {
     if (value != 0 && value != 1) return fail(c, GM_ERR_ARG, "variant must be 0 or 1");
     c->variant = value;
+    return GM_OK;
+  }
+  if (option == GM_OPT_PERSISTENT) {
+    if (value != 0 && value != 1) return fail(c, GM_ERR_ARG, "persistent must be 0 or 1");
+    c->persistent = value;
+    return GM_OK;
+  }
+  if (option == GM_OPT_WAVES) {
+    if (value != 1 && value != 2) return fail(c, GM_ERR_ARG, "waves must be 1 or 2");
+    c->waves = value;
+    return GM_OK;
+  }
+  if (option == GM_OPT_UNIT) {
+    if (value < 64 || value > (1 << 20)) return fail(c, GM_ERR_ARG, "unit size must be in [64, 2^20]");
+    c->unit = value;
     return GM_OK;
   }
   return fail(c, GM_ERR_ARG, "unknown option %d", option);
@@ -180,6 +208,7 @@ struct Plan {
   int ninst = 0, nfreq = 0;
   std::vector<int> q, l;
   std::vector<double> omre;
+  std::vector<int> uq, us, ue;      // work units (frequency slot, instance range)
 };
 }
 
@@ -215,6 +244,20 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
     for (int i = 0; i < nl[f]; ++i) { P.q.push_back(f); P.l.push_back(lmin[f] + i); }
   }
   P.ninst = (int)P.q.size();
+  {
+    // work units: degree ranges of one frequency, highest frequency first and
+    // low degrees (the longest integrations) first within a frequency
+    std::vector<int> off(nfreq + 1, 0);
+    for (int f = 0; f < nfreq; ++f) off[f + 1] = off[f] + nl[f];
+    std::vector<int> order(nfreq);
+    for (int f = 0; f < nfreq; ++f) order[f] = f;
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return freqs[a] > freqs[b]; });
+    const int usz = c->persistent ? c->unit : 256;   // one-lane-per-job kernels: one unit per block
+    for (int f : order)
+      for (int a = off[f]; a < off[f + 1]; a += usz) {
+        P.uq.push_back(f); P.us.push_back(a); P.ue.push_back(std::min(a + usz, off[f + 1]));
+      }
+  }
   const size_t n = P.ninst;
   const int nsp = istyp == 1 ? 10 : 5;
   HIPCHK(c, hipSetDevice(c->device));
@@ -236,6 +279,9 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   size_t o_gfre = (full && (gfre && !dev)) ? take(4 * nsp * nnod * n + 4) : 0;
   size_t o_gfim = (full && (gfim && !dev)) ? take(4 * nsp * nnod * n + 4) : 0;
   size_t o_hist = take(sizeof(unsigned long long) * 6 * 8);
+  const int nunits = (int)P.uq.size();
+  size_t o_uq = take(sizeof(int) * (nunits + 1)), o_ust = take(sizeof(int) * (nunits + 1));
+  size_t o_ue = take(sizeof(int) * (nunits + 1)), o_uc = take(sizeof(int) * 4);
   if (off > c->dbuf_size) {
     if (c->dbuf) { hipFree(c->dbuf); c->dbuf = nullptr; c->dbuf_size = 0; }
     if (hipMalloc(&c->dbuf, off) != hipSuccess) return fail(c, GM_ERR_NOMEM, "device allocation of %zu bytes failed", off);
@@ -249,6 +295,9 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   HIPCHK(c, hipMemcpyAsync(base + o_omre, P.omre.data(), sizeof(double) * nfreq, hipMemcpyHostToDevice, s));
   HIPCHK(c, hipMemcpyAsync(base + o_q, P.q.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
   HIPCHK(c, hipMemcpyAsync(base + o_l, P.l.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(base + o_uq, P.uq.data(), sizeof(int) * nunits, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(base + o_ust, P.us.data(), sizeof(int) * nunits, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(base + o_ue, P.ue.data(), sizeof(int) * nunits, hipMemcpyHostToDevice, s));
   std::vector<unsigned long long> zero(48, 0ULL);
   HIPCHK(c, hipMemcpyAsync(base + o_hist, zero.data(), sizeof(unsigned long long) * 48, hipMemcpyHostToDevice, s));
   GmBatch B;
@@ -269,19 +318,29 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
     if ((B.gfre == nullptr) != (B.gfim == nullptr)) return fail(c, GM_ERR_ARG, "gfre and gfim must both be given");
   }
   B.hist = (unsigned long long*)(base + o_hist);
+  B.nunits = nunits;
+  B.unit_q = (const int*)(base + o_uq); B.unit_start = (const int*)(base + o_ust); B.unit_end = (const int*)(base + o_ue);
+  B.unit_counter = (int*)(base + o_uc);
   auto launch = c->variant == 0 ? gm_exact::launch_stage : gm_fast::launch_stage;
+  auto bpc = c->variant == 0 ? gm_exact::persist_blocks_per_cu : gm_fast::persist_blocks_per_cu;
+  const size_t lds = sizeof(GmKnot) * (size_t)(m.nk + 1);
+  const int pgrid = std::max(1, std::min(nunits, c->ncu * bpc(lds, c->waves)));
   GmKnot* dT = (GmKnot*)(base + o_tables);
   const double* dF = (const double*)(base + o_freqs);
   HIPCHK(c, hipEventRecord(c->ev[0], s));
-  launch(0, dM, B, nfreq, dF, attmode, dT, s);
+  const bool pers = c->persistent != 0;
+  launch(0, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
   HIPCHK(c, hipEventRecord(c->ev[1], s));
-  launch(1, dM, B, nfreq, dF, attmode, dT, s);
+  launch(1, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
   HIPCHK(c, hipEventRecord(c->ev[2], s));
-  launch(2, dM, B, nfreq, dF, attmode, dT, s);
+  launch(pers ? 6 : 2, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
   HIPCHK(c, hipEventRecord(c->ev[3], s));
-  launch(3, dM, B, nfreq, dF, attmode, dT, s);
+  launch(pers ? 7 : 3, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
   HIPCHK(c, hipEventRecord(c->ev[4], s));
-  if (full) { launch(4, dM, B, nfreq, dF, attmode, dT, s); launch(5, dM, B, nfreq, dF, attmode, dT, s); }
+  if (full) {
+    launch(4, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
+    launch(5, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
+  }
   HIPCHK(c, hipEventRecord(c->ev[5], s));
   HIPCHK(c, hipGetLastError());
   // results

@@ -43,6 +43,8 @@ struct GmKnot {
   double ro, ro2;
   double er[7], ei[7];
   double e2r[7], e2i[7];
+  double rh;     // 1 / (rk(j+1) - rk(j)) inside the layer (0 at a layer top)
+  double pad;    // keeps the record a multiple of 16 bytes
 };
 
 // one batch of (omega, l) instances and its device buffers
@@ -69,5 +71,11 @@ struct GmBatch {
   double* __restrict__ gf;         // FK column, complex [nnod][nsp][n] (may be null)
   float* __restrict__ gfre;        // FP32 real part [nnod][nsp][n] (may be null)
   float* __restrict__ gfim;
-  unsigned long long* __restrict__ hist;   // [5 systems][8]
+  unsigned long long* __restrict__ hist;   // [6][8]: 5 systems + misc counters
+  // work units of the persistent kernels: instance ranges of one frequency
+  int nunits;
+  const int* __restrict__ unit_q;
+  const int* __restrict__ unit_start;
+  const int* __restrict__ unit_end;
+  int* __restrict__ unit_counter;          // one counter per persistent kernel launch
 };

@@ -3,6 +3,8 @@
 #include "gm_common.h"
 #pragma clang fp contract(fast)
 #define GM_NS gm_fast
+namespace gm_fast { static constexpr bool kFast = true; }
 #include "gm_device.inc"
 #include "gm_kernels.inc"
+#include "gm_persist.inc"
 #include "gm_launch.inc"

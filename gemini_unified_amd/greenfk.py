@@ -72,6 +72,17 @@ class GreenFKEngine:
         check(self.h, _lib.lib().gm_set_option(self.h, 1, VARIANTS[variant]))
         self.variant = variant
 
+    def configure(self, persistent=None, unit=None, waves=None):
+        """Kernel schedule options (all result-preserving): persistent substep-scheduled kernels or
+        one lane per job, work-unit size, waves per SIMD of the persistent kernels."""
+        L = _lib.lib()
+        if persistent is not None:
+            check(self.h, L.gm_set_option(self.h, 2, int(bool(persistent))))
+        if unit is not None:
+            check(self.h, L.gm_set_option(self.h, 3, int(unit)))
+        if waves is not None:
+            check(self.h, L.gm_set_option(self.h, 4, int(waves)))
+
     def timing(self):
         ms = np.zeros(6)
         _lib.lib().gm_last_timing(self.h, ms.ctypes.data_as(D))

@@ -54,6 +54,9 @@ typedef struct gm_ctx gm_ctx;
 
 /* options (gm_set_option) */
 #define GM_OPT_VARIANT 1       /* 0: exact (no FMA contraction, bitwise operation order), 1: fast (FMA) */
+#define GM_OPT_PERSISTENT 2    /* 1 (default): persistent substep-scheduled kernels, 0: one lane per job */
+#define GM_OPT_UNIT 3          /* (omega, l) instances per work unit of the persistent kernels */
+#define GM_OPT_WAVES 4         /* waves per SIMD the persistent kernels are built for: 1 or 2 */
 
 /* work counters returned in stats[10] (same meaning as the CPU oracle's) */
 #define GM_NSTAT 10            /* propagations, accepted steps, rejected steps, MMP sequences,

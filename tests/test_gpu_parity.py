@@ -27,11 +27,17 @@ def rel_l2(a, b):
     return np.where(den > 0, num / np.where(den > 0, den, 1), num)
 
 
-@pytest.fixture(scope="module")
-def engines(c1_setup):
+SCHEDULES = [dict(persistent=1, waves=1, unit=64), dict(persistent=1, waves=2, unit=1024),
+             dict(persistent=0)]
+
+
+@pytest.fixture(scope="module", params=SCHEDULES, ids=["persist-w1-u64", "persist-w2", "lane-per-job"])
+def engines(c1_setup, request):
     nem, nodes, jre = c1_setup
     ex = GreenFKEngine(nem, nodes, variant="exact")
     fa = GreenFKEngine(nem, nodes, variant="fast")
+    ex.configure(**request.param)
+    fa.configure(**request.param)
     yield ex, fa
     ex.close()
     fa.close()
