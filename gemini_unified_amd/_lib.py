@@ -17,7 +17,9 @@ VP = C.c_void_p
 
 # every symbol declared in include/gemini_fk.h
 EXPORTS = ["gm_ctx_create", "gm_ctx_destroy", "gm_set_option", "gm_last_error", "gm_model_set",
-           "gm_nodes_set", "gm_minors_batch", "gm_gfk_batch", "gm_last_timing", "gm_last_system_stats"]
+           "gm_nodes_set", "gm_minors_batch", "gm_gfk_batch", "gm_last_timing", "gm_last_system_stats",
+           "gm_mlp_nparams", "gm_mlp_forward", "gm_mlp_grad", "gm_mlp_train", "gm_set_proposer",
+           "gm_collect_steplog"]
 
 
 class GeminiFKError(RuntimeError):
@@ -45,6 +47,14 @@ def lib():
                                    C.c_int, D, F, F, I, LL]
         L.gm_last_timing.argtypes = [VP, D]
         L.gm_last_system_stats.argtypes = [VP, C.c_int, LL]
+        L.gm_mlp_nparams.restype = C.c_int
+        L.gm_mlp_forward.argtypes = [C.c_int, F, C.c_int, F, F]
+        L.gm_mlp_grad.argtypes = [C.c_int, F, C.c_int, F, F, C.c_float, C.c_float, F, F]
+        L.gm_mlp_train.argtypes = [C.c_int, F, C.c_int, F, F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_float,
+                                   C.c_uint, F]
+        L.gm_set_proposer.argtypes = [VP, F, F, C.c_float, C.c_int, C.c_int]
+        L.gm_collect_steplog.argtypes = [VP, C.c_int, D, I, I, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                                         C.c_int, F, I, LL]
         _lib = L
     return _lib
 

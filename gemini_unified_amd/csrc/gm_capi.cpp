@@ -27,7 +27,7 @@ static const int kESYS[5] = {219, 148, 156, 196, 206};
 struct gm_ctx {
   int device = 0;
   int variant = 1;
-  int persistent = 1;      // 1: substep-scheduled persistent kernels, 0: one lane per job
+  int persistent = 0;      // 1: substep-scheduled persistent kernels, 0: one lane per job (default, faster here)
   int unit = 1024;         // instances per work unit of the persistent kernels
   int waves = 1;           // waves per SIMD the persistent kernels are compiled for (1 or 2)
   int ncu = 0;
@@ -42,6 +42,14 @@ struct gm_ctx {
   hipEvent_t ev[7];
   double timing[6] = {0};
   unsigned long long last_hist[48] = {0};
+  // learned step proposer
+  float* dmlp = nullptr;
+  GmProposer prop;
+  int prop_enabled = 0;
+  // step logging of the next run (gm_collect_steplog)
+  int log_max = 0;
+  float* log_host = nullptr;
+  int log_count = 0;
 };
 
 static std::string g_create_err;
@@ -85,6 +93,7 @@ extern "C" int gm_ctx_destroy(gm_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   if (c->dbuf) hipFree(c->dbuf);
   if (c->dmodel) hipFree(c->dmodel);
+  if (c->dmlp) hipFree(c->dmlp);
   for (int i = 0; i < 7; ++i) hipEventDestroy(c->ev[i]);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -282,6 +291,9 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   const int nunits = (int)P.uq.size();
   size_t o_uq = take(sizeof(int) * (nunits + 1)), o_ust = take(sizeof(int) * (nunits + 1));
   size_t o_ue = take(sizeof(int) * (nunits + 1)), o_uc = take(sizeof(int) * 4);
+  const int ctl_mode = c->log_max > 0 ? 1 : (c->prop_enabled ? 2 : 0);
+  size_t o_log = ctl_mode == 1 ? take(sizeof(float) * GM_LOGREC * (size_t)c->log_max) : 0;
+  size_t o_logc = take(sizeof(int) * 4);
   if (off > c->dbuf_size) {
     if (c->dbuf) { hipFree(c->dbuf); c->dbuf = nullptr; c->dbuf_size = 0; }
     if (hipMalloc(&c->dbuf, off) != hipSuccess) return fail(c, GM_ERR_NOMEM, "device allocation of %zu bytes failed", off);
@@ -321,6 +333,16 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   B.nunits = nunits;
   B.unit_q = (const int*)(base + o_uq); B.unit_start = (const int*)(base + o_ust); B.unit_end = (const int*)(base + o_ue);
   B.unit_counter = (int*)(base + o_uc);
+  B.ctl_mode = ctl_mode;
+  B.mlp = c->dmlp;
+  B.prop = c->prop;
+  B.logbuf = ctl_mode == 1 ? (float*)(base + o_log) : nullptr;
+  B.logcount = (int*)(base + o_logc);
+  B.logmax = c->log_max;
+  {
+    int z4[4] = {0, 0, 0, 0};
+    HIPCHK(c, hipMemcpyAsync(base + o_logc, z4, sizeof z4, hipMemcpyHostToDevice, s));
+  }
   auto launch = c->variant == 0 ? gm_exact::launch_stage : gm_fast::launch_stage;
   auto bpc = c->variant == 0 ? gm_exact::persist_blocks_per_cu : gm_fast::persist_blocks_per_cu;
   const size_t lds = sizeof(GmKnot) * (size_t)(m.nk + 1);
@@ -328,7 +350,7 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   GmKnot* dT = (GmKnot*)(base + o_tables);
   const double* dF = (const double*)(base + o_freqs);
   HIPCHK(c, hipEventRecord(c->ev[0], s));
-  const bool pers = c->persistent != 0;
+  const bool pers = c->persistent != 0 && ctl_mode == 0;   // step logging / learned control: one lane per job
   launch(0, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
   HIPCHK(c, hipEventRecord(c->ev[1], s));
   launch(1, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
@@ -366,6 +388,15 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   HIPCHK(c, hipMemcpyAsync(hstatus.data(), base + o_status, sizeof(int) * n, hipMemcpyDeviceToHost, s));
   unsigned long long hist[48];
   HIPCHK(c, hipMemcpyAsync(hist, base + o_hist, sizeof hist, hipMemcpyDeviceToHost, s));
+  if (ctl_mode == 1) {
+    int cnt = 0;
+    HIPCHK(c, hipMemcpyAsync(&cnt, base + o_logc, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    c->log_count = cnt;
+    int ncopy = cnt < c->log_max ? cnt : c->log_max;
+    if (ncopy > 0 && c->log_host)
+      HIPCHK(c, hipMemcpyAsync(c->log_host, base + o_log, sizeof(float) * GM_LOGREC * (size_t)ncopy, hipMemcpyDeviceToHost, s));
+  }
   HIPCHK(c, hipStreamSynchronize(s));
   float ms;
   for (int i = 0; i < 5; ++i) { hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]); c->timing[i] = ms; }
@@ -400,4 +431,39 @@ extern "C" int gm_last_system_stats(gm_ctx* c, int sys, long long* stats) {
   for (int i = 0; i < GM_NSTAT; ++i) stats[i] = 0;
   derive_stats(c->last_hist, stats, sys);
   return GM_OK;
+}
+
+extern "C" int gm_set_proposer(gm_ctx* c, const float* params, const float* norm, float safety, int kcheck, int enable) {
+  if (!c) return GM_ERR_ARG;
+  if (!enable) { c->prop_enabled = 0; return GM_OK; }
+  if (!params || !norm || !(safety > 0.0f)) return fail(c, GM_ERR_ARG, "proposer needs parameters, normalisation and safety > 0");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (!c->dmlp && hipMalloc(&c->dmlp, sizeof(float) * GM_MLP_NPARAM) != hipSuccess)
+    return fail(c, GM_ERR_NOMEM, "proposer parameter allocation failed");
+  HIPCHK(c, hipMemcpy(c->dmlp, params, sizeof(float) * GM_MLP_NPARAM, hipMemcpyHostToDevice));
+  memset(&c->prop, 0, sizeof c->prop);
+  for (int i = 0; i < GM_MLP_IN; ++i) {
+    c->prop.mu[i] = norm[i];
+    c->prop.isd[i] = norm[GM_MLP_IN + i] > 0.0f ? 1.0f / norm[GM_MLP_IN + i] : 1.0f;
+  }
+  for (int o = 0; o < GM_MLP_OUT; ++o) { c->prop.omu[o] = norm[2 * GM_MLP_IN + o]; c->prop.osd[o] = norm[2 * GM_MLP_IN + 2 + o]; }
+  c->prop.safety = safety;
+  c->prop.kcheck = kcheck ? 1 : 0;
+  c->prop_enabled = 1;
+  return GM_OK;
+}
+
+extern "C" int gm_collect_steplog(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const int* nl,
+                                  double sigma, double eps, int attmode, int jsl, int jsr, int maxrec, float* records,
+                                  int* nrec, long long* stats) {
+  if (!c || maxrec <= 0 || !records) return GM_ERR_ARG;
+  c->log_max = maxrec;
+  c->log_host = records;
+  c->log_count = 0;
+  int rc = run(c, nfreq, freqs, lmin, nl, sigma, eps, attmode, jsl, jsr, jsr, 1, false, 0, nullptr, nullptr, nullptr,
+               nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, stats);
+  c->log_max = 0;
+  c->log_host = nullptr;
+  if (nrec) *nrec = c->log_count;
+  return rc;
 }

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include "gm_mlp.h"
 
 #define GM_MAXK 512      // maximum earth-model knots
 #define GM_MAXL 32       // maximum layers
@@ -78,4 +79,13 @@ struct GmBatch {
   const int* __restrict__ unit_start;
   const int* __restrict__ unit_end;
   int* __restrict__ unit_counter;          // one counter per persistent kernel launch
+  // step control of the minors system: 0 plain (reference), 1 plain with step
+  // logging, 2 learned proposer
+  int ctl_mode;
+  const float* __restrict__ mlp;           // proposer parameters (GM_MLP_NPARAM floats)
+  float* __restrict__ logbuf;              // step log records [logmax][GM_LOGREC]
+  int* __restrict__ logcount;
+  int logmax;
+  GmProposer prop;
 };
+#define GM_LOGREC 16

@@ -115,6 +115,30 @@ class GreenFKEngine:
         check(self.h, rc)
         return st
 
+    def set_proposer(self, params=None, norm=None, safety=0.9, kcheck=False, enable=True):
+        """Install the learned step proposer (or remove it with enable=False)."""
+        if not enable:
+            check(self.h, _lib.lib().gm_set_proposer(self.h, None, None, 1.0, 0, 0))
+            return
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        nv = np.ascontiguousarray(norm, dtype=np.float32)
+        check(self.h, _lib.lib().gm_set_proposer(self.h, p.ctypes.data_as(F), nv.ctypes.data_as(F), float(safety),
+                                                 int(bool(kcheck)), 1))
+
+    def collect_steplog(self, freqs, lmin, nl, sigma, eps, jsl, jsr, maxrec=2_000_000,
+                        attmode="ATTENUATION_AND_DISPERSION"):
+        """Step records of the plain spheroidal-minors integration, float32 [n, 16]."""
+        freqs, lmin, nl, nk = self._plan(freqs, lmin, nl)
+        rec = np.zeros((maxrec, 16), np.float32)
+        nrec = C.c_int()
+        st = np.zeros(10, np.int64)
+        am = ATTMODES[attmode] if isinstance(attmode, str) else int(attmode)
+        check(self.h, _lib.lib().gm_collect_steplog(self.h, freqs.size, freqs.ctypes.data_as(D),
+                                                    lmin.ctypes.data_as(I), nl.ctypes.data_as(I), float(sigma),
+                                                    float(eps), am, int(jsl), int(jsr), int(maxrec),
+                                                    rec.ctypes.data_as(F), C.byref(nrec), st.ctypes.data_as(LL)))
+        return rec[:min(nrec.value, maxrec)], nrec.value, st
+
     @staticmethod
     def _plan(freqs, lmin, nl):
         freqs = np.ascontiguousarray(np.atleast_1d(freqs), dtype=np.float64)

@@ -1,0 +1,119 @@
+"""Learned Bulirsch-Stoer step proposer: data, targets, GPU training.
+
+The plain integrator (reference step control) is run over a training subset of
+the (omega, l) grid with step logging: for every spheroidal-minors step it
+records the features the network sees and the Bulirsch error estimate after
+each of the 3..7 modified-midpoint sequences (all seven are computed).  With
+the error estimate after k sequences scaling as h^p_k — p = 2, 4, 4, 6, 6 for
+extrapolation in h of the even modified-midpoint error expansion — each record
+gives the largest step h_k that the error test would accept after k sequences,
+h_k = h (eps / err_k)^(1/p_k), and the work per unit length MV(k) / h_k (MV =
+matvecs of k sequences).  The regression targets are the cheapest k* and
+log10(h_k*/R).  The network is trained on the GPU by the library's own
+forward / backward / Adam kernels and then proposes every step; the reference
+error test still decides, and a rejected step is halved as in the reference.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import F, check
+
+P_ORDER = np.array([2.0, 4.0, 4.0, 6.0, 6.0])            # k = 3..7
+MATVECS = np.array([15.0, 24.0, 37.0, 54.0, 79.0])       # sum_{j<=k} (n_j + 1), k = 3..7
+NIN, NOUT = 9, 2
+FEATURES = ["log10 omega", "l/1000", "r/R", "vp km/s", "vs km/s", "rho g/cc", "log10 Q_mu", "log10 h_prev/R",
+            "k_prev"]
+
+
+def nparams() -> int:
+    return int(_lib.lib().gm_mlp_nparams())
+
+
+def init_params(seed=0):
+    rng = np.random.default_rng(seed)
+    shapes = [(NIN, 64), (64,), (64, 64), (64,), (64, 64), (64,), (64, NOUT), (NOUT,)]
+    parts = [rng.normal(0.0, np.sqrt(2.0 / s[0]), size=s).ravel() if len(s) == 2 else np.zeros(s) for s in shapes]
+    return np.concatenate(parts).astype(np.float32)
+
+
+def targets_from_log(rec):
+    """(features X[n,9], targets T[n,2] = (log10(h*/R), k*), mask of usable records)."""
+    X = rec[:, :NIN].astype(np.float64)
+    lh = rec[:, 9].astype(np.float64)
+    e = rec[:, 10:15].astype(np.float64)                       # log10(err_k / eps), k = 3..7
+    ok = np.all(np.isfinite(e), axis=1) & np.isfinite(lh) & np.all(np.isfinite(X), axis=1)
+    lhk = lh[:, None] - e / P_ORDER[None, :]                   # log10(h_k / R)
+    cost = np.log10(MATVECS)[None, :] - lhk                    # log10(work per unit length)
+    kbest = np.argmin(np.where(np.isfinite(cost), cost, np.inf), axis=1)
+    T = np.stack([lhk[np.arange(len(lh)), kbest], kbest + 3.0], axis=1)
+    return X[ok].astype(np.float32), T[ok].astype(np.float32), ok
+
+
+def normalisation(X, T):
+    mu, sd = X.mean(0), X.std(0)
+    sd = np.where(sd > 1e-6, sd, 1.0)
+    omu, osd = T.mean(0), T.std(0)
+    osd = np.where(osd > 1e-6, osd, 1.0)
+    return np.concatenate([mu, sd, omu, osd]).astype(np.float32)
+
+
+def train(X, T, epochs=30, batch=1024, lr=2e-3, seed=1, device=0, w=(1.0, 0.25)):
+    """Normalise, train on the GPU, return (params, norm, per-epoch losses)."""
+    norm = normalisation(X, T)
+    mu, sd, omu, osd = norm[:NIN], norm[NIN:2 * NIN], norm[2 * NIN:2 * NIN + 2], norm[2 * NIN + 2:]
+    Xn = np.ascontiguousarray((X - mu) / sd, dtype=np.float32)
+    Tn = np.ascontiguousarray((T - omu) / osd, dtype=np.float32)
+    p = init_params(seed)
+    losses = np.zeros(epochs, np.float32)
+    check(None, _lib.lib().gm_mlp_train(int(device), p.ctypes.data_as(F), Xn.shape[0], Xn.ctypes.data_as(F),
+                                        Tn.ctypes.data_as(F), float(w[0]), float(w[1]), int(epochs), int(batch),
+                                        float(lr), int(seed), losses.ctypes.data_as(F)))
+    return p, norm, losses
+
+
+def forward(params, X, device=0):
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    Y = np.zeros((X.shape[0], NOUT), np.float32)
+    check(None, _lib.lib().gm_mlp_forward(int(device), np.ascontiguousarray(params, np.float32).ctypes.data_as(F),
+                                          X.shape[0], X.ctypes.data_as(F), Y.ctypes.data_as(F)))
+    return Y
+
+
+def grad(params, X, T, w=(1.0, 1.0), device=0):
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    T = np.ascontiguousarray(T, dtype=np.float32)
+    g = np.zeros(nparams(), np.float32)
+    loss = C.c_float()
+    check(None, _lib.lib().gm_mlp_grad(int(device), np.ascontiguousarray(params, np.float32).ctypes.data_as(F),
+                                       X.shape[0], X.ctypes.data_as(F), T.ctypes.data_as(F), float(w[0]),
+                                       float(w[1]), C.byref(loss), g.ctypes.data_as(F)))
+    return loss.value, g
+
+
+def split_grid(grid, frac=0.1, seed=4):
+    """Seeded split of the (omega, l) grid: returns (train, test) as lists of (freq, l-array)."""
+    rng = np.random.default_rng(seed)
+    train, test = [], []
+    for f, n in zip(grid.freqs, grid.nwn):
+        m = rng.random(int(n)) < frac
+        ls = np.arange(int(n))
+        train.append((f, ls[m]))
+        test.append((f, ls[~m]))
+    return train, test
+
+
+def as_batch(pairs):
+    """(freq, l-array) pairs -> per-slot arrays (freqs, lmin, nl) of contiguous l runs."""
+    fr, lm, nl = [], [], []
+    for f, ls in pairs:
+        if len(ls) == 0:
+            continue
+        # contiguous runs of degrees
+        br = np.flatnonzero(np.diff(ls) != 1) + 1
+        for run in np.split(ls, br):
+            fr.append(f); lm.append(int(run[0])); nl.append(len(run))
+    return np.array(fr), np.array(lm, np.int32), np.array(nl, np.int32)

@@ -54,7 +54,7 @@ typedef struct gm_ctx gm_ctx;
 
 /* options (gm_set_option) */
 #define GM_OPT_VARIANT 1       /* 0: exact (no FMA contraction, bitwise operation order), 1: fast (FMA) */
-#define GM_OPT_PERSISTENT 2    /* 1 (default): persistent substep-scheduled kernels, 0: one lane per job */
+#define GM_OPT_PERSISTENT 2    /* 0 (default): one lane per job, 1: persistent substep-scheduled kernels */
 #define GM_OPT_UNIT 3          /* (omega, l) instances per work unit of the persistent kernels */
 #define GM_OPT_WAVES 4         /* waves per SIMD the persistent kernels are built for: 1 or 2 */
 
@@ -119,6 +119,32 @@ int gm_last_timing(gm_ctx* ctx, double* ms6);
 /* Work counters of the last batch call restricted to one ODE system
  * (0 minors, 1 toroidal, 2 fluid, 3 radial, 4 adjoint), for per-kernel rooflines. */
 int gm_last_system_stats(gm_ctx* ctx, int sys, long long* stats);
+
+/* ---- learned step proposer (no counterpart in the reference) ----------------
+ * A 9 -> 64 -> 64 -> 64 -> 2 FP32 MLP (ReLU) proposes log10(h/R) and the try
+ * count for each Bulirsch-Stoer step of the spheroidal minors system; the
+ * reference error test still accepts or rejects every step and a rejected step
+ * falls back to the reference halving.  Features: log10 omega, l/1000, r/R,
+ * vp, vs (km/s), rho (g/cc), log10 Q_mu, log10(h_prev/R), previous try count. */
+int gm_mlp_nparams(void);
+/* forward pass of n samples X[n][9] -> Y[n][2] on `device` */
+int gm_mlp_forward(int device, const float* params, int n, const float* X, float* Y);
+/* loss = mean_b (w0 (y0-t0)^2 + w1 (y1-t1)^2) and its parameter gradient */
+int gm_mlp_grad(int device, const float* params, int n, const float* X, const float* T, float w0, float w1,
+                float* loss, float* grad);
+/* Adam training on the GPU; params in/out; losses[epochs] = mean loss per epoch */
+int gm_mlp_train(int device, float* params, int n, const float* X, const float* T, float w0, float w1, int epochs,
+                 int batch, float lr, unsigned seed, float* losses);
+/* install (enable = 1) or remove (enable = 0) the proposer of a context;
+ * norm = mu[9], sd[9] of the features and mu[2], sd[2] of the outputs */
+int gm_set_proposer(gm_ctx* ctx, const float* params, const float* norm, float safety, int kcheck, int enable);
+/* run the plain minors integration over the given (f, l) and record every
+ * spheroidal-minors step: 9 features, log10(h/R), log10(err_k/eps) for the
+ * error tests after 3..7 sequences (all 7 sequences are computed), accepted
+ * try (0 = rejected): 16 floats per record */
+int gm_collect_steplog(gm_ctx* ctx, int nfreq, const double* freqs, const int* lmin, const int* nl, double sigma,
+                       double eps, int attmode, int jsl, int jsr, int maxrec, float* records, int* nrec,
+                       long long* stats);
 
 #ifdef __cplusplus
 }
