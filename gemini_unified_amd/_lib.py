@@ -19,7 +19,7 @@ VP = C.c_void_p
 EXPORTS = ["gm_ctx_create", "gm_ctx_destroy", "gm_set_option", "gm_last_error", "gm_model_set",
            "gm_nodes_set", "gm_minors_batch", "gm_gfk_batch", "gm_last_timing", "gm_last_system_stats",
            "gm_mlp_nparams", "gm_mlp_forward", "gm_mlp_grad", "gm_mlp_train", "gm_set_proposer",
-           "gm_collect_steplog"]
+           "gm_collect_steplog", "gm_legendre_batch", "gm_moment_sums", "gm_synthetics", "gm_last_synth_timing"]
 
 
 class GeminiFKError(RuntimeError):
@@ -55,6 +55,12 @@ def lib():
         L.gm_set_proposer.argtypes = [VP, F, F, C.c_float, C.c_int, C.c_int]
         L.gm_collect_steplog.argtypes = [VP, C.c_int, D, I, I, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
                                          C.c_int, F, I, LL]
+        # array arguments of the synthetics entry points are void* so that device pointers (ints) pass too
+        L.gm_legendre_batch.argtypes = [VP, C.c_int, D, C.c_int, D, D, D, D, D]
+        L.gm_moment_sums.argtypes = [VP, C.c_int, C.c_int, I, C.c_int, C.c_longlong, VP, VP, C.c_int, D, C.c_int, VP]
+        L.gm_synthetics.argtypes = [VP, C.c_int, C.c_int, C.c_double, C.c_double, I, C.c_int, C.c_longlong, VP, VP,
+                                    C.c_int, D, D, D, D, D, C.c_int, C.c_int, C.c_double, C.c_int, VP, VP]
+        L.gm_last_synth_timing.argtypes = [VP, D]
         _lib = L
     return _lib
 

@@ -8,8 +8,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgemini_fk.so")
-SOURCES = ["gm_exact.hip", "gm_fast.hip", "gm_mlp.hip", "gm_capi.cpp"]
-DEPS = SOURCES + ["gm_common.h", "gm_mlp.h", "gm_device.inc", "gm_kernels.inc", "gm_launch.inc", "gm_persist.inc"]
+SOURCES = ["gm_exact.hip", "gm_fast.hip", "gm_mlp.hip", "gm_synth.hip", "gm_capi.cpp"]
+DEPS = SOURCES + ["gm_common.h", "gm_ctx.h", "gm_mlp.h", "gm_device.inc", "gm_kernels.inc", "gm_launch.inc", "gm_persist.inc"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

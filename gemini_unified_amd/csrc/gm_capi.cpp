@@ -8,6 +8,7 @@
 #include <vector>
 #include <algorithm>
 #include "gm_common.h"
+#include "gm_ctx.h"
 #include "../../include/gemini_fk.h"
 
 namespace gm_exact {
@@ -24,37 +25,9 @@ static const int kN[5] = {5, 2, 2, 2, 4};
 static const int kNNZ[5] = {19, 4, 3, 4, 14};
 static const int kESYS[5] = {219, 148, 156, 196, 206};
 
-struct gm_ctx {
-  int device = 0;
-  int variant = 1;
-  int persistent = 0;      // 1: substep-scheduled persistent kernels, 0: one lane per job (default, faster here)
-  int unit = 1024;         // instances per work unit of the persistent kernels
-  int waves = 1;           // waves per SIMD the persistent kernels are compiled for (1 or 2)
-  int ncu = 0;
-  hipStream_t stream = nullptr;
-  std::string err;
-  GmModel hmodel;
-  bool has_model = false, has_nodes = false;
-  GmModel* dmodel = nullptr;
-  // growable device scratch
-  void* dbuf = nullptr;
-  size_t dbuf_size = 0;
-  hipEvent_t ev[7];
-  double timing[6] = {0};
-  unsigned long long last_hist[48] = {0};
-  // learned step proposer
-  float* dmlp = nullptr;
-  GmProposer prop;
-  int prop_enabled = 0;
-  // step logging of the next run (gm_collect_steplog)
-  int log_max = 0;
-  float* log_host = nullptr;
-  int log_count = 0;
-};
-
 static std::string g_create_err;
 
-static int fail(gm_ctx* c, int code, const char* fmt, ...) {
+int gm_fail(gm_ctx* c, int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
@@ -63,23 +36,23 @@ static int fail(gm_ctx* c, int code, const char* fmt, ...) {
   if (c) c->err = buf; else g_create_err = buf;
   return code;
 }
-#define HIPCHK(c, x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return fail((c), GM_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); } while (0)
 
 extern "C" int gm_ctx_create(int device, gm_ctx** out) {
   if (!out) return GM_ERR_ARG;
   *out = nullptr;
   int n = 0;
   hipError_t e = hipGetDeviceCount(&n);
-  if (e != hipSuccess || n == 0) return fail(nullptr, GM_ERR_HIP, "no HIP device available (%s)", hipGetErrorString(e));
-  if (device < 0 || device >= n) return fail(nullptr, GM_ERR_ARG, "device %d out of range", device);
+  if (e != hipSuccess || n == 0) return gm_fail(nullptr, GM_ERR_HIP, "no HIP device available (%s)", hipGetErrorString(e));
+  if (device < 0 || device >= n) return gm_fail(nullptr, GM_ERR_ARG, "device %d out of range", device);
   gm_ctx* c = new gm_ctx();
   c->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(&c->dmodel, sizeof(GmModel)) != hipSuccess) {
     delete c;
-    return fail(nullptr, GM_ERR_HIP, "context creation failed on device %d", device);
+    return gm_fail(nullptr, GM_ERR_HIP, "context creation failed on device %d", device);
   }
   for (int i = 0; i < 7; ++i) hipEventCreate(&c->ev[i]);
+  for (int i = 0; i < 6; ++i) hipEventCreate(&c->sev[i]);
   hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device);
   if (c->ncu <= 0) c->ncu = 1;
   memset(&c->hmodel, 0, sizeof(GmModel));
@@ -95,6 +68,9 @@ extern "C" int gm_ctx_destroy(gm_ctx* c) {
   if (c->dmodel) hipFree(c->dmodel);
   if (c->dmlp) hipFree(c->dmlp);
   for (int i = 0; i < 7; ++i) hipEventDestroy(c->ev[i]);
+  for (int i = 0; i < 6; ++i) hipEventDestroy(c->sev[i]);
+  if (c->sbuf) hipFree(c->sbuf);
+  if (c->dtw) hipFree(c->dtw);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return GM_OK;
@@ -103,26 +79,32 @@ extern "C" int gm_ctx_destroy(gm_ctx* c) {
 extern "C" int gm_set_option(gm_ctx* c, int option, int value) {
   if (!c) return GM_ERR_ARG;
   if (option == GM_OPT_VARIANT) {
-    if (value != 0 && value != 1) return fail(c, GM_ERR_ARG, "variant must be 0 or 1");
+    if (value != 0 && value != 1) return gm_fail(c, GM_ERR_ARG, "variant must be 0 or 1");
     c->variant = value;
     return GM_OK;
   }
   if (option == GM_OPT_PERSISTENT) {
-    if (value != 0 && value != 1) return fail(c, GM_ERR_ARG, "persistent must be 0 or 1");
+    if (value != 0 && value != 1) return gm_fail(c, GM_ERR_ARG, "persistent must be 0 or 1");
     c->persistent = value;
     return GM_OK;
   }
   if (option == GM_OPT_WAVES) {
-    if (value != 1 && value != 2) return fail(c, GM_ERR_ARG, "waves must be 1 or 2");
+    if (value != 1 && value != 2) return gm_fail(c, GM_ERR_ARG, "waves must be 1 or 2");
     c->waves = value;
     return GM_OK;
   }
+  if (option == GM_OPT_PROPOSE_EVERY) {
+    if (value < 1 || value > 1000) return gm_fail(c, GM_ERR_ARG, "proposal interval must be in [1, 1000]");
+    c->prop_every = value;
+    c->prop.every = value;
+    return GM_OK;
+  }
   if (option == GM_OPT_UNIT) {
-    if (value < 64 || value > (1 << 20)) return fail(c, GM_ERR_ARG, "unit size must be in [64, 2^20]");
+    if (value < 64 || value > (1 << 20)) return gm_fail(c, GM_ERR_ARG, "unit size must be in [64, 2^20]");
     c->unit = value;
     return GM_OK;
   }
-  return fail(c, GM_ERR_ARG, "unknown option %d", option);
+  return gm_fail(c, GM_ERR_ARG, "unknown option %d", option);
 }
 
 extern "C" int gm_last_error(gm_ctx* c, char* buf, int len) {
@@ -144,8 +126,8 @@ extern "C" int gm_model_set(gm_ctx* c, int nk, const double* rk, const double* r
                             int nic, int noc, int fsflag, int equidis, double rearth, double ric, double roc,
                             double fref) {
   if (!c) return GM_ERR_ARG;
-  if (nk < 2 || nk > GM_MAXK) return fail(c, GM_ERR_ARG, "nk=%d outside [2,%d]", nk, GM_MAXK);
-  if (nlay < 2 || nlay > GM_MAXL) return fail(c, GM_ERR_ARG, "nlay=%d outside [2,%d]", nlay, GM_MAXL);
+  if (nk < 2 || nk > GM_MAXK) return gm_fail(c, GM_ERR_ARG, "nk=%d outside [2,%d]", nk, GM_MAXK);
+  if (nlay < 2 || nlay > GM_MAXL) return gm_fail(c, GM_ERR_ARG, "nlay=%d outside [2,%d]", nlay, GM_MAXL);
   GmModel& m = c->hmodel;
   int nnod = m.nnod;
   double rn[GM_MAXN + 1]; int stt[GM_MAXN + 1], lay[GM_MAXN + 1];
@@ -159,7 +141,7 @@ extern "C" int gm_model_set(gm_ctx* c, int nk, const double* rk, const double* r
     m.vsh[i + 1] = vsh[i]; m.eta[i + 1] = eta[i]; m.qkinv[i + 1] = qkinv[i]; m.qminv[i + 1] = qminv[i];
   }
   for (int i = 0; i < nlay; ++i) { m.iktop[i + 1] = iktop[i]; m.rb[i + 1] = rb[i]; m.iflso[i + 1] = iflso[i]; }
-  if (m.iktop[nlay] != nk) return fail(c, GM_ERR_ARG, "iktop(nlay)=%d differs from nk=%d", m.iktop[nlay], nk);
+  if (m.iktop[nlay] != nk) return gm_fail(c, GM_ERR_ARG, "iktop(nlay)=%d differs from nk=%d", m.iktop[nlay], nk);
   m.ikbot[1] = 0;
   for (int nl = 2; nl <= nlay; ++nl) m.ikbot[nl] = m.iktop[nl - 1] + 1;
   for (int nl = 1; nl <= nlay; ++nl) m.drk[nl] = -1.0;
@@ -175,7 +157,7 @@ extern "C" int gm_model_set(gm_ctx* c, int nk, const double* rk, const double* r
 
 extern "C" int gm_nodes_set(gm_ctx* c, int nnod, const double* rnod, const int* state, const int* layer) {
   if (!c) return GM_ERR_ARG;
-  if (nnod < 1 || nnod > GM_MAXN) return fail(c, GM_ERR_ARG, "nnod=%d outside [1,%d]", nnod, GM_MAXN);
+  if (nnod < 1 || nnod > GM_MAXN) return gm_fail(c, GM_ERR_ARG, "nnod=%d outside [1,%d]", nnod, GM_MAXN);
   GmModel& m = c->hmodel;
   m.nnod = nnod;
   for (int i = 0; i < nnod; ++i) { m.rnod[i + 1] = rnod[i]; m.state[i + 1] = state[i]; m.layer[i + 1] = layer[i]; }
@@ -227,28 +209,28 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
                double* u_tor, double* d_tor, double* dettor, double* u_sph, double* d_sph, double* ulg, double* dlg,
                double* detsph, double* gf, float* gfre, float* gfim, int* status, long long* stats) {
   if (!c) return GM_ERR_ARG;
-  if (!c->has_model || !c->has_nodes) return fail(c, GM_ERR_ARG, "model and nodes must be set first");
-  if (nfreq < 0 || (nfreq > 0 && (!freqs || !lmin || !nl))) return fail(c, GM_ERR_ARG, "invalid frequency arguments");
-  if (attmode < 0 || attmode > 3) return fail(c, GM_ERR_ARG, "invalid attenuation mode %d", attmode);
-  if (!(eps > 0.0)) return fail(c, GM_ERR_ARG, "eps must be positive");
+  if (!c->has_model || !c->has_nodes) return gm_fail(c, GM_ERR_ARG, "model and nodes must be set first");
+  if (nfreq < 0 || (nfreq > 0 && (!freqs || !lmin || !nl))) return gm_fail(c, GM_ERR_ARG, "invalid frequency arguments");
+  if (attmode < 0 || attmode > 3) return gm_fail(c, GM_ERR_ARG, "invalid attenuation mode %d", attmode);
+  if (!(eps > 0.0)) return gm_fail(c, GM_ERR_ARG, "eps must be positive");
   GmModel& m = c->hmodel;
   const int nnod = m.nnod;
-  if (jsl < 1 || jsl > nnod || jsr < 1 || jsr > nnod) return fail(c, GM_ERR_NODEIDX, "Invalid index for source node");
+  if (jsl < 1 || jsl > nnod || jsr < 1 || jsr > nnod) return gm_fail(c, GM_ERR_NODEIDX, "Invalid index for source node");
   if (full) {
-    if (istyp != 0 && istyp != 1) return fail(c, GM_ERR_ARG, "istyp must be 0 (FORCE) or 1 (MOMENT)");
-    if (m.state[jre] != 0) return fail(c, GM_ERR_UNSUPPORTED, "receiver node in a fluid layer is not supported");
+    if (istyp != 0 && istyp != 1) return gm_fail(c, GM_ERR_ARG, "istyp must be 0 (FORCE) or 1 (MOMENT)");
+    if (m.state[jre] != 0) return gm_fail(c, GM_ERR_UNSUPPORTED, "receiver node in a fluid layer is not supported");
     for (int k = 1; k <= nnod; ++k)
       if (m.state[k] != 0 || !(m.rnod[k] > m.roc))
-        return fail(c, GM_ERR_UNSUPPORTED, "node %d lies in the core or a fluid layer; only mantle nodes are supported", k);
+        return gm_fail(c, GM_ERR_UNSUPPORTED, "node %d lies in the core or a fluid layer; only mantle nodes are supported", k);
   }
-  if (m.fsflag == 1) return fail(c, GM_ERR_UNSUPPORTED, "full-space models are not supported");
+  if (m.fsflag == 1) return gm_fail(c, GM_ERR_UNSUPPORTED, "full-space models are not supported");
   // (omega, l) plan
   Plan P;
   P.nfreq = nfreq;
   P.omre.resize(nfreq);
   for (int f = 0; f < nfreq; ++f) {
-    if (nl[f] < 0 || lmin[f] < 0) return fail(c, GM_ERR_ARG, "negative degree range at frequency %d", f);
-    if (!(freqs[f] > 0.0)) return fail(c, GM_ERR_ARG, "frequency %d must be positive", f);
+    if (nl[f] < 0 || lmin[f] < 0) return gm_fail(c, GM_ERR_ARG, "negative degree range at frequency %d", f);
+    if (!(freqs[f] > 0.0)) return gm_fail(c, GM_ERR_ARG, "frequency %d must be positive", f);
     P.omre[f] = 2.0 * (double)3.1415927410125732 * freqs[f];
     for (int i = 0; i < nl[f]; ++i) { P.q.push_back(f); P.l.push_back(lmin[f] + i); }
   }
@@ -296,7 +278,7 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   size_t o_logc = take(sizeof(int) * 4);
   if (off > c->dbuf_size) {
     if (c->dbuf) { hipFree(c->dbuf); c->dbuf = nullptr; c->dbuf_size = 0; }
-    if (hipMalloc(&c->dbuf, off) != hipSuccess) return fail(c, GM_ERR_NOMEM, "device allocation of %zu bytes failed", off);
+    if (hipMalloc(&c->dbuf, off) != hipSuccess) return gm_fail(c, GM_ERR_NOMEM, "device allocation of %zu bytes failed", off);
     c->dbuf_size = off;
   }
   char* base = (char*)c->dbuf;
@@ -327,7 +309,7 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
     B.gf = gf ? (dev ? gf : (double*)(base + o_gf)) : nullptr;
     B.gfre = gfre ? (dev ? gfre : (float*)(base + o_gfre)) : nullptr;
     B.gfim = gfim ? (dev ? gfim : (float*)(base + o_gfim)) : nullptr;
-    if ((B.gfre == nullptr) != (B.gfim == nullptr)) return fail(c, GM_ERR_ARG, "gfre and gfim must both be given");
+    if ((B.gfre == nullptr) != (B.gfim == nullptr)) return gm_fail(c, GM_ERR_ARG, "gfre and gfim must both be given");
   }
   B.hist = (unsigned long long*)(base + o_hist);
   B.nunits = nunits;
@@ -407,7 +389,7 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   if (stats) derive_stats(hist, stats);
   int first = 0;
   for (size_t i = 0; i < n; ++i) if (hstatus[i]) { first = hstatus[i]; break; }
-  if (first) return fail(c, first, "at least one (omega,l) instance failed with error %d", first);
+  if (first) return gm_fail(c, first, "at least one (omega,l) instance failed with error %d", first);
   return GM_OK;
 }
 
@@ -436,10 +418,10 @@ extern "C" int gm_last_system_stats(gm_ctx* c, int sys, long long* stats) {
 extern "C" int gm_set_proposer(gm_ctx* c, const float* params, const float* norm, float safety, int kcheck, int enable) {
   if (!c) return GM_ERR_ARG;
   if (!enable) { c->prop_enabled = 0; return GM_OK; }
-  if (!params || !norm || !(safety > 0.0f)) return fail(c, GM_ERR_ARG, "proposer needs parameters, normalisation and safety > 0");
+  if (!params || !norm || !(safety > 0.0f)) return gm_fail(c, GM_ERR_ARG, "proposer needs parameters, normalisation and safety > 0");
   HIPCHK(c, hipSetDevice(c->device));
   if (!c->dmlp && hipMalloc(&c->dmlp, sizeof(float) * GM_MLP_NPARAM) != hipSuccess)
-    return fail(c, GM_ERR_NOMEM, "proposer parameter allocation failed");
+    return gm_fail(c, GM_ERR_NOMEM, "proposer parameter allocation failed");
   HIPCHK(c, hipMemcpy(c->dmlp, params, sizeof(float) * GM_MLP_NPARAM, hipMemcpyHostToDevice));
   memset(&c->prop, 0, sizeof c->prop);
   for (int i = 0; i < GM_MLP_IN; ++i) {
@@ -449,6 +431,7 @@ extern "C" int gm_set_proposer(gm_ctx* c, const float* params, const float* norm
   for (int o = 0; o < GM_MLP_OUT; ++o) { c->prop.omu[o] = norm[2 * GM_MLP_IN + o]; c->prop.osd[o] = norm[2 * GM_MLP_IN + 2 + o]; }
   c->prop.safety = safety;
   c->prop.kcheck = kcheck ? 1 : 0;
+  c->prop.every = c->prop_every;
   c->prop_enabled = 1;
   return GM_OK;
 }

@@ -88,4 +88,4 @@ struct GmBatch {
   int logmax;
   GmProposer prop;
 };
-#define GM_LOGREC 16
+#define GM_LOGREC 20

@@ -1,0 +1,45 @@
+// Context of the Green-FK engine (private to the library).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string>
+#include "gm_common.h"
+
+struct gm_ctx {
+  int device = 0;
+  int variant = 1;
+  int persistent = 0;      // 1: substep-scheduled persistent kernels, 0: one lane per job (default, faster here)
+  int unit = 1024;         // instances per work unit of the persistent kernels
+  int waves = 1;           // waves per SIMD the persistent kernels are compiled for (1 or 2)
+  int ncu = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  GmModel hmodel;
+  bool has_model = false, has_nodes = false;
+  GmModel* dmodel = nullptr;
+  // growable device scratch
+  void* dbuf = nullptr;
+  size_t dbuf_size = 0;
+  hipEvent_t ev[7];
+  double timing[6] = {0};
+  unsigned long long last_hist[48] = {0};
+  // learned step proposer
+  float* dmlp = nullptr;
+  GmProposer prop;
+  int prop_enabled = 0;
+  int prop_every = 1;      // network consulted every prop_every accepted steps
+  // step logging of the next run (gm_collect_steplog)
+  int log_max = 0;
+  float* log_host = nullptr;
+  int log_count = 0;
+  // synthetics half: growable device scratch, stage events and timing
+  void* sbuf = nullptr;
+  size_t sbuf_size = 0;
+  double* dtw = nullptr;   // twiddle table of the last transform length
+  int tw_n = 0;
+  hipEvent_t sev[6];
+  double stiming[6] = {0};
+};
+
+// records the message of a failure on c (or of a failed context creation) and returns code
+int gm_fail(gm_ctx* c, int code, const char* fmt, ...);
+#define HIPCHK(c, x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return gm_fail((c), GM_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); } while (0)

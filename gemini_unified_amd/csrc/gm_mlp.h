@@ -13,7 +13,8 @@ struct GmProposer {
   float omu[GM_MLP_OUT], osd[GM_MLP_OUT]; // y = y_norm * osd + omu
   float safety;                           // proposed step = safety * predicted step
   int kcheck;                             // 1: first error test at the predicted try - 1
-  int pad[2];
+  int every;                              // consult the network every `every` accepted steps
+  int pad;
 };
 
 // One forward pass for one lane.  Every lane reads the same weight at the same

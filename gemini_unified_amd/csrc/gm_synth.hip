@@ -1,0 +1,647 @@
+// Synthetics half of the Green-FK path on MI355X: Legendre basis, moment-tensor
+// harmonic degree sums as a split-bf16 MFMA contraction, the per-frequency
+// displacement combination, and the frequency-to-time transform.
+//
+// Pipeline of gm_synthetics (one source node, R receivers, nf frequencies):
+//   k_twiddle   e^{+2 pi i k / ns}, k < ns/2, FP64
+//   k_legendre  per receiver: P_l^m, dP/dtheta, P/sin (m = 0..2) by the FP64
+//               upward recurrence, times (2l+1)/(4 pi) and 2/(l(l+1)), as
+//               8 basis rows; each value is split into three bf16 pieces
+//               (hi, mid, lo: 24 significant bits).  Layout [piece][row][Rp][Lp].
+//   k_gprep     spectra G_s(l, f) * taper_f(l) in FP64, split into three bf16
+//               pieces.  Layout [piece][s][Np][Lp], column n = 2 f' (+1 imag).
+//   k_contract  the 14 sums S_j[r][n] = sum_l B_b(j)[r][l] G_s(j)[n][l] on the
+//               matrix cores: v_mfma_f32_16x16x32_bf16 over the six piece
+//               products whose exponent sum keeps 24 bits (hi*hi, hi*mid,
+//               mid*hi, hi*lo, mid*mid, lo*hi).  Products of bf16 pieces are
+//               exact in FP32; each 32-degree chunk is accumulated in FP32 by
+//               the MFMA and then added to an FP64 running sum (compensation).
+//   k_combine   displacement (r, theta, phi) from the 14 sums and the moment
+//               tensor, time shift, optional velocity, optional ZNE rotation.
+//   k_ifft      per trace: Hermitian spectrum -> real time series through an
+//               ns/2-point complex FFT in LDS (real-output split), times df
+//               e^{sigma t}, stored as FP32.
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#include "gm_common.h"
+#include "gm_ctx.h"
+#include "../../include/gemini_fk.h"
+
+#pragma clang fp contract(off)
+
+namespace gm_syn {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr int kRows = 8;       // basis rows
+constexpr int kSums = 14;      // moment-tensor degree sums
+constexpr int kPieces = 3;     // bf16 pieces per value
+constexpr int TM = 128, TN = 64, TK = 32, LDSK = 40;   // tile sizes; LDS row pitch (bf16)
+constexpr int kMaxFFT = 8192;  // longest time series (ns/2 complex FP64 = 64 KB of LDS)
+
+// sum j: basis row, spectrum index (offset of (component, jump) in the GFK
+// file for dsvmask (U, V, W)) and the power-of-two factor relating the sum's
+// weight to the basis row (harmonicDegreeSums.f90:251-307)
+__constant__ int c_row[kSums] = {0, 0, 1, 2, 3, 3, 4, 5, 6, 7, 4, 5, 6, 7};
+__constant__ int c_spec[kSums] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 8, 9, 6, 7};
+__constant__ double c_fac[kSums] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 1, -1, 1, -2};
+
+__device__ __forceinline__ void split3(double v, __bf16& a, __bf16& b, __bf16& c) {
+  a = (__bf16)(float)v;
+  double r = v - (double)(float)a;
+  b = (__bf16)(float)r;
+  r = r - (double)(float)b;
+  c = (__bf16)(float)r;
+}
+
+__global__ void k_twiddle(int ns, double* __restrict__ tw) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ns / 2) return;
+  double s, c;
+  sincospi(2.0 * k / ns, &s, &c);
+  tw[2 * k] = c;
+  tw[2 * k + 1] = s;
+}
+
+// One receiver per thread.  mode 0: basis pieces Bs[piece][row][Rp][Lp] for
+// degrees 0..Lp-1 (zero beyond lmax); mode 1: FP64 arrays [R][3][lmax+1] of
+// plm, dpdt, pdst, d2pdt2, dtpdst (computeLegendreFunctions,
+// legendreFunctions.f90:48-157, same operation order).
+__global__ void __launch_bounds__(64) k_legendre(int R, int Rp, int lmax, int Lp, const double* __restrict__ theta,
+                                                 __bf16* __restrict__ Bs, double* __restrict__ o_plm,
+                                                 double* __restrict__ o_dpdt, double* __restrict__ o_pdst,
+                                                 double* __restrict__ o_d2, double* __restrict__ o_ds, int mode) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= (mode == 0 ? Rp : R)) return;
+  const size_t plane = (size_t)Rp * Lp;
+  const int L1 = lmax + 1;
+  const bool pad = r >= R;
+  const double th = pad ? 0.5 * kPi : theta[r];
+  const double ct = cos(th), st = sin(th);
+  const bool poles = !(st > 0.0);
+  const double rst = poles ? 0.0 : 1.0 / st;
+  const double sg = ct < 0.0 ? -1.0 : 1.0;
+  const double fourpi = 4.0 * kPi;
+  const double pmm[3] = {1.0, (1.0 * 1.0) * st, ((1.0 * 1.0) * st * 3.0) * st};
+  double p1[3] = {0.0, 0.0, 0.0}, p2[3] = {0.0, 0.0, 0.0};
+  __bf16 buf[kPieces][kRows][8];
+  for (int l = 0; l < (mode == 0 ? Lp : L1); ++l) {
+    double P[3], D[3], S[3], D2[3], DS[3];
+    const double ll1 = (double)(l * (l + 1));
+    const double sl = (l & 1) ? sg : 1.0, sl1 = (l & 1) ? 1.0 : sg;
+    const double q = (double)(l + 2) * (l + 1) * l * (l - 1);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      double p;
+      if (l > lmax || l < m) p = 0.0;
+      else if (l == m) p = pmm[m];
+      else if (l == m + 1) p = ct * (double)(2 * m + 1) * pmm[m];
+      else p = (ct * (double)(2 * l - 1) * p1[m] - (double)(l + m - 1) * p2[m]) / (double)(l - m);
+      double d = 0.0, s = 0.0, d2 = 0.0, ds = 0.0;
+      if (l <= lmax) {
+        if (!poles) {
+          if (l == m) d = (double)m * ct * p * rst;
+          else if (l > m) d = ((double)l * ct * p - (double)(l + m) * p1[m]) * rst;
+          if (m >= 1 && l >= m) s = p * rst;
+          if (l >= m) d2 = -ct * rst * d + ((double)(m * m) * (rst * rst) - ll1) * p;
+          if (m >= 1 && l >= m) ds = rst * (d - ct * rst * p);
+        } else {
+          if (m == 1) d = 0.5 * ll1 * sl;
+          if (m == 1 && l >= 1) s = 0.5 * ll1 * sl1;
+          if (m == 0) d2 = 0.5 * ll1 * sl;
+          if (m == 2) d2 = 0.25 * q * sl;
+          if (m == 2 && l >= 1) ds = 0.125 * q * sl1;
+        }
+      }
+      P[m] = p; D[m] = d; S[m] = s; D2[m] = d2; DS[m] = ds;
+      p2[m] = p1[m];
+      p1[m] = p;
+    }
+    if (mode == 1) {
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        const size_t o = ((size_t)r * 3 + m) * L1 + l;
+        o_plm[o] = P[m]; o_dpdt[o] = D[m]; o_pdst[o] = S[m]; o_d2[o] = D2[m]; o_ds[o] = DS[m];
+      }
+      continue;
+    }
+    double b[kRows];
+    const double w = (2.0 * l + 1.0) / fourpi;
+    const double g = l >= 1 ? 2.0 / ll1 : 0.0;
+    b[0] = P[0] * w;
+    b[1] = l >= 1 ? P[1] * g * w : 0.0;
+    b[2] = l >= 2 ? P[2] * g * w : 0.0;
+    b[3] = l >= 1 ? D[0] * w : 0.0;
+    b[4] = l >= 1 ? D[1] * g * w : 0.0;
+    b[5] = l >= 2 ? D[2] * g * w : 0.0;
+    b[6] = l >= 1 ? S[1] * g * w : 0.0;
+    b[7] = l >= 2 ? S[2] * g * w : 0.0;
+    const int e = l & 7;
+#pragma unroll
+    for (int k = 0; k < kRows; ++k) {
+      double v = (pad || l > lmax) ? 0.0 : b[k];
+      split3(v, buf[0][k][e], buf[1][k][e], buf[2][k][e]);
+    }
+    if (e == 7) {   // 8 degrees done: one 16-byte store per (piece, row)
+#pragma unroll
+      for (int pc = 0; pc < kPieces; ++pc)
+#pragma unroll
+        for (int k = 0; k < kRows; ++k) {
+          bf16x8 v;
+#pragma unroll
+          for (int i = 0; i < 8; ++i) v[i] = buf[pc][k][i];
+          *(bf16x8*)(Bs + ((size_t)pc * kRows + k) * plane + (size_t)r * Lp + (l - 7)) = v;
+        }
+    }
+  }
+}
+
+// Tapered spectra split into bf16 pieces: Gs[piece][s][Np][Lp].
+// taperHarmonicDegreeSums (harmonicDegreeSums.f90:356-369) per frequency.
+__global__ void k_gprep(int nsp, int nf, int Np, int Lp, const int* __restrict__ nwn, const long long* __restrict__ koff,
+                        long long nkf, const float* __restrict__ gr, const float* __restrict__ gi, double tapfrac,
+                        __bf16* __restrict__ Gs) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t per_s = (size_t)Np * Lp;
+  if (idx >= per_s * nsp) return;
+  const int s = (int)(idx / per_s);
+  const int n = (int)((idx % per_s) / Lp);
+  const int l = (int)(idx % Lp);
+  const int f = n >> 1;
+  double v = 0.0;
+  if (f < nf && l < nwn[f]) {
+    const long long k = (long long)s * nkf + koff[f] + l;
+    const double gv = (n & 1) ? (double)gi[k] : (double)gr[k];
+    const int lmax = nwn[f] - 1;
+    long jt = lround(lmax * (1.0 - tapfrac));
+    if (jt > lmax) jt = lmax;
+    if (jt < 1) jt = 1;
+    double w = 1.0;
+    if (l >= jt) {
+      const double c1 = lmax * tapfrac * 2.0 / kPi;
+      const double c2 = (1.0 - tapfrac) / tapfrac * 0.5 * kPi;
+      const double c = cos((double)l / c1 - c2);
+      w = c * c;
+    }
+    v = gv * w;
+  }
+  __bf16 a, b, c;
+  split3(v, a, b, c);
+  Gs[idx] = a;
+  Gs[idx + per_s * nsp] = b;
+  Gs[idx + 2 * per_s * nsp] = c;
+}
+
+// H[j][Rp][Np] (FP64) = c_fac[j] * sum over degrees of basis row c_row[j]
+// times spectrum c_spec[j] (the j-th moment-tensor sum); grid (Np/TN, Rp/TM,
+// 14), 256 threads: 4 waves of 64 x 32.
+__global__ void __launch_bounds__(256) k_contract(int Rp, int Np, int Lp, int nsp, const __bf16* __restrict__ Bs,
+                                                  const __bf16* __restrict__ Gs, double* __restrict__ H) {
+  __shared__ __attribute__((aligned(16))) __bf16 sA[kPieces][TM][LDSK];
+  __shared__ __attribute__((aligned(16))) __bf16 sB[kPieces][TN][LDSK];
+  const int j = blockIdx.z;
+  const int rowb = c_row[j], spec = c_spec[j];
+  const double fac = c_fac[j];
+  const int r0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wr = (w >> 1) * 64, wc = (w & 1) * 32;
+  const size_t planeA = (size_t)Rp * Lp, planeG = (size_t)Np * Lp;
+  // global -> register staging: A: 3 pieces x 128 rows x 4 groups of 8 degrees; B: 3 x 64 x 4
+  uint4 ra[6], rb[3];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      const int id = t + 256 * i, pc = id >> 9, rem = id & 511, row = rem >> 2, g = rem & 3;
+      ra[i] = *(const uint4*)(Bs + ((size_t)pc * kRows + rowb) * planeA + (size_t)(r0 + row) * Lp + k0 + 8 * g);
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const int id = t + 256 * i, pc = id >> 8, rem = id & 255, row = rem >> 2, g = rem & 3;
+      rb[i] = *(const uint4*)(Gs + ((size_t)pc * nsp + spec) * planeG + (size_t)(n0 + row) * Lp + k0 + 8 * g);
+    }
+  };
+  auto lstore = [&]() {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      const int id = t + 256 * i, pc = id >> 9, rem = id & 511, row = rem >> 2, g = rem & 3;
+      *(uint4*)&sA[pc][row][8 * g] = ra[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const int id = t + 256 * i, pc = id >> 8, rem = id & 255, row = rem >> 2, g = rem & 3;
+      *(uint4*)&sB[pc][row][8 * g] = rb[i];
+    }
+  };
+  floatx4 acc[4][2];
+  double sum[4][2][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      acc[a][b] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int e = 0; e < 4; ++e) sum[a][b][e] = 0.0;
+    }
+  const int fr = lane & 15, fk = (lane >> 4) * 8;
+  gload(0);
+  for (int k0 = 0; k0 < Lp; k0 += TK) {
+    lstore();
+    __syncthreads();
+    if (k0 + TK < Lp) gload(k0 + TK);
+    bf16x8 bq[2][kPieces];
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int pc = 0; pc < kPieces; ++pc) bq[b][pc] = *(const bf16x8*)&sB[pc][wc + b * 16 + fr][fk];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      bf16x8 aq[kPieces];
+#pragma unroll
+      for (int pc = 0; pc < kPieces; ++pc) aq[pc] = *(const bf16x8*)&sA[pc][wr + a * 16 + fr][fk];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        floatx4 c = acc[a][b];
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aq[2], bq[b][0], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aq[1], bq[b][1], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aq[0], bq[b][2], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aq[1], bq[b][0], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aq[0], bq[b][1], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aq[0], bq[b][0], c, 0, 0, 0);
+        acc[a][b] = c;
+      }
+    }
+    // compensation: the FP32 partial of this chunk goes into the FP64 sum
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) sum[a][b][e] += (double)acc[a][b][e];
+        acc[a][b] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+      }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = r0 + wr + a * 16 + (lane >> 4) * 4 + e;
+        const int col = n0 + wc + b * 16 + (lane & 15);
+        H[((size_t)j * Rp + row) * Np + col] = sum[a][b][e] * fac;
+      }
+}
+
+struct cd { double re, im; };
+__device__ __forceinline__ cd cmul(cd a, cd b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ __forceinline__ cd cadd(cd a, cd b) { return {a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ cd csub(cd a, cd b) { return {a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ cd crs(cd a, double s) { return {a.re * s, a.im * s}; }
+
+// Displacement spectra Z[r][3][nf2] (zero below nf1).
+// momentSingleFrequencyDisplacement (singleFrequencyDisplacement.f90:108-125),
+// time shift (computeSyntheticSeismograms.f90:361-365), velocity (:376-378),
+// ZNE rotation (axesRotation.f90:560-568).
+__global__ void k_combine(int R, int nf1, int nf2, int Rp, int Np, const double* __restrict__ H,
+                          const double* __restrict__ phi, const double* __restrict__ propdir,
+                          const double* __restrict__ tred, const double* __restrict__ mt, int velocity, double df,
+                          double sigma, double* __restrict__ Z) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= R * nf2) return;
+  const int r = id / nf2, jf = id % nf2 + 1;
+  cd u[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+  if (jf >= nf1) {
+    const int f = jf - nf1;
+    cd h[kSums];
+#pragma unroll
+    for (int j = 0; j < kSums; ++j) {
+      const size_t o = ((size_t)j * Rp + r) * Np + 2 * f;
+      h[j] = cd{H[o], H[o + 1]};
+    }
+    const double ph = phi[r];
+    const double sp = sin(ph), cp = cos(ph), s2 = sin(2.0 * ph), c2 = cos(2.0 * ph);
+    const double a = mt[3] * cp + mt[4] * sp, b = (mt[2] - mt[1]) * c2 - 2.0 * mt[5] * s2;
+    const double c = mt[4] * cp - mt[3] * sp, d = (mt[2] - mt[1]) * s2 + 2.0 * mt[5] * c2;
+    u[0] = cadd(cadd(cadd(crs(h[0], mt[0]), crs(h[1], mt[1] + mt[2])), crs(h[2], a)), crs(h[3], b));
+    u[1] = cadd(cadd(cadd(crs(h[4], mt[0]), crs(h[5], mt[1] + mt[2])), crs(cadd(h[6], h[8]), a)),
+                crs(cadd(h[7], h[9]), b));
+    u[2] = cadd(crs(cadd(h[10], h[12]), c), crs(cadd(h[11], h[13]), d));
+    const double om = 2.0 * kPi * (jf - 1) * df;
+    if (tred) {
+      const double tr = tred[r];
+      double sn, cs;
+      sincos(om * tr, &sn, &cs);
+      const double ex = exp(sigma * tr);
+      const cd tsf{cs * ex, sn * ex};
+#pragma unroll
+      for (int k = 0; k < 3; ++k) u[k] = cmul(u[k], tsf);
+    }
+    if (velocity) {
+      const cd iz{sigma, om};      // i * (om - i sigma)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) u[k] = cmul(u[k], iz);
+    }
+    if (propdir) {
+      const double ca = cos(propdir[r]), sa = sin(propdir[r]);
+      const cd ue = cadd(crs(u[1], sa), crs(u[2], ca)), un = cadd(crs(u[1], -ca), crs(u[2], sa));
+      u[1] = un;
+      u[2] = ue;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const size_t o = (((size_t)r * 3 + k) * nf2 + jf - 1) * 2;
+    Z[o] = u[k].re;
+    Z[o + 1] = u[k].im;
+  }
+}
+
+// One trace per block: spectrum X_j (j = nf1-1 .. nf2-1, Hermitian mirror,
+// zero elsewhere) -> x_t = sum_j X_j e^{+2 pi i j t / ns}, t < ns, then
+// ts = x df e^{sigma t dt} (transformFrequencyTime, frequencyTime.f90:55-113).
+// Real output from an M = ns/2 point complex transform of
+// Z_k = (X_k + conj X_{M-k}) + i W^k (X_k - conj X_{M-k}), W = e^{2 pi i / ns}:
+// z_n = x_{2n} + i x_{2n+1}.
+__global__ void __launch_bounds__(256) k_ifft(int nf1, int nf2, int ns, int logm, const double* __restrict__ Z,
+                                              const double* __restrict__ tw, double df, double sigma, double dt,
+                                              float* __restrict__ ts) {
+  __shared__ cd sx[kMaxFFT / 2];
+  const int M = ns >> 1;
+  const double* X = Z + (size_t)blockIdx.x * nf2 * 2;
+  auto xin = [&](int j) -> cd {
+    if (j < nf1 - 1 || j > nf2 - 1) return {0.0, 0.0};
+    return {X[2 * j], X[2 * j + 1]};
+  };
+  for (int k = threadIdx.x; k < M; k += blockDim.x) {
+    const cd a = xin(k);
+    const cd bc = k == 0 ? cd{0.0, 0.0} : xin(M - k);
+    const cd b{bc.re, -bc.im};
+    const cd wk{tw[2 * k], tw[2 * k + 1]};
+    const cd s = cadd(a, b), d = csub(a, b);
+    const cd wd = cmul(wk, d);
+    const cd zk{s.re - wd.im, s.im + wd.re};       // s + i * wd
+    sx[__builtin_bitreverse32((unsigned)k) >> (32 - logm)] = zk;
+  }
+  __syncthreads();
+  for (int st = 1; st <= logm; ++st) {
+    const int half = 1 << (st - 1);
+    const int tstep = ns >> st;                     // twiddle e^{2 pi i pos / (2 half)} = W^{pos ns / (2 half)}
+    for (int bf = threadIdx.x; bf < M / 2; bf += blockDim.x) {
+      const int pos = bf & (half - 1);
+      const int i0 = ((bf >> (st - 1)) << st) + pos, i1 = i0 + half;
+      const cd wv{tw[2 * pos * tstep], tw[2 * pos * tstep + 1]};
+      const cd x1 = cmul(wv, sx[i1]);
+      const cd x0 = sx[i0];
+      sx[i0] = cadd(x0, x1);
+      sx[i1] = csub(x0, x1);
+    }
+    __syncthreads();
+  }
+  float* out = ts + (size_t)blockIdx.x * ns;
+  for (int n = threadIdx.x; n < M; n += blockDim.x) {
+    const cd z = sx[n];
+    out[2 * n] = (float)(z.re * df * exp(sigma * dt * (2 * n)));
+    out[2 * n + 1] = (float)(z.im * df * exp(sigma * dt * (2 * n + 1)));
+  }
+}
+
+inline size_t rup(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+}  // namespace gm_syn
+
+using namespace gm_syn;
+
+// ---------------------------------------------------------------------------
+// host side
+
+namespace {
+
+struct SynPlan {
+  int R, Rp, L1, Lp, nf, Np, nsp;
+  long long nkf;
+  std::vector<int> nwnf;
+  std::vector<long long> koff;
+};
+
+int plan_sums(gm_ctx* c, int nf1, int nf2, const int* nwn, int nsp, long long nkf, int nrcv, SynPlan& P) {
+  if (nf1 < 1 || nf2 < nf1) return gm_fail(c, GM_ERR_ARG, "invalid frequency index range %d..%d", nf1, nf2);
+  if (!nwn) return gm_fail(c, GM_ERR_ARG, "nwn is required");
+  if (nsp != 10) return gm_fail(c, GM_ERR_UNSUPPORTED, "moment-tensor sums need nsp = 10 spectra (U, V x 4 jumps, W x 2)");
+  if (nrcv < 1) return gm_fail(c, GM_ERR_ARG, "nrcv must be positive");
+  P.nf = nf2 - nf1 + 1;
+  P.nsp = nsp;
+  P.nwnf.resize(P.nf);
+  P.koff.resize(P.nf);
+  long long off = 0;
+  int lmax = 0;
+  for (int f = 0; f < P.nf; ++f) {
+    const int n = nwn[nf1 - 1 + f];
+    if (n < 1 || n > 46000) return gm_fail(c, GM_ERR_ARG, "nwn[%d] = %d out of range [1, 46000]", nf1 + f, n);
+    P.nwnf[f] = n;
+    P.koff[f] = off;
+    off += n;
+    lmax = std::max(lmax, n - 1);
+  }
+  if (nkf < off) return gm_fail(c, GM_ERR_ARG, "nkf = %lld is smaller than the sum of nwn (%lld)", nkf, off);
+  P.nkf = nkf;
+  P.R = nrcv;
+  P.Rp = (int)rup(nrcv, TM);
+  P.L1 = lmax + 1;
+  P.Lp = (int)rup(P.L1, TK);
+  P.Np = (int)rup(2 * (size_t)P.nf, TN);
+  return GM_OK;
+}
+
+struct Carve {
+  size_t off = 0;
+  size_t take(size_t bytes) { size_t o = off; off += rup(bytes, 256); return o; }
+};
+
+int ensure_sbuf(gm_ctx* c, size_t bytes) {
+  if (bytes <= c->sbuf_size) return GM_OK;
+  if (c->sbuf) { (void)hipFree(c->sbuf); c->sbuf = nullptr; c->sbuf_size = 0; }
+  if (hipMalloc(&c->sbuf, bytes) != hipSuccess) return gm_fail(c, GM_ERR_NOMEM, "device allocation of %zu bytes failed", bytes);
+  c->sbuf_size = bytes;
+  return GM_OK;
+}
+
+// basis, spectra and the contraction into H; offsets of the carved buffer in `o`
+struct SumBufs { size_t theta, Bs, Gs, H, nwn, koff, gr, gi; };
+
+int run_sums(gm_ctx* c, const SynPlan& P, const double* theta, const float* gr, const float* gi, bool dev_in,
+             char* base, const SumBufs& o, hipStream_t s) {
+  HIPCHK(c, hipMemcpyAsync(base + o.theta, theta, sizeof(double) * P.R, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(base + o.nwn, P.nwnf.data(), sizeof(int) * P.nf, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(base + o.koff, P.koff.data(), sizeof(long long) * P.nf, hipMemcpyHostToDevice, s));
+  const float *dgr = gr, *dgi = gi;
+  if (!dev_in) {
+    HIPCHK(c, hipMemcpyAsync(base + o.gr, gr, sizeof(float) * P.nsp * P.nkf, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(base + o.gi, gi, sizeof(float) * P.nsp * P.nkf, hipMemcpyHostToDevice, s));
+    dgr = (const float*)(base + o.gr);
+    dgi = (const float*)(base + o.gi);
+  }
+  HIPCHK(c, hipEventRecord(c->sev[0], s));
+  hipLaunchKernelGGL(k_legendre, dim3((P.Rp + 63) / 64), dim3(64), 0, s, P.R, P.Rp, P.L1 - 1, P.Lp,
+                     (const double*)(base + o.theta), (__bf16*)(base + o.Bs), nullptr, nullptr, nullptr, nullptr,
+                     nullptr, 0);
+  HIPCHK(c, hipEventRecord(c->sev[1], s));
+  const size_t ng = (size_t)P.nsp * P.Np * P.Lp;
+  hipLaunchKernelGGL(k_gprep, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, s, P.nsp, P.nf, P.Np, P.Lp,
+                     (const int*)(base + o.nwn), (const long long*)(base + o.koff), P.nkf, dgr, dgi, 0.2,
+                     (__bf16*)(base + o.Gs));
+  HIPCHK(c, hipEventRecord(c->sev[2], s));
+  hipLaunchKernelGGL(k_contract, dim3(P.Np / TN, P.Rp / TM, kSums), dim3(256), 0, s, P.Rp, P.Np, P.Lp, P.nsp,
+                     (const __bf16*)(base + o.Bs), (const __bf16*)(base + o.Gs), (double*)(base + o.H));
+  HIPCHK(c, hipEventRecord(c->sev[3], s));
+  HIPCHK(c, hipGetLastError());
+  return GM_OK;
+}
+
+SumBufs carve_sums(const SynPlan& P, bool dev_in, Carve& cv) {
+  SumBufs o;
+  o.theta = cv.take(sizeof(double) * P.R);
+  o.nwn = cv.take(sizeof(int) * P.nf);
+  o.koff = cv.take(sizeof(long long) * P.nf);
+  o.Bs = cv.take(sizeof(__bf16) * kPieces * kRows * (size_t)P.Rp * P.Lp);
+  o.Gs = cv.take(sizeof(__bf16) * kPieces * (size_t)P.nsp * P.Np * P.Lp);
+  o.H = cv.take(sizeof(double) * kSums * (size_t)P.Rp * P.Np);
+  o.gr = dev_in ? 0 : cv.take(sizeof(float) * P.nsp * P.nkf);
+  o.gi = dev_in ? 0 : cv.take(sizeof(float) * P.nsp * P.nkf);
+  return o;
+}
+
+}  // namespace
+
+extern "C" int gm_legendre_batch(gm_ctx* c, int nrcv, const double* theta, int lmax, double* plm, double* dpdt,
+                                 double* pdst, double* d2pdt2, double* dtpdst) {
+  if (!c) return GM_ERR_ARG;
+  if (nrcv < 1 || !theta) return gm_fail(c, GM_ERR_ARG, "nrcv must be positive and theta given");
+  if (lmax < 0 || lmax > 46000) return gm_fail(c, GM_ERR_ARG, "lmax = %d out of range [0, 46000]", lmax);
+  if (!plm || !dpdt || !pdst || !d2pdt2 || !dtpdst) return gm_fail(c, GM_ERR_ARG, "all five output arrays are required");
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t n = (size_t)nrcv * 3 * (lmax + 1);
+  Carve cv;
+  const size_t ot = cv.take(sizeof(double) * nrcv);
+  size_t oo[5];
+  for (int i = 0; i < 5; ++i) oo[i] = cv.take(sizeof(double) * n);
+  int rc = ensure_sbuf(c, cv.off);
+  if (rc) return rc;
+  char* base = (char*)c->sbuf;
+  hipStream_t s = c->stream;
+  HIPCHK(c, hipMemcpyAsync(base + ot, theta, sizeof(double) * nrcv, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_legendre, dim3((nrcv + 63) / 64), dim3(64), 0, s, nrcv, nrcv, lmax, 0,
+                     (const double*)(base + ot), nullptr, (double*)(base + oo[0]), (double*)(base + oo[1]),
+                     (double*)(base + oo[2]), (double*)(base + oo[3]), (double*)(base + oo[4]), 1);
+  HIPCHK(c, hipGetLastError());
+  double* outs[5] = {plm, dpdt, pdst, d2pdt2, dtpdst};
+  for (int i = 0; i < 5; ++i) HIPCHK(c, hipMemcpyAsync(outs[i], base + oo[i], sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  return GM_OK;
+}
+
+extern "C" int gm_moment_sums(gm_ctx* c, int nf1, int nf2, const int* nwn, int nsp, long long nkf, const float* gfkspr,
+                              const float* gfkspi, int nrcv, const double* theta, int flags, double* hdsum) {
+  if (!c) return GM_ERR_ARG;
+  SynPlan P;
+  int rc = plan_sums(c, nf1, nf2, nwn, nsp, nkf, nrcv, P);
+  if (rc) return rc;
+  if (!gfkspr || !gfkspi || !theta || !hdsum) return gm_fail(c, GM_ERR_ARG, "spectra, theta and hdsum are required");
+  const bool dev = (flags & GM_FLAG_DEVICE_PTRS) != 0;
+  HIPCHK(c, hipSetDevice(c->device));
+  Carve cv;
+  SumBufs o = carve_sums(P, dev, cv);
+  if ((rc = ensure_sbuf(c, cv.off))) return rc;
+  char* base = (char*)c->sbuf;
+  hipStream_t s = c->stream;
+  if ((rc = run_sums(c, P, theta, gfkspr, gfkspi, dev, base, o, s))) return rc;
+  // H[j][Rp][Np] -> hdsum[j][R][nf] complex
+  for (int j = 0; j < kSums; ++j)
+    HIPCHK(c, hipMemcpy2DAsync(hdsum + (size_t)j * P.R * P.nf * 2, sizeof(double) * 2 * P.nf,
+                               base + o.H + sizeof(double) * (size_t)j * P.Rp * P.Np, sizeof(double) * P.Np,
+                               sizeof(double) * 2 * P.nf, P.R, dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  float ms;
+  for (int i = 0; i < 3; ++i) { (void)hipEventElapsedTime(&ms, c->sev[i], c->sev[i + 1]); c->stiming[i] = ms; }
+  c->stiming[3] = c->stiming[4] = 0.0;
+  (void)hipEventElapsedTime(&ms, c->sev[0], c->sev[3]);
+  c->stiming[5] = ms;
+  return GM_OK;
+}
+
+extern "C" int gm_synthetics(gm_ctx* c, int nf1, int nf2, double df, double sigma, const int* nwn, int nsp,
+                             long long nkf, const float* gfkspr, const float* gfkspi, int nrcv, const double* theta,
+                             const double* phi, const double* propdir, const double* tred, const double* mt,
+                             int velocity, int nsamp, double dt, int flags, float* ts, double* zsp) {
+  if (!c) return GM_ERR_ARG;
+  SynPlan P;
+  int rc = plan_sums(c, nf1, nf2, nwn, nsp, nkf, nrcv, P);
+  if (rc) return rc;
+  if (!gfkspr || !gfkspi || !theta || !phi || !mt || !ts)
+    return gm_fail(c, GM_ERR_ARG, "spectra, theta, phi, mt and ts are required");
+  if (!(df > 0.0) || !(dt > 0.0)) return gm_fail(c, GM_ERR_ARG, "df and dt must be positive");
+  // transformFrequencyTime (frequencyTime.f90:68-85): only the case without decimation
+  int nslog = (int)std::ceil((double)std::log((float)nf2) / std::log(2.0)) + 1;
+  if ((1 << nslog) > nsamp) return gm_fail(c, GM_ERR_UNSUPPORTED, "nsamp = %d needs decimation (minimum %d)", nsamp, 1 << nslog);
+  nslog = (int)std::ceil(-std::log(df * dt) / std::log(2.0));
+  if (nslog < 2 || (1 << nslog) != nsamp)
+    return gm_fail(c, GM_ERR_ARG, "nsamp = %d must equal 2^ceil(-log2(df dt)) = %d", nsamp, nslog >= 2 ? 1 << nslog : 0);
+  if (nsamp > kMaxFFT) return gm_fail(c, GM_ERR_UNSUPPORTED, "nsamp = %d exceeds %d", nsamp, kMaxFFT);
+  const bool dev = (flags & GM_FLAG_DEVICE_PTRS) != 0;
+  HIPCHK(c, hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  if (c->tw_n != nsamp) {
+    if (c->dtw) { (void)hipFree(c->dtw); c->dtw = nullptr; c->tw_n = 0; }
+    if (hipMalloc(&c->dtw, sizeof(double) * nsamp) != hipSuccess) return gm_fail(c, GM_ERR_NOMEM, "twiddle allocation failed");
+    hipLaunchKernelGGL(k_twiddle, dim3((nsamp / 2 + 255) / 256), dim3(256), 0, s, nsamp, c->dtw);
+    c->tw_n = nsamp;
+  }
+  Carve cv;
+  SumBufs o = carve_sums(P, dev, cv);
+  const size_t o_phi = cv.take(sizeof(double) * P.R), o_pd = cv.take(sizeof(double) * P.R);
+  const size_t o_tr = cv.take(sizeof(double) * P.R), o_mt = cv.take(sizeof(double) * 6);
+  const size_t o_z = cv.take(sizeof(double) * 2 * 3 * (size_t)P.R * nf2);
+  const size_t o_ts = dev ? 0 : cv.take(sizeof(float) * 3 * (size_t)P.R * nsamp);
+  if ((rc = ensure_sbuf(c, cv.off))) return rc;
+  char* base = (char*)c->sbuf;
+  HIPCHK(c, hipMemcpyAsync(base + o_phi, phi, sizeof(double) * P.R, hipMemcpyHostToDevice, s));
+  if (propdir) HIPCHK(c, hipMemcpyAsync(base + o_pd, propdir, sizeof(double) * P.R, hipMemcpyHostToDevice, s));
+  if (tred) HIPCHK(c, hipMemcpyAsync(base + o_tr, tred, sizeof(double) * P.R, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(base + o_mt, mt, sizeof(double) * 6, hipMemcpyHostToDevice, s));
+  if ((rc = run_sums(c, P, theta, gfkspr, gfkspi, dev, base, o, s))) return rc;
+  double* dZ = (double*)(base + o_z);
+  hipLaunchKernelGGL(k_combine, dim3((unsigned)(((size_t)P.R * nf2 + 255) / 256)), dim3(256), 0, s, P.R, nf1, nf2,
+                     P.Rp, P.Np, (const double*)(base + o.H), (const double*)(base + o_phi),
+                     propdir ? (const double*)(base + o_pd) : nullptr, tred ? (const double*)(base + o_tr) : nullptr,
+                     (const double*)(base + o_mt), velocity ? 1 : 0, df, sigma, dZ);
+  HIPCHK(c, hipEventRecord(c->sev[4], s));
+  float* dts = dev ? ts : (float*)(base + o_ts);
+  int logm = 0;
+  while ((1 << (logm + 1)) <= nsamp / 2) ++logm;
+  hipLaunchKernelGGL(k_ifft, dim3(3 * P.R), dim3(256), 0, s, nf1, nf2, nsamp, logm, (const double*)dZ, c->dtw, df, sigma,
+                     dt, dts);
+  HIPCHK(c, hipEventRecord(c->sev[5], s));
+  HIPCHK(c, hipGetLastError());
+  if (!dev) HIPCHK(c, hipMemcpyAsync(ts, dts, sizeof(float) * 3 * (size_t)P.R * nsamp, hipMemcpyDeviceToHost, s));
+  if (zsp)
+    HIPCHK(c, hipMemcpyAsync(zsp, dZ, sizeof(double) * 2 * 3 * (size_t)P.R * nf2,
+                             dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  float ms;
+  for (int i = 0; i < 5; ++i) { (void)hipEventElapsedTime(&ms, c->sev[i], c->sev[i + 1]); c->stiming[i] = ms; }
+  (void)hipEventElapsedTime(&ms, c->sev[0], c->sev[5]);
+  c->stiming[5] = ms;
+  return GM_OK;
+}
+
+extern "C" int gm_last_synth_timing(gm_ctx* c, double* ms6) {
+  if (!c || !ms6) return GM_ERR_ARG;
+  for (int i = 0; i < 6; ++i) ms6[i] = c->stiming[i];
+  return GM_OK;
+}

@@ -72,10 +72,13 @@ class GreenFKEngine:
         check(self.h, _lib.lib().gm_set_option(self.h, 1, VARIANTS[variant]))
         self.variant = variant
 
-    def configure(self, persistent=None, unit=None, waves=None):
+    def configure(self, persistent=None, unit=None, waves=None, propose_every=None):
         """Kernel schedule options (all result-preserving): persistent substep-scheduled kernels or
-        one lane per job, work-unit size, waves per SIMD of the persistent kernels."""
+        one lane per job, work-unit size, waves per SIMD of the persistent kernels; and how often the
+        learned proposer is consulted (every N accepted steps, plus sub-interval starts)."""
         L = _lib.lib()
+        if propose_every is not None:
+            check(self.h, L.gm_set_option(self.h, 5, int(propose_every)))
         if persistent is not None:
             check(self.h, L.gm_set_option(self.h, 2, int(bool(persistent))))
         if unit is not None:
@@ -127,9 +130,9 @@ class GreenFKEngine:
 
     def collect_steplog(self, freqs, lmin, nl, sigma, eps, jsl, jsr, maxrec=2_000_000,
                         attmode="ATTENUATION_AND_DISPERSION"):
-        """Step records of the plain spheroidal-minors integration, float32 [n, 16]."""
+        """Step records of the plain spheroidal-minors integration, float32 [n, 20] (layout in include/gemini_fk.h)."""
         freqs, lmin, nl, nk = self._plan(freqs, lmin, nl)
-        rec = np.zeros((maxrec, 16), np.float32)
+        rec = np.zeros((maxrec, 20), np.float32)
         nrec = C.c_int()
         st = np.zeros(10, np.int64)
         am = ATTMODES[attmode] if isinstance(attmode, str) else int(attmode)

@@ -1,0 +1,109 @@
+"""Synthetics half of the Green-FK path on the GPU: Legendre functions, moment
+tensor harmonic degree sums, displacement spectra and the frequency-to-time
+transform (the global, spherical-harmonics branch of
+f90/computeSyntheticSeismograms.f90:311-388).
+
+The Green FK spectra of one source node come in the layout the reference's
+synthetics stage reads from the GFK file (readDataNodeGreenFKSpectra,
+f90/greenFKSpectra.f90:171-212): FP32 real and imaginary parts [nsp][nkf], k
+running over the frequencies nf1..nf2 with the degrees of each frequency
+fastest.  That is also one node's slice of GreenFKEngine.gfk()'s FP32 output.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import D, I, VP, check
+
+MOMENT_SUMS = 14
+
+
+class SyntheticsEngine:
+    def __init__(self, device: int = 0):
+        self.device = device
+        h = VP()
+        rc = _lib.lib().gm_ctx_create(int(device), C.byref(h))
+        if rc != 0:
+            raise _lib.GeminiFKError(rc, _lib.last_error(None))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.lib().gm_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def timing(self):
+        """ms of the last call: Legendre basis, spectra preparation, contraction, combination, transform, total."""
+        ms = np.zeros(6)
+        check(self.h, _lib.lib().gm_last_synth_timing(self.h, ms.ctypes.data_as(D)))
+        return ms
+
+    def legendre(self, lmax, theta):
+        """computeLegendreFunctions (f90/legendreFunctions.f90:48) at each theta: dict of [nrcv, 3, lmax+1]."""
+        theta = np.ascontiguousarray(np.atleast_1d(theta), dtype=np.float64)
+        out = {k: np.zeros((theta.size, 3, lmax + 1)) for k in ("plm", "dpdt", "pdst", "d2pdt2", "dtpdst")}
+        check(self.h, _lib.lib().gm_legendre_batch(self.h, theta.size, theta.ctypes.data_as(D), int(lmax),
+                                                   *[out[k].ctypes.data_as(D) for k in ("plm", "dpdt", "pdst",
+                                                                                        "d2pdt2", "dtpdst")]))
+        return out
+
+    @staticmethod
+    def _spectra(gr, gi):
+        gr = np.ascontiguousarray(gr, dtype=np.float32)
+        gi = np.ascontiguousarray(gi, dtype=np.float32)
+        if gr.shape != gi.shape or gr.ndim != 2:
+            raise ValueError("gr, gi must be [nsp, nkf] arrays of one shape")
+        return gr, gi
+
+    def moment_sums(self, nf1, nf2, nwn, gr, gi, theta):
+        """14 moment harmonic degree sums, complex [14, nrcv, nf2-nf1+1] (momentHarmonicDegreeSums)."""
+        nwn = np.ascontiguousarray(nwn, dtype=np.int32)
+        gr, gi = self._spectra(gr, gi)
+        theta = np.ascontiguousarray(np.atleast_1d(theta), dtype=np.float64)
+        out = np.zeros((MOMENT_SUMS, theta.size, nf2 - nf1 + 1), np.complex128)
+        check(self.h, _lib.lib().gm_moment_sums(self.h, int(nf1), int(nf2), nwn.ctypes.data_as(I), gr.shape[0],
+                                                gr.shape[1], gr.ctypes.data, gi.ctypes.data, theta.size,
+                                                theta.ctypes.data_as(D), 0, out.ctypes.data))
+        return out
+
+    def synthetics(self, nf1, nf2, df, sigma, nwn, gr, gi, theta, phi, mt, nsamp, dt, propdir=None, tred=None,
+                   velocity=False, want_spectra=False, device_ptrs=None):
+        """Seismograms [nrcv, 3, nsamp] FP32 (and displacement spectra [nrcv, 3, nf2] if asked).
+
+        device_ptrs = (gr_ptr, gi_ptr, nsp, nkf, ts_ptr): spectra and output already in HBM
+        (integers, e.g. torch tensors' data_ptr()); gr, gi are then ignored and ts is written in place.
+        """
+        nwn = np.ascontiguousarray(nwn, dtype=np.int32)
+        theta = np.ascontiguousarray(np.atleast_1d(theta), dtype=np.float64)
+        phi = np.ascontiguousarray(np.atleast_1d(phi), dtype=np.float64)
+        mt = np.ascontiguousarray(mt, dtype=np.float64)
+        if phi.size != theta.size or mt.size != 6:
+            raise ValueError("phi must match theta and mt must hold 6 entries")
+        pd = None if propdir is None else np.ascontiguousarray(propdir, dtype=np.float64)
+        tr = None if tred is None else np.ascontiguousarray(tred, dtype=np.float64)
+        zsp = np.zeros((theta.size, 3, nf2), np.complex128) if want_spectra else None
+        if device_ptrs is None:
+            gr, gi = self._spectra(gr, gi)
+            nsp, nkf, pgr, pgi, flags = gr.shape[0], gr.shape[1], gr.ctypes.data, gi.ctypes.data, 0
+            ts = np.zeros((theta.size, 3, nsamp), np.float32)
+            pts = ts.ctypes.data
+        else:
+            if want_spectra:
+                raise ValueError("want_spectra needs host outputs")
+            pgr, pgi, nsp, nkf, pts = device_ptrs
+            ts, flags = None, 1
+        check(self.h, _lib.lib().gm_synthetics(
+            self.h, int(nf1), int(nf2), float(df), float(sigma), nwn.ctypes.data_as(I), int(nsp), int(nkf), pgr, pgi,
+            theta.size, theta.ctypes.data_as(D), phi.ctypes.data_as(D), None if pd is None else pd.ctypes.data_as(D),
+            None if tr is None else tr.ctypes.data_as(D), mt.ctypes.data_as(D), int(bool(velocity)), int(nsamp),
+            float(dt), flags, pts, None if zsp is None else zsp.ctypes.data))
+        return (ts, zsp) if want_spectra else ts

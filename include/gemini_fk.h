@@ -57,6 +57,8 @@ typedef struct gm_ctx gm_ctx;
 #define GM_OPT_PERSISTENT 2    /* 0 (default): one lane per job, 1: persistent substep-scheduled kernels */
 #define GM_OPT_UNIT 3          /* (omega, l) instances per work unit of the persistent kernels */
 #define GM_OPT_WAVES 4         /* waves per SIMD the persistent kernels are built for: 1 or 2 */
+#define GM_OPT_PROPOSE_EVERY 5 /* learned proposer: consult the network at sub-interval starts, after a
+                                  rejection and every N-th accepted step (default 1: every step) */
 
 /* work counters returned in stats[10] (same meaning as the CPU oracle's) */
 #define GM_NSTAT 10            /* propagations, accepted steps, rejected steps, MMP sequences,
@@ -141,10 +143,56 @@ int gm_set_proposer(gm_ctx* ctx, const float* params, const float* norm, float s
 /* run the plain minors integration over the given (f, l) and record every
  * spheroidal-minors step: 9 features, log10(h/R), log10(err_k/eps) for the
  * error tests after 3..7 sequences (all 7 sequences are computed), accepted
- * try (0 = rejected): 16 floats per record */
+ * try (0 = rejected), instance index, running step number of the instance,
+ * log10(distance to the end of the sub-interval / R), 1 if the step was cut to
+ * end the sub-interval: 20 floats per record */
 int gm_collect_steplog(gm_ctx* ctx, int nfreq, const double* freqs, const int* lmin, const int* nl, double sigma,
                        double eps, int attmode, int jsl, int jsr, int maxrec, float* records, int* nrec,
                        long long* stats);
+
+/* ---- synthetics half: Legendre functions, degree sums, frequency-time ---- */
+
+/* Unnormalised associated Legendre functions without the Condon-Shortley sign
+ * for m = 0..2 and their theta derivatives at each theta[r] (radians), FP64.
+ * Replaces computeLegendreFunctions (f90/legendreFunctions.f90:48-157).
+ * Outputs [nrcv][3][lmax+1] each (m = 0 rows of pdst and dtpdst are zero). */
+int gm_legendre_batch(gm_ctx* ctx, int nrcv, const double* theta, int lmax, double* plm, double* dpdt, double* pdst,
+                      double* d2pdt2, double* dtpdst);
+
+/* The 14 moment-tensor harmonic degree sums (kint 1..14) of one source node
+ * for receivers at epicentral distances theta[r] (radians) and frequency
+ * indices jf = nf1..nf2 (Fortran numbering, f = (jf-1) df) with nwn[jf-1]
+ * degrees each.  gfkspr/gfkspi: FP32 [nsp = 10][nkf], k over the frequencies
+ * nf1..nf2 with degrees fastest (readDataNodeGreenFKSpectra's layout,
+ * f90/greenFKSpectra.f90:171-212).  hdsum: complex [14][nrcv][nf2-nf1+1].
+ * Replaces momentHarmonicDegreeSums with taperHarmonicDegreeSums
+ * (f90/harmonicDegreeSums.f90:231-369) in the frequency loop of
+ * sphericalHarmonicsDisplacementSpectra (f90/displacementSpectra.f90:49-106).
+ * The contraction runs on the matrix cores in split bf16 with FP32
+ * accumulation per 32 degrees and FP64 accumulation across them.
+ * GM_FLAG_DEVICE_PTRS: gfkspr, gfkspi and hdsum are device pointers. */
+int gm_moment_sums(gm_ctx* ctx, int nf1, int nf2, const int* nwn, int nsp, long long nkf, const float* gfkspr,
+                   const float* gfkspi, int nrcv, const double* theta, int flags, double* hdsum);
+
+/* Seismograms of one source node: degree sums, moment-tensor combination
+ * (f90/singleFrequencyDisplacement.f90:108-125, mt = Mrr, Mtt, Mff, Mrt, Mrf,
+ * Mtf), time shift by tred[r] (NULL: none), velocity (velocity = 1) and
+ * rotation to Z, N, E by propdir[r] (NULL: components r, theta, phi), then the
+ * frequency-to-time transform to nsamp FP32 samples (transformFrequencyTime,
+ * f90/frequencyTime.f90:55-113; nsamp = 2^ceil(-log2(df dt)) <= 8192, no
+ * decimation).  Event and station filters are unity.  Replaces the station
+ * loop body of f90/computeSyntheticSeismograms.f90:311-388.
+ * ts: [nrcv][3][nsamp]; zsp (optional): complex [nrcv][3][nf2].
+ * GM_FLAG_DEVICE_PTRS: gfkspr, gfkspi, ts and zsp are device pointers. */
+int gm_synthetics(gm_ctx* ctx, int nf1, int nf2, double df, double sigma, const int* nwn, int nsp, long long nkf,
+                  const float* gfkspr, const float* gfkspi, int nrcv, const double* theta, const double* phi,
+                  const double* propdir, const double* tred, const double* mt, int velocity, int nsamp, double dt,
+                  int flags, float* ts, double* zsp);
+
+/* Kernel time of the last gm_moment_sums / gm_synthetics call in milliseconds:
+ * 0 Legendre basis, 1 spectra preparation, 2 contraction, 3 combination,
+ * 4 frequency-time transform, 5 total. */
+int gm_last_synth_timing(gm_ctx* ctx, double* ms6);
 
 #ifdef __cplusplus
 }

@@ -22,8 +22,8 @@ L = C.POINTER(C.c_int64)
 
 
 def build(force: bool = False) -> str:
-    src = os.path.join(_HERE, "gemini_oracle.cpp")
-    if force or not os.path.exists(_LIB) or os.path.getmtime(_LIB) < os.path.getmtime(src):
+    srcs = [os.path.join(_HERE, f) for f in ("gemini_oracle.cpp", "synth_oracle.cpp")]
+    if force or not os.path.exists(_LIB) or os.path.getmtime(_LIB) < max(os.path.getmtime(f) for f in srcs):
         subprocess.check_call(["make", "-s", "-B" if force else "-s", "-C", _HERE, "liboracle.so"])
     return _LIB
 
@@ -220,3 +220,78 @@ def vti(fratio, ro, alfv, alfh, betv, beth, eta, qkinv, qminv):
     z = np.zeros(7, np.complex128)
     lib().orc_vti(fratio, ro, alfv, alfh, betv, beth, eta, qkinv, qminv, z.ctypes.data_as(D))
     return z
+
+
+# ---- synthetics half: Legendre functions, degree sums, frequency-time transform ----
+
+def legendre(lmax, theta):
+    """dict of plm, dpdt, pdst, d2pdt2, dtpdst, each [3, lmax+1] (m = 0..2; pdst/dtpdst row 0 unused)."""
+    out = {k: np.zeros((3, lmax + 1)) for k in ("plm", "dpdt", "pdst", "d2pdt2", "dtpdst")}
+    rc = lib().orc_legendre(int(lmax), C.c_double(theta), *[_d(out[k]) for k in ("plm", "dpdt", "pdst", "d2pdt2",
+                                                                                   "dtpdst")])
+    assert rc == 0
+    return out
+
+
+def taper(lmax, tapfrac=0.2):
+    w = np.zeros(lmax + 1)
+    lib().orc_taper(int(lmax), C.c_double(tapfrac), _d(w))
+    return w
+
+
+def fft(x, y, is_=1):
+    """In-place-style radix-2 FFT of the reference (no normalisation); returns (x, y)."""
+    x = np.ascontiguousarray(x, dtype=np.float64).copy()
+    y = np.ascontiguousarray(y, dtype=np.float64).copy()
+    n = int(round(np.log2(x.size)))
+    assert 1 << n == x.size
+    lib().orc_fft(n, int(is_), _d(x), _d(y))
+    return x, y
+
+
+def moment_sums(nf1, nf2, nwn, gr, gi, theta, tapfrac=0.2, nthreads=1):
+    """14 moment harmonic degree sums: complex [nrcv, nf, 14].  gr, gi: FP32 [nsp, nkf] of one node."""
+    nwn = np.ascontiguousarray(nwn, dtype=np.int32)
+    gr = np.ascontiguousarray(gr, dtype=np.float32)
+    gi = np.ascontiguousarray(gi, dtype=np.float32)
+    theta = np.ascontiguousarray(np.atleast_1d(theta), dtype=np.float64)
+    nf = nf2 - nf1 + 1
+    h = np.zeros((theta.size, nf, 14), np.complex128)
+    F = C.POINTER(C.c_float)
+    lib().orc_moment_sums(int(nf1), int(nf2), nwn.ctypes.data_as(I), C.c_long(gr.shape[1]), gr.ctypes.data_as(F),
+                          gi.ctypes.data_as(F), theta.size, _d(theta), C.c_double(tapfrac),
+                          h.ctypes.data_as(D), int(nthreads))
+    return h
+
+
+def synthetics(nf1, nf2, df, sigma, nwn, gr, gi, theta, phi, mt, nsamp, dt, propdir=None, tred=None, velocity=False,
+               nthreads=1):
+    """(ts float32 [nrcv, 3, nsamp], zsp complex [nrcv, 3, nf2]) of one source node."""
+    nwn = np.ascontiguousarray(nwn, dtype=np.int32)
+    gr = np.ascontiguousarray(gr, dtype=np.float32)
+    gi = np.ascontiguousarray(gi, dtype=np.float32)
+    theta = np.ascontiguousarray(theta, dtype=np.float64)
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    mt = np.ascontiguousarray(mt, dtype=np.float64)
+    nr = theta.size
+    ts = np.zeros((nr, 3, nsamp), np.float32)
+    zsp = np.zeros((nr, 3, nf2), np.complex128)
+    F = C.POINTER(C.c_float)
+    pd = None if propdir is None else np.ascontiguousarray(propdir, dtype=np.float64)
+    tr = None if tred is None else np.ascontiguousarray(tred, dtype=np.float64)
+    rc = lib().orc_synthetics(int(nf1), int(nf2), C.c_double(df), C.c_double(sigma), nwn.ctypes.data_as(I),
+                              C.c_long(gr.shape[1]), gr.ctypes.data_as(F), gi.ctypes.data_as(F), nr, _d(theta),
+                              _d(phi), None if pd is None else _d(pd), None if tr is None else _d(tr), _d(mt),
+                              int(bool(velocity)), int(nsamp), C.c_double(dt), ts.ctypes.data_as(F),
+                              zsp.ctypes.data_as(D), int(nthreads))
+    assert rc == 0, rc
+    return ts, zsp
+
+
+def freq_time(nf1, nf2, df, sigma, sp, nsamp, dt):
+    """transformFrequencyTime of one spectrum sp (complex [nf2]); returns (status, ts float32 [nsamp])."""
+    sp = np.ascontiguousarray(sp, dtype=np.complex128)
+    ts = np.zeros(nsamp, np.float32)
+    rc = lib().orc_freq_time(int(nf1), int(nf2), C.c_double(df), C.c_double(sigma), sp.ctypes.data_as(D), int(nsamp),
+                             C.c_double(dt), ts.ctypes.data_as(C.POINTER(C.c_float)))
+    return rc, ts

@@ -1,0 +1,284 @@
+// CPU restatement of the synthetics half of the Green-FK path: Legendre
+// functions, moment-tensor harmonic degree sums, the single-frequency
+// displacement combination, time shift / velocity / ZNE rotation and the
+// frequency-to-time transform.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker the GPU kernels are
+// compared against; no product code path may call it.
+//
+// Each function cites the reference routine it restates.  Arithmetic is FP64
+// throughout, in the reference's operation order where that is cheap to keep.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <vector>
+
+namespace syn {
+
+using cd = std::complex<double>;
+static const double kPi = 3.14159265358979323846;     // mc_pid (f90/mathConstants.f90)
+
+// Unnormalised associated Legendre functions P_l^m(cos theta), m = 0..2, without
+// the Condon-Shortley sign, and their theta derivatives / ratios to sin(theta).
+// Restates computeLegendreFunctions, f90/legendreFunctions.f90:48-157.
+// Layout: plm, dpdt, d2pdt2: [3][lmax+1]; pdst, dtpdst: [3][lmax+1] (m = 0 row unused).
+void legendre(int lmax, double theta, double* plm, double* dpdt, double* pdst, double* d2pdt2, double* dtpdst) {
+  const int L1 = lmax + 1;
+  for (int i = 0; i < 3 * L1; ++i) plm[i] = dpdt[i] = pdst[i] = d2pdt2[i] = dtpdst[i] = 0.0;
+  auto P = [&](int l, int m) -> double& { return plm[m * L1 + l]; };
+  auto D = [&](int l, int m) -> double& { return dpdt[m * L1 + l]; };
+  auto S = [&](int l, int m) -> double& { return pdst[m * L1 + l]; };
+  auto D2 = [&](int l, int m) -> double& { return d2pdt2[m * L1 + l]; };
+  auto DS = [&](int l, int m) -> double& { return dtpdst[m * L1 + l]; };
+  const double ct = std::cos(theta), st = std::sin(theta);
+  const bool poles = !(st > 0.0);
+  // legendreFunctions.f90:84-95: P_m^m = (2m-1)!! sin^m, upward recurrence in l
+  for (int m = 0; m <= 2 && m <= lmax; ++m) {
+    double pmm = 1.0, odd = 1.0;
+    for (int j = 1; j <= m; ++j) { pmm = pmm * odd * st; odd = odd + 2.0; }
+    P(m, m) = pmm;
+    if (m + 1 <= lmax) P(m + 1, m) = ct * (2 * m + 1) * P(m, m);
+    for (int l = m + 2; l <= lmax; ++l)
+      P(l, m) = (ct * double(2 * l - 1) * P(l - 1, m) - (l + m - 1) * P(l - 2, m)) / double(l - m);
+  }
+  if (!poles) {
+    // legendreFunctions.f90:99-112
+    const double rst = 1.0 / st;
+    for (int m = 0; m <= 2 && m <= lmax; ++m) {
+      D(m, m) = m * ct * P(m, m) * rst;
+      for (int l = m + 1; l <= lmax; ++l) D(l, m) = (l * ct * P(l, m) - (l + m) * P(l - 1, m)) * rst;
+    }
+    for (int m = 1; m <= 2; ++m)
+      for (int l = m; l <= lmax; ++l) S(l, m) = P(l, m) * rst;
+    // legendreFunctions.f90:128-141
+    for (int m = 0; m <= 2 && m <= lmax; ++m)
+      for (int l = m; l <= lmax; ++l) D2(l, m) = -ct * rst * D(l, m) + (m * m * (rst * rst) - double(l * (l + 1))) * P(l, m);
+    for (int m = 1; m <= 2; ++m)
+      for (int l = m; l <= lmax; ++l) DS(l, m) = rst * (D(l, m) - ct * rst * P(l, m));
+  } else {
+    // legendreFunctions.f90:113-127, 142-155 (pole limits; sign(ct)^l)
+    const double s = ct < 0.0 ? -1.0 : 1.0;
+    for (int l = 0; l <= lmax; ++l) {
+      const double sl = (l % 2) ? s : 1.0, sl1 = ((l + 1) % 2) ? s : 1.0;
+      const double ll1 = double(l) * (l + 1);
+      const double q = double(l + 2) * (l + 1) * l * (l - 1);   // 64-bit: the reference's default integer overflows for l > 215
+      D(l, 0) = 0.0; D(l, 1) = 0.5 * ll1 * sl; D(l, 2) = 0.0;
+      D2(l, 0) = 0.5 * ll1 * sl; D2(l, 1) = 0.0; D2(l, 2) = 0.25 * q * sl;
+      if (l >= 1) {
+        S(l, 1) = 0.5 * ll1 * sl1; S(l, 2) = 0.0;
+        DS(l, 1) = 0.0; DS(l, 2) = 0.125 * q * sl1;
+      }
+    }
+  }
+}
+
+// cos^2 taper over the last tapfrac of the degrees 0..lmax.
+// Restates taperHarmonicDegreeSums, f90/harmonicDegreeSums.f90:356-369.
+void taper(int lmax, double tapfrac, double* w) {
+  int jt = (int)std::lround(lmax * (1.0 - tapfrac));
+  if (jt > lmax) jt = lmax;
+  if (jt < 1) jt = 1;
+  const double c1 = lmax * tapfrac * 2.0 / kPi;
+  const double c2 = (1.0 - tapfrac) / tapfrac * 0.5 * kPi;
+  for (int j = 0; j <= lmax; ++j) {
+    if (j < jt) { w[j] = 1.0; continue; }
+    const double c = std::cos(double(j) / c1 - c2);
+    w[j] = c * c;
+  }
+}
+
+// (basis row, Green spectrum index, factor) of the 14 moment-tensor sums.
+// Restates the case table of momentHarmonicDegreeSums
+// (f90/harmonicDegreeSums.f90:251-307) and the (component, jump) -> spectrum
+// offset map (:44-48 with positionDataGreenFKSpectra, f90/greenFKSpectra.f90:204-212
+// for dsvmask (U, V, W) and 4 spheroidal / 2 toroidal jumps).
+// basis: 0 P_l0, 1 P_l1, 2 P_l2, 3 dP_l0, 4 dP_l1, 5 dP_l2, 6 P_l1/sin, 7 P_l2/sin
+struct SumDef { int basis, isp, lmin; double fac; bool over_ll1; };
+static const SumDef kSums[14] = {
+    {0, 0, 0, 1.0, false}, {0, 1, 0, 1.0, false}, {1, 2, 1, 2.0, true}, {2, 3, 2, 2.0, true},
+    {3, 4, 1, 1.0, false}, {3, 5, 1, 1.0, false}, {4, 6, 1, 2.0, true}, {5, 7, 2, 2.0, true},
+    {6, 8, 1, 2.0, true},  {7, 9, 2, 4.0, true},  {4, 8, 1, 2.0, true}, {5, 9, 2, -2.0, true},
+    {6, 6, 1, 2.0, true},  {7, 7, 2, -4.0, true}};
+
+// The 14 moment harmonic degree sums at one receiver for each frequency.
+// gr, gi: [nsp][nkf] FP32 spectra of one node, k over frequencies nf1..nf2 with
+// nwn[jf-1] degrees each (l fastest) — the layout readDataNodeGreenFKSpectra
+// returns; hdsum: [nf][14] complex.
+// Restates the frequency loop of sphericalHarmonicsDisplacementSpectra
+// (f90/displacementSpectra.f90:49-106) with momentHarmonicDegreeSums.
+void moment_sums(int nf1, int nf2, const int* nwn, long nkf, const float* gr, const float* gi, double theta,
+                 double tapfrac, cd* hdsum) {
+  int lmax = 0;
+  for (int jf = nf1; jf <= nf2; ++jf) lmax = std::max(lmax, nwn[jf - 1] - 1);
+  const int L1 = lmax + 1;
+  std::vector<double> plm(3 * L1), dpdt(3 * L1), pdst(3 * L1), d2(3 * L1), ds(3 * L1), w(L1);
+  legendre(lmax, theta, plm.data(), dpdt.data(), pdst.data(), d2.data(), ds.data());
+  const double* rows[8] = {&plm[0], &plm[L1], &plm[2 * L1], &dpdt[0], &dpdt[L1], &dpdt[2 * L1], &pdst[L1], &pdst[2 * L1]};
+  const double fourpi = 4.0 * kPi;
+  long off = 0;
+  for (int jf = nf1; jf <= nf2; ++jf) {
+    const int n = nwn[jf - 1];
+    const int lm = n - 1;
+    taper(lm, tapfrac, w.data());
+    for (int s = 0; s < 14; ++s) {
+      const SumDef& d = kSums[s];
+      cd z = 0.0;
+      for (int l = d.lmin; l <= lm; ++l) {
+        double b = rows[d.basis][l];
+        if (d.over_ll1) b = b * d.fac / double(l * (l + 1));
+        const cd g(gr[(long)d.isp * nkf + off + l], gi[(long)d.isp * nkf + off + l]);
+        z += b * g * (2.0 * l + 1.0) / fourpi * w[l];
+      }
+      hdsum[(long)(jf - nf1) * 14 + s] = z;
+    }
+    off += n;
+  }
+}
+
+// Displacement (r, theta, phi) at one frequency from the 14 sums and the moment
+// tensor mt = (Mrr, Mtt, Mff, Mrt, Mrf, Mtf).
+// Restates momentSingleFrequencyDisplacement, f90/singleFrequencyDisplacement.f90:108-125.
+void moment_displacement(double phi, const double* mt, const cd* w, cd* zsp) {
+  const double sp = std::sin(phi), cp = std::cos(phi), s2 = std::sin(2.0 * phi), c2 = std::cos(2.0 * phi);
+  const double a = mt[3] * cp + mt[4] * sp, b = (mt[2] - mt[1]) * c2 - 2.0 * mt[5] * s2;
+  const double c = mt[4] * cp - mt[3] * sp, d = (mt[2] - mt[1]) * s2 + 2.0 * mt[5] * c2;
+  zsp[0] = w[0] * mt[0] + w[1] * (mt[1] + mt[2]) + w[2] * a + w[3] * b;
+  zsp[1] = w[4] * mt[0] + w[5] * (mt[1] + mt[2]) + (w[6] + w[8]) * a + (w[7] + w[9]) * b;
+  zsp[2] = (w[10] + w[12]) * c + (w[11] + w[13]) * d;
+}
+
+// In-place radix-2 complex FFT, e^{+i...} for is > 0, no normalisation.
+// Restates doubleFastFourierTransform (f90/fourierTransform.f90:118-192):
+// decimation in frequency with bit-reversed reordering at the end.
+void fft(std::vector<double>& x, std::vector<double>& y, int n, int is) {
+  const int ng = 1 << n;
+  const double piz = 2.0 * kPi;
+  for (int m = 1; m <= n; ++m) {
+    const int nar = 1 << (m - 1);
+    const int lar = ng / nar, larh = lar / 2;
+    const double alpha = piz / double(is > 0 ? lar : -lar);
+    for (int jr = 1; jr <= larh; ++jr) {
+      const double beta = alpha * double(jr - 1);
+      const double ec = std::cos(beta), es = std::sin(beta);
+      int ja = jr - lar;
+      for (int nr = 1; nr <= nar; ++nr) {
+        ja += lar;
+        const int jb = ja + larh;
+        const double zx = x[ja - 1] - x[jb - 1], zy = y[ja - 1] - y[jb - 1];
+        x[ja - 1] += x[jb - 1];
+        y[ja - 1] += y[jb - 1];
+        x[jb - 1] = zx * ec - zy * es;
+        y[jb - 1] = zx * es + zy * ec;
+      }
+    }
+  }
+  for (int j = 0; j < ng; ++j) {
+    int k = 0;
+    for (int b = 0; b < n; ++b) if (j & (1 << b)) k |= 1 << (n - 1 - b);
+    if (j < k) { std::swap(x[j], x[k]); std::swap(y[j], y[k]); }
+  }
+}
+
+// Spectrum sp[jf-1], jf = 1..nf2 (entries below nf1 ignored) -> nsamp real
+// samples.  Restates transformFrequencyTime (f90/frequencyTime.f90:55-113) for
+// the case ns <= nsamp (no decimation).  Returns 0, or 1 if decimation would be needed.
+int freq_time(int nf1, int nf2, double df, double sigma, const cd* sp, int nsamp, double dt, float* ts) {
+  // the reference takes this log in single precision (log of real(nf2))
+  int nslog = (int)std::ceil((double)std::log((float)nf2) / std::log(2.0)) + 1;
+  int ns = 1 << nslog;
+  if (ns > nsamp) return 1;
+  ns = nsamp;
+  nslog = (int)std::ceil(-std::log(df * dt) / std::log(2.0));
+  if ((1 << nslog) != ns) return 2;
+  std::vector<double> x(ns, 0.0), y(ns, 0.0);
+  for (int jf = nf1; jf <= nf2; ++jf) { x[jf - 1] = sp[jf - 1].real(); y[jf - 1] = sp[jf - 1].imag(); }
+  for (int i = ns / 2 + 2; i <= ns; ++i) { x[i - 1] = x[ns - i + 1]; y[i - 1] = -y[ns - i + 1]; }
+  fft(x, y, nslog, +1);
+  double fac = 1.0;
+  const double expo = std::exp(sigma * dt);
+  for (int i = 0; i < nsamp; ++i) { ts[i] = (float)(x[i] * df * fac); fac = fac * expo; }
+  return 0;
+}
+
+}  // namespace syn
+
+extern "C" {
+
+int orc_legendre(int lmax, double theta, double* plm, double* dpdt, double* pdst, double* d2pdt2, double* dtpdst) {
+  if (lmax < 0) return 9;
+  syn::legendre(lmax, theta, plm, dpdt, pdst, d2pdt2, dtpdst);
+  return 0;
+}
+
+int orc_taper(int lmax, double tapfrac, double* w) {
+  syn::taper(lmax, tapfrac, w);
+  return 0;
+}
+
+int orc_fft(int n, int is, double* x, double* y) {
+  std::vector<double> a(x, x + (1 << n)), b(y, y + (1 << n));
+  syn::fft(a, b, n, is);
+  std::memcpy(x, a.data(), sizeof(double) * a.size());
+  std::memcpy(y, b.data(), sizeof(double) * b.size());
+  return 0;
+}
+
+// sp: complex [nf2] (entries below nf1 ignored) -> ts [nsamp]; returns transformFrequencyTime's status
+int orc_freq_time(int nf1, int nf2, double df, double sigma, const double* sp, int nsamp, double dt, float* ts) {
+  return syn::freq_time(nf1, nf2, df, sigma, reinterpret_cast<const syn::cd*>(sp), nsamp, dt, ts);
+}
+
+// hdsum: [nrcv][nf][14] complex (interleaved)
+int orc_moment_sums(int nf1, int nf2, const int* nwn, long nkf, const float* gr, const float* gi, int nrcv,
+                    const double* theta, double tapfrac, double* hdsum, int nthreads) {
+  const int nf = nf2 - nf1 + 1;
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads > 0 ? nthreads : 1)
+  for (int r = 0; r < nrcv; ++r)
+    syn::moment_sums(nf1, nf2, nwn, nkf, gr, gi, theta[r], tapfrac,
+                     reinterpret_cast<syn::cd*>(hdsum) + (long)r * nf * 14);
+  return 0;
+}
+
+// Full synthetics of one source node: sums, displacement, time shift, velocity,
+// ZNE rotation (propdir given) and the frequency-to-time transform.
+// Restates the station loop of computeSyntheticSeismograms
+// (f90/computeSyntheticSeismograms.f90:311-388) with unit event and station
+// filters.  ts: [nrcv][3][nsamp]; zsp (optional): [nrcv][3][nf2] complex.
+int orc_synthetics(int nf1, int nf2, double df, double sigma, const int* nwn, long nkf, const float* gr,
+                   const float* gi, int nrcv, const double* theta, const double* phi, const double* propdir,
+                   const double* tred, const double* mt, int velocity, int nsamp, double dt, float* ts, double* zsp,
+                   int nthreads) {
+  const int nf = nf2 - nf1 + 1;
+  int bad = 0;
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads > 0 ? nthreads : 1) reduction(| : bad)
+  for (int r = 0; r < nrcv; ++r) {
+    std::vector<syn::cd> h((size_t)nf * 14), z((size_t)3 * nf2, 0.0);
+    syn::moment_sums(nf1, nf2, nwn, nkf, gr, gi, theta[r], 0.2, h.data());
+    const double tr = tred ? tred[r] : 0.0;
+    for (int jf = nf1; jf <= nf2; ++jf) {
+      syn::cd u[3];
+      syn::moment_displacement(phi[r], mt, &h[(size_t)(jf - nf1) * 14], u);
+      // time shift filter, f90/computeSyntheticSeismograms.f90:361-365
+      const double om = 2.0 * syn::kPi * (jf - 1) * df;
+      const syn::cd tsf = std::exp(syn::cd(0.0, om * tr)) * std::exp(sigma * tr);
+      for (int c = 0; c < 3; ++c) u[c] *= tsf;
+      if (velocity) {                                   // :376-378
+        const syn::cd zom(om, -sigma);
+        for (int c = 0; c < 3; ++c) u[c] = u[c] * syn::cd(0.0, 1.0) * zom;
+      }
+      if (propdir) {                                    // f90/axesRotation.f90:560-568
+        const double ca = std::cos(propdir[r]), sa = std::sin(propdir[r]);
+        const syn::cd uz = u[0], ue = sa * u[1] + ca * u[2], un = -ca * u[1] + sa * u[2];
+        u[0] = uz; u[1] = un; u[2] = ue;
+      }
+      for (int c = 0; c < 3; ++c) z[(size_t)c * nf2 + jf - 1] = u[c];
+    }
+    if (zsp) std::memcpy(zsp + (size_t)r * 3 * nf2 * 2, z.data(), sizeof(syn::cd) * 3 * nf2);
+    for (int c = 0; c < 3; ++c)
+      bad |= syn::freq_time(nf1, nf2, df, sigma, &z[(size_t)c * nf2], nsamp, dt, ts + ((size_t)r * 3 + c) * nsamp);
+  }
+  return bad;
+}
+
+}  // extern "C"

@@ -1,0 +1,119 @@
+"""GPU parity of the synthetics half (Legendre functions, degree sums on the
+matrix cores, seismograms) against the CPU oracle, through the C ABI."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "oracle"))
+import oracle  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def syn():
+    from gemini_unified_amd.synthetics import SyntheticsEngine
+    e = SyntheticsEngine(0)
+    yield e
+    e.close()
+
+
+def test_legendre_matches_oracle(syn):
+    thetas = np.array([0.0, 0.004, 0.3, 1.2, 2.5, 3.1])
+    L = 2500
+    g = syn.legendre(L, thetas)
+    print(f"legendre: {thetas.size} receivers, lmax {L}", flush=True)
+    for i, th in enumerate(thetas):
+        o = oracle.legendre(L, th)
+        for k in ("plm", "dpdt", "pdst", "d2pdt2", "dtpdst"):
+            for m in range(3):
+                ref = o[k][m]
+                scale = max(np.abs(ref).max(), 1e-300)
+                err = np.abs(g[k][i, m] - ref).max() / scale
+                assert err < 1e-10, (th, k, m, err)
+
+
+def _random_spectra(rng, nwn):
+    nkf = int(nwn.sum())
+    # spectra decaying with degree like evanescent FK spectra, FP32 as in the GFK file
+    l = np.concatenate([np.arange(n) for n in nwn])
+    env = np.exp(-l / 80.0) + 0.05
+    gr = (rng.normal(size=(10, nkf)) * env).astype(np.float32)
+    gi = (rng.normal(size=(10, nkf)) * env).astype(np.float32)
+    return gr, gi
+
+
+def test_moment_sums_match_oracle(syn):
+    rng = np.random.default_rng(11)
+    nf1, nf2 = 2, 9
+    nwn_all = np.zeros(nf2, np.int32)
+    nwn_all[nf1 - 1:] = np.linspace(150, 360, nf2 - nf1 + 1).astype(np.int32)
+    gr, gi = _random_spectra(rng, nwn_all[nf1 - 1:])
+    theta = np.sort(rng.uniform(0.05, 3.1, size=37))
+    print(f"moment sums: {theta.size} receivers, {nf2 - nf1 + 1} frequencies", flush=True)
+    g = syn.moment_sums(nf1, nf2, nwn_all, gr, gi, theta)                     # [14, R, nf]
+    o = oracle.moment_sums(nf1, nf2, nwn_all, gr, gi, theta).transpose(2, 0, 1)
+    scale = np.abs(o).max(axis=2, keepdims=True) + 1e-300
+    err = np.abs(g - o) / scale
+    print("max error relative to the largest sum of each (kint, receiver):", err.max(), flush=True)
+    assert err.max() < 1e-5
+
+
+def test_moment_sums_generating_function(syn):
+    """sum_l P_l t^l = (1 - 2 x t + t^2)^(-1/2) through the GPU contraction."""
+    L, t = 400, 0.6
+    nwn = np.array([L], np.int32)
+    gr = np.zeros((10, L), np.float32)
+    gi = np.zeros((10, L), np.float32)
+    l = np.arange(L)
+    gr[0] = t ** l * 4 * np.pi / (2 * l + 1)
+    theta = np.linspace(0.1, 3.0, 129)
+    h = syn.moment_sums(1, 1, nwn, gr, gi, theta)[0, :, 0]
+    ref = (1 - 2 * np.cos(theta) * t + t * t) ** -0.5
+    err = np.abs(h - ref) / ref
+    print("generating function max rel error", err.max(), flush=True)
+    assert err.max() < 1e-6
+
+
+def test_synthetics_match_oracle(syn):
+    rng = np.random.default_rng(12)
+    nf1, nf2, nsamp = 2, 64, 256
+    T = 256.0
+    df, dt = 1.0 / T, T / nsamp
+    sigma = 5 * df
+    nwn_all = np.zeros(nf2, np.int32)
+    nwn_all[nf1 - 1:] = np.linspace(40, 120, nf2 - nf1 + 1).astype(np.int32)
+    gr, gi = _random_spectra(rng, nwn_all[nf1 - 1:])
+    R = 6
+    theta = rng.uniform(0.1, 3.0, R)
+    phi = rng.uniform(0, 2 * np.pi, R)
+    propdir = rng.uniform(0, 2 * np.pi, R)
+    tred = rng.uniform(0, 20, R)
+    mt = rng.normal(size=6) * 1e18
+    print(f"synthetics: {R} receivers, nf2 {nf2}, nsamp {nsamp}", flush=True)
+    for vel, pd, tr in [(False, None, None), (True, propdir, tred)]:
+        ts, zsp = syn.synthetics(nf1, nf2, df, sigma, nwn_all, gr, gi, theta, phi, mt, nsamp, dt, propdir=pd,
+                                 tred=tr, velocity=vel, want_spectra=True)
+        ots, ozsp = oracle.synthetics(nf1, nf2, df, sigma, nwn_all, gr, gi, theta, phi, mt, nsamp, dt, propdir=pd,
+                                      tred=tr, velocity=vel)
+        zerr = np.abs(zsp - ozsp).max() / np.abs(ozsp).max()
+        ts64, ots64 = ts.astype(np.float64), ots.astype(np.float64)       # FP32 squares overflow at 1e18 N m
+        terr = np.linalg.norm(ts64 - ots64, axis=2) / np.linalg.norm(ots64, axis=2)
+        print(f"velocity={vel}: spectra max rel {zerr:.2e}, seismogram rel-L2 max {terr.max():.2e}", flush=True)
+        assert zerr < 1e-5
+        assert terr.max() < 1e-5
+
+
+def test_synthetics_errors(syn):
+    from gemini_unified_amd._lib import GeminiFKError
+    nwn = np.array([10, 10], np.int32)
+    g = np.zeros((5, 20), np.float32)
+    with pytest.raises(GeminiFKError) as e:
+        syn.synthetics(1, 2, 1 / 64, 0.0, nwn, g, g, [1.0], [0.0], np.ones(6), 64, 1.0)
+    assert e.value.code == 8
+    g = np.zeros((10, 20), np.float32)
+    with pytest.raises(GeminiFKError) as e:
+        syn.synthetics(1, 2, 1 / 64, 0.0, nwn, g, g, [1.0], [0.0], np.ones(6), 128, 1.0)
+    assert e.value.code == 9

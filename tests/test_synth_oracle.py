@@ -1,0 +1,162 @@
+"""Pins of the synthetics half of the CPU oracle (Legendre functions, degree
+sums, displacement combination, FFT and frequency-time transform) against
+closed forms, scipy/numpy and invariants.  No GPU needed."""
+import os
+import sys
+
+import numpy as np
+import pytest
+from scipy.special import lpmv
+
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "oracle"))
+import oracle  # noqa: E402
+
+
+@pytest.mark.parametrize("theta", [0.3, 1.1, 2.0, 2.9])
+def test_legendre_vs_scipy(theta):
+    """P_l^m without the Condon-Shortley phase: (-1)^m scipy.special.lpmv."""
+    L = 60
+    o = oracle.legendre(L, theta)
+    x = np.cos(theta)
+    for m in range(3):
+        ref = np.array([(-1) ** m * lpmv(m, l, x) if l >= m else 0.0 for l in range(L + 1)])
+        scale = np.maximum(np.abs(ref), 1e-300)
+        np.testing.assert_allclose(o["plm"][m], ref, rtol=1e-11, atol=1e-13 * np.abs(ref).max())
+
+
+def test_legendre_generating_function_high_degree():
+    """sum_l t^l P_l(x) = (1 - 2xt + t^2)^(-1/2) with lmax = 2500 (recurrence stability)."""
+    L, t = 2500, 0.99        # t^L ~ 1e-11: truncation below the tolerance
+    for theta in [0.01, 0.7, 1.5707, 3.1]:
+        x = np.cos(theta)
+        p0 = oracle.legendre(L, theta)["plm"][0]
+        s = np.sum(p0 * t ** np.arange(L + 1))
+        ref = (1 - 2 * x * t + t * t) ** -0.5
+        assert abs(s - ref) / abs(ref) < 1e-9, (theta, s, ref)
+
+
+def test_legendre_theta_derivatives_by_finite_differences():
+    L, th, h = 40, 0.9, 1e-5
+    o, op, om = oracle.legendre(L, th), oracle.legendre(L, th + h), oracle.legendre(L, th - h)
+    fd = (op["plm"] - om["plm"]) / (2 * h)
+    np.testing.assert_allclose(o["dpdt"], fd, rtol=1e-6, atol=1e-6 * np.abs(fd).max())
+    fd2 = (op["dpdt"] - om["dpdt"]) / (2 * h)
+    np.testing.assert_allclose(o["d2pdt2"], fd2, rtol=1e-5, atol=1e-5 * np.abs(fd2).max())
+    st = np.sin(th)
+    np.testing.assert_allclose(o["pdst"][1:], o["plm"][1:] / st, rtol=1e-14)
+    fds = (op["pdst"] - om["pdst"]) / (2 * h)
+    np.testing.assert_allclose(o["dtpdst"][1:], fds[1:], rtol=1e-5, atol=1e-5 * np.abs(fds).max())
+
+
+@pytest.mark.parametrize("pole", [0.0])
+def test_legendre_pole_limits(pole):
+    """The pole formulas equal the limit theta -> 0 of the regular branch.  (theta = pi in FP64 has
+    sin(theta) = 1.2e-16 > 0, so the reference's pole branch is reached at theta = 0 only.)"""
+    L = 30
+    at = oracle.legendre(L, pole)
+    eps = 1e-4 if pole == 0.0 else -1e-4
+    n1, n2 = oracle.legendre(L, pole + eps), oracle.legendre(L, pole + 2 * eps)
+    for key in ("dpdt", "pdst", "d2pdt2", "dtpdst"):
+        a, b = at[key], 2 * n1[key] - n2[key]      # linear extrapolation of the regular branch to the pole
+        scale = np.abs(b).max()
+        rows = [1] if key in ("pdst", "dtpdst") else [0, 1, 2]
+        for m in rows:
+            if key == "dtpdst" and m == 1:
+                continue        # d/dtheta (P_l1/sin) -> 0 at the pole, the regular branch is O(theta)
+            if key == "d2pdt2" and m == 0:
+                # legendreFunctions.f90:146 sets +l(l+1)/2 sign(ct)^l; the limit is -l(l+1)/2 sign(ct)^l.
+                # The oracle keeps the reference's value (only strain sums, kint >= 15, use it).
+                np.testing.assert_allclose(a[m], -b[m], rtol=1e-4, atol=1e-4 * scale)
+                continue
+            np.testing.assert_allclose(a[m], b[m], rtol=1e-4, atol=1e-4 * scale, err_msg=f"{key} m={m}")
+    np.testing.assert_allclose(at["dtpdst"][2], (2 * n1["dtpdst"] - n2["dtpdst"])[2], rtol=1e-4)
+
+
+def test_taper_closed_form():
+    lmax, tf = 1000, 0.2
+    w = oracle.taper(lmax, tf)
+    jt = int(round(lmax * (1 - tf)))
+    assert np.all(w[:jt] == 1.0)
+    assert abs(w[jt] - 1.0) < 1e-12            # cos^2(0) at the start of the taper
+    assert abs(w[lmax]) < 1e-12                # cos^2(pi/2) at lmax
+    assert np.all(np.diff(w[jt:]) <= 1e-15)    # monotone
+    assert abs(w[(jt + lmax) // 2] - 0.5) < 1e-2
+
+
+def test_fft_matches_numpy():
+    rng = np.random.default_rng(0)
+    n = 2 ** 10
+    x, y = rng.normal(size=n), rng.normal(size=n)
+    a, b = oracle.fft(x, y, +1)
+    ref = np.fft.ifft(x + 1j * y) * n
+    np.testing.assert_allclose(a + 1j * b, ref, rtol=0, atol=1e-10 * np.abs(ref).max())
+    a, b = oracle.fft(x, y, -1)
+    ref = np.fft.fft(x + 1j * y)
+    np.testing.assert_allclose(a + 1j * b, ref, rtol=0, atol=1e-10 * np.abs(ref).max())
+
+
+def test_freq_time_matches_irfft():
+    """ts = df e^{sigma t} N irfft(X) for the Hermitian spectrum X_j, j = nf1-1..nf2-1."""
+    rng = np.random.default_rng(1)
+    nf1, nf2, nsamp = 3, 100, 512
+    T = 256.0
+    df, dt = 1.0 / T, T / nsamp
+    sigma = 5 * df
+    sp = np.zeros(nf2, np.complex128)
+    sp[nf1 - 1:] = rng.normal(size=nf2 - nf1 + 1) + 1j * rng.normal(size=nf2 - nf1 + 1)
+    rc, ts = oracle.freq_time(nf1, nf2, df, sigma, sp, nsamp, dt)
+    assert rc == 0
+    X = np.zeros(nsamp // 2 + 1, np.complex128)
+    X[:nf2] = sp
+    ref = np.fft.irfft(X, nsamp) * nsamp * df * np.exp(sigma * dt * np.arange(nsamp))
+    np.testing.assert_allclose(ts, ref, rtol=0, atol=2e-6 * np.abs(ref).max())
+    rc, _ = oracle.freq_time(nf1, 300, df, sigma, np.zeros(300, complex), 512, dt)
+    assert rc != 0                                  # would need decimation: not handled
+
+
+def _gen_spectra(nf, L, t, kind):
+    """Spectra G_s(l) that turn a degree sum into a generating function: G = t^l 4 pi/(2l+1) (x l(l+1)/2)."""
+    l = np.arange(L)
+    g = t ** l * 4 * np.pi / (2 * l + 1)
+    if kind == "m1":
+        g = g * l * (l + 1) / 2.0
+    return g
+
+
+@pytest.mark.parametrize("theta", [0.4, 1.3, 2.6])
+def test_moment_sums_generating_functions(theta):
+    """sum_l P_l t^l, sum_l dP_l/dtheta t^l, sum_l P_l^1 t^l in closed form (taper region negligible)."""
+    L, t = 240, 0.5
+    nwn = np.array([L], np.int32)
+    nsp = 10
+    gr = np.zeros((nsp, L), np.float32)
+    gi = np.zeros((nsp, L), np.float32)
+    g0 = _gen_spectra(1, L, t, "m0")
+    g1 = _gen_spectra(1, L, t, "m1")
+    gr[0] = g0          # kint 1: P_l0 x U jump 1
+    gi[4] = g0          # kint 5: dP_l0/dtheta x V jump 1 (imaginary part)
+    gr[2] = g1          # kint 3: P_l1 2/(l(l+1)) x U jump 3
+    h = oracle.moment_sums(1, 1, nwn, gr, gi, [theta])[0, 0]
+    x, s = np.cos(theta), np.sin(theta)
+    q = 1 - 2 * x * t + t * t
+    # FP32 storage of the spectra limits the agreement to ~1e-7
+    assert abs(h[0] - q ** -0.5) < 2e-7 * q ** -0.5
+    assert abs(h[4] - 1j * (-t * s * q ** -1.5)) < 2e-7 * abs(t * s * q ** -1.5)
+    assert abs(h[2] - s * t * q ** -1.5) < 2e-7 * abs(s * t * q ** -1.5)
+    assert abs(h[1]) == 0.0 and abs(h[3]) == 0.0     # spectra 1 and 3 are zero
+
+
+def test_displacement_isotropic_source_is_azimuth_independent():
+    """An explosion (Mrr = Mtt = Mff) has no transverse motion and no azimuth dependence."""
+    rng = np.random.default_rng(2)
+    L = 50
+    nsp = 10
+    gr = rng.normal(size=(nsp, L)).astype(np.float32)
+    gi = rng.normal(size=(nsp, L)).astype(np.float32)
+    nwn = np.array([L], np.int32)
+    mt = np.array([1.0, 1.0, 1.0, 0.0, 0.0, 0.0])
+    _, zsp = oracle.synthetics(1, 1, 1.0 / 64, 0.0, nwn, gr, gi, [0.8] * 3, [0.1, 1.7, 4.0], mt, 4, 16.0)
+    z = zsp[:, :, 0]
+    assert np.all(np.abs(z[:, 2]) < 1e-12 * np.abs(z).max())
+    np.testing.assert_allclose(z[1], z[0], rtol=1e-13)
+    np.testing.assert_allclose(z[2], z[0], rtol=1e-13)
