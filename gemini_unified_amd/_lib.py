@@ -19,7 +19,7 @@ VP = C.c_void_p
 EXPORTS = ["gm_ctx_create", "gm_ctx_destroy", "gm_set_option", "gm_last_error", "gm_model_set",
            "gm_nodes_set", "gm_minors_batch", "gm_gfk_batch", "gm_last_timing", "gm_last_system_stats",
            "gm_mlp_nparams", "gm_mlp_forward", "gm_mlp_grad", "gm_mlp_train", "gm_set_proposer",
-           "gm_collect_steplog", "gm_legendre_batch", "gm_moment_sums", "gm_synthetics", "gm_last_synth_timing"]
+           "gm_collect_steplog", "gm_minors_batch_wn", "gm_gfk_batch_wn", "gm_legendre_batch", "gm_moment_sums", "gm_synthetics", "gm_last_synth_timing"]
 
 
 class GeminiFKError(RuntimeError):
@@ -53,8 +53,12 @@ def lib():
         L.gm_mlp_train.argtypes = [C.c_int, F, C.c_int, F, F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_float,
                                    C.c_uint, F]
         L.gm_set_proposer.argtypes = [VP, F, F, C.c_float, C.c_int, C.c_int]
-        L.gm_collect_steplog.argtypes = [VP, C.c_int, D, I, I, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
-                                         C.c_int, F, I, LL]
+        L.gm_collect_steplog.argtypes = [VP, C.c_int, D, I, I, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                         C.c_int, C.c_int, F, I, LL]
+        L.gm_minors_batch_wn.argtypes = [VP, C.c_int, D, I, I, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                         C.c_int, C.c_int, D, D, D, D, D, D, D, D, I, LL]
+        L.gm_gfk_batch_wn.argtypes = [VP, C.c_int, D, I, I, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                      C.c_int, C.c_int, D, F, F, I, LL]
         # array arguments of the synthetics entry points are void* so that device pointers (ints) pass too
         L.gm_legendre_batch.argtypes = [VP, C.c_int, D, C.c_int, D, D, D, D, D]
         L.gm_moment_sums.argtypes = [VP, C.c_int, C.c_int, I, C.c_int, C.c_longlong, VP, VP, C.c_int, D, C.c_int, VP]

@@ -204,8 +204,8 @@ struct Plan {
 }
 
 // common driver of gm_minors_batch / gm_gfk_batch
-static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const int* nl, double sigma, double eps,
-               int attmode, int jsl, int jsr, int jre, int istyp, bool full, int flags,
+static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const int* nl, double dwn, double sigma,
+               double eps, int attmode, int jsl, int jsr, int jre, int istyp, bool full, int flags,
                double* u_tor, double* d_tor, double* dettor, double* u_sph, double* d_sph, double* ulg, double* dlg,
                double* detsph, double* gf, float* gfre, float* gfim, int* status, long long* stats) {
   if (!c) return GM_ERR_ARG;
@@ -230,6 +230,7 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   P.omre.resize(nfreq);
   for (int f = 0; f < nfreq; ++f) {
     if (nl[f] < 0 || lmin[f] < 0) return gm_fail(c, GM_ERR_ARG, "negative degree range at frequency %d", f);
+    if (dwn > 0.0 && nl[f] > 0 && lmin[f] < 1) return gm_fail(c, GM_ERR_ARG, "wavenumber indices are 1-based (frequency %d)", f);
     if (!(freqs[f] > 0.0)) return gm_fail(c, GM_ERR_ARG, "frequency %d must be positive", f);
     P.omre[f] = 2.0 * (double)3.1415927410125732 * freqs[f];
     for (int i = 0; i < nl[f]; ++i) { P.q.push_back(f); P.l.push_back(lmin[f] + i); }
@@ -297,6 +298,7 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   GmBatch B;
   memset(&B, 0, sizeof B);
   B.n = (int)n; B.nnod = nnod; B.jsl = jsl; B.jsr = jsr; B.jre = jre; B.istyp = istyp;
+  B.dwn = dwn;
   B.omim = -sigma; B.eps = eps;
   B.q = (const int*)(base + o_q); B.l = (const int*)(base + o_l); B.omre = (const double*)(base + o_omre);
   B.nkp1 = m.nk + 1; B.tables = (const GmKnot*)(base + o_tables);
@@ -397,15 +399,34 @@ extern "C" int gm_minors_batch(gm_ctx* c, int nfreq, const double* freqs, const 
                                double eps, int attmode, int jsl, int jsr, int flags, double* u_tor, double* d_tor,
                                double* dettor, double* u_sph, double* d_sph, double* ulg, double* dlg,
                                double* detsph, int* status, long long* stats) {
-  return run(c, nfreq, freqs, lmin, nl, sigma, eps, attmode, jsl, jsr, jsr, 1, false, flags, u_tor, d_tor, dettor,
+  return run(c, nfreq, freqs, lmin, nl, 0.0, sigma, eps, attmode, jsl, jsr, jsr, 1, false, flags, u_tor, d_tor, dettor,
              u_sph, d_sph, ulg, dlg, detsph, nullptr, nullptr, nullptr, status, stats);
+}
+
+extern "C" int gm_minors_batch_wn(gm_ctx* c, int nfreq, const double* freqs, const int* iwnmin, const int* nwn,
+                                  double dwn, double sigma, double eps, int attmode, int jsl, int jsr, int flags,
+                                  double* u_tor, double* d_tor, double* dettor, double* u_sph, double* d_sph,
+                                  double* ulg, double* dlg, double* detsph, int* status, long long* stats) {
+  if (!c) return GM_ERR_ARG;
+  if (!(dwn > 0.0)) return gm_fail(c, GM_ERR_ARG, "dwn must be positive");
+  return run(c, nfreq, freqs, iwnmin, nwn, dwn, sigma, eps, attmode, jsl, jsr, jsr, 1, false, flags, u_tor, d_tor,
+             dettor, u_sph, d_sph, ulg, dlg, detsph, nullptr, nullptr, nullptr, status, stats);
 }
 
 extern "C" int gm_gfk_batch(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const int* nl, double sigma,
                             double eps, int attmode, int jre, int istyp, int flags, double* gf, float* gfre,
                             float* gfim, int* status, long long* stats) {
-  return run(c, nfreq, freqs, lmin, nl, sigma, eps, attmode, jre, jre, jre, istyp, true, flags, nullptr, nullptr,
+  return run(c, nfreq, freqs, lmin, nl, 0.0, sigma, eps, attmode, jre, jre, jre, istyp, true, flags, nullptr, nullptr,
              nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, gf, gfre, gfim, status, stats);
+}
+
+extern "C" int gm_gfk_batch_wn(gm_ctx* c, int nfreq, const double* freqs, const int* iwnmin, const int* nwn, double dwn,
+                               double sigma, double eps, int attmode, int jre, int istyp, int flags, double* gf,
+                               float* gfre, float* gfim, int* status, long long* stats) {
+  if (!c) return GM_ERR_ARG;
+  if (!(dwn > 0.0)) return gm_fail(c, GM_ERR_ARG, "dwn must be positive");
+  return run(c, nfreq, freqs, iwnmin, nwn, dwn, sigma, eps, attmode, jre, jre, jre, istyp, true, flags, nullptr,
+             nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, gf, gfre, gfim, status, stats);
 }
 
 extern "C" int gm_last_system_stats(gm_ctx* c, int sys, long long* stats) {
@@ -437,13 +458,13 @@ extern "C" int gm_set_proposer(gm_ctx* c, const float* params, const float* norm
 }
 
 extern "C" int gm_collect_steplog(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const int* nl,
-                                  double sigma, double eps, int attmode, int jsl, int jsr, int maxrec, float* records,
-                                  int* nrec, long long* stats) {
-  if (!c || maxrec <= 0 || !records) return GM_ERR_ARG;
+                                  double dwn, double sigma, double eps, int attmode, int jsl, int jsr, int maxrec,
+                                  float* records, int* nrec, long long* stats) {
+  if (!c || maxrec <= 0 || !records || dwn < 0.0) return GM_ERR_ARG;
   c->log_max = maxrec;
   c->log_host = records;
   c->log_count = 0;
-  int rc = run(c, nfreq, freqs, lmin, nl, sigma, eps, attmode, jsl, jsr, jsr, 1, false, 0, nullptr, nullptr, nullptr,
+  int rc = run(c, nfreq, freqs, lmin, nl, dwn, sigma, eps, attmode, jsl, jsr, jsr, 1, false, 0, nullptr, nullptr, nullptr,
                nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, stats);
   c->log_max = 0;
   c->log_host = nullptr;

@@ -54,7 +54,8 @@ struct GmBatch {
   int nnod, jsl, jsr, jre, istyp;
   double omim, eps;
   const int* __restrict__ q;       // frequency slot per instance
-  const int* __restrict__ l;       // harmonic degree per instance
+  const int* __restrict__ l;       // harmonic degree (dwn == 0) or 1-based wavenumber index (dwn > 0)
+  double dwn;                      // wavenumber spacing of the regional mode, 0 for harmonic degrees
   const double* __restrict__ omre; // per frequency slot
   int nkp1;                        // table stride (nk + 1)
   const GmKnot* __restrict__ tables;

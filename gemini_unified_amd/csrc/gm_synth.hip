@@ -51,6 +51,9 @@ constexpr int kMaxFFT = 8192;  // longest time series (ns/2 complex FP64 = 64 KB
 __constant__ int c_row[kSums] = {0, 0, 1, 2, 3, 3, 4, 5, 6, 7, 4, 5, 6, 7};
 __constant__ int c_spec[kSums] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 8, 9, 6, 7};
 __constant__ double c_fac[kSums] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 1, -1, 1, -2};
+// first degree each spectrum enters a sum with (the sums' loop bounds); the V
+// spectra at l = 0 hold 0/0 from the reciprocity factor 1/dll1 and are never read
+__constant__ int c_lmin[10] = {0, 0, 1, 2, 1, 1, 1, 2, 1, 2};
 
 __device__ __forceinline__ void split3(double v, __bf16& a, __bf16& b, __bf16& c) {
   a = (__bf16)(float)v;
@@ -175,7 +178,7 @@ __global__ void k_gprep(int nsp, int nf, int Np, int Lp, const int* __restrict__
   const int l = (int)(idx % Lp);
   const int f = n >> 1;
   double v = 0.0;
-  if (f < nf && l < nwn[f]) {
+  if (f < nf && l < nwn[f] && l >= c_lmin[s]) {
     const long long k = (long long)s * nkf + koff[f] + l;
     const double gv = (n & 1) ? (double)gi[k] : (double)gr[k];
     const int lmax = nwn[f] - 1;

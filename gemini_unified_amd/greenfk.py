@@ -96,25 +96,37 @@ class GreenFKEngine:
         check(self.h, _lib.lib().gm_last_system_stats(self.h, int(sys), st.ctypes.data_as(LL)))
         return st
 
-    def run_minors_resident(self, freqs, lmin, nl, sigma, eps, jsl, jsr, attmode="ATTENUATION_AND_DISPERSION"):
+    def run_minors_resident(self, freqs, lmin, nl, sigma, eps, jsl, jsr, attmode="ATTENUATION_AND_DISPERSION",
+                            dwn=None):
         """Minors of all (f, l) pairs left resident in the engine's HBM buffers (no copy-out)."""
         freqs, lmin, nl, nk = self._plan(freqs, lmin, nl)
         st = np.zeros(10, np.int64)
         am = ATTMODES[attmode] if isinstance(attmode, str) else int(attmode)
-        rc = _lib.lib().gm_minors_batch(self.h, freqs.size, freqs.ctypes.data_as(D), lmin.ctypes.data_as(I),
-                                        nl.ctypes.data_as(I), float(sigma), float(eps), am, int(jsl), int(jsr), 0,
-                                        *([None] * 8), None, st.ctypes.data_as(LL))
+        if dwn:
+            rc = _lib.lib().gm_minors_batch_wn(self.h, freqs.size, freqs.ctypes.data_as(D), lmin.ctypes.data_as(I),
+                                               nl.ctypes.data_as(I), float(dwn), float(sigma), float(eps), am,
+                                               int(jsl), int(jsr), 0, *([None] * 8), None, st.ctypes.data_as(LL))
+        else:
+            rc = _lib.lib().gm_minors_batch(self.h, freqs.size, freqs.ctypes.data_as(D), lmin.ctypes.data_as(I),
+                                            nl.ctypes.data_as(I), float(sigma), float(eps), am, int(jsl), int(jsr), 0,
+                                            *([None] * 8), None, st.ctypes.data_as(LL))
         check(self.h, rc)
         return st
 
-    def run_gfk_resident(self, freqs, lmin, nl, sigma, eps, jre, istyp=1, attmode="ATTENUATION_AND_DISPERSION"):
+    def run_gfk_resident(self, freqs, lmin, nl, sigma, eps, jre, istyp=1, attmode="ATTENUATION_AND_DISPERSION",
+                         dwn=None):
         """Green-FK columns of all (f, l) pairs left resident in HBM (no copy-out)."""
         freqs, lmin, nl, nk = self._plan(freqs, lmin, nl)
         st = np.zeros(10, np.int64)
         am = ATTMODES[attmode] if isinstance(attmode, str) else int(attmode)
-        rc = _lib.lib().gm_gfk_batch(self.h, freqs.size, freqs.ctypes.data_as(D), lmin.ctypes.data_as(I),
-                                     nl.ctypes.data_as(I), float(sigma), float(eps), am, int(jre), int(istyp), 0,
-                                     None, None, None, None, st.ctypes.data_as(LL))
+        if dwn:
+            rc = _lib.lib().gm_gfk_batch_wn(self.h, freqs.size, freqs.ctypes.data_as(D), lmin.ctypes.data_as(I),
+                                            nl.ctypes.data_as(I), float(dwn), float(sigma), float(eps), am, int(jre),
+                                            int(istyp), 0, None, None, None, None, st.ctypes.data_as(LL))
+        else:
+            rc = _lib.lib().gm_gfk_batch(self.h, freqs.size, freqs.ctypes.data_as(D), lmin.ctypes.data_as(I),
+                                         nl.ctypes.data_as(I), float(sigma), float(eps), am, int(jre), int(istyp), 0,
+                                         None, None, None, None, st.ctypes.data_as(LL))
         check(self.h, rc)
         return st
 
@@ -129,7 +141,7 @@ class GreenFKEngine:
                                                  int(bool(kcheck)), 1))
 
     def collect_steplog(self, freqs, lmin, nl, sigma, eps, jsl, jsr, maxrec=2_000_000,
-                        attmode="ATTENUATION_AND_DISPERSION"):
+                        attmode="ATTENUATION_AND_DISPERSION", dwn=None):
         """Step records of the plain spheroidal-minors integration, float32 [n, 20] (layout in include/gemini_fk.h)."""
         freqs, lmin, nl, nk = self._plan(freqs, lmin, nl)
         rec = np.zeros((maxrec, 20), np.float32)
@@ -137,7 +149,8 @@ class GreenFKEngine:
         st = np.zeros(10, np.int64)
         am = ATTMODES[attmode] if isinstance(attmode, str) else int(attmode)
         check(self.h, _lib.lib().gm_collect_steplog(self.h, freqs.size, freqs.ctypes.data_as(D),
-                                                    lmin.ctypes.data_as(I), nl.ctypes.data_as(I), float(sigma),
+                                                    lmin.ctypes.data_as(I), nl.ctypes.data_as(I),
+                                                    float(dwn or 0.0), float(sigma),
                                                     float(eps), am, int(jsl), int(jsr), int(maxrec),
                                                     rec.ctypes.data_as(F), C.byref(nrec), st.ctypes.data_as(LL)))
         return rec[:min(nrec.value, maxrec)], nrec.value, st
@@ -150,8 +163,10 @@ class GreenFKEngine:
         nl = np.ascontiguousarray(np.broadcast_to(nl, (nf,)), dtype=np.int32)
         return freqs, lmin, nl, int(nl.sum())
 
-    def minors(self, freqs, lmin, nl, sigma, eps, jsl, jsr, attmode="ATTENUATION_AND_DISPERSION", raise_on_error=True):
-        """Toroidal + spheroidal minors of all (f, l) pairs; arrays indexed [node, comp, k]."""
+    def minors(self, freqs, lmin, nl, sigma, eps, jsl, jsr, attmode="ATTENUATION_AND_DISPERSION", raise_on_error=True,
+               dwn=None):
+        """Toroidal + spheroidal minors of all (f, l) pairs; arrays indexed [node, comp, k].
+        With dwn, lmin/nl are 1-based wavenumber indices (regional mode, k = (iwn-1) dwn)."""
         freqs, lmin, nl, nk = self._plan(freqs, lmin, nl)
         nn = self.nodes.nnod
         out = dict(u_tor=np.zeros((nn, 2, nk), np.complex128), d_tor=np.zeros((nn, 2, nk), np.complex128),
@@ -160,11 +175,17 @@ class GreenFKEngine:
                    det_sph=np.zeros((nn, nk), np.complex128), status=np.zeros(nk, np.int32))
         st = np.zeros(10, np.int64)
         am = ATTMODES[attmode] if isinstance(attmode, str) else int(attmode)
-        rc = _lib.lib().gm_minors_batch(self.h, freqs.size, freqs.ctypes.data_as(D), lmin.ctypes.data_as(I),
-                                        nl.ctypes.data_as(I), float(sigma), float(eps), am, int(jsl), int(jsr), 0,
-                                        *[out[k].ctypes.data_as(D) for k in
-                                          ("u_tor", "d_tor", "det_tor", "u_sph", "d_sph", "ulg", "dlg", "det_sph")],
-                                        out["status"].ctypes.data_as(I), st.ctypes.data_as(LL))
+        outs = [out[k].ctypes.data_as(D) for k in ("u_tor", "d_tor", "det_tor", "u_sph", "d_sph", "ulg", "dlg",
+                                                    "det_sph")]
+        if dwn:
+            rc = _lib.lib().gm_minors_batch_wn(self.h, freqs.size, freqs.ctypes.data_as(D), lmin.ctypes.data_as(I),
+                                               nl.ctypes.data_as(I), float(dwn), float(sigma), float(eps), am,
+                                               int(jsl), int(jsr), 0, *outs, out["status"].ctypes.data_as(I),
+                                               st.ctypes.data_as(LL))
+        else:
+            rc = _lib.lib().gm_minors_batch(self.h, freqs.size, freqs.ctypes.data_as(D), lmin.ctypes.data_as(I),
+                                            nl.ctypes.data_as(I), float(sigma), float(eps), am, int(jsl), int(jsr), 0,
+                                            *outs, out["status"].ctypes.data_as(I), st.ctypes.data_as(LL))
         self.stats += st
         out["stats"] = st
         if raise_on_error:
@@ -172,7 +193,7 @@ class GreenFKEngine:
         return out
 
     def gfk(self, freqs, lmin, nl, sigma, eps, jre, istyp=1, attmode="ATTENUATION_AND_DISPERSION",
-            want_fp32=True, raise_on_error=True):
+            want_fp32=True, raise_on_error=True, dwn=None):
         """Green-FK columns; gf[node, isp, k] complex128 (and FP32 re/im as written to the file)."""
         freqs, lmin, nl, nk = self._plan(freqs, lmin, nl)
         nn = self.nodes.nnod
@@ -183,11 +204,18 @@ class GreenFKEngine:
         status = np.zeros(nk, np.int32)
         st = np.zeros(10, np.int64)
         am = ATTMODES[attmode] if isinstance(attmode, str) else int(attmode)
-        rc = _lib.lib().gm_gfk_batch(self.h, freqs.size, freqs.ctypes.data_as(D), lmin.ctypes.data_as(I),
-                                     nl.ctypes.data_as(I), float(sigma), float(eps), am, int(jre), int(istyp), 0,
-                                     gf.ctypes.data_as(D), None if gfre is None else gfre.ctypes.data_as(F),
-                                     None if gfim is None else gfim.ctypes.data_as(F), status.ctypes.data_as(I),
-                                     st.ctypes.data_as(LL))
+        fre = None if gfre is None else gfre.ctypes.data_as(F)
+        fim = None if gfim is None else gfim.ctypes.data_as(F)
+        if dwn:
+            rc = _lib.lib().gm_gfk_batch_wn(self.h, freqs.size, freqs.ctypes.data_as(D), lmin.ctypes.data_as(I),
+                                            nl.ctypes.data_as(I), float(dwn), float(sigma), float(eps), am, int(jre),
+                                            int(istyp), 0, gf.ctypes.data_as(D), fre, fim, status.ctypes.data_as(I),
+                                            st.ctypes.data_as(LL))
+        else:
+            rc = _lib.lib().gm_gfk_batch(self.h, freqs.size, freqs.ctypes.data_as(D), lmin.ctypes.data_as(I),
+                                         nl.ctypes.data_as(I), float(sigma), float(eps), am, int(jre), int(istyp), 0,
+                                         gf.ctypes.data_as(D), fre, fim, status.ctypes.data_as(I),
+                                         st.ctypes.data_as(LL))
         self.stats += st
         if raise_on_error:
             check(self.h, rc)

@@ -113,6 +113,20 @@ int gm_gfk_batch(gm_ctx* ctx, int nfreq, const double* freqs, const int* lmin, c
                  double eps, int attmode, int jre, int istyp, int flags, double* gf, float* gfre, float* gfim,
                  int* status, long long* stats);
 
+/* Regional (wavenumber) mode of gm_minors_batch / gm_gfk_batch: the instances
+ * of frequency q are the 1-based wavenumber indices iwn = iwnmin[q] ..
+ * iwnmin[q]+nwn[q]-1 with k = (iwn-1) dwn and dll1 = (k R)^2
+ * (computeGreenFKSpectraForSynthetics.f90:410-413 with GLOBAL = 0; the
+ * reference starts at iwn = 2 and writes zeros for k = 0).  Everything else is
+ * as for the degree-based calls. */
+int gm_minors_batch_wn(gm_ctx* ctx, int nfreq, const double* freqs, const int* iwnmin, const int* nwn, double dwn,
+                       double sigma, double eps, int attmode, int jsl, int jsr, int flags, double* u_tor,
+                       double* d_tor, double* dettor, double* u_sph, double* d_sph, double* ulg, double* dlg,
+                       double* detsph, int* status, long long* stats);
+int gm_gfk_batch_wn(gm_ctx* ctx, int nfreq, const double* freqs, const int* iwnmin, const int* nwn, double dwn,
+                    double sigma, double eps, int attmode, int jre, int istyp, int flags, double* gf, float* gfre,
+                    float* gfim, int* status, long long* stats);
+
 /* Kernel time of the last batch call in milliseconds (HIP events on the
  * context stream), per stage: 0 tables, 1 setup, 2 toroidal, 3 spheroidal,
  * 4 adjoint+assembly, 5 total. */
@@ -140,14 +154,15 @@ int gm_mlp_train(int device, float* params, int n, const float* X, const float* 
 /* install (enable = 1) or remove (enable = 0) the proposer of a context;
  * norm = mu[9], sd[9] of the features and mu[2], sd[2] of the outputs */
 int gm_set_proposer(gm_ctx* ctx, const float* params, const float* norm, float safety, int kcheck, int enable);
-/* run the plain minors integration over the given (f, l) and record every
+/* run the plain minors integration over the given (f, l) (or, with dwn > 0,
+ * the wavenumber indices of gm_minors_batch_wn) and record every
  * spheroidal-minors step: 9 features, log10(h/R), log10(err_k/eps) for the
  * error tests after 3..7 sequences (all 7 sequences are computed), accepted
  * try (0 = rejected), instance index, running step number of the instance,
  * log10(distance to the end of the sub-interval / R), 1 if the step was cut to
  * end the sub-interval: 20 floats per record */
-int gm_collect_steplog(gm_ctx* ctx, int nfreq, const double* freqs, const int* lmin, const int* nl, double sigma,
-                       double eps, int attmode, int jsl, int jsr, int maxrec, float* records, int* nrec,
+int gm_collect_steplog(gm_ctx* ctx, int nfreq, const double* freqs, const int* lmin, const int* nl, double dwn,
+                       double sigma, double eps, int attmode, int jsl, int jsr, int maxrec, float* records, int* nrec,
                        long long* stats);
 
 /* ---- synthetics half: Legendre functions, degree sums, frequency-time ---- */

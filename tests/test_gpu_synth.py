@@ -51,11 +51,15 @@ def test_moment_sums_match_oracle(syn):
     nwn_all = np.zeros(nf2, np.int32)
     nwn_all[nf1 - 1:] = np.linspace(150, 360, nf2 - nf1 + 1).astype(np.int32)
     gr, gi = _random_spectra(rng, nwn_all[nf1 - 1:])
+    # l = 0 of the V spectra is 0/0 in real FK spectra; no sum reads it
+    off = np.concatenate([[0], np.cumsum(nwn_all[nf1 - 1:])[:-1]])
+    gr[4:8, off] = np.nan
     theta = np.sort(rng.uniform(0.05, 3.1, size=37))
     print(f"moment sums: {theta.size} receivers, {nf2 - nf1 + 1} frequencies", flush=True)
     g = syn.moment_sums(nf1, nf2, nwn_all, gr, gi, theta)                     # [14, R, nf]
     o = oracle.moment_sums(nf1, nf2, nwn_all, gr, gi, theta).transpose(2, 0, 1)
     scale = np.abs(o).max(axis=2, keepdims=True) + 1e-300
+    assert np.all(np.isfinite(g)) and np.all(np.isfinite(o))
     err = np.abs(g - o) / scale
     print("max error relative to the largest sum of each (kint, receiver):", err.max(), flush=True)
     assert err.max() < 1e-5
