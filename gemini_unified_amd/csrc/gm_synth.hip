@@ -3,7 +3,7 @@
 // displacement combination, and the frequency-to-time transform.
 //
 // Pipeline of gm_synthetics (one source node, R receivers, nf frequencies):
-//   k_twiddle   e^{+2 pi i k / ns}, k < ns/2, FP64
+//   k_twiddle   e^{+2 pi i k / ns}, k < ns/2, FP64 (pi as in the reference)
 //   k_legendre  per receiver: P_l^m, dP/dtheta, P/sin (m = 0..2) by the FP64
 //               upward recurrence, times (2l+1)/(4 pi) and 2/(l(l+1)), as
 //               8 basis rows; each value is split into three bf16 pieces
@@ -38,7 +38,10 @@ namespace gm_syn {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-constexpr double kPi = 3.14159265358979323846;
+// pi of the reference's synthetics code (mc_pid, mathConstants.f90:34): its
+// literal is single precision, so the value is FP32 pi; used for 4 pi, the
+// tapers, 2 pi f and the transform's twiddle angles as the reference does
+constexpr double kPi = 3.1415927410125732;
 constexpr int kRows = 8;       // basis rows
 constexpr int kSums = 14;      // moment-tensor degree sums
 constexpr int kPieces = 3;     // bf16 pieces per value
@@ -51,6 +54,10 @@ constexpr int kMaxFFT = 8192;  // longest time series (ns/2 complex FP64 = 64 KB
 __constant__ int c_row[kSums] = {0, 0, 1, 2, 3, 3, 4, 5, 6, 7, 4, 5, 6, 7};
 __constant__ int c_spec[kSums] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 8, 9, 6, 7};
 __constant__ double c_fac[kSums] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 1, -1, 1, -2};
+// regional mode: the 14 wavenumber-integral kernels map onto 8 distinct rows
+// with the same row and spectrum tables and these signs
+// (momentWavenumberIntegrals, wavenumberIntegrals.f90:353-373)
+__constant__ double c_fac_wn[kSums] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, -1, 1, -1};
 // first degree each spectrum enters a sum with (the sums' loop bounds); the V
 // spectra at l = 0 hold 0/0 from the reciprocity factor 1/dll1 and are never read
 __constant__ int c_lmin[10] = {0, 0, 1, 2, 1, 1, 1, 2, 1, 2};
@@ -67,7 +74,7 @@ __global__ void k_twiddle(int ns, double* __restrict__ tw) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= ns / 2) return;
   double s, c;
-  sincospi(2.0 * k / ns, &s, &c);
+  sincos((2.0 * kPi / ns) * k, &s, &c);
   tw[2 * k] = c;
   tw[2 * k + 1] = s;
 }
@@ -167,9 +174,22 @@ __global__ void __launch_bounds__(64) k_legendre(int R, int Rp, int lmax, int Lp
 
 // Tapered spectra split into bf16 pieces: Gs[piece][s][Np][Lp].
 // taperHarmonicDegreeSums (harmonicDegreeSums.f90:356-369) per frequency.
+// Regional mode (dwn > 0): taperWavenumberIntegrals (wavenumberIntegrals.f90:
+// 499-512) per frequency; the k = 0 column (index 0) is never summed.
+__device__ __forceinline__ double taperWn(int l, int n, double dwn, double tapfrac) {
+  long jt = lround(n * (1.0 - tapfrac));
+  if (jt > n) jt = n;
+  if (jt < 1) jt = 1;
+  if (l + 1 < jt) return 1.0;
+  const double c1 = (n - 1) * dwn * tapfrac * 2.0 / kPi;
+  const double c2 = (1.0 - tapfrac) / tapfrac * 0.5 * kPi;
+  const double c = cos(l * dwn / c1 - c2);
+  return c * c;
+}
+
 __global__ void k_gprep(int nsp, int nf, int Np, int Lp, const int* __restrict__ nwn, const long long* __restrict__ koff,
                         long long nkf, const float* __restrict__ gr, const float* __restrict__ gi, double tapfrac,
-                        __bf16* __restrict__ Gs) {
+                        double dwn, __bf16* __restrict__ Gs) {
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t per_s = (size_t)Np * Lp;
   if (idx >= per_s * nsp) return;
@@ -178,7 +198,13 @@ __global__ void k_gprep(int nsp, int nf, int Np, int Lp, const int* __restrict__
   const int l = (int)(idx % Lp);
   const int f = n >> 1;
   double v = 0.0;
-  if (f < nf && l < nwn[f] && l >= c_lmin[s]) {
+  if (dwn > 0.0) {
+    if (f < nf && l < nwn[f] && l >= 1) {
+      const long long k = (long long)s * nkf + koff[f] + l;
+      const double gv = (n & 1) ? (double)gi[k] : (double)gr[k];
+      v = gv * taperWn(l, nwn[f], dwn, tapfrac);
+    }
+  } else if (f < nf && l < nwn[f] && l >= c_lmin[s]) {
     const long long k = (long long)s * nkf + koff[f] + l;
     const double gv = (n & 1) ? (double)gi[k] : (double)gr[k];
     const int lmax = nwn[f] - 1;
@@ -205,12 +231,12 @@ __global__ void k_gprep(int nsp, int nf, int Np, int Lp, const int* __restrict__
 // times spectrum c_spec[j] (the j-th moment-tensor sum); grid (Np/TN, Rp/TM,
 // 14), 256 threads: 4 waves of 64 x 32.
 __global__ void __launch_bounds__(256) k_contract(int Rp, int Np, int Lp, int nsp, const __bf16* __restrict__ Bs,
-                                                  const __bf16* __restrict__ Gs, double* __restrict__ H) {
+                                                  const __bf16* __restrict__ Gs, double* __restrict__ H, int regional) {
   __shared__ __attribute__((aligned(16))) __bf16 sA[kPieces][TM][LDSK];
   __shared__ __attribute__((aligned(16))) __bf16 sB[kPieces][TN][LDSK];
   const int j = blockIdx.z;
   const int rowb = c_row[j], spec = c_spec[j];
-  const double fac = c_fac[j];
+  const double fac = regional ? c_fac_wn[j] : c_fac[j];
   const int r0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wr = (w >> 1) * 64, wc = (w & 1) * 32;
@@ -307,6 +333,135 @@ __device__ __forceinline__ cd cmul(cd a, cd b) { return {a.re * b.re - a.im * b.
 __device__ __forceinline__ cd cadd(cd a, cd b) { return {a.re + b.re, a.im + b.im}; }
 __device__ __forceinline__ cd csub(cd a, cd b) { return {a.re - b.re, a.im - b.im}; }
 __device__ __forceinline__ cd crs(cd a, double s) { return {a.re * s, a.im * s}; }
+
+// ---- regional branch: wavenumber integrals --------------------------------
+
+// the 8 distinct kernels of the 14 moment wavenumber integrals at k for
+// Bessel (or Neumann) values b0..b2 of k x (wavenumberIntegrals.f90:353-373)
+__device__ __forceinline__ void wnRows(double k, double x, double re, double b0, double b1, double b2, double* w) {
+  w[0] = k * b0;                                          // kint 1, 2
+  w[1] = 2.0 / re * b1;                                   // 3
+  w[2] = k * 2.0 * b2;                                    // 4
+  w[3] = k * (-k * re) * b1;                              // 5, 6
+  w[4] = k * 2.0 * (b0 - b1 / (k * x));                   // 7, 11
+  w[5] = k * 2.0 * k * re * (b1 - 2.0 * b2 / (k * x));    // 8, -12
+  w[6] = k * 2.0 * b1 / (k * x);                          // 9, 13
+  w[7] = k * 4.0 * k * re * b2 / (k * x);                 // 10, -14
+}
+
+// quadrature geometry of one receiver (momentWavenumberIntegrals, :333-346, 427)
+struct WnGeom { double x, xdk, cF, sF; int j3; };
+__device__ __forceinline__ WnGeom wnGeom(double xx, double dwn) {
+  WnGeom g;
+  g.x = xx < 3.0 / (dwn * 1.0e9) ? 3.0 / (dwn * 1.0e9) : xx;
+  g.j3 = (int)((3.0 / g.x) / dwn) + 2;
+  g.xdk = g.x * dwn;
+  g.cF = (1.0 - cos(g.xdk)) / g.xdk / g.x;               // Filon weight of the end points (interior: 2 cF)
+  g.sF = (1.0 - sin(g.xdk) / g.xdk) / g.x;               // weight of the Neumann end terms
+  return g;
+}
+
+// Basis rows of the regional contraction: B_row[r][j] = quadrature weight(j) x
+// kernel_row(k_j), j = 1-based wavenumber index, k_j = (j-1) dwn.  Weights:
+// 2 <= j < j3: dwn (trapezoid); j3: dwn/2 + cF (trapezoid end, Filon start);
+// j > j3: 2 cF (Filon interior).  They are those of a frequency whose last
+// wavenumber lies beyond j3; k_wnfix corrects each frequency's last point.
+// One thread per (receiver, wavenumber); layout [piece][row][Rp][Lp].
+__global__ void k_bessel_basis(int R, int Rp, int Lp, const double* __restrict__ xr, double dwn, double re,
+                               __bf16* __restrict__ Bs) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)Rp * Lp) return;
+  const int r = (int)(idx / Lp), l = (int)(idx % Lp);
+  const size_t plane = (size_t)Rp * Lp;
+  double w[kRows] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (r < R && l >= 1) {
+    const double xx = xr[r];
+    const WnGeom g = wnGeom(xx, dwn);
+    const int j = l + 1;
+    double q;
+    if (j < g.j3) q = dwn;
+    else if (j == g.j3) q = 0.5 * dwn + g.cF;
+    else q = 2.0 * g.cF;
+    const double k = l * dwn;
+    const double b0 = j0(k * xx), b1 = j1(k * xx);
+    const double b2 = -b0 + 2.0 / (k * xx) * b1;
+    wnRows(k, g.x, re, b0, b1, b2, w);
+#pragma unroll
+    for (int i = 0; i < kRows; ++i) w[i] *= q;
+  }
+#pragma unroll
+  for (int i = 0; i < kRows; ++i) {
+    __bf16 a, b, c;
+    split3(w[i], a, b, c);
+    Bs[((size_t)0 * kRows + i) * plane + idx] = a;
+    Bs[((size_t)1 * kRows + i) * plane + idx] = b;
+    Bs[((size_t)2 * kRows + i) * plane + idx] = c;
+  }
+}
+
+// End corrections and the R^2/(2 pi) factor of the regional sums, in place on
+// H[j][Rp][Np], one thread per (receiver, frequency), n = the frequency's last
+// wavenumber index:
+//  - n > j3: the last point carries cF instead of 2 cF, plus the Neumann end
+//    terms (G(n) Y-kernel(k_n) - G(j3) Y-kernel(k_j3)) sF;
+//  - n == j3: trapezoid only, the point carries dwn/2 instead of dwn/2 + cF;
+//  - n < j3: trapezoid only, the last point carries dwn/2 instead of dwn.
+__global__ void k_wnfix(int R, int nf, int Rp, int Np, const int* __restrict__ nwn, const long long* __restrict__ koff,
+                        long long nkf, const float* __restrict__ gr, const float* __restrict__ gi,
+                        const double* __restrict__ xr, double dwn, double re, double tapfrac, double* __restrict__ H) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= R * nf) return;
+  const int r = id / nf, f = id % nf;
+  const double xx = xr[r];
+  const WnGeom g = wnGeom(xx, dwn);
+  const int n = nwn[f];
+  auto rowsAt = [&](int j, double* w) {      // Bessel kernels at wavenumber index j
+    const double k = (j - 1) * dwn;
+    const double b0 = j0(k * xx), b1 = j1(k * xx);
+    wnRows(k, g.x, re, b0, b1, -b0 + 2.0 / (k * xx) * b1, w);
+  };
+  auto spec = [&](int s, int j) -> cd {      // tapered spectrum s at wavenumber index j
+    const long long k = (long long)s * nkf + koff[f] + (j - 1);
+    const double t = taperWn(j - 1, n, dwn, tapfrac);
+    return cd{(double)gr[k] * t, (double)gi[k] * t};
+  };
+  double wn_[kRows], w3[kRows], yn[kRows], y3[kRows];
+  const bool filon = g.j3 < n;
+  rowsAt(n, wn_);
+  if (filon) {
+    const double wn3 = (g.j3 - 1) * dwn, wne = (n - 1) * dwn;
+    double n0 = y0(wn3 * g.x), n1 = y1(wn3 * g.x);
+    wnRows(wn3, g.x, re, n0, n1, -n0 + 2.0 / (wn3 * g.x) * n1, y3);
+    n0 = y0(wne * g.x); n1 = y1(wne * g.x);
+    wnRows(wne, g.x, re, n0, n1, -n0 + 2.0 / (wne * g.x) * n1, yn);
+  } else if (n == g.j3) {
+    rowsAt(g.j3, w3);
+  }
+  const double scale = re * re / (2.0 * kPi);
+#pragma unroll 1
+  for (int j = 0; j < kSums; ++j) {
+    const int row = c_row[j], s = c_spec[j];
+    const double fac = c_fac_wn[j];
+    const size_t o = ((size_t)j * Rp + r) * Np + 2 * f;
+    cd z{H[o], H[o + 1]};
+    const cd gn = spec(s, n);
+    double corr;
+    if (filon) corr = -g.cF;                       // 2 cF/2 -> cF/2 at the last point
+    else if (n == g.j3) corr = -g.cF;              // (dwn/2 + cF/2 x 2) -> dwn/2
+    else corr = -0.5 * dwn;                        // dwn -> dwn/2
+    const double kern = (n == g.j3 && !filon) ? w3[row] : wn_[row];
+    z.re += fac * corr * kern * gn.re;
+    z.im += fac * corr * kern * gn.im;
+    if (filon) {
+      const cd g3 = spec(s, g.j3);
+      z.re += fac * (gn.re * yn[row] - g3.re * y3[row]) * g.sF;
+      z.im += fac * (gn.im * yn[row] - g3.im * y3[row]) * g.sF;
+    }
+    H[o] = z.re * scale;
+    H[o + 1] = z.im * scale;
+  }
+}
+
 
 // Displacement spectra Z[r][3][nf2] (zero below nf1).
 // momentSingleFrequencyDisplacement (singleFrequencyDisplacement.f90:108-125),
@@ -478,8 +633,10 @@ int ensure_sbuf(gm_ctx* c, size_t bytes) {
 // basis, spectra and the contraction into H; offsets of the carved buffer in `o`
 struct SumBufs { size_t theta, Bs, Gs, H, nwn, koff, gr, gi; };
 
+// dwn == 0: harmonic degree sums at epicentral distances theta (radians);
+// dwn > 0: wavenumber integrals at epicentral distances theta (metres), rearth = re
 int run_sums(gm_ctx* c, const SynPlan& P, const double* theta, const float* gr, const float* gi, bool dev_in,
-             char* base, const SumBufs& o, hipStream_t s) {
+             char* base, const SumBufs& o, hipStream_t s, double dwn = 0.0, double re = 0.0) {
   HIPCHK(c, hipMemcpyAsync(base + o.theta, theta, sizeof(double) * P.R, hipMemcpyHostToDevice, s));
   HIPCHK(c, hipMemcpyAsync(base + o.nwn, P.nwnf.data(), sizeof(int) * P.nf, hipMemcpyHostToDevice, s));
   HIPCHK(c, hipMemcpyAsync(base + o.koff, P.koff.data(), sizeof(long long) * P.nf, hipMemcpyHostToDevice, s));
@@ -490,18 +647,30 @@ int run_sums(gm_ctx* c, const SynPlan& P, const double* theta, const float* gr, 
     dgr = (const float*)(base + o.gr);
     dgi = (const float*)(base + o.gi);
   }
+  const bool regional = dwn > 0.0;
+  const double* dth = (const double*)(base + o.theta);
   HIPCHK(c, hipEventRecord(c->sev[0], s));
-  hipLaunchKernelGGL(k_legendre, dim3((P.Rp + 63) / 64), dim3(64), 0, s, P.R, P.Rp, P.L1 - 1, P.Lp,
-                     (const double*)(base + o.theta), (__bf16*)(base + o.Bs), nullptr, nullptr, nullptr, nullptr,
-                     nullptr, 0);
+  if (regional) {
+    const size_t nb = (size_t)P.Rp * P.Lp;
+    hipLaunchKernelGGL(k_bessel_basis, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, P.R, P.Rp, P.Lp, dth, dwn,
+                       re, (__bf16*)(base + o.Bs));
+  } else {
+    hipLaunchKernelGGL(k_legendre, dim3((P.Rp + 63) / 64), dim3(64), 0, s, P.R, P.Rp, P.L1 - 1, P.Lp, dth,
+                       (__bf16*)(base + o.Bs), nullptr, nullptr, nullptr, nullptr, nullptr, 0);
+  }
   HIPCHK(c, hipEventRecord(c->sev[1], s));
   const size_t ng = (size_t)P.nsp * P.Np * P.Lp;
   hipLaunchKernelGGL(k_gprep, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, s, P.nsp, P.nf, P.Np, P.Lp,
-                     (const int*)(base + o.nwn), (const long long*)(base + o.koff), P.nkf, dgr, dgi, 0.2,
+                     (const int*)(base + o.nwn), (const long long*)(base + o.koff), P.nkf, dgr, dgi, 0.2, dwn,
                      (__bf16*)(base + o.Gs));
   HIPCHK(c, hipEventRecord(c->sev[2], s));
   hipLaunchKernelGGL(k_contract, dim3(P.Np / TN, P.Rp / TM, kSums), dim3(256), 0, s, P.Rp, P.Np, P.Lp, P.nsp,
-                     (const __bf16*)(base + o.Bs), (const __bf16*)(base + o.Gs), (double*)(base + o.H));
+                     (const __bf16*)(base + o.Bs), (const __bf16*)(base + o.Gs), (double*)(base + o.H),
+                     regional ? 1 : 0);
+  if (regional)
+    hipLaunchKernelGGL(k_wnfix, dim3((P.R * P.nf + 255) / 256), dim3(256), 0, s, P.R, P.nf, P.Rp, P.Np,
+                       (const int*)(base + o.nwn), (const long long*)(base + o.koff), P.nkf, dgr, dgi, dth, dwn, re,
+                       0.2, (double*)(base + o.H));
   HIPCHK(c, hipEventRecord(c->sev[3], s));
   HIPCHK(c, hipGetLastError());
   return GM_OK;
@@ -549,8 +718,9 @@ extern "C" int gm_legendre_batch(gm_ctx* c, int nrcv, const double* theta, int l
   return GM_OK;
 }
 
-extern "C" int gm_moment_sums(gm_ctx* c, int nf1, int nf2, const int* nwn, int nsp, long long nkf, const float* gfkspr,
-                              const float* gfkspi, int nrcv, const double* theta, int flags, double* hdsum) {
+static int sums_impl(gm_ctx* c, int nf1, int nf2, const int* nwn, int nsp, long long nkf, const float* gfkspr,
+                     const float* gfkspi, int nrcv, const double* theta, int flags, double* hdsum, double dwn,
+                     double re) {
   if (!c) return GM_ERR_ARG;
   SynPlan P;
   int rc = plan_sums(c, nf1, nf2, nwn, nsp, nkf, nrcv, P);
@@ -563,7 +733,7 @@ extern "C" int gm_moment_sums(gm_ctx* c, int nf1, int nf2, const int* nwn, int n
   if ((rc = ensure_sbuf(c, cv.off))) return rc;
   char* base = (char*)c->sbuf;
   hipStream_t s = c->stream;
-  if ((rc = run_sums(c, P, theta, gfkspr, gfkspi, dev, base, o, s))) return rc;
+  if ((rc = run_sums(c, P, theta, gfkspr, gfkspi, dev, base, o, s, dwn, re))) return rc;
   // H[j][Rp][Np] -> hdsum[j][R][nf] complex
   for (int j = 0; j < kSums; ++j)
     HIPCHK(c, hipMemcpy2DAsync(hdsum + (size_t)j * P.R * P.nf * 2, sizeof(double) * 2 * P.nf,
@@ -578,10 +748,10 @@ extern "C" int gm_moment_sums(gm_ctx* c, int nf1, int nf2, const int* nwn, int n
   return GM_OK;
 }
 
-extern "C" int gm_synthetics(gm_ctx* c, int nf1, int nf2, double df, double sigma, const int* nwn, int nsp,
-                             long long nkf, const float* gfkspr, const float* gfkspi, int nrcv, const double* theta,
-                             const double* phi, const double* propdir, const double* tred, const double* mt,
-                             int velocity, int nsamp, double dt, int flags, float* ts, double* zsp) {
+static int synth_impl(gm_ctx* c, int nf1, int nf2, double df, double sigma, const int* nwn, int nsp, long long nkf,
+                      const float* gfkspr, const float* gfkspi, int nrcv, const double* theta, const double* phi,
+                      const double* propdir, const double* tred, const double* mt, int velocity, int nsamp,
+                      double dt, int flags, float* ts, double* zsp, double dwn, double re) {
   if (!c) return GM_ERR_ARG;
   SynPlan P;
   int rc = plan_sums(c, nf1, nf2, nwn, nsp, nkf, nrcv, P);
@@ -617,7 +787,7 @@ extern "C" int gm_synthetics(gm_ctx* c, int nf1, int nf2, double df, double sigm
   if (propdir) HIPCHK(c, hipMemcpyAsync(base + o_pd, propdir, sizeof(double) * P.R, hipMemcpyHostToDevice, s));
   if (tred) HIPCHK(c, hipMemcpyAsync(base + o_tr, tred, sizeof(double) * P.R, hipMemcpyHostToDevice, s));
   HIPCHK(c, hipMemcpyAsync(base + o_mt, mt, sizeof(double) * 6, hipMemcpyHostToDevice, s));
-  if ((rc = run_sums(c, P, theta, gfkspr, gfkspi, dev, base, o, s))) return rc;
+  if ((rc = run_sums(c, P, theta, gfkspr, gfkspi, dev, base, o, s, dwn, re))) return rc;
   double* dZ = (double*)(base + o_z);
   hipLaunchKernelGGL(k_combine, dim3((unsigned)(((size_t)P.R * nf2 + 255) / 256)), dim3(256), 0, s, P.R, nf1, nf2,
                      P.Rp, P.Np, (const double*)(base + o.H), (const double*)(base + o_phi),
@@ -641,6 +811,42 @@ extern "C" int gm_synthetics(gm_ctx* c, int nf1, int nf2, double df, double sigm
   (void)hipEventElapsedTime(&ms, c->sev[0], c->sev[5]);
   c->stiming[5] = ms;
   return GM_OK;
+}
+
+extern "C" int gm_moment_sums(gm_ctx* c, int nf1, int nf2, const int* nwn, int nsp, long long nkf, const float* gfkspr,
+                              const float* gfkspi, int nrcv, const double* theta, int flags, double* hdsum) {
+  return sums_impl(c, nf1, nf2, nwn, nsp, nkf, gfkspr, gfkspi, nrcv, theta, flags, hdsum, 0.0, 0.0);
+}
+
+extern "C" int gm_bessel_sums(gm_ctx* c, int nf1, int nf2, const int* nwn, int nsp, long long nkf, const float* gfkspr,
+                              const float* gfkspi, double dwn, double rearth, int nrcv, const double* x, int flags,
+                              double* wnint) {
+  if (!c) return GM_ERR_ARG;
+  if (!(dwn > 0.0) || !(rearth > 0.0)) return gm_fail(c, GM_ERR_ARG, "dwn and rearth must be positive");
+  for (int r = 0; r < nrcv && x; ++r)
+    if (!(x[r] > 0.0)) return gm_fail(c, GM_ERR_ARG, "epicentral distance %d must be positive", r);
+  return sums_impl(c, nf1, nf2, nwn, nsp, nkf, gfkspr, gfkspi, nrcv, x, flags, wnint, dwn, rearth);
+}
+
+extern "C" int gm_synthetics(gm_ctx* c, int nf1, int nf2, double df, double sigma, const int* nwn, int nsp,
+                             long long nkf, const float* gfkspr, const float* gfkspi, int nrcv, const double* theta,
+                             const double* phi, const double* propdir, const double* tred, const double* mt,
+                             int velocity, int nsamp, double dt, int flags, float* ts, double* zsp) {
+  return synth_impl(c, nf1, nf2, df, sigma, nwn, nsp, nkf, gfkspr, gfkspi, nrcv, theta, phi, propdir, tred, mt,
+                    velocity, nsamp, dt, flags, ts, zsp, 0.0, 0.0);
+}
+
+extern "C" int gm_synthetics_regional(gm_ctx* c, int nf1, int nf2, double df, double sigma, const int* nwn, int nsp,
+                                      long long nkf, const float* gfkspr, const float* gfkspi, double dwn,
+                                      double rearth, int nrcv, const double* x, const double* phi,
+                                      const double* propdir, const double* tred, const double* mt, int velocity,
+                                      int nsamp, double dt, int flags, float* ts, double* zsp) {
+  if (!c) return GM_ERR_ARG;
+  if (!(dwn > 0.0) || !(rearth > 0.0)) return gm_fail(c, GM_ERR_ARG, "dwn and rearth must be positive");
+  for (int r = 0; r < nrcv && x; ++r)
+    if (!(x[r] > 0.0)) return gm_fail(c, GM_ERR_ARG, "epicentral distance %d must be positive", r);
+  return synth_impl(c, nf1, nf2, df, sigma, nwn, nsp, nkf, gfkspr, gfkspi, nrcv, x, phi, propdir, tred, mt, velocity,
+                    nsamp, dt, flags, ts, zsp, dwn, rearth);
 }
 
 extern "C" int gm_last_synth_timing(gm_ctx* c, double* ms6) {

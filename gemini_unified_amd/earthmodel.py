@@ -213,6 +213,42 @@ def global_model_json(seed: int = 1, nk: int = 200) -> dict:
                          nic=2, noc=3, ric=ric, roc=roc)
 
 
+def regional_model_json(seed: int = 6) -> dict:
+    """Seeded regional model: 6 crust/upper-mantle layers over 0-200 km depth on a homogeneous sphere.
+
+    Layer thicknesses of the upper five layers are uniform in 2-40 km, the sixth reaches down to
+    200 km.  vp 4.5 -> 8.2 km/s, vs 2.6 -> 4.6 km/s, rho 2.4 -> 3.4 g/cc increase with depth, linear
+    within each layer (2 knots), with seeded +-2 % knot perturbations; Qmu 100 -> 300, Qkappa 57823.
+    Below 200 km the model is the homogeneous sphere (layer 1) with the values at 200 km.
+    """
+    rng = np.random.default_rng(seed)
+    R, zmax = 6371000.0, 200000.0
+    th = list(rng.uniform(2000.0, 40000.0, size=5))
+    tops = np.concatenate([[0.0], np.cumsum(th)])                   # depths of the layer tops, surface first
+    bots = np.concatenate([tops[1:], [zmax]])
+
+    def prop(z, a, b):
+        return a + (b - a) * (np.asarray(z) / zmax)
+
+    def pert(v):
+        return np.asarray(v) * (1.0 + 0.02 * rng.uniform(-1.0, 1.0, size=np.shape(v)))
+
+    layers = []
+    for zt, zb in reversed(list(zip(tops, bots))):                  # bottom-up
+        z = np.array([zb, zt])
+        layers.append(dict(r=R - z, fluid=False, vp=pert(prop(z, 4.5e3, 8.2e3)), vs=pert(prop(z, 2.6e3, 4.6e3)),
+                           density=pert(prop(z, 2.4e3, 3.4e3)), qk=np.full(2, 57823.0),
+                           qm=prop(z, 100.0, 300.0)))
+    base = layers[0]
+    sph = dict(r=np.array([R - zmax]), fluid=False, vp=base["vp"][:1].copy(), vs=base["vs"][:1].copy(),
+               density=base["density"][:1].copy(), qk=np.array([57823.0]), qm=base["qm"][:1].copy())
+    return _layered_json(f"seeded regional model, seed {seed}", R, [sph] + layers, nic=0, noc=0)
+
+
+def regional_model(seed: int = 6) -> NodeEarthmodel:
+    return NodeEarthmodel.from_json(regional_model_json(seed))
+
+
 def global_model(seed: int = 1, nk: int = 200) -> NodeEarthmodel:
     return NodeEarthmodel.from_json(global_model_json(seed, nk))
 
@@ -331,6 +367,17 @@ class FrequencyGrid:
         nf1 = max(int(np.rint(fmin / df)), 2)
         nwn = np.full(nf2 - nf1 + 1, lmax + 1, dtype=np.int64)
         return cls(df=df, nf1=nf1, nf2=nf2, sigma=5.0 * df, nwn=nwn)
+
+    @classmethod
+    def regional_fixed(cls, tlen: float, fmax: float, nwn: int, xlen: float, fmin: float = 0.0):
+        """Regional grid: nwn wavenumbers k = (iwn-1) dwn, dwn = 2 pi / xlen, at every frequency
+        (computeGreenFKSpectraForSynthetics.f90:195, 231-232)."""
+        df = 1.0 / tlen
+        nf2 = int(np.rint(fmax / df)) + 1
+        nf1 = max(int(np.rint(fmin / df)), 2)
+        g = cls(df=df, nf1=nf1, nf2=nf2, sigma=5.0 * df, nwn=np.full(nf2 - nf1 + 1, nwn, dtype=np.int64))
+        g.dwn = MC_TWO_PID / xlen
+        return g
 
     @classmethod
     def global_from_slowness(cls, tlen, fmax, p1lim, p2lim, frac, rearth, fmin=0.0):

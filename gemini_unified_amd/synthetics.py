@@ -1,7 +1,7 @@
-"""Synthetics half of the Green-FK path on the GPU: Legendre functions, moment
-tensor harmonic degree sums, displacement spectra and the frequency-to-time
-transform (the global, spherical-harmonics branch of
-f90/computeSyntheticSeismograms.f90:311-388).
+"""Synthetics half of the Green-FK path on the GPU: Legendre functions and
+harmonic degree sums (global branch) or Bessel wavenumber integrals (regional
+branch), displacement spectra and the frequency-to-time transform
+(f90/computeSyntheticSeismograms.f90:311-388).
 
 The Green FK spectra of one source node come in the layout the reference's
 synthetics stage reads from the GFK file (readDataNodeGreenFKSpectra,
@@ -75,10 +75,24 @@ class SyntheticsEngine:
                                                 theta.ctypes.data_as(D), 0, out.ctypes.data))
         return out
 
+    def bessel_sums(self, nf1, nf2, nwn, gr, gi, dwn, rearth, x):
+        """14 moment wavenumber integrals, complex [14, nrcv, nf2-nf1+1] (momentWavenumberIntegrals);
+        x = epicentral distances in metres, wavenumbers k = (j-1) dwn."""
+        nwn = np.ascontiguousarray(nwn, dtype=np.int32)
+        gr, gi = self._spectra(gr, gi)
+        x = np.ascontiguousarray(np.atleast_1d(x), dtype=np.float64)
+        out = np.zeros((MOMENT_SUMS, x.size, nf2 - nf1 + 1), np.complex128)
+        check(self.h, _lib.lib().gm_bessel_sums(self.h, int(nf1), int(nf2), nwn.ctypes.data_as(I), gr.shape[0],
+                                                gr.shape[1], gr.ctypes.data, gi.ctypes.data, float(dwn),
+                                                float(rearth), x.size, x.ctypes.data_as(D), 0, out.ctypes.data))
+        return out
+
     def synthetics(self, nf1, nf2, df, sigma, nwn, gr, gi, theta, phi, mt, nsamp, dt, propdir=None, tred=None,
-                   velocity=False, want_spectra=False, device_ptrs=None):
+                   velocity=False, want_spectra=False, device_ptrs=None, dwn=None, rearth=None):
         """Seismograms [nrcv, 3, nsamp] FP32 (and displacement spectra [nrcv, 3, nf2] if asked).
 
+        Global branch: theta = epicentral distances in radians.  Regional branch (dwn given):
+        theta = epicentral distances in metres, wavenumber integrals with k = (j-1) dwn.
         device_ptrs = (gr_ptr, gi_ptr, nsp, nkf, ts_ptr): spectra and output already in HBM
         (integers, e.g. torch tensors' data_ptr()); gr, gi are then ignored and ts is written in place.
         """
@@ -101,9 +115,15 @@ class SyntheticsEngine:
                 raise ValueError("want_spectra needs host outputs")
             pgr, pgi, nsp, nkf, pts = device_ptrs
             ts, flags = None, 1
-        check(self.h, _lib.lib().gm_synthetics(
-            self.h, int(nf1), int(nf2), float(df), float(sigma), nwn.ctypes.data_as(I), int(nsp), int(nkf), pgr, pgi,
-            theta.size, theta.ctypes.data_as(D), phi.ctypes.data_as(D), None if pd is None else pd.ctypes.data_as(D),
-            None if tr is None else tr.ctypes.data_as(D), mt.ctypes.data_as(D), int(bool(velocity)), int(nsamp),
-            float(dt), flags, pts, None if zsp is None else zsp.ctypes.data))
+        tail = (theta.size, theta.ctypes.data_as(D), phi.ctypes.data_as(D), None if pd is None else pd.ctypes.data_as(D),
+                None if tr is None else tr.ctypes.data_as(D), mt.ctypes.data_as(D), int(bool(velocity)), int(nsamp),
+                float(dt), flags, pts, None if zsp is None else zsp.ctypes.data)
+        L = _lib.lib()
+        if dwn:
+            rc = L.gm_synthetics_regional(self.h, int(nf1), int(nf2), float(df), float(sigma), nwn.ctypes.data_as(I),
+                                          int(nsp), int(nkf), pgr, pgi, float(dwn), float(rearth), *tail)
+        else:
+            rc = L.gm_synthetics(self.h, int(nf1), int(nf2), float(df), float(sigma), nwn.ctypes.data_as(I), int(nsp),
+                                 int(nkf), pgr, pgi, *tail)
+        check(self.h, rc)
         return (ts, zsp) if want_spectra else ts

@@ -204,6 +204,27 @@ int gm_synthetics(gm_ctx* ctx, int nf1, int nf2, double df, double sigma, const 
                   const double* propdir, const double* tred, const double* mt, int velocity, int nsamp, double dt,
                   int flags, float* ts, double* zsp);
 
+/* Regional (Bessel) branch: the 14 moment wavenumber integrals (kint 1..14) of
+ * one source node at epicentral distances x[r] in metres, for spectra sampled
+ * at k = (j-1) dwn, j = 1..nwn[jf-1] (j = 1 is k = 0 and is not summed).
+ * Trapezoid rule below k x = 3, Filon's rule with Neumann end terms above,
+ * cos^2 taper over the last 20 %, times rearth^2 / (2 pi).  Replaces
+ * momentWavenumberIntegrals with taperWavenumberIntegrals
+ * (f90/wavenumberIntegrals.f90:321-512) and computeBesselFunctions
+ * (f90/besselFunctions.f90:33-51) in besselDisplacementSpectra
+ * (f90/displacementSpectra.f90:119-184).  The sum over wavenumbers runs on
+ * the matrix cores as for gm_moment_sums.  wnint: complex [14][nrcv][nf2-nf1+1]. */
+int gm_bessel_sums(gm_ctx* ctx, int nf1, int nf2, const int* nwn, int nsp, long long nkf, const float* gfkspr,
+                   const float* gfkspi, double dwn, double rearth, int nrcv, const double* x, int flags,
+                   double* wnint);
+
+/* gm_synthetics for the regional branch: receivers at epicentral distances
+ * x[r] (metres) and azimuths phi[r]; otherwise as gm_synthetics. */
+int gm_synthetics_regional(gm_ctx* ctx, int nf1, int nf2, double df, double sigma, const int* nwn, int nsp,
+                           long long nkf, const float* gfkspr, const float* gfkspi, double dwn, double rearth,
+                           int nrcv, const double* x, const double* phi, const double* propdir, const double* tred,
+                           const double* mt, int velocity, int nsamp, double dt, int flags, float* ts, double* zsp);
+
 /* Kernel time of the last gm_moment_sums / gm_synthetics call in milliseconds:
  * 0 Legendre basis, 1 spectra preparation, 2 contraction, 3 combination,
  * 4 frequency-time transform, 5 total. */

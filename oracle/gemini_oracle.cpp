@@ -1640,8 +1640,15 @@ int orc_gfk_column(void* h, double omre, double omim, double dll1, int jre, int 
 // computeGreenFKSpectraForSynthetics.f90:410-494).  gf layout: complex
 // [n][isp][il] (Fortran gf(il, isp, n)).  status[il] = error code per degree.
 // The spline tables must have been built for this frequency.
-int orc_gfk_degrees(void* h, double omre, double omim, double rearth, int lmin, int nl, int jre, int istyp,
-                    double eps, int nthreads, double* gf, int* status, int64_t* stats) {
+// dll1 = (k R)^2 of degree l (dwn = 0) or of the 1-based wavenumber index l (dwn > 0):
+// computeGreenFKSpectraForSynthetics.f90:411-412 with geminiIntegrationEnvironment.f90:71
+static double instanceDll1(int l, double dwn, double rearth) {
+  double wn = dwn > 0.0 ? (double)(l - 1) * dwn : std::sqrt((double)(l * (l + 1))) / rearth;
+  return (wn * rearth) * (wn * rearth);
+}
+
+static int gfkInstances(void* h, double omre, double omim, double rearth, int lmin, int nl, double dwn, int jre,
+                        int istyp, double eps, int nthreads, double* gf, int* status, int64_t* stats) {
   Ctx* c = (Ctx*)h;
   int nnod = c->nod.nnod;
   int nsp = istyp == 1 ? 10 : 5;
@@ -1657,8 +1664,7 @@ int orc_gfk_degrees(void* h, double omre, double omim, double rearth, int lmin, 
 #endif
     for (int il = 0; il < nl; ++il) {
       int l = lmin + il;
-      double wn = std::sqrt((double)(l * (l + 1))) / rearth;
-      double dll1 = (wn * rearth) * (wn * rearth);
+      double dll1 = instanceDll1(l, dwn, rearth);
       Column col;
       int rc = gfkColumn(*c, omre, omim, dll1, jre, istyp, eps, col, st);
       status[il] = rc;
@@ -1678,6 +1684,17 @@ int orc_gfk_degrees(void* h, double omre, double omim, double rearth, int lmin, 
   int bad = 0;
   for (int il = 0; il < nl; ++il) if (status[il] != OK) bad = status[il];
   return bad;
+}
+
+int orc_gfk_degrees(void* h, double omre, double omim, double rearth, int lmin, int nl, int jre, int istyp,
+                    double eps, int nthreads, double* gf, int* status, int64_t* stats) {
+  return gfkInstances(h, omre, omim, rearth, lmin, nl, 0.0, jre, istyp, eps, nthreads, gf, status, stats);
+}
+
+// regional mode: 1-based wavenumber indices iwn = iwnmin .. iwnmin+nwn-1, k = (iwn-1) dwn
+int orc_gfk_wavenumbers(void* h, double omre, double omim, double rearth, int iwnmin, int nwn, double dwn, int jre,
+                        int istyp, double eps, int nthreads, double* gf, int* status, int64_t* stats) {
+  return gfkInstances(h, omre, omim, rearth, iwnmin, nwn, dwn, jre, istyp, eps, nthreads, gf, status, stats);
 }
 
 // Engine test: integrate y' = A y (A constant, nvar <= 5, row-major complex) from x1 to x2.
@@ -1734,8 +1751,8 @@ int orc_vti(double fratio, double ro, double alfv, double alfh, double betv, dou
 extern "C" {
 // minors only (toroidal + spheroidal) for l = lmin .. lmin+nl-1 at the current
 // frequency: the unit of the solves/s metric.  Results are discarded.
-int orc_minors_degrees(void* h, double omre, double omim, double rearth, int lmin, int nl, int jsl, int jsr,
-                       double eps, int nthreads, int* status, int64_t* stats) {
+static int minorsInstances(void* h, double omre, double omim, double rearth, int lmin, int nl, double dwn, int jsl,
+                           int jsr, double eps, int nthreads, int* status, int64_t* stats) {
   orc::Ctx* c = (orc::Ctx*)h;
   orc::Stats total;
 #ifdef _OPENMP
@@ -1749,8 +1766,7 @@ int orc_minors_degrees(void* h, double omre, double omim, double rearth, int lmi
 #endif
     for (int il = 0; il < nl; ++il) {
       int l = lmin + il;
-      double wn = std::sqrt((double)(l * (l + 1))) / rearth;
-      double dll1 = (wn * rearth) * (wn * rearth);
+      double dll1 = instanceDll1(l, dwn, rearth);
       orc::Env env; env.m = &c->m; env.nod = &c->nod; env.s = &c->s; env.omre = omre; env.omim = omim; env.dll1 = dll1; env.nl = 1;
       orc::ToroidalMinors T; orc::SpheroidalMinors S;
       int rc = orc::toroidalMinors(env, jsl, jsr, eps, T, st, nullptr);
@@ -1764,5 +1780,15 @@ int orc_minors_degrees(void* h, double omre, double omim, double rearth, int lmi
   }
   if (stats) for (int i = 0; i < 10; ++i) stats[i] += total.v[i];
   return 0;
+}
+
+int orc_minors_degrees(void* h, double omre, double omim, double rearth, int lmin, int nl, int jsl, int jsr,
+                       double eps, int nthreads, int* status, int64_t* stats) {
+  return minorsInstances(h, omre, omim, rearth, lmin, nl, 0.0, jsl, jsr, eps, nthreads, status, stats);
+}
+
+int orc_minors_wavenumbers(void* h, double omre, double omim, double rearth, int iwnmin, int nwn, double dwn, int jsl,
+                           int jsr, double eps, int nthreads, int* status, int64_t* stats) {
+  return minorsInstances(h, omre, omim, rearth, iwnmin, nwn, dwn, jsl, jsr, eps, nthreads, status, stats);
 }
 }

@@ -62,6 +62,10 @@ def lib():
         _lib.orc_vti.argtypes = [C.c_double] * 9 + [D]
         _lib.orc_minors_degrees.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                                             C.c_int, C.c_int, C.c_double, C.c_int, I, L]
+        _lib.orc_gfk_wavenumbers.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                             C.c_double, C.c_int, C.c_int, C.c_double, C.c_int, D, I, L]
+        _lib.orc_minors_wavenumbers.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                                C.c_double, C.c_int, C.c_int, C.c_double, C.c_int, I, L]
     return _lib
 
 
@@ -172,6 +176,25 @@ class Oracle:
                                  status.ctypes.data_as(I), self.stats.ctypes.data_as(L))
         return status
 
+    def gfk_wavenumbers(self, omre, omim, iwnmin, nwn, dwn, jre, istyp, eps, nthreads=1):
+        """Regional-mode columns for wavenumber indices iwnmin..iwnmin+nwn-1 (k = (iwn-1) dwn); gf[n, isp, i]."""
+        n = self.nodes.nnod
+        nsp = 10 if istyp == 1 else 5
+        gf = np.zeros((n, nsp, nwn), np.complex128)
+        status = np.zeros(nwn, np.int32)
+        rc = lib().orc_gfk_wavenumbers(self.h, C.c_double(omre), C.c_double(omim), C.c_double(self.nem.rearth),
+                                       int(iwnmin), int(nwn), C.c_double(dwn), int(jre), int(istyp), C.c_double(eps),
+                                       int(nthreads), _c(gf), status.ctypes.data_as(I),
+                                       self.stats.ctypes.data_as(L))
+        return rc, gf, status
+
+    def minors_wavenumbers(self, omre, omim, iwnmin, nwn, dwn, jsl, jsr, eps, nthreads=1):
+        status = np.zeros(nwn, np.int32)
+        lib().orc_minors_wavenumbers(self.h, C.c_double(omre), C.c_double(omim), C.c_double(self.nem.rearth),
+                                     int(iwnmin), int(nwn), C.c_double(dwn), int(jsl), int(jsr), C.c_double(eps),
+                                     int(nthreads), status.ctypes.data_as(I), self.stats.ctypes.data_as(L))
+        return status
+
     def gfk_degrees(self, omre, omim, lmin, nl, jre, istyp, eps, nthreads=1):
         """Columns for l = lmin..lmin+nl-1 at the current frequency; gf[n, isp, il]."""
         n = self.nodes.nnod
@@ -264,9 +287,34 @@ def moment_sums(nf1, nf2, nwn, gr, gi, theta, tapfrac=0.2, nthreads=1):
     return h
 
 
+def besselj(order, x):
+    lib().orc_besselj.restype = C.c_double
+    return lib().orc_besselj(int(order), C.c_double(x))
+
+
+def bessely(order, x):
+    lib().orc_bessely.restype = C.c_double
+    return lib().orc_bessely(int(order), C.c_double(x))
+
+
+def bessel_sums(nf1, nf2, nwn, gr, gi, dwn, rearth, x, tapfrac=0.2, nthreads=1):
+    """14 moment wavenumber integrals: complex [nrcv, nf, 14]; x = epicentral distances in m."""
+    nwn = np.ascontiguousarray(nwn, dtype=np.int32)
+    gr = np.ascontiguousarray(gr, dtype=np.float32)
+    gi = np.ascontiguousarray(gi, dtype=np.float32)
+    x = np.ascontiguousarray(np.atleast_1d(x), dtype=np.float64)
+    h = np.zeros((x.size, nf2 - nf1 + 1, 14), np.complex128)
+    F = C.POINTER(C.c_float)
+    lib().orc_bessel_sums(int(nf1), int(nf2), nwn.ctypes.data_as(I), C.c_long(gr.shape[1]), gr.ctypes.data_as(F),
+                          gi.ctypes.data_as(F), C.c_double(dwn), C.c_double(rearth), x.size, _d(x),
+                          C.c_double(tapfrac), h.ctypes.data_as(D), int(nthreads))
+    return h
+
+
 def synthetics(nf1, nf2, df, sigma, nwn, gr, gi, theta, phi, mt, nsamp, dt, propdir=None, tred=None, velocity=False,
-               nthreads=1):
-    """(ts float32 [nrcv, 3, nsamp], zsp complex [nrcv, 3, nf2]) of one source node."""
+               nthreads=1, dwn=0.0, rearth=0.0):
+    """(ts float32 [nrcv, 3, nsamp], zsp complex [nrcv, 3, nf2]) of one source node; dwn > 0 selects the
+    regional (wavenumber integral) branch with distances theta * rearth."""
     nwn = np.ascontiguousarray(nwn, dtype=np.int32)
     gr = np.ascontiguousarray(gr, dtype=np.float32)
     gi = np.ascontiguousarray(gi, dtype=np.float32)
@@ -279,8 +327,9 @@ def synthetics(nf1, nf2, df, sigma, nwn, gr, gi, theta, phi, mt, nsamp, dt, prop
     F = C.POINTER(C.c_float)
     pd = None if propdir is None else np.ascontiguousarray(propdir, dtype=np.float64)
     tr = None if tred is None else np.ascontiguousarray(tred, dtype=np.float64)
-    rc = lib().orc_synthetics(int(nf1), int(nf2), C.c_double(df), C.c_double(sigma), nwn.ctypes.data_as(I),
-                              C.c_long(gr.shape[1]), gr.ctypes.data_as(F), gi.ctypes.data_as(F), nr, _d(theta),
+    rc = lib().orc_synthetics_mode(int(nf1), int(nf2), C.c_double(df), C.c_double(sigma), nwn.ctypes.data_as(I),
+                              C.c_long(gr.shape[1]), gr.ctypes.data_as(F), gi.ctypes.data_as(F), C.c_double(dwn),
+                              C.c_double(rearth), nr, _d(theta),
                               _d(phi), None if pd is None else _d(pd), None if tr is None else _d(tr), _d(mt),
                               int(bool(velocity)), int(nsamp), C.c_double(dt), ts.ctypes.data_as(F),
                               zsp.ctypes.data_as(D), int(nthreads))

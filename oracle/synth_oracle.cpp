@@ -17,7 +17,9 @@
 namespace syn {
 
 using cd = std::complex<double>;
-static const double kPi = 3.14159265358979323846;     // mc_pid (f90/mathConstants.f90)
+// mc_pid (f90/mathConstants.f90:34): the literal has no d0 exponent, so the
+// reference's "double precision" pi is the FP32 value 3.1415927410125732
+static const double kPi = 3.1415927410125732;
 
 // Unnormalised associated Legendre functions P_l^m(cos theta), m = 0..2, without
 // the Condon-Shortley sign, and their theta derivatives / ratios to sin(theta).
@@ -201,6 +203,102 @@ int freq_time(int nf1, int nf2, double df, double sigma, const cd* sp, int nsamp
   return 0;
 }
 
+// ---- regional (Bessel) branch ----
+
+// cos^2 taper over the last tapfrac of the wavenumbers 1..nwn (1-based j, k = (j-1) dwn).
+// Restates taperWavenumberIntegrals, f90/wavenumberIntegrals.f90:499-512.
+void taper_wn(int nwn, double dwn, double tapfrac, double* w) {
+  int jt = (int)std::lround(nwn * (1.0 - tapfrac));
+  if (jt > nwn) jt = nwn;
+  if (jt < 1) jt = 1;
+  const double c1 = (nwn - 1) * dwn * tapfrac * 2.0 / kPi;
+  const double c2 = (1.0 - tapfrac) / tapfrac * 0.5 * kPi;
+  for (int j = 1; j <= nwn; ++j) {
+    if (j < jt) { w[j - 1] = 1.0; continue; }
+    const double c = std::cos((j - 1) * dwn / c1 - c2);
+    w[j - 1] = c * c;
+  }
+}
+
+// Kernel of moment wavenumber integral kint (1..14) at wavenumber k for Bessel
+// (or Neumann) values b0, b1, b2 of order 0..2 at k x.
+// Restates the case tables of momentWavenumberIntegrals, f90/wavenumberIntegrals.f90:353-373 (and :379-399).
+static double wn_kernel(int kint, double k, double x, double re, double b0, double b1, double b2) {
+  switch (kint) {
+    case 1: case 2: return k * b0;
+    case 3: return 2.0 / re * b1;
+    case 4: return k * 2.0 * b2;
+    case 5: case 6: return k * (-k * re) * b1;
+    case 7: case 11: return k * 2.0 * (b0 - b1 / (k * x));
+    case 8: return k * 2.0 * k * re * (b1 - 2.0 * b2 / (k * x));
+    case 9: case 13: return k * 2.0 * b1 / (k * x);
+    case 10: return k * 4.0 * k * re * b2 / (k * x);
+    case 12: return -k * 2.0 * k * re * (b1 - 2.0 * b2 / (k * x));
+    case 14: return -k * 4.0 * k * re * b2 / (k * x);
+  }
+  return 0.0;
+}
+
+// The 14 moment wavenumber integrals at epicentral distance xx (m) for each frequency.
+// gr, gi as in moment_sums (wavenumbers of frequency jf: j = 1..nwn[jf-1], k = (j-1) dwn).
+// Restates momentWavenumberIntegrals (f90/wavenumberIntegrals.f90:321-477) with
+// computeBesselFunctions (f90/besselFunctions.f90:33-51) in the frequency loop of
+// besselDisplacementSpectra (f90/displacementSpectra.f90:119-184).  J0, J1, Y0,
+// Y1 come from the C library (the reference vendors Cody's SPECFUN routines).
+static const int kSpecWn[14] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 8, 9, 6, 7};
+void bessel_sums(int nf1, int nf2, const int* nwn, long nkf, const float* gr, const float* gi, double dwn, double re,
+                 double xx, double tapfrac, cd* wnint) {
+  int nmax = 0;
+  for (int jf = nf1; jf <= nf2; ++jf) nmax = std::max(nmax, nwn[jf - 1]);
+  // computeBesselFunctions: besselj(j, 0..2) at k = (j-1) dwn, j = 1 .. nwnmax
+  std::vector<double> J0(nmax), J1(nmax), J2(nmax);
+  J0[0] = 1.0; J1[0] = 0.0; J2[0] = 0.0;
+  double wn = dwn;
+  for (int j = 2; j <= nmax; ++j) {
+    J0[j - 1] = ::j0(wn * xx);
+    J1[j - 1] = ::j1(wn * xx);
+    J2[j - 1] = -J0[j - 1] + 2.0 / (wn * xx) * J1[j - 1];
+    wn = wn + dwn;
+  }
+  double x = xx;
+  if (xx < 3.0 / (dwn * 1.0e9)) x = 3.0 / (dwn * 1.0e9);
+  double wn3 = 3.0 / x;
+  const int j3 = (int)(wn3 / dwn) + 2;
+  wn3 = (j3 - 1) * dwn;
+  const double xdk = x * dwn;
+  std::vector<double> w(nmax), kern(nmax);
+  long off = 0;
+  for (int jf = nf1; jf <= nf2; ++jf) {
+    const int n = nwn[jf - 1];
+    taper_wn(n, dwn, tapfrac, w.data());
+    const int j33 = std::min(j3, n);
+    const double wne = (n - 1) * dwn;
+    for (int kint = 1; kint <= 14; ++kint) {
+      const int s = kSpecWn[kint - 1];
+      auto G = [&](int j) { return cd(gr[(long)s * nkf + off + j - 1], gi[(long)s * nkf + off + j - 1]); };
+      for (int j = 1; j <= n; ++j) kern[j - 1] = wn_kernel(kint, (j - 1) * dwn, x, re, J0[j - 1], J1[j - 1], J2[j - 1]);
+      cd z = 0.0;
+      for (int j = 2; j <= j33 - 1; ++j) z += G(j) * kern[j - 1] * w[j - 1];
+      z += 0.5 * G(j33) * kern[j33 - 1] * w[j33 - 1];
+      z *= dwn;
+      if (j33 != n) {
+        double y0 = ::y0(wn3 * x), y1 = ::y1(wn3 * x), y2 = -y0 + 2.0 / (wn3 * x) * y1;
+        const double b3 = wn_kernel(kint, wn3, x, re, y0, y1, y2);
+        y0 = ::y0(wne * x); y1 = ::y1(wne * x); y2 = -y0 + 2.0 / (wne * x) * y1;
+        const double be = wn_kernel(kint, wne, x, re, y0, y1, y2);
+        cd zs = G(j3) * kern[j3 - 1] * w[j3 - 1];
+        for (int j = j3 + 1; j <= n - 1; ++j) zs += 2.0 * G(j) * kern[j - 1] * w[j - 1];
+        zs += G(n) * kern[n - 1] * w[n - 1];
+        z += zs / x * (1.0 - std::cos(xdk)) / xdk;
+        z += (G(n) * be * w[n - 1] - G(j3) * b3 * w[j3 - 1]) / x * (1.0 - std::sin(xdk) / xdk);
+      }
+      z *= re * re / (2.0 * kPi);
+      wnint[(long)(jf - nf1) * 14 + kint - 1] = z;
+    }
+    off += n;
+  }
+}
+
 }  // namespace syn
 
 extern "C" {
@@ -240,21 +338,39 @@ int orc_moment_sums(int nf1, int nf2, const int* nwn, long nkf, const float* gr,
   return 0;
 }
 
+// wnint: [nrcv][nf][14] complex; x: epicentral distances in m
+int orc_bessel_sums(int nf1, int nf2, const int* nwn, long nkf, const float* gr, const float* gi, double dwn,
+                    double re, int nrcv, const double* x, double tapfrac, double* wnint, int nthreads) {
+  const int nf = nf2 - nf1 + 1;
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads > 0 ? nthreads : 1)
+  for (int r = 0; r < nrcv; ++r)
+    syn::bessel_sums(nf1, nf2, nwn, nkf, gr, gi, dwn, re, x[r], tapfrac,
+                     reinterpret_cast<syn::cd*>(wnint) + (long)r * nf * 14);
+  return 0;
+}
+
+double orc_besselj(int order, double x) { return order == 0 ? ::j0(x) : ::j1(x); }
+double orc_bessely(int order, double x) { return order == 0 ? ::y0(x) : ::y1(x); }
+
 // Full synthetics of one source node: sums, displacement, time shift, velocity,
 // ZNE rotation (propdir given) and the frequency-to-time transform.
 // Restates the station loop of computeSyntheticSeismograms
 // (f90/computeSyntheticSeismograms.f90:311-388) with unit event and station
 // filters.  ts: [nrcv][3][nsamp]; zsp (optional): [nrcv][3][nf2] complex.
-int orc_synthetics(int nf1, int nf2, double df, double sigma, const int* nwn, long nkf, const float* gr,
-                   const float* gi, int nrcv, const double* theta, const double* phi, const double* propdir,
-                   const double* tred, const double* mt, int velocity, int nsamp, double dt, float* ts, double* zsp,
-                   int nthreads) {
+// dwn > 0: regional mode (besselDisplacementSpectra, f90/displacementSpectra.f90:119-184):
+// theta[r] is the epicentral distance in radians as in the global mode and the
+// wavenumber integrals are taken at x = theta[r] * re.
+int orc_synthetics_mode(int nf1, int nf2, double df, double sigma, const int* nwn, long nkf, const float* gr,
+                        const float* gi, double dwn, double re, int nrcv, const double* theta, const double* phi,
+                        const double* propdir, const double* tred, const double* mt, int velocity, int nsamp,
+                        double dt, float* ts, double* zsp, int nthreads) {
   const int nf = nf2 - nf1 + 1;
   int bad = 0;
 #pragma omp parallel for schedule(dynamic) num_threads(nthreads > 0 ? nthreads : 1) reduction(| : bad)
   for (int r = 0; r < nrcv; ++r) {
     std::vector<syn::cd> h((size_t)nf * 14), z((size_t)3 * nf2, 0.0);
-    syn::moment_sums(nf1, nf2, nwn, nkf, gr, gi, theta[r], 0.2, h.data());
+    if (dwn > 0.0) syn::bessel_sums(nf1, nf2, nwn, nkf, gr, gi, dwn, re, theta[r] * re, 0.2, h.data());
+    else syn::moment_sums(nf1, nf2, nwn, nkf, gr, gi, theta[r], 0.2, h.data());
     const double tr = tred ? tred[r] : 0.0;
     for (int jf = nf1; jf <= nf2; ++jf) {
       syn::cd u[3];
@@ -279,6 +395,14 @@ int orc_synthetics(int nf1, int nf2, double df, double sigma, const int* nwn, lo
       bad |= syn::freq_time(nf1, nf2, df, sigma, &z[(size_t)c * nf2], nsamp, dt, ts + ((size_t)r * 3 + c) * nsamp);
   }
   return bad;
+}
+
+int orc_synthetics(int nf1, int nf2, double df, double sigma, const int* nwn, long nkf, const float* gr,
+                   const float* gi, int nrcv, const double* theta, const double* phi, const double* propdir,
+                   const double* tred, const double* mt, int velocity, int nsamp, double dt, float* ts, double* zsp,
+                   int nthreads) {
+  return orc_synthetics_mode(nf1, nf2, df, sigma, nwn, nkf, gr, gi, 0.0, 0.0, nrcv, theta, phi, propdir, tred, mt,
+                             velocity, nsamp, dt, ts, zsp, nthreads);
 }
 
 }  // extern "C"

@@ -72,7 +72,7 @@ def test_moment_sums_generating_function(syn):
     gr = np.zeros((10, L), np.float32)
     gi = np.zeros((10, L), np.float32)
     l = np.arange(L)
-    gr[0] = t ** l * 4 * np.pi / (2 * l + 1)
+    gr[0] = t ** l * 4 * float(np.float32(np.pi)) / (2 * l + 1)      # 4 pi of the reference (FP32 pi)
     theta = np.linspace(0.1, 3.0, 129)
     h = syn.moment_sums(1, 1, nwn, gr, gi, theta)[0, :, 0]
     ref = (1 - 2 * np.cos(theta) * t + t * t) ** -0.5
@@ -121,3 +121,92 @@ def test_synthetics_errors(syn):
     with pytest.raises(GeminiFKError) as e:
         syn.synthetics(1, 2, 1 / 64, 0.0, nwn, g, g, [1.0], [0.0], np.ones(6), 128, 1.0)
     assert e.value.code == 9
+
+
+def test_bessel_sums_match_oracle(syn):
+    """Regional wavenumber integrals: receivers on both sides of the trapezoid/Filon switch
+    (j3 = 3/(x dwn) + 2 beyond, at and below the last wavenumber of each frequency)."""
+    rng = np.random.default_rng(21)
+    nf1, nf2 = 2, 7
+    nwn_all = np.zeros(nf2, np.int32)
+    nwn_all[nf1 - 1:] = [800, 900, 1000, 1100, 1200, 1300]
+    nkf = int(nwn_all.sum())
+    dwn, re = 1e-5, 6371000.0
+    k = np.concatenate([np.arange(n) * dwn for n in nwn_all[nf1 - 1:]])
+    env = np.exp(-k / 4e-3) + 0.05
+    gr = (rng.normal(size=(10, nkf)) * env).astype(np.float32)
+    gi = (rng.normal(size=(10, nkf)) * env).astype(np.float32)
+    x = np.array([150.0, 250.0, 375.0, 3.0 / (998.5 * dwn), 700.0, 1500.0, 5e3, 2e4, 1e5, 2e5])  # 4th: j3 = 1000
+    print(f"bessel sums: {x.size} receivers, {nf2 - nf1 + 1} frequencies", flush=True)
+    g = syn.bessel_sums(nf1, nf2, nwn_all, gr, gi, dwn, re, x)
+    o = oracle.bessel_sums(nf1, nf2, nwn_all, gr, gi, dwn, re, x).transpose(2, 0, 1)
+    assert np.all(np.isfinite(g))
+    scale = np.abs(o).max(axis=2, keepdims=True) + 1e-300
+    err = np.abs(g - o) / scale
+    print("max error relative to the largest integral of each (kint, receiver):", err.max(), flush=True)
+    assert err.max() < 1e-5
+
+
+def test_bessel_sums_closed_form(syn):
+    """(2/x) int J1(kx) exp(-ak) dk = (2/x)(1 - a/sqrt(a^2+x^2))/x through the GPU path."""
+    pi32, re, a, dwn, nwn = float(np.float32(np.pi)), 6371000.0, 1000.0, 5e-6, 8000
+    k = np.arange(nwn) * dwn
+    gr = np.zeros((10, nwn), np.float32)
+    gi = np.zeros((10, nwn), np.float32)
+    gr[8] = (np.exp(-a * k) * 2 * pi32 / re ** 2 * 1e20).astype(np.float32)
+    x = np.array([1000.0, 5000.0])
+    h = syn.bessel_sums(1, 1, np.array([nwn], np.int32), gr, gi, dwn, re, x)[8, :, 0].real / 1e20
+    ref = 2 / x * (1 - a / np.sqrt(a * a + x * x)) / x
+    err = np.abs(h - ref) / ref
+    print("closed form rel error", err, flush=True)
+    assert np.all(err < 1e-4)
+
+
+def test_regional_synthetics_match_oracle(syn):
+    rng = np.random.default_rng(22)
+    nf1, nf2, nsamp = 2, 40, 128
+    T = 51.2
+    df, dt = 1.0 / T, T / nsamp
+    sigma = 5 * df
+    dwn, re = 2e-5, 6371000.0
+    nwn_all = np.zeros(nf2, np.int32)
+    nwn_all[nf1 - 1:] = np.linspace(300, 600, nf2 - nf1 + 1).astype(np.int32)
+    nkf = int(nwn_all.sum())
+    k = np.concatenate([np.arange(n) * dwn for n in nwn_all[nf1 - 1:]])
+    env = np.exp(-k / 3e-3) + 0.02
+    gr = (rng.normal(size=(10, nkf)) * env).astype(np.float32)
+    gi = (rng.normal(size=(10, nkf)) * env).astype(np.float32)
+    R = 5
+    x = rng.uniform(1e3, 2e5, R)
+    phi = rng.uniform(0, 2 * np.pi, R)
+    mt = rng.normal(size=6) * 1e18
+    ts, zsp = syn.synthetics(nf1, nf2, df, sigma, nwn_all, gr, gi, x, phi, mt, nsamp, dt, velocity=True,
+                             want_spectra=True, dwn=dwn, rearth=re)
+    ots, ozsp = oracle.synthetics(nf1, nf2, df, sigma, nwn_all, gr, gi, x / re, phi, mt, nsamp, dt, velocity=True,
+                                  dwn=dwn, rearth=re)
+    zerr = np.abs(zsp - ozsp).max() / np.abs(ozsp).max()
+    ts64, ots64 = ts.astype(np.float64), ots.astype(np.float64)
+    terr = np.linalg.norm(ts64 - ots64, axis=2) / np.linalg.norm(ots64, axis=2)
+    print(f"regional: spectra max rel {zerr:.2e}, seismogram rel-L2 max {terr.max():.2e}", flush=True)
+    assert zerr < 1e-5 and terr.max() < 1e-5
+
+
+def test_regional_gfk_columns_match_oracle():
+    """Wavenumber-mode FK columns of the regional model, exact variant, against the oracle."""
+    from gemini_unified_amd import ExternalRadialNodes, FrequencyGrid, GreenFKEngine, MC_PI, regional_model
+    nem = regional_model(6)
+    nodes = ExternalRadialNodes.from_blocks(nem, [8], [30000.0 / 8])
+    jre = nodes.node_index(nem.rearth)
+    grid = FrequencyGrid.regional_fixed(102.4, 5.0, 20000, 8.0e6)
+    eng = GreenFKEngine(nem, nodes, device=0, variant="exact")
+    orc = oracle.Oracle(nem, nodes)
+    for f, iw0 in [(1.0, 2), (3.0, 4000), (5.0, 12000)]:
+        n = 12
+        r = eng.gfk([f], [iw0], [n], grid.sigma, 1e-6, jre, 1, want_fp32=False, dwn=grid.dwn)
+        orc.set_frequency(f, 3)
+        rc, ref, st = orc.gfk_wavenumbers(2.0 * MC_PI * f, -grid.sigma, iw0, n, grid.dwn, jre, 1, 1e-6)
+        assert rc == 0 and np.all(r["status"] == 0)
+        d = np.linalg.norm((r["gf"] - ref).reshape(-1, n), axis=0) / np.linalg.norm(ref.reshape(-1, n), axis=0)
+        print(f"regional f={f} iwn={iw0}..{iw0 + n - 1}: max rel-L2 {d.max():.2e}", flush=True)
+        assert d.max() < 1e-12
+    eng.close()
