@@ -22,7 +22,10 @@ Procedure of the default run:
      l <= 600 workload at eps = 1e-6 (rejected-step ratio, solves/s);
   5. the degree-sum contraction and frequency-time transform of 4096
      receivers on spectra of the same shape as the main workload's;
-  6. the CPU oracle on a bounded sample of the workload at the plain eps.
+  6. the regional branch: minors over 512 frequencies x 19,999 wavenumbers of a
+     6-layer crustal model and the wavenumber-integral synthetics of 1024
+     receivers;
+  7. the CPU oracle on a bounded sample of the workload at the plain eps.
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -222,6 +225,70 @@ def synth_report(grid, nrcv=4096, nsamp=8192, seed=3):
     return out
 
 
+def c5_report(variant, nrcv=1024, seed=7):
+    """Regional (Bessel) branch: minors over the whole regional grid and the wavenumber-integral
+    synthetics of nrcv receivers at 1-200 km on spectra of the same shape."""
+    from gemini_unified_amd import ExternalRadialNodes, FrequencyGrid, GreenFKEngine, regional_model
+    from gemini_unified_amd.synthetics import SyntheticsEngine
+    nem = regional_model(6)
+    nodes = ExternalRadialNodes.from_blocks(nem, [32], [30000.0 / 32])
+    jre = nodes.node_index(nem.rearth)
+    grid = FrequencyGrid.regional_fixed(102.4, 5.0, 20000, 8.0e6)
+    eps = 1e-6
+    nf = grid.freqs.size
+    # the reference skips k = 0 (iwn = 1): wavenumber indices 2..20000
+    iw = np.full(nf, 2, np.int32)
+    nw = np.full(nf, 19999, np.int32)
+    ninst = int(nw.sum())
+    eng = GreenFKEngine(nem, nodes, device=0, variant=variant)
+    eng.configure(persistent=0)
+    eng.run_minors_resident(grid.freqs, iw, nw, grid.sigma, eps, jre, jre, dwn=grid.dwn)
+    t0 = time.perf_counter()
+    st = eng.run_minors_resident(grid.freqs, iw, nw, grid.sigma, eps, jre, jre, dwn=grid.dwn)
+    wall = time.perf_counter() - t0
+    tm = eng.timing()
+    flops = sum(int(eng.system_stats(s_)[7]) for s_ in (0, 2, 3))
+    eng.close()
+    out = {"workload": ("seeded regional model (6 layers of 2-40 km over a halfspace), T=102.4 s, f<=5 Hz "
+                        f"(512 frequencies) x 19,999 wavenumbers (dwn = 2 pi / 8000 km) = {ninst:,} solves, "
+                        f"32 source depths 0.9-30 km + surface receiver, eps {eps:g}, plain control"),
+           "solves_per_s": round(ninst / wall, 1), "ms_per_pass": round(wall * 1e3, 1),
+           "stage_ms": {"toroidal": round(tm[2], 1), "spheroidal": round(tm[3], 1)},
+           "k_spheroidal_tflops": round(flops / (tm[3] * 1e-3) / 1e12, 3),
+           "rejected_ratio": round(float(st[2]) / max(1, float(st[1] + st[2])), 5)}
+    log(f"C5 minors: {out['solves_per_s']:.0f} solves/s ({wall:.2f} s per pass)")
+    rng = np.random.default_rng(seed)
+    nwn = np.zeros(grid.nf2, np.int32)
+    nwn[grid.nf1 - 1:] = 20000
+    nkf = int(nwn.sum())
+    k = np.tile(np.arange(20000) * grid.dwn, nf)
+    env = (np.exp(-k / 4e-3) + 0.01).astype(np.float32)
+    gr = rng.standard_normal((10, nkf), dtype=np.float32) * env
+    gi = rng.standard_normal((10, nkf), dtype=np.float32) * env
+    x = rng.uniform(1e3, 2e5, nrcv)
+    phi = rng.uniform(0.0, 2 * np.pi, nrcv)
+    mt = rng.normal(size=6) * 1e18
+    nsamp = 2048
+    dt = 1.0 / (grid.df * nsamp)
+    syn = SyntheticsEngine(0)
+    ms = []
+    for _ in range(3):
+        syn.synthetics(grid.nf1, grid.nf2, grid.df, grid.sigma, nwn, gr, gi, x, phi, mt, nsamp, dt, dwn=grid.dwn,
+                       rearth=nem.rearth)
+        ms.append(syn.timing())
+    syn.close()
+    ms = np.mean(np.array(ms[1:]), axis=0)
+    alg = 4.0 * nrcv * nf * 20000 * 14
+    out["synthetics"] = {"receivers": nrcv, "wavenumbers": 20000, "nsamp": nsamp,
+                         "stage_ms": {"bessel_basis": round(ms[0], 3), "spectra_prep": round(ms[1], 3),
+                                      "contraction_and_end_terms": round(ms[2], 3), "combine": round(ms[3], 3),
+                                      "ifft": round(ms[4], 3), "total": round(ms[5], 3)},
+                         "contraction_algorithmic_tflops": round(alg / (ms[2] * 1e-3) / 1e12, 2),
+                         "data": "synthetic FP32 spectra with an exp(-k/4e-3) envelope (stage times exclude copies)"}
+    log(f"C5 synthetics: {out['synthetics']['stage_ms']}")
+    return out
+
+
 def cpu_baseline(nem, nodes, grid, jre, eps, seconds, threads, seed=4):
     """CPU oracle (scalar C++ restatement, OpenMP over l) on a seeded sample of the workload."""
     oracle = oracle_mod()
@@ -288,6 +355,7 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--no-synth", action="store_true")
+    ap.add_argument("--no-c5", action="store_true")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -367,6 +435,7 @@ def main():
 
     c3 = None if (args.no_learned or args.no_c3) else c3_report(args.variant, args.safety, args.propose_every)
     synth = None if args.no_synth else synth_report(grid)
+    c5 = None if args.no_c5 else c5_report(args.variant)
 
     cpu = None
     if not args.no_cpu:
@@ -393,6 +462,7 @@ def main():
         "learned": learned,
         "c3_learned_proposer": c3,
         "synthetics": synth,
+        "c5_regional": c5,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

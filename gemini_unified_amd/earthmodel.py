@@ -214,18 +214,21 @@ def global_model_json(seed: int = 1, nk: int = 200) -> dict:
 
 
 def regional_model_json(seed: int = 6) -> dict:
-    """Seeded regional model: 6 crust/upper-mantle layers over 0-200 km depth on a homogeneous sphere.
+    """Seeded regional model: 6 crust/upper-mantle layers on a homogeneous sphere (halfspace).
 
-    Layer thicknesses of the upper five layers are uniform in 2-40 km, the sixth reaches down to
-    200 km.  vp 4.5 -> 8.2 km/s, vs 2.6 -> 4.6 km/s, rho 2.4 -> 3.4 g/cc increase with depth, linear
-    within each layer (2 knots), with seeded +-2 % knot perturbations; Qmu 100 -> 300, Qkappa 57823.
-    Below 200 km the model is the homogeneous sphere (layer 1) with the values at 200 km.
+    Layer thicknesses are uniform in 2-40 km (scaled down if they add up to more than 200 km); the
+    halfspace starts below the sixth layer.  vp 4.5 -> 8.2 km/s, vs 2.6 -> 4.6 km/s, rho 2.4 -> 3.4
+    g/cc increase with depth over 0-200 km, linear within each layer (2 knots), with seeded +-2 %
+    knot perturbations; Qmu 100 -> 300, Qkappa 57823.  The halfspace (layer 1) carries the values
+    at the bottom of the sixth layer.
     """
     rng = np.random.default_rng(seed)
     R, zmax = 6371000.0, 200000.0
-    th = list(rng.uniform(2000.0, 40000.0, size=5))
-    tops = np.concatenate([[0.0], np.cumsum(th)])                   # depths of the layer tops, surface first
-    bots = np.concatenate([tops[1:], [zmax]])
+    th = rng.uniform(2000.0, 40000.0, size=6)
+    if th.sum() > zmax:
+        th *= zmax / th.sum()
+    tops = np.concatenate([[0.0], np.cumsum(th)[:-1]])              # depths of the layer tops, surface first
+    bots = np.cumsum(th)
 
     def prop(z, a, b):
         return a + (b - a) * (np.asarray(z) / zmax)
@@ -240,7 +243,7 @@ def regional_model_json(seed: int = 6) -> dict:
                            density=pert(prop(z, 2.4e3, 3.4e3)), qk=np.full(2, 57823.0),
                            qm=prop(z, 100.0, 300.0)))
     base = layers[0]
-    sph = dict(r=np.array([R - zmax]), fluid=False, vp=base["vp"][:1].copy(), vs=base["vs"][:1].copy(),
+    sph = dict(r=np.array([R - bots[-1]]), fluid=False, vp=base["vp"][:1].copy(), vs=base["vs"][:1].copy(),
                density=base["density"][:1].copy(), qk=np.array([57823.0]), qm=base["qm"][:1].copy())
     return _layered_json(f"seeded regional model, seed {seed}", R, [sph] + layers, nic=0, noc=0)
 
