@@ -140,10 +140,12 @@ def train_proposer(eng, grid, jre, eps, frac, seed, maxrec=3_000_000):
     from gemini_unified_amd import proposer
     train, _ = proposer.split_grid(grid, frac, seed)
     fr, lm, nl = proposer.as_batch(train)
+    log(f"proposer: logging the steps of {int(nl.sum())} solves at eps {eps:g}")
     t0 = time.perf_counter()
     rec, nrec, _ = eng.collect_steplog(fr, lm, nl, grid.sigma, eps, jre, jre, maxrec=maxrec)
     X, T, _ = proposer.targets_from_log(rec)
     t1 = time.perf_counter()
+    log(f"proposer: {nrec} step records in {t1 - t0:.1f} s; training")
     params, norm, losses = proposer.train(X, T, epochs=12, batch=2048, lr=2e-3)
     t2 = time.perf_counter()
     log(f"proposer: {nrec} step records ({X.shape[0]} used) in {t1 - t0:.1f} s, trained in {t2 - t1:.1f} s, "

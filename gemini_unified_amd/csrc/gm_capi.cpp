@@ -93,6 +93,11 @@ extern "C" int gm_set_option(gm_ctx* c, int option, int value) {
     c->waves = value;
     return GM_OK;
   }
+  if (option == GM_OPT_SPH_SPLIT) {
+    if (value != 1 && value != 2 && value != 4) return gm_fail(c, GM_ERR_ARG, "split must be 1, 2 or 4");
+    c->sph_split = value;
+    return GM_OK;
+  }
   if (option == GM_OPT_PROPOSE_EVERY) {
     if (value < 1 || value > 1000) return gm_fail(c, GM_ERR_ARG, "proposal interval must be in [1, 1000]");
     c->prop_every = value;
@@ -318,6 +323,7 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   B.unit_q = (const int*)(base + o_uq); B.unit_start = (const int*)(base + o_ust); B.unit_end = (const int*)(base + o_ue);
   B.unit_counter = (int*)(base + o_uc);
   B.ctl_mode = ctl_mode;
+  B.sph_split = c->sph_split;
   B.mlp = c->dmlp;
   B.prop = c->prop;
   B.logbuf = ctl_mode == 1 ? (float*)(base + o_log) : nullptr;

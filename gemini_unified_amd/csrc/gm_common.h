@@ -82,6 +82,7 @@ struct GmBatch {
   int* __restrict__ unit_counter;          // one counter per persistent kernel launch
   // step control of the minors system: 0 plain (reference), 1 plain with step
   // logging, 2 learned proposer
+  int sph_split;                           // blocks per work unit of the spheroidal kernel (1, 2 or 4)
   int ctl_mode;
   const float* __restrict__ mlp;           // proposer parameters (GM_MLP_NPARAM floats)
   float* __restrict__ logbuf;              // step log records [logmax][GM_LOGREC]

@@ -72,13 +72,15 @@ class GreenFKEngine:
         check(self.h, _lib.lib().gm_set_option(self.h, 1, VARIANTS[variant]))
         self.variant = variant
 
-    def configure(self, persistent=None, unit=None, waves=None, propose_every=None):
+    def configure(self, persistent=None, unit=None, waves=None, propose_every=None, sph_split=None):
         """Kernel schedule options (all result-preserving): persistent substep-scheduled kernels or
         one lane per job, work-unit size, waves per SIMD of the persistent kernels; and how often the
         learned proposer is consulted (every N accepted steps, plus sub-interval starts)."""
         L = _lib.lib()
         if propose_every is not None:
             check(self.h, L.gm_set_option(self.h, 5, int(propose_every)))
+        if sph_split is not None:
+            check(self.h, L.gm_set_option(self.h, 6, int(sph_split)))
         if persistent is not None:
             check(self.h, L.gm_set_option(self.h, 2, int(bool(persistent))))
         if unit is not None:

@@ -57,8 +57,11 @@ typedef struct gm_ctx gm_ctx;
 #define GM_OPT_PERSISTENT 2    /* 0 (default): one lane per job, 1: persistent substep-scheduled kernels */
 #define GM_OPT_UNIT 3          /* (omega, l) instances per work unit of the persistent kernels */
 #define GM_OPT_WAVES 4         /* waves per SIMD the persistent kernels are built for: 1 or 2 */
-#define GM_OPT_PROPOSE_EVERY 5 /* learned proposer: consult the network at sub-interval starts, after a
-                                  rejection and every N-th accepted step (default 1: every step) */
+#define GM_OPT_PROPOSE_EVERY 5 /* learned proposer: consult the network after every N-th accepted step,
+                                  and at a sub-interval start once N accepted steps have passed since the
+                                  last consultation (default 1: every step) */
+#define GM_OPT_SPH_SPLIT 6     /* blocks per 256-instance work unit of the spheroidal kernel: 1, 2
+                                  (default) or 4 */
 
 /* work counters returned in stats[10] (same meaning as the CPU oracle's) */
 #define GM_NSTAT 10            /* propagations, accepted steps, rejected steps, MMP sequences,
