@@ -400,7 +400,7 @@ def main():
                 "traffic": traffic, "kernel_ms": round(tm_p[3], 3), "algorithmic_flops_per_launch": sph_flops,
                 "toroidal": {"kernel": "k_toroidal", "kernel_ms": round(tm_p[2], 3),
                              "achieved": round(tor_flops / (tm_p[2] * 1e-3) / 1e12, 4)}}
-    log(f"plain: eps {eps_p:.0e}, {value_p:.1f} solves/s; k_spheroidal {tm_p[3]:.1f} ms, {achieved:.3f} TFLOP/s")
+    log(f"plain: eps {eps_p:g}, {value_p:.1f} solves/s; k_spheroidal {tm_p[3]:.1f} ms, {achieved:.3f} TFLOP/s")
     plain = {"value": round(value_p, 3), "ms_per_step": round(ms_p, 3), "eps": eps_p,
              "spectrum_rel_err": {"max": emax, "p99": ep99}, "eps_ladder": ladder_p,
              "rejected_ratio": round(float(stats_p[2]) / max(1, float(stats_p[1] + stats_p[2])), 5),
@@ -431,7 +431,7 @@ def main():
                        "safety": args.safety, "propose_every": args.propose_every, "train": info,
                        "trained_on": "step logs of the plain integrator on a seeded 0.4 % of the (omega,l) grid",
                        "learned_over_plain": round(value_l / value_p, 4)}
-            log(f"learned: eps {eps_l:.0e}, {value_l:.1f} solves/s ({value_l / value_p:.3f} x plain)")
+            log(f"learned: eps {eps_l:g}, {value_l:.1f} solves/s ({value_l / value_p:.3f} x plain)")
         eng.set_proposer(enable=False)
     eng.close()
 
@@ -441,7 +441,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        log(f"cpu baseline: oracle on {threads} threads for ~{args.cpu_seconds:.0f} s at eps {eps_p:.0e}")
+        log(f"cpu baseline: oracle on {threads} threads for ~{args.cpu_seconds:.0f} s at eps {eps_p:g}")
         rate, nsolves, nfreq = cpu_baseline(nem, nodes, grid, jre, eps_p, args.cpu_seconds, threads)
         cpu = {"value": round(rate, 3), "unit": "solves/s", "cores": threads, "kind": "port",
                "sample": f"{nsolves} solves: {nfreq} seeded random frequencies x 32 consecutive degrees at a "
