@@ -51,9 +51,15 @@ extern "C" int gm_ctx_create(int device, gm_ctx** out) {
     delete c;
     return gm_fail(nullptr, GM_ERR_HIP, "context creation failed on device %d", device);
   }
-  for (int i = 0; i < 7; ++i) hipEventCreate(&c->ev[i]);
-  for (int i = 0; i < 6; ++i) hipEventCreate(&c->sev[i]);
-  hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device);
+  bool ok = true;
+  for (int i = 0; i < 7; ++i) ok = ok && hipEventCreate(&c->ev[i]) == hipSuccess;
+  for (int i = 0; i < 6; ++i) ok = ok && hipEventCreate(&c->sev[i]) == hipSuccess;
+  ok = ok && hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess;
+  ok = ok && hipDeviceGetAttribute(&c->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess;
+  if (!ok) {
+    gm_ctx_destroy(c);
+    return gm_fail(nullptr, GM_ERR_HIP, "context creation failed on device %d", device);
+  }
   if (c->ncu <= 0) c->ncu = 1;
   memset(&c->hmodel, 0, sizeof(GmModel));
   *out = c;
@@ -62,16 +68,17 @@ extern "C" int gm_ctx_create(int device, gm_ctx** out) {
 
 extern "C" int gm_ctx_destroy(gm_ctx* c) {
   if (!c) return GM_OK;
-  hipSetDevice(c->device);
-  if (c->stream) hipStreamSynchronize(c->stream);
-  if (c->dbuf) hipFree(c->dbuf);
-  if (c->dmodel) hipFree(c->dmodel);
-  if (c->dmlp) hipFree(c->dmlp);
-  for (int i = 0; i < 7; ++i) hipEventDestroy(c->ev[i]);
-  for (int i = 0; i < 6; ++i) hipEventDestroy(c->sev[i]);
-  if (c->sbuf) hipFree(c->sbuf);
-  if (c->dtw) hipFree(c->dtw);
-  if (c->stream) hipStreamDestroy(c->stream);
+  // teardown: release everything even if one call reports an error
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->dbuf) (void)hipFree(c->dbuf);
+  if (c->dmodel) (void)hipFree(c->dmodel);
+  if (c->dmlp) (void)hipFree(c->dmlp);
+  for (int i = 0; i < 7; ++i) if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+  for (int i = 0; i < 6; ++i) if (c->sev[i]) (void)hipEventDestroy(c->sev[i]);
+  if (c->sbuf) (void)hipFree(c->sbuf);
+  if (c->dtw) (void)hipFree(c->dtw);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return GM_OK;
 }
@@ -133,6 +140,10 @@ extern "C" int gm_model_set(gm_ctx* c, int nk, const double* rk, const double* r
   if (!c) return GM_ERR_ARG;
   if (nk < 2 || nk > GM_MAXK) return gm_fail(c, GM_ERR_ARG, "nk=%d outside [2,%d]", nk, GM_MAXK);
   if (nlay < 2 || nlay > GM_MAXL) return gm_fail(c, GM_ERR_ARG, "nlay=%d outside [2,%d]", nlay, GM_MAXL);
+  // the minors kernels stage one frequency's spline table in LDS
+  if (sizeof(GmKnot) * (size_t)(nk + 1) > (size_t)c->max_lds)
+    return gm_fail(c, GM_ERR_UNSUPPORTED, "nk=%d: spline table of %zu B exceeds the %d B of LDS per block", nk,
+                   sizeof(GmKnot) * (size_t)(nk + 1), c->max_lds);
   GmModel& m = c->hmodel;
   int nnod = m.nnod;
   double rn[GM_MAXN + 1]; int stt[GM_MAXN + 1], lay[GM_MAXN + 1];
@@ -283,7 +294,7 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   size_t o_log = ctl_mode == 1 ? take(sizeof(float) * GM_LOGREC * (size_t)c->log_max) : 0;
   size_t o_logc = take(sizeof(int) * 4);
   if (off > c->dbuf_size) {
-    if (c->dbuf) { hipFree(c->dbuf); c->dbuf = nullptr; c->dbuf_size = 0; }
+    if (c->dbuf) { HIPCHK(c, hipFree(c->dbuf)); c->dbuf = nullptr; c->dbuf_size = 0; }
     if (hipMalloc(&c->dbuf, off) != hipSuccess) return gm_fail(c, GM_ERR_NOMEM, "device allocation of %zu bytes failed", off);
     c->dbuf_size = off;
   }
@@ -389,8 +400,8 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   }
   HIPCHK(c, hipStreamSynchronize(s));
   float ms;
-  for (int i = 0; i < 5; ++i) { hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]); c->timing[i] = ms; }
-  hipEventElapsedTime(&ms, c->ev[0], c->ev[5]);
+  for (int i = 0; i < 5; ++i) { HIPCHK(c, hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1])); c->timing[i] = ms; }
+  HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[5]));
   c->timing[5] = ms;
   memcpy(c->last_hist, hist, sizeof hist);
   if (status) memcpy(status, hstatus.data(), sizeof(int) * n);

@@ -45,8 +45,12 @@ struct GmKnot {
   double er[7], ei[7];
   double e2r[7], e2i[7];
   double rh;     // 1 / (rk(j+1) - rk(j)) inside the layer (0 at a layer top)
-  double pad;    // keeps the record a multiple of 16 bytes
+  // 272-byte records: a multiple of 16 bytes, and consecutive knots start 4
+  // LDS banks apart, so lanes reading one field of different knots spread
+  // over 8 bank groups instead of all hitting the same banks
+  double pad[3];
 };
+static_assert(sizeof(GmKnot) == 272, "GmKnot record size");
 
 // one batch of (omega, l) instances and its device buffers
 struct GmBatch {

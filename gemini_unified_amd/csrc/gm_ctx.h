@@ -12,6 +12,7 @@ struct gm_ctx {
   int waves = 1;           // waves per SIMD the persistent kernels are compiled for (1 or 2)
   int sph_split = 2;       // blocks per work unit of the one-lane-per-job spheroidal kernel
   int ncu = 0;
+  int max_lds = 65536;      // LDS bytes per block the device allows
   hipStream_t stream = nullptr;
   std::string err;
   GmModel hmodel;
@@ -20,7 +21,7 @@ struct gm_ctx {
   // growable device scratch
   void* dbuf = nullptr;
   size_t dbuf_size = 0;
-  hipEvent_t ev[7];
+  hipEvent_t ev[7] = {};
   double timing[6] = {0};
   unsigned long long last_hist[48] = {0};
   // learned step proposer
@@ -37,7 +38,7 @@ struct gm_ctx {
   size_t sbuf_size = 0;
   double* dtw = nullptr;   // twiddle table of the last transform length
   int tw_n = 0;
-  hipEvent_t sev[6];
+  hipEvent_t sev[6] = {};
   double stiming[6] = {0};
 };
 

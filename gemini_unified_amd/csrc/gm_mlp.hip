@@ -184,23 +184,38 @@ using namespace gm_mlp;
 // ---------------------------------------------------------------------------
 #define MCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { fprintf(stderr, "gm_mlp: %s: %s\n", #x, hipGetErrorString(e_)); return GM_ERR_HIP; } } while (0)
 
+namespace {
+// device buffer released on every exit path (hipFree waits for pending work)
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  hipError_t alloc(size_t n) { return hipMalloc(&p, sizeof(T) * n); }
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  operator T*() const { return p; }
+};
+struct StreamGuard {
+  hipStream_t s = nullptr;
+  ~StreamGuard() { if (s) (void)hipStreamDestroy(s); }
+};
+}  // namespace
+
 extern "C" int gm_mlp_nparams(void) { return GM_MLP_NPARAM; }
 
 // forward of n samples (host arrays): X[n][9] -> Y[n][2]
 extern "C" int gm_mlp_forward(int device, const float* params, int n, const float* X, float* Y) {
   if (n <= 0 || !params || !X || !Y) return GM_ERR_ARG;
   MCHK(hipSetDevice(device));
-  float *dp, *dx, *dw;
-  MCHK(hipMalloc(&dp, sizeof(float) * GM_MLP_NPARAM));
-  MCHK(hipMalloc(&dx, sizeof(float) * (size_t)n * GM_MLP_IN));
-  MCHK(hipMalloc(&dw, sizeof(float) * work_floats(n)));
+  DevBuf<float> dp, dx, dw;
+  MCHK(dp.alloc(GM_MLP_NPARAM));
+  MCHK(dx.alloc((size_t)n * GM_MLP_IN));
+  MCHK(dw.alloc(work_floats(n)));
   MCHK(hipMemcpy(dp, params, sizeof(float) * GM_MLP_NPARAM, hipMemcpyHostToDevice));
   MCHK(hipMemcpy(dx, X, sizeof(float) * (size_t)n * GM_MLP_IN, hipMemcpyHostToDevice));
   Work w;
   carve(dw, n, w);
   forward(0, dp, dx, n, w);
+  MCHK(hipGetLastError());
   MCHK(hipMemcpy(Y, w.y4, sizeof(float) * (size_t)n * GM_MLP_OUT, hipMemcpyDeviceToHost));
-  hipFree(dp); hipFree(dx); hipFree(dw);
   return GM_OK;
 }
 
@@ -209,13 +224,13 @@ extern "C" int gm_mlp_grad(int device, const float* params, int n, const float* 
                            float* loss, float* grad) {
   if (n <= 0 || !params || !X || !T || !grad) return GM_ERR_ARG;
   MCHK(hipSetDevice(device));
-  float *dp, *dx, *dt, *dw, *dg, *dl;
-  MCHK(hipMalloc(&dp, sizeof(float) * GM_MLP_NPARAM));
-  MCHK(hipMalloc(&dg, sizeof(float) * GM_MLP_NPARAM));
-  MCHK(hipMalloc(&dx, sizeof(float) * (size_t)n * GM_MLP_IN));
-  MCHK(hipMalloc(&dt, sizeof(float) * (size_t)n * 2));
-  MCHK(hipMalloc(&dw, sizeof(float) * work_floats(n)));
-  MCHK(hipMalloc(&dl, sizeof(float)));
+  DevBuf<float> dp, dx, dt, dw, dg, dl;
+  MCHK(dp.alloc(GM_MLP_NPARAM));
+  MCHK(dg.alloc(GM_MLP_NPARAM));
+  MCHK(dx.alloc((size_t)n * GM_MLP_IN));
+  MCHK(dt.alloc((size_t)n * 2));
+  MCHK(dw.alloc(work_floats(n)));
+  MCHK(dl.alloc(1));
   MCHK(hipMemcpy(dp, params, sizeof(float) * GM_MLP_NPARAM, hipMemcpyHostToDevice));
   MCHK(hipMemcpy(dx, X, sizeof(float) * (size_t)n * GM_MLP_IN, hipMemcpyHostToDevice));
   MCHK(hipMemcpy(dt, T, sizeof(float) * (size_t)n * 2, hipMemcpyHostToDevice));
@@ -226,9 +241,9 @@ extern "C" int gm_mlp_grad(int device, const float* params, int n, const float* 
   forward(0, dp, dx, n, w);
   hipLaunchKernelGGL(k_loss, dim3((n + 255) / 256), dim3(256), 0, 0, n, w.y4, dt, w0, w1, w.d4, dl);
   backward(0, dp, dx, n, w, dg);
+  MCHK(hipGetLastError());
   MCHK(hipMemcpy(grad, dg, sizeof(float) * GM_MLP_NPARAM, hipMemcpyDeviceToHost));
   if (loss) MCHK(hipMemcpy(loss, dl, sizeof(float), hipMemcpyDeviceToHost));
-  hipFree(dp); hipFree(dg); hipFree(dx); hipFree(dt); hipFree(dw); hipFree(dl);
   return GM_OK;
 }
 
@@ -238,21 +253,23 @@ extern "C" int gm_mlp_train(int device, float* params, int n, const float* X, co
   if (n <= 0 || batch <= 0 || epochs <= 0 || !params || !X || !T) return GM_ERR_ARG;
   if (batch > n) batch = n;
   MCHK(hipSetDevice(device));
-  hipStream_t s;
-  MCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-  float *dp, *dg, *dm1, *dm2, *dx, *dt, *xb, *tb, *dw, *dl;
-  int* didx;
-  MCHK(hipMalloc(&dp, sizeof(float) * GM_MLP_NPARAM));
-  MCHK(hipMalloc(&dg, sizeof(float) * GM_MLP_NPARAM));
-  MCHK(hipMalloc(&dm1, sizeof(float) * GM_MLP_NPARAM));
-  MCHK(hipMalloc(&dm2, sizeof(float) * GM_MLP_NPARAM));
-  MCHK(hipMalloc(&dx, sizeof(float) * (size_t)n * GM_MLP_IN));
-  MCHK(hipMalloc(&dt, sizeof(float) * (size_t)n * 2));
-  MCHK(hipMalloc(&xb, sizeof(float) * (size_t)batch * GM_MLP_IN));
-  MCHK(hipMalloc(&tb, sizeof(float) * (size_t)batch * 2));
-  MCHK(hipMalloc(&dw, sizeof(float) * work_floats(batch)));
-  MCHK(hipMalloc(&dl, sizeof(float) * (epochs + 1)));
-  MCHK(hipMalloc(&didx, sizeof(int) * (size_t)n));
+  // buffers are declared before the stream so that the stream is destroyed first on every exit
+  DevBuf<float> dp, dg, dm1, dm2, dx, dt, xb, tb, dw, dl;
+  DevBuf<int> didx;
+  StreamGuard sg;
+  MCHK(hipStreamCreateWithFlags(&sg.s, hipStreamNonBlocking));
+  hipStream_t s = sg.s;
+  MCHK(dp.alloc(GM_MLP_NPARAM));
+  MCHK(dg.alloc(GM_MLP_NPARAM));
+  MCHK(dm1.alloc(GM_MLP_NPARAM));
+  MCHK(dm2.alloc(GM_MLP_NPARAM));
+  MCHK(dx.alloc((size_t)n * GM_MLP_IN));
+  MCHK(dt.alloc((size_t)n * 2));
+  MCHK(xb.alloc((size_t)batch * GM_MLP_IN));
+  MCHK(tb.alloc((size_t)batch * 2));
+  MCHK(dw.alloc(work_floats(batch)));
+  MCHK(dl.alloc((size_t)epochs + 1));
+  MCHK(didx.alloc((size_t)n));
   MCHK(hipMemcpyAsync(dp, params, sizeof(float) * GM_MLP_NPARAM, hipMemcpyHostToDevice, s));
   MCHK(hipMemcpyAsync(dx, X, sizeof(float) * (size_t)n * GM_MLP_IN, hipMemcpyHostToDevice, s));
   MCHK(hipMemcpyAsync(dt, T, sizeof(float) * (size_t)n * 2, hipMemcpyHostToDevice, s));
@@ -283,6 +300,7 @@ extern "C" int gm_mlp_train(int device, float* params, int n, const float* X, co
                          lr, b1, b2, ae, c1, c2);
     }
     // sync the per-epoch RNG upload (perm is reused on the host)
+    MCHK(hipGetLastError());
     MCHK(hipStreamSynchronize(s));
   }
   MCHK(hipMemcpyAsync(params, dp, sizeof(float) * GM_MLP_NPARAM, hipMemcpyDeviceToHost, s));
@@ -290,8 +308,5 @@ extern "C" int gm_mlp_train(int device, float* params, int n, const float* X, co
   MCHK(hipMemcpyAsync(hl.data(), dl, sizeof(float) * epochs, hipMemcpyDeviceToHost, s));
   MCHK(hipStreamSynchronize(s));
   if (losses) for (int ep = 0; ep < epochs; ++ep) losses[ep] = hl[ep] / (float)std::max(nb, 1);
-  hipFree(dp); hipFree(dg); hipFree(dm1); hipFree(dm2); hipFree(dx); hipFree(dt); hipFree(xb); hipFree(tb);
-  hipFree(dw); hipFree(dl); hipFree(didx);
-  hipStreamDestroy(s);
   return GM_OK;
 }
