@@ -140,10 +140,10 @@ extern "C" int gm_model_set(gm_ctx* c, int nk, const double* rk, const double* r
   if (!c) return GM_ERR_ARG;
   if (nk < 2 || nk > GM_MAXK) return gm_fail(c, GM_ERR_ARG, "nk=%d outside [2,%d]", nk, GM_MAXK);
   if (nlay < 2 || nlay > GM_MAXL) return gm_fail(c, GM_ERR_ARG, "nlay=%d outside [2,%d]", nlay, GM_MAXL);
-  // the minors kernels stage one frequency's spline table in LDS
-  if (sizeof(GmKnot) * (size_t)(nk + 1) > (size_t)c->max_lds)
-    return gm_fail(c, GM_ERR_UNSUPPORTED, "nk=%d: spline table of %zu B exceeds the %d B of LDS per block", nk,
-                   sizeof(GmKnot) * (size_t)(nk + 1), c->max_lds);
+  // the minors kernels stage one frequency's spline table image in LDS
+  if (gmLdsBytes(nk, nlay) > (size_t)c->max_lds)
+    return gm_fail(c, GM_ERR_UNSUPPORTED, "nk=%d: spline table image of %zu B exceeds the %d B of LDS per block", nk,
+                   gmLdsBytes(nk, nlay), c->max_lds);
   GmModel& m = c->hmodel;
   int nnod = m.nnod;
   double rn[GM_MAXN + 1]; int stt[GM_MAXN + 1], lay[GM_MAXN + 1];
@@ -276,6 +276,8 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   size_t o_freqs = take(sizeof(double) * (nfreq + 1));
   size_t o_omre = take(sizeof(double) * (nfreq + 1));
   size_t o_tables = take(sizeof(GmKnot) * (size_t)(m.nk + 1) * (nfreq + 1));
+  const size_t lstride = gmLdsBytes(m.nk, m.nlay);
+  size_t o_ltab = take(lstride * (size_t)(nfreq + 1));
   size_t o_q = take(sizeof(int) * (n + 1)), o_l = take(sizeof(int) * (n + 1)), o_status = take(sizeof(int) * (n + 1));
   size_t o_rs = take(sizeof(double) * 2 * (n + 1));
   size_t o_ut = take(16 * 2 * nnod * n + 16), o_dt = take(16 * 2 * nnod * n + 16), o_dett = take(16 * nnod * n + 16);
@@ -318,6 +320,7 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   B.omim = -sigma; B.eps = eps;
   B.q = (const int*)(base + o_q); B.l = (const int*)(base + o_l); B.omre = (const double*)(base + o_omre);
   B.nkp1 = m.nk + 1; B.tables = (const GmKnot*)(base + o_tables);
+  B.ltab = (const unsigned char*)(base + o_ltab); B.lstride = (int)lstride;
   B.status = (int*)(base + o_status); B.rs = (double*)(base + o_rs);
   B.ut = (double*)(base + o_ut); B.dt = (double*)(base + o_dt); B.dett = (double*)(base + o_dett);
   B.us = (double*)(base + o_us); B.ds = (double*)(base + o_ds); B.ulg = (double*)(base + o_ulg);
@@ -346,7 +349,7 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   }
   auto launch = c->variant == 0 ? gm_exact::launch_stage : gm_fast::launch_stage;
   auto bpc = c->variant == 0 ? gm_exact::persist_blocks_per_cu : gm_fast::persist_blocks_per_cu;
-  const size_t lds = sizeof(GmKnot) * (size_t)(m.nk + 1);
+  const size_t lds = lstride;
   const int pgrid = std::max(1, std::min(nunits, c->ncu * bpc(lds, c->waves)));
   GmKnot* dT = (GmKnot*)(base + o_tables);
   const double* dF = (const double*)(base + o_freqs);

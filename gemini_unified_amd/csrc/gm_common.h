@@ -52,6 +52,27 @@ struct GmKnot {
 };
 static_assert(sizeof(GmKnot) == 272, "GmKnot record size");
 
+// Compact record of one knot staged in LDS: density and the moduli 1..5
+// (A, C, F, L, N) with their spline coefficients and 1/h, the set the minors
+// systems evaluate at every grid point.  184 bytes: consecutive records start
+// 14 banks apart (16 distinct bank offsets), and a 200-knot table fits four
+// times into one CU's LDS.  Moduli 6 and 7 (kappa, mu: start radius and
+// initial values only) are read from the full table in global memory.
+struct GmKnotL {
+  double ro, ro2, rh;
+  double er[5], ei[5], e2r[5], e2i[5];
+};
+static_assert(sizeof(GmKnotL) == 184, "GmKnotL record size");
+
+// LDS image of one frequency's table, staged by every block of the minors
+// kernels (and laid out the same in global memory, one image per frequency):
+//   GmKnotL rec[nk+1] | double rk[nk+1] | double drk[nlay+1] | int iktop[nlay+1]
+// so that the knot lookup of a spline evaluation touches LDS only.
+__host__ __device__ inline size_t gmLdsBytes(int nk, int nlay) {
+  size_t b = (size_t)(nk + 1) * (sizeof(GmKnotL) + sizeof(double)) + (size_t)(nlay + 1) * (sizeof(double) + sizeof(int));
+  return (b + 15) & ~(size_t)15;
+}
+
 // one batch of (omega, l) instances and its device buffers
 struct GmBatch {
   int n;                           // number of (omega, l) instances
@@ -63,6 +84,8 @@ struct GmBatch {
   const double* __restrict__ omre; // per frequency slot
   int nkp1;                        // table stride (nk + 1)
   const GmKnot* __restrict__ tables;
+  const unsigned char* __restrict__ ltab;  // LDS images, one per frequency slot, lstride bytes apart
+  int lstride;
   int* __restrict__ status;
   double* __restrict__ rs;         // [2][n]: toroidal, spheroidal start radius
   double* __restrict__ ut;         // toroidal up   [nnod][2][n] complex (re,im interleaved)

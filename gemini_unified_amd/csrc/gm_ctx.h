@@ -10,7 +10,7 @@ struct gm_ctx {
   int persistent = 0;      // 1: substep-scheduled persistent kernels, 0: one lane per job (default, faster here)
   int unit = 1024;         // instances per work unit of the persistent kernels
   int waves = 1;           // waves per SIMD the persistent kernels are compiled for (1 or 2)
-  int sph_split = 2;       // blocks per work unit of the one-lane-per-job spheroidal kernel
+  int sph_split = 4;       // blocks per work unit of the one-lane-per-job spheroidal kernel
   int ncu = 0;
   int max_lds = 65536;      // LDS bytes per block the device allows
   hipStream_t stream = nullptr;

@@ -61,7 +61,7 @@ typedef struct gm_ctx gm_ctx;
                                   and at a sub-interval start once N accepted steps have passed since the
                                   last consultation (default 1: every step) */
 #define GM_OPT_SPH_SPLIT 6     /* blocks per 256-instance work unit of the spheroidal kernel: 1, 2
-                                  (default) or 4 */
+                                  or 4 (default) */
 
 /* work counters returned in stats[10] (same meaning as the CPU oracle's) */
 #define GM_NSTAT 10            /* propagations, accepted steps, rejected steps, MMP sequences,
