@@ -291,7 +291,7 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   size_t o_hist = take(sizeof(unsigned long long) * 6 * 8);
   const int nunits = (int)P.uq.size();
   size_t o_uq = take(sizeof(int) * (nunits + 1)), o_ust = take(sizeof(int) * (nunits + 1));
-  size_t o_ue = take(sizeof(int) * (nunits + 1)), o_uc = take(sizeof(int) * 4);
+  size_t o_ue = take(sizeof(int) * (nunits + 1)), o_uc = take(sizeof(int) * GM_NCOUNTERS);
   const int ctl_mode = c->log_max > 0 ? 1 : (c->prop_enabled ? 2 : 0);
   size_t o_log = ctl_mode == 1 ? take(sizeof(float) * GM_LOGREC * (size_t)c->log_max) : 0;
   size_t o_logc = take(sizeof(int) * 4);

@@ -106,7 +106,7 @@ struct GmBatch {
   const int* __restrict__ unit_q;
   const int* __restrict__ unit_start;
   const int* __restrict__ unit_end;
-  int* __restrict__ unit_counter;          // one counter per persistent kernel launch
+  int* __restrict__ unit_counter;          // GM_NCOUNTERS work counters, zeroed by k_setup
   // step control of the minors system: 0 plain (reference), 1 plain with step
   // logging, 2 learned proposer
   int sph_split;                           // blocks per work unit of the spheroidal kernel (1, 2 or 4)
@@ -118,3 +118,6 @@ struct GmBatch {
   GmProposer prop;
 };
 #define GM_LOGREC 20
+// work counters (GmBatch::unit_counter): persistent toroidal / spheroidal
+// kernels, and the block claims of the one-lane-per-job kernels
+enum { GM_CNT_TOR_PERSIST = 0, GM_CNT_SPH_PERSIST = 1, GM_CNT_TOR = 2, GM_CNT_SPH = 3, GM_NCOUNTERS = 4 };
