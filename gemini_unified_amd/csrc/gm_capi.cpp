@@ -276,8 +276,9 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   size_t o_freqs = take(sizeof(double) * (nfreq + 1));
   size_t o_omre = take(sizeof(double) * (nfreq + 1));
   size_t o_tables = take(sizeof(GmKnot) * (size_t)(m.nk + 1) * (nfreq + 1));
-  const size_t lstride = gmLdsBytes(m.nk, m.nlay);
+  const size_t lstride = gmImageBytes<GmKnotL>(m.nk, m.nlay), tstride = gmImageBytes<GmKnotT>(m.nk, m.nlay);
   size_t o_ltab = take(lstride * (size_t)(nfreq + 1));
+  size_t o_ttab = take(tstride * (size_t)(nfreq + 1));
   size_t o_q = take(sizeof(int) * (n + 1)), o_l = take(sizeof(int) * (n + 1)), o_status = take(sizeof(int) * (n + 1));
   size_t o_rs = take(sizeof(double) * 2 * (n + 1));
   size_t o_ut = take(16 * 2 * nnod * n + 16), o_dt = take(16 * 2 * nnod * n + 16), o_dett = take(16 * nnod * n + 16);
@@ -321,6 +322,8 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   B.q = (const int*)(base + o_q); B.l = (const int*)(base + o_l); B.omre = (const double*)(base + o_omre);
   B.nkp1 = m.nk + 1; B.tables = (const GmKnot*)(base + o_tables);
   B.ltab = (const unsigned char*)(base + o_ltab); B.lstride = (int)lstride;
+  B.ttab = (const unsigned char*)(base + o_ttab); B.tstride = (int)tstride;
+  B.tor_split = 4;
   B.status = (int*)(base + o_status); B.rs = (double*)(base + o_rs);
   B.ut = (double*)(base + o_ut); B.dt = (double*)(base + o_dt); B.dett = (double*)(base + o_dett);
   B.us = (double*)(base + o_us); B.ds = (double*)(base + o_ds); B.ulg = (double*)(base + o_ulg);

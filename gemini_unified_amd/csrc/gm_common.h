@@ -59,19 +59,31 @@ static_assert(sizeof(GmKnot) == 272, "GmKnot record size");
 // times into one CU's LDS.  Moduli 6 and 7 (kappa, mu: start radius and
 // initial values only) are read from the full table in global memory.
 struct GmKnotL {
+  static constexpr int kFirst = 0, kCount = 5;   // moduli held: kFirst .. kFirst + kCount - 1 (0-based)
   double ro, ro2, rh;
   double er[5], ei[5], e2r[5], e2i[5];
 };
 static_assert(sizeof(GmKnotL) == 184, "GmKnotL record size");
+// The toroidal kernel's record: density and the moduli 4, 5 (L, N) only.
+// 88 bytes (22 banks apart): a 200-knot image of 19 KB lets eight one-wave
+// blocks, two per SIMD, share a CU.
+struct GmKnotT {
+  static constexpr int kFirst = 3, kCount = 2;
+  double ro, ro2, rh;
+  double er[2], ei[2], e2r[2], e2i[2];
+};
+static_assert(sizeof(GmKnotT) == 88, "GmKnotT record size");
 
 // LDS image of one frequency's table, staged by every block of the minors
 // kernels (and laid out the same in global memory, one image per frequency):
-//   GmKnotL rec[nk+1] | double rk[nk+1] | double drk[nlay+1] | int iktop[nlay+1]
+//   R rec[nk+1] | double rk[nk+1] | double drk[nlay+1] | int iktop[nlay+1]
 // so that the knot lookup of a spline evaluation touches LDS only.
-__host__ __device__ inline size_t gmLdsBytes(int nk, int nlay) {
-  size_t b = (size_t)(nk + 1) * (sizeof(GmKnotL) + sizeof(double)) + (size_t)(nlay + 1) * (sizeof(double) + sizeof(int));
+template <class R>
+__host__ __device__ inline size_t gmImageBytes(int nk, int nlay) {
+  size_t b = (size_t)(nk + 1) * (sizeof(R) + sizeof(double)) + (size_t)(nlay + 1) * (sizeof(double) + sizeof(int));
   return (b + 15) & ~(size_t)15;
 }
+__host__ __device__ inline size_t gmLdsBytes(int nk, int nlay) { return gmImageBytes<GmKnotL>(nk, nlay); }
 
 // one batch of (omega, l) instances and its device buffers
 struct GmBatch {
@@ -84,8 +96,11 @@ struct GmBatch {
   const double* __restrict__ omre; // per frequency slot
   int nkp1;                        // table stride (nk + 1)
   const GmKnot* __restrict__ tables;
-  const unsigned char* __restrict__ ltab;  // LDS images, one per frequency slot, lstride bytes apart
+  const unsigned char* __restrict__ ltab;  // LDS images (GmKnotL), one per frequency slot, lstride bytes apart
   int lstride;
+  const unsigned char* __restrict__ ttab;  // toroidal LDS images (GmKnotT), tstride bytes apart
+  int tstride;
+  int tor_split;                           // blocks per work unit of the toroidal kernel
   int* __restrict__ status;
   double* __restrict__ rs;         // [2][n]: toroidal, spheroidal start radius
   double* __restrict__ ut;         // toroidal up   [nnod][2][n] complex (re,im interleaved)
