@@ -45,8 +45,9 @@ METRIC = "minor-ODE (ω,l) solves/s, learned vs plain step control at equal rel-
 FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 vector peak
 BF16_PEAK_TFLOPS = 2500.0    # MI355X dense bf16 MFMA peak
 HBM_PEAK_GBS = 8000.0
-EPS_LADDER = [1e-6, 3e-7, 1e-7, 5e-8, 3e-8, 2e-8, 1.4e-8, 1e-8, 7e-9, 5e-9, 3.5e-9, 2.5e-9, 1.8e-9, 1.3e-9,
-              1e-9, 5e-10]
+# both controllers are timed at the loosest eps of this ladder (12 rungs per decade, 1e-6 .. 5e-10)
+# whose sampled spectrum error is within ERR_TARGET
+EPS_LADDER = [float(f"{10 ** (-6 - k / 12):.3g}") for k in range(40)]
 ERR_TARGET = 1e-6
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_spheroidal_traffic.txt")
 
@@ -350,8 +351,8 @@ def main():
     ap.add_argument("--workload", choices=["c1", "c2"], default="c2")
     ap.add_argument("--variant", choices=["exact", "fast"], default="fast")
     ap.add_argument("--eps", type=float, default=None, help="skip the error ladder: time plain (and learned) at eps")
-    ap.add_argument("--safety", type=float, default=0.7)
-    ap.add_argument("--propose-every", type=int, default=16)
+    ap.add_argument("--safety", type=float, default=0.8)
+    ap.add_argument("--propose-every", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
