@@ -80,8 +80,9 @@ int gm_last_error(gm_ctx* ctx, char* buf, int len);
  * Replaces createNodeEarthmodel's result (f90/nodeEarthmodel.f90:98-257).
  * Limits: nk <= 512 knots, nlay <= 32 layers, and the per-frequency spline
  * image that every block stages in LDS, (nk+1)*192 + (nlay+1)*12 bytes, must
- * fit the device's LDS per block (GM_ERR_UNSUPPORTED otherwise: 850 knots on
- * MI355X's 160 KB).  Up to about 205 knots, four spheroidal blocks share a CU. */
+ * fit the device's LDS per block (GM_ERR_UNSUPPORTED otherwise; at most 99 KB
+ * within these limits, so always on MI355X's 160 KB).  Up to about 205 knots,
+ * four spheroidal blocks share a CU. */
 int gm_model_set(gm_ctx* ctx, int nk, const double* rk, const double* ro, const double* vpv, const double* vph,
                  const double* vsv, const double* vsh, const double* eta, const double* qkinv, const double* qminv,
                  int nlay, const int* iktop, const double* rb, const int* iflso, int nic, int noc, int fsflag,
