@@ -102,3 +102,24 @@ def test_gfk_columns_match_oracle(engines, orc, c1_setup, f, lmin, nl):
     print(f"fast variant  f={f:.5f}: max rel-L2 {errf.max():.2e}, p99 {np.quantile(errf, 0.99):.2e}")
     assert np.quantile(errf, 0.99) < 1e-8
     assert errf.max() < 1e-5
+
+
+@pytest.mark.parametrize("variant,tol", [("exact", 1e-12), ("fast", 1e-9)])
+def test_large_model_gfk_matches_oracle(variant, tol):
+    """480 knots: the spline table image staged in LDS is 92 KB, above the
+    64 KB of the 200-knot workloads, so fewer blocks share a CU."""
+    from gemini_unified_amd import ExternalRadialNodes, global_model
+    nem = global_model(3, 480)
+    nodes = ExternalRadialNodes.from_blocks(nem, [1], [30000.0])
+    jre = nodes.node_index(nem.rearth)
+    f, ls = 40 / TLEN, 24
+    eng = GreenFKEngine(nem, nodes, variant=variant)
+    r = eng.gfk([f], 0, ls, SIGMA, EPS, jre, 1, want_fp32=False)
+    eng.close()
+    assert (r["status"] == 0).all()
+    orc = oracle.Oracle(nem, nodes)
+    orc.set_frequency(f, 3)
+    rc, ref, _ = orc.gfk_degrees(2.0 * MC_PI * f, -SIGMA, 0, ls, jre, 1, EPS)
+    assert rc == 0
+    err = rel_l2(r["gf"], ref)
+    assert err.max() < tol, err.max()
