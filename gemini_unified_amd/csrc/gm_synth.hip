@@ -46,6 +46,8 @@ constexpr int kRows = 8;       // basis rows
 constexpr int kSums = 14;      // moment-tensor degree sums
 constexpr int kPieces = 3;     // bf16 pieces per value
 constexpr int TM = 128, TN = 64, TK = 32, LDSK = 40;   // tile sizes; LDS row pitch (bf16)
+constexpr int kXcd = 8;        // XCDs of the MI355X the contraction's tile order is laid out for
+constexpr int kStrip = 8;      // row tiles per strip of that order
 constexpr int kMaxFFT = 8192;  // longest time series (ns/2 complex FP64 = 64 KB of LDS)
 
 // sum j: basis row, spectrum index (offset of (component, jump) in the GFK
@@ -234,10 +236,23 @@ __global__ void __launch_bounds__(256) k_contract(int Rp, int Np, int Lp, int ns
                                                   const __bf16* __restrict__ Gs, double* __restrict__ H, int regional) {
   __shared__ __attribute__((aligned(16))) __bf16 sA[kPieces][TM][LDSK];
   __shared__ __attribute__((aligned(16))) __bf16 sB[kPieces][TN][LDSK];
-  const int j = blockIdx.z;
+  // XCD-aware tile order.  The hardware deals blocks to the kXcd XCDs
+  // round-robin by block index, so block b runs on XCD b % kXcd as the
+  // (b / kXcd)-th of its XCD.  Each XCD is given one contiguous range of the
+  // (sum, tile) list, ordered in strips of kStrip row tiles with the row
+  // tile fastest: blocks that run together on one XCD then share column
+  // tiles of the spectra and row tiles of the basis in that XCD's L2.
+  const int gx = Np / TN, gy = Rp / TM, ntiles = gx * gy, nwork = kSums * ntiles;
+  const int b = blockIdx.x, xcd = b % kXcd, sq = b / kXcd;
+  const int per = (nwork + kXcd - 1) / kXcd, rem = nwork - (per - 1) * kXcd;   // rem XCDs get per items
+  const int wk = xcd < rem ? xcd * per + sq : rem * per + (xcd - rem) * (per - 1) + sq;
+  const int j = wk / ntiles, tile = wk % ntiles;
+  const int strip = tile / (kStrip * gx), sidx = tile % (kStrip * gx);
+  const int sh = min(kStrip, gy - strip * kStrip);
+  const int ty = strip * kStrip + sidx % sh, tx = sidx / sh;
   const int rowb = c_row[j], spec = c_spec[j];
   const double fac = regional ? c_fac_wn[j] : c_fac[j];
-  const int r0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+  const int r0 = ty * TM, n0 = tx * TN;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wr = (w >> 1) * 64, wc = (w & 1) * 32;
   const size_t planeA = (size_t)Rp * Lp, planeG = (size_t)Np * Lp;
@@ -664,7 +679,7 @@ int run_sums(gm_ctx* c, const SynPlan& P, const double* theta, const float* gr, 
                      (const int*)(base + o.nwn), (const long long*)(base + o.koff), P.nkf, dgr, dgi, 0.2, dwn,
                      (__bf16*)(base + o.Gs));
   HIPCHK(c, hipEventRecord(c->sev[2], s));
-  hipLaunchKernelGGL(k_contract, dim3(P.Np / TN, P.Rp / TM, kSums), dim3(256), 0, s, P.Rp, P.Np, P.Lp, P.nsp,
+  hipLaunchKernelGGL(k_contract, dim3((P.Np / TN) * (P.Rp / TM) * kSums), dim3(256), 0, s, P.Rp, P.Np, P.Lp, P.nsp,
                      (const __bf16*)(base + o.Bs), (const __bf16*)(base + o.Gs), (double*)(base + o.H),
                      regional ? 1 : 0);
   if (regional)
