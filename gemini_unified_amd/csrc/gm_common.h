@@ -135,4 +135,5 @@ struct GmBatch {
 #define GM_LOGREC 20
 // work counters (GmBatch::unit_counter): persistent toroidal / spheroidal
 // kernels, and the block claims of the one-lane-per-job kernels
-enum { GM_CNT_TOR_PERSIST = 0, GM_CNT_SPH_PERSIST = 1, GM_CNT_TOR = 2, GM_CNT_SPH = 3, GM_NCOUNTERS = 4 };
+enum { GM_CNT_TOR_PERSIST = 0, GM_CNT_SPH_PERSIST = 1, GM_CNT_TOR = 2, GM_CNT_SPH = 3, GM_CNT_ADJ = 4,
+       GM_NCOUNTERS = 5 };
