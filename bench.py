@@ -228,6 +228,22 @@ def synth_report(grid, nrcv=4096, nsamp=8192, seed=3):
     return out
 
 
+def gfk_report(eng, grid, jre, eps, ninst):
+    """The whole drop-in batch gm_gfk_batch on the main workload: minors, adjoint, unit-jump Green
+    functions, reciprocity, source terms and FP32 packing, outputs left in HBM (plain control)."""
+    eng.run_gfk_resident(grid.freqs, 0, grid.nwn, grid.sigma, eps, jre, 1)
+    t0 = time.perf_counter()
+    st = eng.run_gfk_resident(grid.freqs, 0, grid.nwn, grid.sigma, eps, jre, 1)
+    dt = time.perf_counter() - t0
+    tm = eng.timing()
+    out = {"columns_per_s": round(ninst / dt, 1), "ms_per_batch": round(dt * 1e3, 1), "eps": eps,
+           "stage_ms": {"toroidal": round(tm[2], 1), "spheroidal": round(tm[3], 1),
+                        "adjoint_and_assembly": round(tm[4], 1)},
+           "propagations": int(st[0])}
+    log(f"gfk columns: {out['columns_per_s']:.0f} columns/s ({out['ms_per_batch']} ms per batch), {out['stage_ms']}")
+    return out
+
+
 def c5_report(variant, nrcv=1024, seed=7):
     """Regional (Bessel) branch: minors over the whole regional grid and the wavenumber-integral
     synthetics of nrcv receivers at 1-200 km on spectra of the same shape."""
@@ -359,6 +375,7 @@ def main():
     ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--no-synth", action="store_true")
     ap.add_argument("--no-c5", action="store_true")
+    ap.add_argument("--no-gfk", action="store_true")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -410,6 +427,8 @@ def main():
              "work": {k: int(v) for k, v in zip(["propagations", "accepted_steps", "rejected_steps", "mmp_sequences",
                                                  "matvecs", "sysmat_evals", "neville_cells", "flops"],
                                                 stats_p[:8])}}
+
+    gfk = None if args.no_gfk else gfk_report(eng, grid, jre, eps_p, ninst)
 
     # 3. learned control
     learned = None
@@ -465,6 +484,7 @@ def main():
         "learned": learned,
         "c3_learned_proposer": c3,
         "synthetics": synth,
+        "gfk_columns": gfk,
         "c5_regional": c5,
         "roofline": roofline,
         "cpu_baseline": cpu,
