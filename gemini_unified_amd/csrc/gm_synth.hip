@@ -4,7 +4,7 @@
 //
 // Pipeline of gm_synthetics (one source node, R receivers, nf frequencies):
 //   k_twiddle   e^{+2 pi i k / ns}, k < ns/2, FP64 (pi as in the reference)
-//   k_legendre  per receiver: P_l^m, dP/dtheta, P/sin (m = 0..2) by the FP64
+//   k_legendre  per (receiver, m): P_l^m, dP/dtheta, P/sin (m = 0..2) by the FP64
 //               upward recurrence, times (2l+1)/(4 pi) and 2/(l(l+1)), as
 //               8 basis rows; each value is split into three bf16 pieces
 //               (hi, mid, lo: 24 significant bits).  Layout [piece][row][Rp][Lp].
@@ -81,15 +81,18 @@ __global__ void k_twiddle(int ns, double* __restrict__ tw) {
   tw[2 * k + 1] = s;
 }
 
-// One receiver per thread.  mode 0: basis pieces Bs[piece][row][Rp][Lp] for
-// degrees 0..Lp-1 (zero beyond lmax); mode 1: FP64 arrays [R][3][lmax+1] of
-// plm, dpdt, pdst, d2pdt2, dtpdst (computeLegendreFunctions,
-// legendreFunctions.f90:48-157, same operation order).
+// One (receiver, order m) per thread, m = blockIdx.y: the three recurrences in
+// l (m = 0, 1, 2) are independent, so they run on three lanes.  mode 0: basis
+// pieces Bs[piece][row][Rp][Lp] of the rows of order m (rows 0, 3 for m = 0;
+// 1, 4, 6 for m = 1; 2, 5, 7 for m = 2) for degrees 0..Lp-1 (zero beyond
+// lmax); mode 1: FP64 arrays [R][3][lmax+1] of plm, dpdt, pdst, d2pdt2, dtpdst
+// (computeLegendreFunctions, legendreFunctions.f90:48-157, same operation order).
 __global__ void __launch_bounds__(64) k_legendre(int R, int Rp, int lmax, int Lp, const double* __restrict__ theta,
                                                  __bf16* __restrict__ Bs, double* __restrict__ o_plm,
                                                  double* __restrict__ o_dpdt, double* __restrict__ o_pdst,
                                                  double* __restrict__ o_d2, double* __restrict__ o_ds, int mode) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = blockIdx.y;
   if (r >= (mode == 0 ? Rp : R)) return;
   const size_t plane = (size_t)Rp * Lp;
   const int L1 = lmax + 1;
@@ -100,63 +103,60 @@ __global__ void __launch_bounds__(64) k_legendre(int R, int Rp, int lmax, int Lp
   const double rst = poles ? 0.0 : 1.0 / st;
   const double sg = ct < 0.0 ? -1.0 : 1.0;
   const double fourpi = 4.0 * kPi;
-  const double pmm[3] = {1.0, (1.0 * 1.0) * st, ((1.0 * 1.0) * st * 3.0) * st};
-  double p1[3] = {0.0, 0.0, 0.0}, p2[3] = {0.0, 0.0, 0.0};
-  __bf16 buf[kPieces][kRows][8];
+  const double pmm = m == 0 ? 1.0 : (m == 1 ? (1.0 * 1.0) * st : ((1.0 * 1.0) * st * 3.0) * st);
+  // basis rows of this order and how many
+  const int nrow = m == 0 ? 2 : 3;
+  const int row0 = m, row1 = m + 3, row2 = m + 5;
+  double p1 = 0.0, p2 = 0.0;
+  __bf16 buf[kPieces][3][8];
   for (int l = 0; l < (mode == 0 ? Lp : L1); ++l) {
-    double P[3], D[3], S[3], D2[3], DS[3];
     const double ll1 = (double)(l * (l + 1));
     const double sl = (l & 1) ? sg : 1.0, sl1 = (l & 1) ? 1.0 : sg;
     const double q = (double)(l + 2) * (l + 1) * l * (l - 1);
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      double p;
-      if (l > lmax || l < m) p = 0.0;
-      else if (l == m) p = pmm[m];
-      else if (l == m + 1) p = ct * (double)(2 * m + 1) * pmm[m];
-      else p = (ct * (double)(2 * l - 1) * p1[m] - (double)(l + m - 1) * p2[m]) / (double)(l - m);
-      double d = 0.0, s = 0.0, d2 = 0.0, ds = 0.0;
-      if (l <= lmax) {
-        if (!poles) {
-          if (l == m) d = (double)m * ct * p * rst;
-          else if (l > m) d = ((double)l * ct * p - (double)(l + m) * p1[m]) * rst;
-          if (m >= 1 && l >= m) s = p * rst;
-          if (l >= m) d2 = -ct * rst * d + ((double)(m * m) * (rst * rst) - ll1) * p;
-          if (m >= 1 && l >= m) ds = rst * (d - ct * rst * p);
-        } else {
-          if (m == 1) d = 0.5 * ll1 * sl;
-          if (m == 1 && l >= 1) s = 0.5 * ll1 * sl1;
-          if (m == 0) d2 = 0.5 * ll1 * sl;
-          if (m == 2) d2 = 0.25 * q * sl;
-          if (m == 2 && l >= 1) ds = 0.125 * q * sl1;
-        }
+    double p;
+    if (l > lmax || l < m) p = 0.0;
+    else if (l == m) p = pmm;
+    else if (l == m + 1) p = ct * (double)(2 * m + 1) * pmm;
+    else p = (ct * (double)(2 * l - 1) * p1 - (double)(l + m - 1) * p2) / (double)(l - m);
+    double d = 0.0, sv = 0.0, d2 = 0.0, ds = 0.0;
+    if (l <= lmax) {
+      if (!poles) {
+        if (l == m) d = (double)m * ct * p * rst;
+        else if (l > m) d = ((double)l * ct * p - (double)(l + m) * p1) * rst;
+        if (m >= 1 && l >= m) sv = p * rst;
+        if (l >= m) d2 = -ct * rst * d + ((double)(m * m) * (rst * rst) - ll1) * p;
+        if (m >= 1 && l >= m) ds = rst * (d - ct * rst * p);
+      } else {
+        if (m == 1) d = 0.5 * ll1 * sl;
+        if (m == 1 && l >= 1) sv = 0.5 * ll1 * sl1;
+        if (m == 0) d2 = 0.5 * ll1 * sl;
+        if (m == 2) d2 = 0.25 * q * sl;
+        if (m == 2 && l >= 1) ds = 0.125 * q * sl1;
       }
-      P[m] = p; D[m] = d; S[m] = s; D2[m] = d2; DS[m] = ds;
-      p2[m] = p1[m];
-      p1[m] = p;
     }
+    p2 = p1;
+    p1 = p;
     if (mode == 1) {
-#pragma unroll
-      for (int m = 0; m < 3; ++m) {
-        const size_t o = ((size_t)r * 3 + m) * L1 + l;
-        o_plm[o] = P[m]; o_dpdt[o] = D[m]; o_pdst[o] = S[m]; o_d2[o] = D2[m]; o_ds[o] = DS[m];
-      }
+      const size_t o = ((size_t)r * 3 + m) * L1 + l;
+      o_plm[o] = p; o_dpdt[o] = d; o_pdst[o] = sv; o_d2[o] = d2; o_ds[o] = ds;
       continue;
     }
-    double b[kRows];
+    double b[3];
     const double w = (2.0 * l + 1.0) / fourpi;
     const double g = l >= 1 ? 2.0 / ll1 : 0.0;
-    b[0] = P[0] * w;
-    b[1] = l >= 1 ? P[1] * g * w : 0.0;
-    b[2] = l >= 2 ? P[2] * g * w : 0.0;
-    b[3] = l >= 1 ? D[0] * w : 0.0;
-    b[4] = l >= 1 ? D[1] * g * w : 0.0;
-    b[5] = l >= 2 ? D[2] * g * w : 0.0;
-    b[6] = l >= 1 ? S[1] * g * w : 0.0;
-    b[7] = l >= 2 ? S[2] * g * w : 0.0;
+    if (m == 0) {
+      b[0] = p * w;
+      b[1] = l >= 1 ? d * w : 0.0;
+      b[2] = 0.0;
+    } else {
+      const bool on = l >= m;
+      b[0] = on ? p * g * w : 0.0;
+      b[1] = on ? d * g * w : 0.0;
+      b[2] = on ? sv * g * w : 0.0;
+    }
     const int e = l & 7;
 #pragma unroll
-    for (int k = 0; k < kRows; ++k) {
+    for (int k = 0; k < 3; ++k) {
       double v = (pad || l > lmax) ? 0.0 : b[k];
       split3(v, buf[0][k][e], buf[1][k][e], buf[2][k][e]);
     }
@@ -164,11 +164,13 @@ __global__ void __launch_bounds__(64) k_legendre(int R, int Rp, int lmax, int Lp
 #pragma unroll
       for (int pc = 0; pc < kPieces; ++pc)
 #pragma unroll
-        for (int k = 0; k < kRows; ++k) {
+        for (int k = 0; k < 3; ++k) {
+          if (k >= nrow) continue;
+          const int row = k == 0 ? row0 : (k == 1 ? row1 : row2);
           bf16x8 v;
 #pragma unroll
           for (int i = 0; i < 8; ++i) v[i] = buf[pc][k][i];
-          *(bf16x8*)(Bs + ((size_t)pc * kRows + k) * plane + (size_t)r * Lp + (l - 7)) = v;
+          *(bf16x8*)(Bs + ((size_t)pc * kRows + row) * plane + (size_t)r * Lp + (l - 7)) = v;
         }
     }
   }
@@ -670,7 +672,7 @@ int run_sums(gm_ctx* c, const SynPlan& P, const double* theta, const float* gr, 
     hipLaunchKernelGGL(k_bessel_basis, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, P.R, P.Rp, P.Lp, dth, dwn,
                        re, (__bf16*)(base + o.Bs));
   } else {
-    hipLaunchKernelGGL(k_legendre, dim3((P.Rp + 63) / 64), dim3(64), 0, s, P.R, P.Rp, P.L1 - 1, P.Lp, dth,
+    hipLaunchKernelGGL(k_legendre, dim3((P.Rp + 63) / 64, 3), dim3(64), 0, s, P.R, P.Rp, P.L1 - 1, P.Lp, dth,
                        (__bf16*)(base + o.Bs), nullptr, nullptr, nullptr, nullptr, nullptr, 0);
   }
   HIPCHK(c, hipEventRecord(c->sev[1], s));
@@ -723,7 +725,7 @@ extern "C" int gm_legendre_batch(gm_ctx* c, int nrcv, const double* theta, int l
   char* base = (char*)c->sbuf;
   hipStream_t s = c->stream;
   HIPCHK(c, hipMemcpyAsync(base + ot, theta, sizeof(double) * nrcv, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_legendre, dim3((nrcv + 63) / 64), dim3(64), 0, s, nrcv, nrcv, lmax, 0,
+  hipLaunchKernelGGL(k_legendre, dim3((nrcv + 63) / 64, 3), dim3(64), 0, s, nrcv, nrcv, lmax, 0,
                      (const double*)(base + ot), nullptr, (double*)(base + oo[0]), (double*)(base + oo[1]),
                      (double*)(base + oo[2]), (double*)(base + oo[3]), (double*)(base + oo[4]), 1);
   HIPCHK(c, hipGetLastError());
