@@ -234,7 +234,7 @@ __global__ void k_gprep(int nsp, int nf, int Np, int Lp, const int* __restrict__
 // H[j][Rp][Np] (FP64) = c_fac[j] * sum over degrees of basis row c_row[j]
 // times spectrum c_spec[j] (the j-th moment-tensor sum); grid (Np/TN, Rp/TM,
 // 14), 256 threads: 4 waves of 64 x 32.
-__global__ void __launch_bounds__(256) k_contract(int Rp, int Np, int Lp, int nsp, const __bf16* __restrict__ Bs,
+__global__ void __launch_bounds__(256, 2) k_contract(int Rp, int Np, int Lp, int nsp, const __bf16* __restrict__ Bs,
                                                   const __bf16* __restrict__ Gs, double* __restrict__ H, int regional) {
   __shared__ __attribute__((aligned(16))) __bf16 sA[kPieces][TM][LDSK];
   __shared__ __attribute__((aligned(16))) __bf16 sB[kPieces][TN][LDSK];
@@ -258,32 +258,33 @@ __global__ void __launch_bounds__(256) k_contract(int Rp, int Np, int Lp, int ns
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wr = (w >> 1) * 64, wc = (w & 1) * 32;
   const size_t planeA = (size_t)Rp * Lp, planeG = (size_t)Np * Lp;
-  // global -> register staging: A: 3 pieces x 128 rows x 4 groups of 8 degrees; B: 3 x 64 x 4
-  uint4 ra[6], rb[3];
-  auto gload = [&](int k0) {
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-      const int id = t + 256 * i, pc = id >> 9, rem = id & 511, row = rem >> 2, g = rem & 3;
-      ra[i] = *(const uint4*)(Bs + ((size_t)pc * kRows + rowb) * planeA + (size_t)(r0 + row) * Lp + k0 + 8 * g);
-    }
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      const int id = t + 256 * i, pc = id >> 8, rem = id & 255, row = rem >> 2, g = rem & 3;
-      rb[i] = *(const uint4*)(Gs + ((size_t)pc * nsp + spec) * planeG + (size_t)(n0 + row) * Lp + k0 + 8 * g);
-    }
-  };
-  auto lstore = [&]() {
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-      const int id = t + 256 * i, pc = id >> 9, rem = id & 511, row = rem >> 2, g = rem & 3;
-      *(uint4*)&sA[pc][row][8 * g] = ra[i];
-    }
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      const int id = t + 256 * i, pc = id >> 8, rem = id & 255, row = rem >> 2, g = rem & 3;
-      *(uint4*)&sB[pc][row][8 * g] = rb[i];
-    }
-  };
+  // global -> register staging: A: 3 pieces x 128 rows x 4 groups of 8 degrees; B: 3 x 64 x 4.
+  // Written out element by element with literal indices: as loops (even with
+  // unroll pragmas) the staging registers were indexed at run time and the
+  // compiler kept them in scratch.
+  uint4 ra0, ra1, ra2, ra3, ra4, ra5, rb0, rb1, rb2;
+#define GM_A_SRC(i, k0)                                                                                        \
+  (Bs + ((size_t)((t + 256 * (i)) >> 9) * kRows + rowb) * planeA +                                           \
+   (size_t)(r0 + (((t + 256 * (i)) & 511) >> 2)) * Lp + (k0) + 8 * ((t + 256 * (i)) & 3))
+#define GM_B_SRC(i, k0)                                                                                        \
+  (Gs + ((size_t)((t + 256 * (i)) >> 8) * nsp + spec) * planeG +                                             \
+   (size_t)(n0 + (((t + 256 * (i)) & 255) >> 2)) * Lp + (k0) + 8 * ((t + 256 * (i)) & 3))
+#define GM_A_DST(i) (&sA[(t + 256 * (i)) >> 9][((t + 256 * (i)) & 511) >> 2][8 * ((t + 256 * (i)) & 3)])
+#define GM_B_DST(i) (&sB[(t + 256 * (i)) >> 8][((t + 256 * (i)) & 255) >> 2][8 * ((t + 256 * (i)) & 3)])
+#define GM_GLOAD(k0)                                                                                           \
+  do {                                                                                                         \
+    ra0 = *(const uint4*)GM_A_SRC(0, k0); ra1 = *(const uint4*)GM_A_SRC(1, k0);                                \
+    ra2 = *(const uint4*)GM_A_SRC(2, k0); ra3 = *(const uint4*)GM_A_SRC(3, k0);                                \
+    ra4 = *(const uint4*)GM_A_SRC(4, k0); ra5 = *(const uint4*)GM_A_SRC(5, k0);                                \
+    rb0 = *(const uint4*)GM_B_SRC(0, k0); rb1 = *(const uint4*)GM_B_SRC(1, k0);                                \
+    rb2 = *(const uint4*)GM_B_SRC(2, k0);                                                                      \
+  } while (0)
+#define GM_LSTORE()                                                                                            \
+  do {                                                                                                         \
+    *(uint4*)GM_A_DST(0) = ra0; *(uint4*)GM_A_DST(1) = ra1; *(uint4*)GM_A_DST(2) = ra2;                        \
+    *(uint4*)GM_A_DST(3) = ra3; *(uint4*)GM_A_DST(4) = ra4; *(uint4*)GM_A_DST(5) = ra5;                        \
+    *(uint4*)GM_B_DST(0) = rb0; *(uint4*)GM_B_DST(1) = rb1; *(uint4*)GM_B_DST(2) = rb2;                        \
+  } while (0)
   floatx4 acc[4][2];
   double sum[4][2][4];
 #pragma unroll
@@ -295,11 +296,11 @@ __global__ void __launch_bounds__(256) k_contract(int Rp, int Np, int Lp, int ns
       for (int e = 0; e < 4; ++e) sum[a][b][e] = 0.0;
     }
   const int fr = lane & 15, fk = (lane >> 4) * 8;
-  gload(0);
+  GM_GLOAD(0);
   for (int k0 = 0; k0 < Lp; k0 += TK) {
-    lstore();
+    GM_LSTORE();
     __syncthreads();
-    if (k0 + TK < Lp) gload(k0 + TK);
+    if (k0 + TK < Lp) GM_GLOAD(k0 + TK);
     bf16x8 bq[2][kPieces];
 #pragma unroll
     for (int b = 0; b < 2; ++b)
@@ -344,6 +345,12 @@ __global__ void __launch_bounds__(256) k_contract(int Rp, int Np, int Lp, int ns
         H[((size_t)j * Rp + row) * Np + col] = sum[a][b][e] * fac;
       }
 }
+#undef GM_GLOAD
+#undef GM_LSTORE
+#undef GM_A_SRC
+#undef GM_B_SRC
+#undef GM_A_DST
+#undef GM_B_DST
 
 struct cd { double re, im; };
 __device__ __forceinline__ cd cmul(cd a, cd b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
