@@ -108,17 +108,16 @@ __global__ void __launch_bounds__(64) k_legendre(int R, int Rp, int lmax, int Lp
   const int nrow = m == 0 ? 2 : 3;
   const int row0 = m, row1 = m + 3, row2 = m + 5;
   double p1 = 0.0, p2 = 0.0;
-  __bf16 buf[kPieces][3][8];
-  for (int l = 0; l < (mode == 0 ? Lp : L1); ++l) {
+  // one degree of the recurrence (values of this order m at degree l)
+  auto step = [&](int l, double& p, double& d, double& sv, double& d2, double& ds) {
     const double ll1 = (double)(l * (l + 1));
     const double sl = (l & 1) ? sg : 1.0, sl1 = (l & 1) ? 1.0 : sg;
     const double q = (double)(l + 2) * (l + 1) * l * (l - 1);
-    double p;
     if (l > lmax || l < m) p = 0.0;
     else if (l == m) p = pmm;
     else if (l == m + 1) p = ct * (double)(2 * m + 1) * pmm;
     else p = (ct * (double)(2 * l - 1) * p1 - (double)(l + m - 1) * p2) / (double)(l - m);
-    double d = 0.0, sv = 0.0, d2 = 0.0, ds = 0.0;
+    d = 0.0; sv = 0.0; d2 = 0.0; ds = 0.0;
     if (l <= lmax) {
       if (!poles) {
         if (l == m) d = (double)m * ct * p * rst;
@@ -136,43 +135,55 @@ __global__ void __launch_bounds__(64) k_legendre(int R, int Rp, int lmax, int Lp
     }
     p2 = p1;
     p1 = p;
-    if (mode == 1) {
+  };
+  if (mode == 1) {
+    for (int l = 0; l < L1; ++l) {
+      double p, d, sv, d2, ds;
+      step(l, p, d, sv, d2, ds);
       const size_t o = ((size_t)r * 3 + m) * L1 + l;
       o_plm[o] = p; o_dpdt[o] = d; o_pdst[o] = sv; o_d2[o] = d2; o_ds[o] = ds;
-      continue;
     }
-    double b[3];
-    const double w = (2.0 * l + 1.0) / fourpi;
-    const double g = l >= 1 ? 2.0 / ll1 : 0.0;
-    if (m == 0) {
-      b[0] = p * w;
-      b[1] = l >= 1 ? d * w : 0.0;
-      b[2] = 0.0;
-    } else {
-      const bool on = l >= m;
-      b[0] = on ? p * g * w : 0.0;
-      b[1] = on ? d * g * w : 0.0;
-      b[2] = on ? sv * g * w : 0.0;
+    return;
+  }
+  // mode 0: groups of 8 degrees (Lp is a multiple of 8), each row's 8 values
+  // split into bf16 pieces in registers and stored as one 16-byte vector
+  for (int l0 = 0; l0 < Lp; l0 += 8) {
+    bf16x8 v[kPieces][3];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int l = l0 + e;
+      double p, d, sv, d2, ds;
+      step(l, p, d, sv, d2, ds);
+      const double ll1 = (double)(l * (l + 1));
+      double b[3];
+      const double w = (2.0 * l + 1.0) / fourpi;
+      const double g = l >= 1 ? 2.0 / ll1 : 0.0;
+      if (m == 0) {
+        b[0] = p * w;
+        b[1] = l >= 1 ? d * w : 0.0;
+        b[2] = 0.0;
+      } else {
+        const bool on = l >= m;
+        b[0] = on ? p * g * w : 0.0;
+        b[1] = on ? d * g * w : 0.0;
+        b[2] = on ? sv * g * w : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double x = (pad || l > lmax) ? 0.0 : b[k];
+        __bf16 h0, h1, h2;
+        split3(x, h0, h1, h2);
+        v[0][k][e] = h0; v[1][k][e] = h1; v[2][k][e] = h2;
+      }
     }
-    const int e = l & 7;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      double v = (pad || l > lmax) ? 0.0 : b[k];
-      split3(v, buf[0][k][e], buf[1][k][e], buf[2][k][e]);
-    }
-    if (e == 7) {   // 8 degrees done: one 16-byte store per (piece, row)
+    for (int pc = 0; pc < kPieces; ++pc)
 #pragma unroll
-      for (int pc = 0; pc < kPieces; ++pc)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          if (k >= nrow) continue;
-          const int row = k == 0 ? row0 : (k == 1 ? row1 : row2);
-          bf16x8 v;
-#pragma unroll
-          for (int i = 0; i < 8; ++i) v[i] = buf[pc][k][i];
-          *(bf16x8*)(Bs + ((size_t)pc * kRows + row) * plane + (size_t)r * Lp + (l - 7)) = v;
-        }
-    }
+      for (int k = 0; k < 3; ++k) {
+        if (k >= nrow) continue;
+        const int row = k == 0 ? row0 : (k == 1 ? row1 : row2);
+        *(bf16x8*)(Bs + ((size_t)pc * kRows + row) * plane + (size_t)r * Lp + l0) = v[pc][k];
+      }
   }
 }
 
@@ -462,9 +473,12 @@ __global__ void k_wnfix(int R, int nf, int Rp, int Np, const int* __restrict__ n
     rowsAt(g.j3, w3);
   }
   const double scale = re * re / (2.0 * kPi);
-#pragma unroll 1
+  // fully unrolled: the kernel-row index is then a constant and the four
+  // 8-row tables stay in registers (a run-time index put them in scratch)
+  constexpr int kRowOf[kSums] = {0, 0, 1, 2, 3, 3, 4, 5, 6, 7, 4, 5, 6, 7};   // = c_row
+#pragma unroll
   for (int j = 0; j < kSums; ++j) {
-    const int row = c_row[j], s = c_spec[j];
+    const int row = kRowOf[j], s = c_spec[j];
     const double fac = c_fac_wn[j];
     const size_t o = ((size_t)j * Rp + r) * Np + 2 * f;
     cd z{H[o], H[o + 1]};
