@@ -18,17 +18,25 @@ struct GmProposer {
 };
 
 // One forward pass for one lane.  Every lane reads the same weight at the same
-// time (uniform address), so the loads go through the scalar data cache and the
-// FMAs take the weight as a scalar operand.  Hidden activations stay in VGPRs.
-__device__ __forceinline__ void gmMlpInfer(const float* __restrict__ P, const float* x, float* y) {
-  const float* W1 = P;
-  const float* b1 = W1 + GM_MLP_IN * GM_MLP_H;
-  const float* W2 = b1 + GM_MLP_H;
-  const float* b2 = W2 + GM_MLP_H * GM_MLP_H;
-  const float* W3 = b2 + GM_MLP_H;
-  const float* b3 = W3 + GM_MLP_H * GM_MLP_H;
-  const float* W4 = b3 + GM_MLP_H;
-  const float* b4 = W4 + GM_MLP_H * GM_MLP_OUT;
+// time, and no kernel writes the weights while it runs, so the pointer is made
+// wave-uniform and constant-address-space: the weights then come in through
+// scalar loads and the FMAs take them as scalar operands (a per-lane generic
+// pointer costs one vector load per four weights, about 2,300 per proposal).
+// Hidden activations stay in VGPRs.
+typedef const __attribute__((address_space(4))) float* GmConstF;
+__device__ __forceinline__ void gmMlpInfer(const float* __restrict__ Pg, const float* x, float* y) {
+  const unsigned long long pa = (unsigned long long)Pg;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)pa);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(pa >> 32));
+  GmConstF P = (GmConstF)((((unsigned long long)hi) << 32) | lo);
+  GmConstF W1 = P;
+  GmConstF b1 = W1 + GM_MLP_IN * GM_MLP_H;
+  GmConstF W2 = b1 + GM_MLP_H;
+  GmConstF b2 = W2 + GM_MLP_H * GM_MLP_H;
+  GmConstF W3 = b2 + GM_MLP_H;
+  GmConstF b3 = W3 + GM_MLP_H * GM_MLP_H;
+  GmConstF W4 = b3 + GM_MLP_H;
+  GmConstF b4 = W4 + GM_MLP_H * GM_MLP_OUT;
   float h1[GM_MLP_H], h2[GM_MLP_H];
 #pragma unroll
   for (int o = 0; o < GM_MLP_H; ++o) {
