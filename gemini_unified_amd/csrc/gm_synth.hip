@@ -1,6 +1,8 @@
-// Synthetics half of the Green-FK path on MI355X: Legendre basis, moment-tensor
-// harmonic degree sums as a split-bf16 MFMA contraction, the per-frequency
-// displacement combination, and the frequency-to-time transform.
+// Synthetics half of the Green-FK path on MI355X: Legendre basis, harmonic
+// degree sums of a moment-tensor (14 sums) or single-force (7 sums) source as
+// a split-bf16 MFMA contraction, the per-frequency displacement combination,
+// and the frequency-to-time transform.  The source type follows the number of
+// spectra per node: 10 (moment) or 5 (force).
 //
 // Pipeline of gm_synthetics (one source node, R receivers, nf frequencies):
 //   k_twiddle   e^{+2 pi i k / ns}, k < ns/2, FP64 (pi as in the reference)
@@ -10,14 +12,15 @@
 //               (hi, mid, lo: 24 significant bits).  Layout [piece][row][Rp][Lp].
 //   k_gprep     spectra G_s(l, f) * taper_f(l) in FP64, split into three bf16
 //               pieces.  Layout [piece][s][Np][Lp], column n = 2 f' (+1 imag).
-//   k_contract  the 14 sums S_j[r][n] = sum_l B_b(j)[r][l] G_s(j)[n][l] on the
+//   k_contract  the sums S_j[r][n] = sum_l B_b(j)[r][l] G_s(j)[n][l] on the
 //               matrix cores: v_mfma_f32_16x16x32_bf16 over the six piece
 //               products whose exponent sum keeps 24 bits (hi*hi, hi*mid,
 //               mid*hi, hi*lo, mid*mid, lo*hi).  Products of bf16 pieces are
 //               exact in FP32; each 32-degree chunk is accumulated in FP32 by
 //               the MFMA and then added to an FP64 running sum (compensation).
-//   k_combine   displacement (r, theta, phi) from the 14 sums and the moment
-//               tensor, time shift, optional velocity, optional ZNE rotation.
+//   k_combine   displacement (r, theta, phi) from the sums and the moment
+//               tensor or force, time shift, optional velocity, optional ZNE
+//               rotation.
 //   k_ifft      per trace: Hermitian spectrum -> real time series through an
 //               ns/2-point complex FFT in LDS (real-output split), times df
 //               e^{sigma t}, stored as FP32.
@@ -43,26 +46,31 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 // tapers, 2 pi f and the transform's twiddle angles as the reference does
 constexpr double kPi = 3.1415927410125732;
 constexpr int kRows = 8;       // basis rows
-constexpr int kSums = 14;      // moment-tensor degree sums
+constexpr int kSums = 14;      // moment-tensor degree sums (the larger set)
+constexpr int kForceSums = 7;  // single-force degree sums
 constexpr int kPieces = 3;     // bf16 pieces per value
 constexpr int TM = 128, TN = 64, TK = 32, LDSK = 40;   // tile sizes; LDS row pitch (bf16)
 constexpr int kXcd = 8;        // XCDs of the MI355X the contraction's tile order is laid out for
 constexpr int kStrip = 8;      // row tiles per strip of that order
 constexpr int kMaxFFT = 8192;  // longest time series (ns/2 complex FP64 = 64 KB of LDS)
 
-// sum j: basis row, spectrum index (offset of (component, jump) in the GFK
-// file for dsvmask (U, V, W)) and the power-of-two factor relating the sum's
-// weight to the basis row (harmonicDegreeSums.f90:251-307)
-__constant__ int c_row[kSums] = {0, 0, 1, 2, 3, 3, 4, 5, 6, 7, 4, 5, 6, 7};
-__constant__ int c_spec[kSums] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 8, 9, 6, 7};
-__constant__ double c_fac[kSums] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 1, -1, 1, -2};
-// regional mode: the 14 wavenumber-integral kernels map onto 8 distinct rows
-// with the same row and spectrum tables and these signs
-// (momentWavenumberIntegrals, wavenumberIntegrals.f90:353-373)
-__constant__ double c_fac_wn[kSums] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, -1, 1, -1};
+// Sum j of source type t (0 single force, kint 1..7; 1 moment tensor, kint
+// 1..14): basis row, spectrum index (offset of (component, jump) in the GFK
+// file for dsvmask (U, V, W): force U1 U2 V1 V2 W1, moment U1..U4 V1..V4 W1 W2)
+// and the factor relating the sum's weight to the basis row
+// (forceHarmonicDegreeSums, harmonicDegreeSums.f90:115-164, with the map at
+// :34-36; momentHarmonicDegreeSums, :251-307, with the map at :42-46)
+__constant__ int c_row[2][kSums] = {{0, 1, 3, 4, 6, 4, 6}, {0, 0, 1, 2, 3, 3, 4, 5, 6, 7, 4, 5, 6, 7}};
+__constant__ int c_spec[2][kSums] = {{0, 1, 2, 3, 4, 4, 3}, {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 8, 9, 6, 7}};
+__constant__ double c_fac[2][kSums] = {{1, -1, 1, -1, -1, 1, 1}, {1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 1, -1, 1, -2}};
+// regional mode: the wavenumber-integral kernels map onto the 8 distinct rows
+// of wnRows with the same row and spectrum tables and these signs
+// (forceWavenumberIntegrals, wavenumberIntegrals.f90:162-176;
+// momentWavenumberIntegrals, :353-373)
+__constant__ double c_fac_wn[2][kSums] = {{1, -1, 1, -1, -1, 1, 1}, {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, -1, 1, -1}};
 // first degree each spectrum enters a sum with (the sums' loop bounds); the V
 // spectra at l = 0 hold 0/0 from the reciprocity factor 1/dll1 and are never read
-__constant__ int c_lmin[10] = {0, 0, 1, 2, 1, 1, 1, 2, 1, 2};
+__constant__ int c_lmin[2][10] = {{0, 1, 1, 1, 1}, {0, 0, 1, 2, 1, 1, 1, 2, 1, 2}};
 
 __device__ __forceinline__ void split3(double v, __bf16& a, __bf16& b, __bf16& c) {
   a = (__bf16)(float)v;
@@ -204,7 +212,7 @@ __device__ __forceinline__ double taperWn(int l, int n, double dwn, double tapfr
 
 __global__ void k_gprep(int nsp, int nf, int Np, int Lp, const int* __restrict__ nwn, const long long* __restrict__ koff,
                         long long nkf, const float* __restrict__ gr, const float* __restrict__ gi, double tapfrac,
-                        double dwn, __bf16* __restrict__ Gs) {
+                        double dwn, int istyp, __bf16* __restrict__ Gs) {
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t per_s = (size_t)Np * Lp;
   if (idx >= per_s * nsp) return;
@@ -219,7 +227,7 @@ __global__ void k_gprep(int nsp, int nf, int Np, int Lp, const int* __restrict__
       const double gv = (n & 1) ? (double)gi[k] : (double)gr[k];
       v = gv * taperWn(l, nwn[f], dwn, tapfrac);
     }
-  } else if (f < nf && l < nwn[f] && l >= c_lmin[s]) {
+  } else if (f < nf && l < nwn[f] && l >= c_lmin[istyp][s]) {
     const long long k = (long long)s * nkf + koff[f] + l;
     const double gv = (n & 1) ? (double)gi[k] : (double)gr[k];
     const int lmax = nwn[f] - 1;
@@ -242,11 +250,12 @@ __global__ void k_gprep(int nsp, int nf, int Np, int Lp, const int* __restrict__
   Gs[idx + 2 * per_s * nsp] = c;
 }
 
-// H[j][Rp][Np] (FP64) = c_fac[j] * sum over degrees of basis row c_row[j]
-// times spectrum c_spec[j] (the j-th moment-tensor sum); grid (Np/TN, Rp/TM,
-// 14), 256 threads: 4 waves of 64 x 32.
+// H[j][Rp][Np] (FP64) = c_fac[t][j] * sum over degrees of basis row c_row[t][j]
+// times spectrum c_spec[t][j] (the j-th sum of source type t = istyp); one
+// block per (sum, tile) of Np/TN x Rp/TM tiles, 256 threads: 4 waves of 64 x 32.
 __global__ void __launch_bounds__(256, 2) k_contract(int Rp, int Np, int Lp, int nsp, const __bf16* __restrict__ Bs,
-                                                  const __bf16* __restrict__ Gs, double* __restrict__ H, int regional) {
+                                                  const __bf16* __restrict__ Gs, double* __restrict__ H, int regional,
+                                                  int istyp) {
   __shared__ __attribute__((aligned(16))) __bf16 sA[kPieces][TM][LDSK];
   __shared__ __attribute__((aligned(16))) __bf16 sB[kPieces][TN][LDSK];
   // XCD-aware tile order.  The hardware deals blocks to the kXcd XCDs
@@ -255,7 +264,8 @@ __global__ void __launch_bounds__(256, 2) k_contract(int Rp, int Np, int Lp, int
   // (sum, tile) list, ordered in strips of kStrip row tiles with the row
   // tile fastest: blocks that run together on one XCD then share column
   // tiles of the spectra and row tiles of the basis in that XCD's L2.
-  const int gx = Np / TN, gy = Rp / TM, ntiles = gx * gy, nwork = kSums * ntiles;
+  const int nsum = istyp ? kSums : kForceSums;
+  const int gx = Np / TN, gy = Rp / TM, ntiles = gx * gy, nwork = nsum * ntiles;
   const int b = blockIdx.x, xcd = b % kXcd, sq = b / kXcd;
   const int per = (nwork + kXcd - 1) / kXcd, rem = nwork - (per - 1) * kXcd;   // rem XCDs get per items
   const int wk = xcd < rem ? xcd * per + sq : rem * per + (xcd - rem) * (per - 1) + sq;
@@ -263,8 +273,8 @@ __global__ void __launch_bounds__(256, 2) k_contract(int Rp, int Np, int Lp, int
   const int strip = tile / (kStrip * gx), sidx = tile % (kStrip * gx);
   const int sh = min(kStrip, gy - strip * kStrip);
   const int ty = strip * kStrip + sidx % sh, tx = sidx / sh;
-  const int rowb = c_row[j], spec = c_spec[j];
-  const double fac = regional ? c_fac_wn[j] : c_fac[j];
+  const int rowb = c_row[istyp][j], spec = c_spec[istyp][j];
+  const double fac = regional ? c_fac_wn[istyp][j] : c_fac[istyp][j];
   const int r0 = ty * TM, n0 = tx * TN;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wr = (w >> 1) * 64, wc = (w & 1) * 32;
@@ -441,6 +451,8 @@ __global__ void k_bessel_basis(int R, int Rp, int Lp, const double* __restrict__
 //    terms (G(n) Y-kernel(k_n) - G(j3) Y-kernel(k_j3)) sF;
 //  - n == j3: trapezoid only, the point carries dwn/2 instead of dwn/2 + cF;
 //  - n < j3: trapezoid only, the last point carries dwn/2 instead of dwn.
+// ISTYP: 0 single force (7 sums), 1 moment tensor (14 sums).
+template <int ISTYP>
 __global__ void k_wnfix(int R, int nf, int Rp, int Np, const int* __restrict__ nwn, const long long* __restrict__ koff,
                         long long nkf, const float* __restrict__ gr, const float* __restrict__ gi,
                         const double* __restrict__ xr, double dwn, double re, double tapfrac, double* __restrict__ H) {
@@ -475,11 +487,12 @@ __global__ void k_wnfix(int R, int nf, int Rp, int Np, const int* __restrict__ n
   const double scale = re * re / (2.0 * kPi);
   // fully unrolled: the kernel-row index is then a constant and the four
   // 8-row tables stay in registers (a run-time index put them in scratch)
-  constexpr int kRowOf[kSums] = {0, 0, 1, 2, 3, 3, 4, 5, 6, 7, 4, 5, 6, 7};   // = c_row
+  constexpr int kN = ISTYP ? kSums : kForceSums;
+  constexpr int kRowOf[2][kSums] = {{0, 1, 3, 4, 6, 4, 6}, {0, 0, 1, 2, 3, 3, 4, 5, 6, 7, 4, 5, 6, 7}};   // = c_row
 #pragma unroll
-  for (int j = 0; j < kSums; ++j) {
-    const int row = kRowOf[j], s = c_spec[j];
-    const double fac = c_fac_wn[j];
+  for (int j = 0; j < kN; ++j) {
+    const int row = kRowOf[ISTYP][j], s = c_spec[ISTYP][j];
+    const double fac = c_fac_wn[ISTYP][j];
     const size_t o = ((size_t)j * Rp + r) * Np + 2 * f;
     cd z{H[o], H[o + 1]};
     const cd gn = spec(s, n);
@@ -502,33 +515,44 @@ __global__ void k_wnfix(int R, int nf, int Rp, int Np, const int* __restrict__ n
 
 
 // Displacement spectra Z[r][3][nf2] (zero below nf1).
-// momentSingleFrequencyDisplacement (singleFrequencyDisplacement.f90:108-125),
-// time shift (computeSyntheticSeismograms.f90:361-365), velocity (:376-378),
-// ZNE rotation (axesRotation.f90:560-568).
+// forceSingleFrequencyDisplacement (istyp 0, fomt = Fr, Ft, Ff) or
+// momentSingleFrequencyDisplacement (istyp 1, fomt = Mrr, Mtt, Mff, Mrt, Mrf,
+// Mtf) (singleFrequencyDisplacement.f90:35-46, 108-125), time shift
+// (computeSyntheticSeismograms.f90:361-365), velocity (:376-378), ZNE rotation
+// (axesRotation.f90:560-568).
 __global__ void k_combine(int R, int nf1, int nf2, int Rp, int Np, const double* __restrict__ H,
                           const double* __restrict__ phi, const double* __restrict__ propdir,
                           const double* __restrict__ tred, const double* __restrict__ mt, int velocity, double df,
-                          double sigma, double* __restrict__ Z) {
+                          double sigma, int istyp, double* __restrict__ Z) {
   const int id = blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= R * nf2) return;
   const int r = id / nf2, jf = id % nf2 + 1;
   cd u[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
   if (jf >= nf1) {
     const int f = jf - nf1;
+    const int nsum = istyp ? kSums : kForceSums;
     cd h[kSums];
 #pragma unroll
     for (int j = 0; j < kSums; ++j) {
       const size_t o = ((size_t)j * Rp + r) * Np + 2 * f;
-      h[j] = cd{H[o], H[o + 1]};
+      h[j] = j < nsum ? cd{H[o], H[o + 1]} : cd{0.0, 0.0};
     }
     const double ph = phi[r];
-    const double sp = sin(ph), cp = cos(ph), s2 = sin(2.0 * ph), c2 = cos(2.0 * ph);
-    const double a = mt[3] * cp + mt[4] * sp, b = (mt[2] - mt[1]) * c2 - 2.0 * mt[5] * s2;
-    const double c = mt[4] * cp - mt[3] * sp, d = (mt[2] - mt[1]) * s2 + 2.0 * mt[5] * c2;
-    u[0] = cadd(cadd(cadd(crs(h[0], mt[0]), crs(h[1], mt[1] + mt[2])), crs(h[2], a)), crs(h[3], b));
-    u[1] = cadd(cadd(cadd(crs(h[4], mt[0]), crs(h[5], mt[1] + mt[2])), crs(cadd(h[6], h[8]), a)),
-                crs(cadd(h[7], h[9]), b));
-    u[2] = cadd(crs(cadd(h[10], h[12]), c), crs(cadd(h[11], h[13]), d));
+    const double sp = sin(ph), cp = cos(ph);
+    if (istyp) {
+      const double s2 = sin(2.0 * ph), c2 = cos(2.0 * ph);
+      const double a = mt[3] * cp + mt[4] * sp, b = (mt[2] - mt[1]) * c2 - 2.0 * mt[5] * s2;
+      const double c = mt[4] * cp - mt[3] * sp, d = (mt[2] - mt[1]) * s2 + 2.0 * mt[5] * c2;
+      u[0] = cadd(cadd(cadd(crs(h[0], mt[0]), crs(h[1], mt[1] + mt[2])), crs(h[2], a)), crs(h[3], b));
+      u[1] = cadd(cadd(cadd(crs(h[4], mt[0]), crs(h[5], mt[1] + mt[2])), crs(cadd(h[6], h[8]), a)),
+                  crs(cadd(h[7], h[9]), b));
+      u[2] = cadd(crs(cadd(h[10], h[12]), c), crs(cadd(h[11], h[13]), d));
+    } else {
+      const double a = mt[1] * cp + mt[2] * sp, b = mt[1] * sp - mt[2] * cp;
+      u[0] = cadd(crs(h[0], mt[0]), crs(h[1], a));
+      u[1] = cadd(crs(h[2], mt[0]), crs(cadd(h[3], h[4]), a));
+      u[2] = crs(cadd(h[5], h[6]), b);
+    }
     const double om = 2.0 * kPi * (jf - 1) * df;
     if (tred) {
       const double tr = tred[r];
@@ -621,6 +645,7 @@ namespace {
 
 struct SynPlan {
   int R, Rp, L1, Lp, nf, Np, nsp;
+  int istyp, nsum;       // source type from nsp (0 force: 5 spectra, 7 sums; 1 moment: 10 spectra, 14 sums)
   long long nkf;
   std::vector<int> nwnf;
   std::vector<long long> koff;
@@ -629,10 +654,16 @@ struct SynPlan {
 int plan_sums(gm_ctx* c, int nf1, int nf2, const int* nwn, int nsp, long long nkf, int nrcv, SynPlan& P) {
   if (nf1 < 1 || nf2 < nf1) return gm_fail(c, GM_ERR_ARG, "invalid frequency index range %d..%d", nf1, nf2);
   if (!nwn) return gm_fail(c, GM_ERR_ARG, "nwn is required");
-  if (nsp != 10) return gm_fail(c, GM_ERR_UNSUPPORTED, "moment-tensor sums need nsp = 10 spectra (U, V x 4 jumps, W x 2)");
+  // dsvmask (U, V, W): numberGreenFKSpectra (greenFKSpectra.f90:216-220) is
+  // 2 x 4 + 2 = 10 for a moment source and 2 x 2 + 1 = 5 for a single force
+  if (nsp != 10 && nsp != 5)
+    return gm_fail(c, GM_ERR_UNSUPPORTED,
+                   "the sums need nsp = 10 (moment: U, V x 4 jumps, W x 2) or 5 (force: U, V x 2 jumps, W x 1), got %d", nsp);
   if (nrcv < 1) return gm_fail(c, GM_ERR_ARG, "nrcv must be positive");
   P.nf = nf2 - nf1 + 1;
   P.nsp = nsp;
+  P.istyp = nsp == 10 ? 1 : 0;
+  P.nsum = P.istyp ? kSums : kForceSums;
   P.nwnf.resize(P.nf);
   P.koff.resize(P.nf);
   long long off = 0;
@@ -700,15 +731,15 @@ int run_sums(gm_ctx* c, const SynPlan& P, const double* theta, const float* gr, 
   const size_t ng = (size_t)P.nsp * P.Np * P.Lp;
   hipLaunchKernelGGL(k_gprep, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, s, P.nsp, P.nf, P.Np, P.Lp,
                      (const int*)(base + o.nwn), (const long long*)(base + o.koff), P.nkf, dgr, dgi, 0.2, dwn,
-                     (__bf16*)(base + o.Gs));
+                     P.istyp, (__bf16*)(base + o.Gs));
   HIPCHK(c, hipEventRecord(c->sev[2], s));
-  hipLaunchKernelGGL(k_contract, dim3((P.Np / TN) * (P.Rp / TM) * kSums), dim3(256), 0, s, P.Rp, P.Np, P.Lp, P.nsp,
-                     (const __bf16*)(base + o.Bs), (const __bf16*)(base + o.Gs), (double*)(base + o.H),
-                     regional ? 1 : 0);
+  hipLaunchKernelGGL(k_contract, dim3((P.Np / TN) * (P.Rp / TM) * P.nsum), dim3(256), 0, s, P.Rp, P.Np, P.Lp,
+                     P.nsp, (const __bf16*)(base + o.Bs), (const __bf16*)(base + o.Gs), (double*)(base + o.H),
+                     regional ? 1 : 0, P.istyp);
   if (regional)
-    hipLaunchKernelGGL(k_wnfix, dim3((P.R * P.nf + 255) / 256), dim3(256), 0, s, P.R, P.nf, P.Rp, P.Np,
-                       (const int*)(base + o.nwn), (const long long*)(base + o.koff), P.nkf, dgr, dgi, dth, dwn, re,
-                       0.2, (double*)(base + o.H));
+    hipLaunchKernelGGL(P.istyp ? k_wnfix<1> : k_wnfix<0>, dim3((P.R * P.nf + 255) / 256), dim3(256), 0, s, P.R, P.nf,
+                       P.Rp, P.Np, (const int*)(base + o.nwn), (const long long*)(base + o.koff), P.nkf, dgr, dgi, dth,
+                       dwn, re, 0.2, (double*)(base + o.H));
   HIPCHK(c, hipEventRecord(c->sev[3], s));
   HIPCHK(c, hipGetLastError());
   return GM_OK;
@@ -721,7 +752,7 @@ SumBufs carve_sums(const SynPlan& P, bool dev_in, Carve& cv) {
   o.koff = cv.take(sizeof(long long) * P.nf);
   o.Bs = cv.take(sizeof(__bf16) * kPieces * kRows * (size_t)P.Rp * P.Lp);
   o.Gs = cv.take(sizeof(__bf16) * kPieces * (size_t)P.nsp * P.Np * P.Lp);
-  o.H = cv.take(sizeof(double) * kSums * (size_t)P.Rp * P.Np);
+  o.H = cv.take(sizeof(double) * P.nsum * (size_t)P.Rp * P.Np);
   o.gr = dev_in ? 0 : cv.take(sizeof(float) * P.nsp * P.nkf);
   o.gi = dev_in ? 0 : cv.take(sizeof(float) * P.nsp * P.nkf);
   return o;
@@ -756,13 +787,15 @@ extern "C" int gm_legendre_batch(gm_ctx* c, int nrcv, const double* theta, int l
   return GM_OK;
 }
 
-static int sums_impl(gm_ctx* c, int nf1, int nf2, const int* nwn, int nsp, long long nkf, const float* gfkspr,
-                     const float* gfkspi, int nrcv, const double* theta, int flags, double* hdsum, double dwn,
-                     double re) {
+static int sums_impl(gm_ctx* c, int istyp, int nf1, int nf2, const int* nwn, int nsp, long long nkf,
+                     const float* gfkspr, const float* gfkspi, int nrcv, const double* theta, int flags, double* hdsum,
+                     double dwn, double re) {
   if (!c) return GM_ERR_ARG;
   SynPlan P;
   int rc = plan_sums(c, nf1, nf2, nwn, nsp, nkf, nrcv, P);
   if (rc) return rc;
+  if (P.istyp != istyp)
+    return gm_fail(c, GM_ERR_ARG, istyp ? "moment sums need nsp = 10 spectra" : "force sums need nsp = 5 spectra");
   if (!gfkspr || !gfkspi || !theta || !hdsum) return gm_fail(c, GM_ERR_ARG, "spectra, theta and hdsum are required");
   const bool dev = (flags & GM_FLAG_DEVICE_PTRS) != 0;
   HIPCHK(c, hipSetDevice(c->device));
@@ -773,7 +806,7 @@ static int sums_impl(gm_ctx* c, int nf1, int nf2, const int* nwn, int nsp, long 
   hipStream_t s = c->stream;
   if ((rc = run_sums(c, P, theta, gfkspr, gfkspi, dev, base, o, s, dwn, re))) return rc;
   // H[j][Rp][Np] -> hdsum[j][R][nf] complex
-  for (int j = 0; j < kSums; ++j)
+  for (int j = 0; j < P.nsum; ++j)
     HIPCHK(c, hipMemcpy2DAsync(hdsum + (size_t)j * P.R * P.nf * 2, sizeof(double) * 2 * P.nf,
                                base + o.H + sizeof(double) * (size_t)j * P.Rp * P.Np, sizeof(double) * P.Np,
                                sizeof(double) * 2 * P.nf, P.R, dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
@@ -795,7 +828,7 @@ static int synth_impl(gm_ctx* c, int nf1, int nf2, double df, double sigma, cons
   int rc = plan_sums(c, nf1, nf2, nwn, nsp, nkf, nrcv, P);
   if (rc) return rc;
   if (!gfkspr || !gfkspi || !theta || !phi || !mt || !ts)
-    return gm_fail(c, GM_ERR_ARG, "spectra, theta, phi, mt and ts are required");
+    return gm_fail(c, GM_ERR_ARG, "spectra, theta, phi, fomt and ts are required");
   if (!(df > 0.0) || !(dt > 0.0)) return gm_fail(c, GM_ERR_ARG, "df and dt must be positive");
   // transformFrequencyTime (frequencyTime.f90:68-85): only the case without decimation
   int nslog = (int)std::ceil((double)std::log((float)nf2) / std::log(2.0)) + 1;
@@ -824,13 +857,13 @@ static int synth_impl(gm_ctx* c, int nf1, int nf2, double df, double sigma, cons
   HIPCHK(c, hipMemcpyAsync(base + o_phi, phi, sizeof(double) * P.R, hipMemcpyHostToDevice, s));
   if (propdir) HIPCHK(c, hipMemcpyAsync(base + o_pd, propdir, sizeof(double) * P.R, hipMemcpyHostToDevice, s));
   if (tred) HIPCHK(c, hipMemcpyAsync(base + o_tr, tred, sizeof(double) * P.R, hipMemcpyHostToDevice, s));
-  HIPCHK(c, hipMemcpyAsync(base + o_mt, mt, sizeof(double) * 6, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(base + o_mt, mt, sizeof(double) * (P.istyp ? 6 : 3), hipMemcpyHostToDevice, s));
   if ((rc = run_sums(c, P, theta, gfkspr, gfkspi, dev, base, o, s, dwn, re))) return rc;
   double* dZ = (double*)(base + o_z);
   hipLaunchKernelGGL(k_combine, dim3((unsigned)(((size_t)P.R * nf2 + 255) / 256)), dim3(256), 0, s, P.R, nf1, nf2,
                      P.Rp, P.Np, (const double*)(base + o.H), (const double*)(base + o_phi),
                      propdir ? (const double*)(base + o_pd) : nullptr, tred ? (const double*)(base + o_tr) : nullptr,
-                     (const double*)(base + o_mt), velocity ? 1 : 0, df, sigma, dZ);
+                     (const double*)(base + o_mt), velocity ? 1 : 0, df, sigma, P.istyp, dZ);
   HIPCHK(c, hipEventRecord(c->sev[4], s));
   float* dts = dev ? ts : (float*)(base + o_ts);
   int logm = 0;
@@ -853,17 +886,34 @@ static int synth_impl(gm_ctx* c, int nf1, int nf2, double df, double sigma, cons
 
 extern "C" int gm_moment_sums(gm_ctx* c, int nf1, int nf2, const int* nwn, int nsp, long long nkf, const float* gfkspr,
                               const float* gfkspi, int nrcv, const double* theta, int flags, double* hdsum) {
-  return sums_impl(c, nf1, nf2, nwn, nsp, nkf, gfkspr, gfkspi, nrcv, theta, flags, hdsum, 0.0, 0.0);
+  return sums_impl(c, 1, nf1, nf2, nwn, nsp, nkf, gfkspr, gfkspi, nrcv, theta, flags, hdsum, 0.0, 0.0);
+}
+
+extern "C" int gm_force_sums(gm_ctx* c, int nf1, int nf2, const int* nwn, int nsp, long long nkf, const float* gfkspr,
+                             const float* gfkspi, int nrcv, const double* theta, int flags, double* hdsum) {
+  return sums_impl(c, 0, nf1, nf2, nwn, nsp, nkf, gfkspr, gfkspi, nrcv, theta, flags, hdsum, 0.0, 0.0);
+}
+
+static int bessel_impl(gm_ctx* c, int istyp, int nf1, int nf2, const int* nwn, int nsp, long long nkf,
+                       const float* gfkspr, const float* gfkspi, double dwn, double rearth, int nrcv, const double* x,
+                       int flags, double* wnint) {
+  if (!c) return GM_ERR_ARG;
+  if (!(dwn > 0.0) || !(rearth > 0.0)) return gm_fail(c, GM_ERR_ARG, "dwn and rearth must be positive");
+  for (int r = 0; r < nrcv && x; ++r)
+    if (!(x[r] > 0.0)) return gm_fail(c, GM_ERR_ARG, "epicentral distance %d must be positive", r);
+  return sums_impl(c, istyp, nf1, nf2, nwn, nsp, nkf, gfkspr, gfkspi, nrcv, x, flags, wnint, dwn, rearth);
 }
 
 extern "C" int gm_bessel_sums(gm_ctx* c, int nf1, int nf2, const int* nwn, int nsp, long long nkf, const float* gfkspr,
                               const float* gfkspi, double dwn, double rearth, int nrcv, const double* x, int flags,
                               double* wnint) {
-  if (!c) return GM_ERR_ARG;
-  if (!(dwn > 0.0) || !(rearth > 0.0)) return gm_fail(c, GM_ERR_ARG, "dwn and rearth must be positive");
-  for (int r = 0; r < nrcv && x; ++r)
-    if (!(x[r] > 0.0)) return gm_fail(c, GM_ERR_ARG, "epicentral distance %d must be positive", r);
-  return sums_impl(c, nf1, nf2, nwn, nsp, nkf, gfkspr, gfkspi, nrcv, x, flags, wnint, dwn, rearth);
+  return bessel_impl(c, 1, nf1, nf2, nwn, nsp, nkf, gfkspr, gfkspi, dwn, rearth, nrcv, x, flags, wnint);
+}
+
+extern "C" int gm_force_bessel_sums(gm_ctx* c, int nf1, int nf2, const int* nwn, int nsp, long long nkf,
+                                    const float* gfkspr, const float* gfkspi, double dwn, double rearth, int nrcv,
+                                    const double* x, int flags, double* wnint) {
+  return bessel_impl(c, 0, nf1, nf2, nwn, nsp, nkf, gfkspr, gfkspi, dwn, rearth, nrcv, x, flags, wnint);
 }
 
 extern "C" int gm_synthetics(gm_ctx* c, int nf1, int nf2, double df, double sigma, const int* nwn, int nsp,

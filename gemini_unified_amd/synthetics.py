@@ -19,6 +19,10 @@ from . import _lib
 from ._lib import D, I, VP, check
 
 MOMENT_SUMS = 14
+FORCE_SUMS = 7
+# spectra per node in the GFK file for dsvmask (U, V, W): numberGreenFKSpectra
+# (f90/greenFKSpectra.f90:216-220) for a single force (istyp 0) and a moment tensor (istyp 1)
+NSP = {0: 5, 1: 10}
 
 
 class SyntheticsEngine:
@@ -64,33 +68,49 @@ class SyntheticsEngine:
             raise ValueError("gr, gi must be [nsp, nkf] arrays of one shape")
         return gr, gi
 
-    def moment_sums(self, nf1, nf2, nwn, gr, gi, theta):
-        """14 moment harmonic degree sums, complex [14, nrcv, nf2-nf1+1] (momentHarmonicDegreeSums)."""
+    def _sums(self, fn, nsum, nf1, nf2, nwn, gr, gi, theta):
         nwn = np.ascontiguousarray(nwn, dtype=np.int32)
         gr, gi = self._spectra(gr, gi)
         theta = np.ascontiguousarray(np.atleast_1d(theta), dtype=np.float64)
-        out = np.zeros((MOMENT_SUMS, theta.size, nf2 - nf1 + 1), np.complex128)
-        check(self.h, _lib.lib().gm_moment_sums(self.h, int(nf1), int(nf2), nwn.ctypes.data_as(I), gr.shape[0],
-                                                gr.shape[1], gr.ctypes.data, gi.ctypes.data, theta.size,
-                                                theta.ctypes.data_as(D), 0, out.ctypes.data))
+        out = np.zeros((nsum, theta.size, nf2 - nf1 + 1), np.complex128)
+        check(self.h, fn(self.h, int(nf1), int(nf2), nwn.ctypes.data_as(I), gr.shape[0], gr.shape[1], gr.ctypes.data,
+                         gi.ctypes.data, theta.size, theta.ctypes.data_as(D), 0, out.ctypes.data))
         return out
+
+    def _wn_sums(self, fn, nsum, nf1, nf2, nwn, gr, gi, dwn, rearth, x):
+        nwn = np.ascontiguousarray(nwn, dtype=np.int32)
+        gr, gi = self._spectra(gr, gi)
+        x = np.ascontiguousarray(np.atleast_1d(x), dtype=np.float64)
+        out = np.zeros((nsum, x.size, nf2 - nf1 + 1), np.complex128)
+        check(self.h, fn(self.h, int(nf1), int(nf2), nwn.ctypes.data_as(I), gr.shape[0], gr.shape[1], gr.ctypes.data,
+                         gi.ctypes.data, float(dwn), float(rearth), x.size, x.ctypes.data_as(D), 0, out.ctypes.data))
+        return out
+
+    def moment_sums(self, nf1, nf2, nwn, gr, gi, theta):
+        """14 moment harmonic degree sums, complex [14, nrcv, nf2-nf1+1] (momentHarmonicDegreeSums);
+        gr, gi: [10, nkf]."""
+        return self._sums(_lib.lib().gm_moment_sums, MOMENT_SUMS, nf1, nf2, nwn, gr, gi, theta)
+
+    def force_sums(self, nf1, nf2, nwn, gr, gi, theta):
+        """7 single-force harmonic degree sums, complex [7, nrcv, nf2-nf1+1] (forceHarmonicDegreeSums);
+        gr, gi: [5, nkf] (U1, U2, V1, V2, W1)."""
+        return self._sums(_lib.lib().gm_force_sums, FORCE_SUMS, nf1, nf2, nwn, gr, gi, theta)
 
     def bessel_sums(self, nf1, nf2, nwn, gr, gi, dwn, rearth, x):
         """14 moment wavenumber integrals, complex [14, nrcv, nf2-nf1+1] (momentWavenumberIntegrals);
         x = epicentral distances in metres, wavenumbers k = (j-1) dwn."""
-        nwn = np.ascontiguousarray(nwn, dtype=np.int32)
-        gr, gi = self._spectra(gr, gi)
-        x = np.ascontiguousarray(np.atleast_1d(x), dtype=np.float64)
-        out = np.zeros((MOMENT_SUMS, x.size, nf2 - nf1 + 1), np.complex128)
-        check(self.h, _lib.lib().gm_bessel_sums(self.h, int(nf1), int(nf2), nwn.ctypes.data_as(I), gr.shape[0],
-                                                gr.shape[1], gr.ctypes.data, gi.ctypes.data, float(dwn),
-                                                float(rearth), x.size, x.ctypes.data_as(D), 0, out.ctypes.data))
-        return out
+        return self._wn_sums(_lib.lib().gm_bessel_sums, MOMENT_SUMS, nf1, nf2, nwn, gr, gi, dwn, rearth, x)
+
+    def force_bessel_sums(self, nf1, nf2, nwn, gr, gi, dwn, rearth, x):
+        """7 single-force wavenumber integrals, complex [7, nrcv, nf2-nf1+1] (forceWavenumberIntegrals)."""
+        return self._wn_sums(_lib.lib().gm_force_bessel_sums, FORCE_SUMS, nf1, nf2, nwn, gr, gi, dwn, rearth, x)
 
     def synthetics(self, nf1, nf2, df, sigma, nwn, gr, gi, theta, phi, mt, nsamp, dt, propdir=None, tred=None,
                    velocity=False, want_spectra=False, device_ptrs=None, dwn=None, rearth=None):
         """Seismograms [nrcv, 3, nsamp] FP32 (and displacement spectra [nrcv, 3, nf2] if asked).
 
+        The source type follows the number of spectra nsp: 10, moment tensor, mt = (Mrr, Mtt, Mff, Mrt,
+        Mrf, Mtf); 5, single force, mt = (Fr, Ft, Ff) (getForceMomentSeismicEvent's fomt).
         Global branch: theta = epicentral distances in radians.  Regional branch (dwn given):
         theta = epicentral distances in metres, wavenumber integrals with k = (j-1) dwn.
         device_ptrs = (gr_ptr, gi_ptr, nsp, nkf, ts_ptr): spectra and output already in HBM
@@ -100,8 +120,9 @@ class SyntheticsEngine:
         theta = np.ascontiguousarray(np.atleast_1d(theta), dtype=np.float64)
         phi = np.ascontiguousarray(np.atleast_1d(phi), dtype=np.float64)
         mt = np.ascontiguousarray(mt, dtype=np.float64)
-        if phi.size != theta.size or mt.size != 6:
-            raise ValueError("phi must match theta and mt must hold 6 entries")
+        nsp_in = device_ptrs[2] if device_ptrs is not None else np.shape(gr)[0]
+        if phi.size != theta.size or mt.size != {10: 6, 5: 3}.get(int(nsp_in), 6):
+            raise ValueError("phi must match theta and mt must hold 6 (moment) or 3 (force, nsp = 5) entries")
         pd = None if propdir is None else np.ascontiguousarray(propdir, dtype=np.float64)
         tr = None if tred is None else np.ascontiguousarray(tred, dtype=np.float64)
         zsp = np.zeros((theta.size, 3, nf2), np.complex128) if want_spectra else None

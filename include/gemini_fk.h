@@ -197,9 +197,23 @@ int gm_legendre_batch(gm_ctx* ctx, int nrcv, const double* theta, int lmax, doub
 int gm_moment_sums(gm_ctx* ctx, int nf1, int nf2, const int* nwn, int nsp, long long nkf, const float* gfkspr,
                    const float* gfkspi, int nrcv, const double* theta, int flags, double* hdsum);
 
-/* Seismograms of one source node: degree sums, moment-tensor combination
- * (f90/singleFrequencyDisplacement.f90:108-125, mt = Mrr, Mtt, Mff, Mrt, Mrf,
- * Mtf), time shift by tred[r] (NULL: none), velocity (velocity = 1) and
+/* The 7 single-force harmonic degree sums (kint 1..7, displacement) of one
+ * source node, arguments as for gm_moment_sums with nsp = 5 spectra (U, V x 2
+ * jumps, W x 1: the FK spectra of a FORCE source, istyp = 0).
+ * hdsum: complex [7][nrcv][nf2-nf1+1].  Replaces forceHarmonicDegreeSums with
+ * taperHarmonicDegreeSums (f90/harmonicDegreeSums.f90:95-167, 356-369) in the
+ * frequency loop of sphericalHarmonicsDisplacementSpectra
+ * (f90/displacementSpectra.f90:49-106). */
+int gm_force_sums(gm_ctx* ctx, int nf1, int nf2, const int* nwn, int nsp, long long nkf, const float* gfkspr,
+                  const float* gfkspi, int nrcv, const double* theta, int flags, double* hdsum);
+
+/* Seismograms of one source node: degree sums, then the moment-tensor or force
+ * combination (evalSingleFrequencyDisplacementSpectra,
+ * f90/displacementSpectra.f90:200-228).  The source type follows nsp as in the
+ * GFK file: nsp = 10, moment source, fomt = Mrr, Mtt, Mff, Mrt, Mrf, Mtf
+ * (f90/singleFrequencyDisplacement.f90:108-125); nsp = 5, single force,
+ * fomt = Fr, Ft, Ff (:35-46).  Then time shift by tred[r] (NULL: none),
+ * velocity (velocity = 1) and
  * rotation to Z, N, E by propdir[r] (NULL: components r, theta, phi), then the
  * frequency-to-time transform to nsamp FP32 samples (transformFrequencyTime,
  * f90/frequencyTime.f90:55-113; nsamp = 2^ceil(-log2(df dt)) <= 8192, no
@@ -209,7 +223,7 @@ int gm_moment_sums(gm_ctx* ctx, int nf1, int nf2, const int* nwn, int nsp, long 
  * GM_FLAG_DEVICE_PTRS: gfkspr, gfkspi, ts and zsp are device pointers. */
 int gm_synthetics(gm_ctx* ctx, int nf1, int nf2, double df, double sigma, const int* nwn, int nsp, long long nkf,
                   const float* gfkspr, const float* gfkspi, int nrcv, const double* theta, const double* phi,
-                  const double* propdir, const double* tred, const double* mt, int velocity, int nsamp, double dt,
+                  const double* propdir, const double* tred, const double* fomt, int velocity, int nsamp, double dt,
                   int flags, float* ts, double* zsp);
 
 /* Regional (Bessel) branch: the 14 moment wavenumber integrals (kint 1..14) of
@@ -226,12 +240,20 @@ int gm_bessel_sums(gm_ctx* ctx, int nf1, int nf2, const int* nwn, int nsp, long 
                    const float* gfkspi, double dwn, double rearth, int nrcv, const double* x, int flags,
                    double* wnint);
 
+/* The 7 single-force wavenumber integrals (kint 1..7) of one source node,
+ * arguments as for gm_bessel_sums with nsp = 5 spectra (U, V x 2 jumps, W x 1).
+ * Replaces forceWavenumberIntegrals (f90/wavenumberIntegrals.f90:122-250).
+ * wnint: complex [7][nrcv][nf2-nf1+1]. */
+int gm_force_bessel_sums(gm_ctx* ctx, int nf1, int nf2, const int* nwn, int nsp, long long nkf, const float* gfkspr,
+                         const float* gfkspi, double dwn, double rearth, int nrcv, const double* x, int flags,
+                         double* wnint);
+
 /* gm_synthetics for the regional branch: receivers at epicentral distances
  * x[r] (metres) and azimuths phi[r]; otherwise as gm_synthetics. */
 int gm_synthetics_regional(gm_ctx* ctx, int nf1, int nf2, double df, double sigma, const int* nwn, int nsp,
                            long long nkf, const float* gfkspr, const float* gfkspi, double dwn, double rearth,
                            int nrcv, const double* x, const double* phi, const double* propdir, const double* tred,
-                           const double* mt, int velocity, int nsamp, double dt, int flags, float* ts, double* zsp);
+                           const double* fomt, int velocity, int nsamp, double dt, int flags, float* ts, double* zsp);
 
 /* Kernel time of the last gm_moment_sums / gm_synthetics call in milliseconds:
  * 0 Legendre basis, 1 spectra preparation, 2 contraction, 3 combination,

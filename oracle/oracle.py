@@ -272,19 +272,28 @@ def fft(x, y, is_=1):
     return x, y
 
 
-def moment_sums(nf1, nf2, nwn, gr, gi, theta, tapfrac=0.2, nthreads=1):
-    """14 moment harmonic degree sums: complex [nrcv, nf, 14].  gr, gi: FP32 [nsp, nkf] of one node."""
+def _degree_sums(fn, nsum, nf1, nf2, nwn, gr, gi, theta, tapfrac, nthreads):
     nwn = np.ascontiguousarray(nwn, dtype=np.int32)
     gr = np.ascontiguousarray(gr, dtype=np.float32)
     gi = np.ascontiguousarray(gi, dtype=np.float32)
     theta = np.ascontiguousarray(np.atleast_1d(theta), dtype=np.float64)
     nf = nf2 - nf1 + 1
-    h = np.zeros((theta.size, nf, 14), np.complex128)
+    h = np.zeros((theta.size, nf, nsum), np.complex128)
     F = C.POINTER(C.c_float)
-    lib().orc_moment_sums(int(nf1), int(nf2), nwn.ctypes.data_as(I), C.c_long(gr.shape[1]), gr.ctypes.data_as(F),
-                          gi.ctypes.data_as(F), theta.size, _d(theta), C.c_double(tapfrac),
-                          h.ctypes.data_as(D), int(nthreads))
+    fn(int(nf1), int(nf2), nwn.ctypes.data_as(I), C.c_long(gr.shape[1]), gr.ctypes.data_as(F),
+       gi.ctypes.data_as(F), theta.size, _d(theta), C.c_double(tapfrac), h.ctypes.data_as(D), int(nthreads))
     return h
+
+
+def moment_sums(nf1, nf2, nwn, gr, gi, theta, tapfrac=0.2, nthreads=1):
+    """14 moment harmonic degree sums: complex [nrcv, nf, 14].  gr, gi: FP32 [nsp = 10, nkf] of one node."""
+    return _degree_sums(lib().orc_moment_sums, 14, nf1, nf2, nwn, gr, gi, theta, tapfrac, nthreads)
+
+
+def force_sums(nf1, nf2, nwn, gr, gi, theta, tapfrac=0.2, nthreads=1):
+    """7 force harmonic degree sums: complex [nrcv, nf, 7].  gr, gi: FP32 [nsp = 5, nkf] of one node
+    (U1, U2, V1, V2, W1)."""
+    return _degree_sums(lib().orc_force_sums, 7, nf1, nf2, nwn, gr, gi, theta, tapfrac, nthreads)
 
 
 def besselj(order, x):
@@ -297,24 +306,26 @@ def bessely(order, x):
     return lib().orc_bessely(int(order), C.c_double(x))
 
 
-def bessel_sums(nf1, nf2, nwn, gr, gi, dwn, rearth, x, tapfrac=0.2, nthreads=1):
-    """14 moment wavenumber integrals: complex [nrcv, nf, 14]; x = epicentral distances in m."""
+def bessel_sums(nf1, nf2, nwn, gr, gi, dwn, rearth, x, tapfrac=0.2, nthreads=1, istyp=1):
+    """Wavenumber integrals: complex [nrcv, nf, 14] (istyp 1, moment) or [nrcv, nf, 7] (istyp 0, force);
+    x = epicentral distances in m."""
     nwn = np.ascontiguousarray(nwn, dtype=np.int32)
     gr = np.ascontiguousarray(gr, dtype=np.float32)
     gi = np.ascontiguousarray(gi, dtype=np.float32)
     x = np.ascontiguousarray(np.atleast_1d(x), dtype=np.float64)
-    h = np.zeros((x.size, nf2 - nf1 + 1, 14), np.complex128)
+    h = np.zeros((x.size, nf2 - nf1 + 1, 14 if istyp == 1 else 7), np.complex128)
     F = C.POINTER(C.c_float)
-    lib().orc_bessel_sums(int(nf1), int(nf2), nwn.ctypes.data_as(I), C.c_long(gr.shape[1]), gr.ctypes.data_as(F),
-                          gi.ctypes.data_as(F), C.c_double(dwn), C.c_double(rearth), x.size, _d(x),
-                          C.c_double(tapfrac), h.ctypes.data_as(D), int(nthreads))
+    lib().orc_bessel_sums_src(int(istyp), int(nf1), int(nf2), nwn.ctypes.data_as(I), C.c_long(gr.shape[1]),
+                              gr.ctypes.data_as(F), gi.ctypes.data_as(F), C.c_double(dwn), C.c_double(rearth),
+                              x.size, _d(x), C.c_double(tapfrac), h.ctypes.data_as(D), int(nthreads))
     return h
 
 
 def synthetics(nf1, nf2, df, sigma, nwn, gr, gi, theta, phi, mt, nsamp, dt, propdir=None, tred=None, velocity=False,
                nthreads=1, dwn=0.0, rearth=0.0):
     """(ts float32 [nrcv, 3, nsamp], zsp complex [nrcv, 3, nf2]) of one source node; dwn > 0 selects the
-    regional (wavenumber integral) branch with distances theta * rearth."""
+    regional (wavenumber integral) branch with distances theta * rearth.  The source type follows the
+    number of spectra: 10 (moment, mt = Mrr, Mtt, Mff, Mrt, Mrf, Mtf) or 5 (force, mt = Fr, Ft, Ff)."""
     nwn = np.ascontiguousarray(nwn, dtype=np.int32)
     gr = np.ascontiguousarray(gr, dtype=np.float32)
     gi = np.ascontiguousarray(gi, dtype=np.float32)
@@ -327,7 +338,9 @@ def synthetics(nf1, nf2, df, sigma, nwn, gr, gi, theta, phi, mt, nsamp, dt, prop
     F = C.POINTER(C.c_float)
     pd = None if propdir is None else np.ascontiguousarray(propdir, dtype=np.float64)
     tr = None if tred is None else np.ascontiguousarray(tred, dtype=np.float64)
-    rc = lib().orc_synthetics_mode(int(nf1), int(nf2), C.c_double(df), C.c_double(sigma), nwn.ctypes.data_as(I),
+    istyp = {10: 1, 5: 0}[gr.shape[0]]
+    assert mt.size == (6 if istyp == 1 else 3)
+    rc = lib().orc_synthetics_src(istyp, int(nf1), int(nf2), C.c_double(df), C.c_double(sigma), nwn.ctypes.data_as(I),
                               C.c_long(gr.shape[1]), gr.ctypes.data_as(F), gi.ctypes.data_as(F), C.c_double(dwn),
                               C.c_double(rearth), nr, _d(theta),
                               _d(phi), None if pd is None else _d(pd), None if tr is None else _d(tr), _d(mt),

@@ -103,14 +103,24 @@ static const SumDef kSums[14] = {
     {6, 8, 1, 2.0, true},  {7, 9, 2, 4.0, true},  {4, 8, 1, 2.0, true}, {5, 9, 2, -2.0, true},
     {6, 6, 1, 2.0, true},  {7, 7, 2, -4.0, true}};
 
-// The 14 moment harmonic degree sums at one receiver for each frequency.
+// (basis row, Green spectrum index, factor) of the 7 force sums (kint 1..7:
+// displacement only).  Restates the case table of forceHarmonicDegreeSums
+// (f90/harmonicDegreeSums.f90:115-143) and the (component, jump) -> spectrum
+// map (:34-36 with positionDataGreenFKSpectra, f90/greenFKSpectra.f90:204-212,
+// for dsvmask (U, V, W) and 2 spheroidal / 1 toroidal jumps: U1 U2 V1 V2 W1).
+static const SumDef kForceSums[7] = {
+    {0, 0, 0, 1.0, false},  {1, 1, 1, -2.0, true}, {3, 2, 1, 1.0, false}, {4, 3, 1, -2.0, true},
+    {6, 4, 1, -2.0, true},  {4, 4, 1, 2.0, true},  {6, 3, 1, 2.0, true}};
+
+// Harmonic degree sums of one source type at one receiver for each frequency.
 // gr, gi: [nsp][nkf] FP32 spectra of one node, k over frequencies nf1..nf2 with
 // nwn[jf-1] degrees each (l fastest) — the layout readDataNodeGreenFKSpectra
-// returns; hdsum: [nf][14] complex.
+// returns; hdsum: [nf][nsum] complex.
 // Restates the frequency loop of sphericalHarmonicsDisplacementSpectra
-// (f90/displacementSpectra.f90:49-106) with momentHarmonicDegreeSums.
-void moment_sums(int nf1, int nf2, const int* nwn, long nkf, const float* gr, const float* gi, double theta,
-                 double tapfrac, cd* hdsum) {
+// (f90/displacementSpectra.f90:49-106) with momentHarmonicDegreeSums or
+// forceHarmonicDegreeSums.
+static void degree_sums(const SumDef* defs, int nsum, int nf1, int nf2, const int* nwn, long nkf, const float* gr,
+                        const float* gi, double theta, double tapfrac, cd* hdsum) {
   int lmax = 0;
   for (int jf = nf1; jf <= nf2; ++jf) lmax = std::max(lmax, nwn[jf - 1] - 1);
   const int L1 = lmax + 1;
@@ -123,8 +133,8 @@ void moment_sums(int nf1, int nf2, const int* nwn, long nkf, const float* gr, co
     const int n = nwn[jf - 1];
     const int lm = n - 1;
     taper(lm, tapfrac, w.data());
-    for (int s = 0; s < 14; ++s) {
-      const SumDef& d = kSums[s];
+    for (int s = 0; s < nsum; ++s) {
+      const SumDef& d = defs[s];
       cd z = 0.0;
       for (int l = d.lmin; l <= lm; ++l) {
         double b = rows[d.basis][l];
@@ -132,10 +142,22 @@ void moment_sums(int nf1, int nf2, const int* nwn, long nkf, const float* gr, co
         const cd g(gr[(long)d.isp * nkf + off + l], gi[(long)d.isp * nkf + off + l]);
         z += b * g * (2.0 * l + 1.0) / fourpi * w[l];
       }
-      hdsum[(long)(jf - nf1) * 14 + s] = z;
+      hdsum[(long)(jf - nf1) * nsum + s] = z;
     }
     off += n;
   }
+}
+
+// The 14 moment harmonic degree sums (kint 1..14); hdsum: [nf][14] complex.
+void moment_sums(int nf1, int nf2, const int* nwn, long nkf, const float* gr, const float* gi, double theta,
+                 double tapfrac, cd* hdsum) {
+  degree_sums(kSums, 14, nf1, nf2, nwn, nkf, gr, gi, theta, tapfrac, hdsum);
+}
+
+// The 7 force harmonic degree sums (kint 1..7); hdsum: [nf][7] complex.
+void force_sums(int nf1, int nf2, const int* nwn, long nkf, const float* gr, const float* gi, double theta,
+                double tapfrac, cd* hdsum) {
+  degree_sums(kForceSums, 7, nf1, nf2, nwn, nkf, gr, gi, theta, tapfrac, hdsum);
 }
 
 // Displacement (r, theta, phi) at one frequency from the 14 sums and the moment
@@ -148,6 +170,17 @@ void moment_displacement(double phi, const double* mt, const cd* w, cd* zsp) {
   zsp[0] = w[0] * mt[0] + w[1] * (mt[1] + mt[2]) + w[2] * a + w[3] * b;
   zsp[1] = w[4] * mt[0] + w[5] * (mt[1] + mt[2]) + (w[6] + w[8]) * a + (w[7] + w[9]) * b;
   zsp[2] = (w[10] + w[12]) * c + (w[11] + w[13]) * d;
+}
+
+// Displacement (r, theta, phi) at one frequency from the 7 force sums and the
+// force f = (Fr, Ft, Ff).
+// Restates forceSingleFrequencyDisplacement, f90/singleFrequencyDisplacement.f90:35-46.
+void force_displacement(double phi, const double* f, const cd* w, cd* zsp) {
+  const double sp = std::sin(phi), cp = std::cos(phi);
+  const double a = f[1] * cp + f[2] * sp, b = f[1] * sp - f[2] * cp;
+  zsp[0] = w[0] * f[0] + w[1] * a;
+  zsp[1] = w[2] * f[0] + (w[3] + w[4]) * a;
+  zsp[2] = (w[5] + w[6]) * b;
 }
 
 // In-place radix-2 complex FFT, e^{+i...} for is > 0, no normalisation.
@@ -239,15 +272,37 @@ static double wn_kernel(int kint, double k, double x, double re, double b0, doub
   return 0.0;
 }
 
-// The 14 moment wavenumber integrals at epicentral distance xx (m) for each frequency.
+// Kernel of force wavenumber integral kint (1..7) at wavenumber k.
+// Restates the case tables of forceWavenumberIntegrals, f90/wavenumberIntegrals.f90:162-176 (and :183-197, :220-234).
+static double wn_kernel_force(int kint, double k, double x, double re, double b0, double b1, double) {
+  switch (kint) {
+    case 1: return k * b0;
+    case 2: return -2.0 * b1 / re;
+    case 3: return -k * k * b1 * re;
+    case 4: return -2.0 * k * (b0 - b1 / (k * x));
+    case 5: return -2.0 * k * b1 / (k * x);
+    case 6: return 2.0 * k * (b0 - b1 / (k * x));
+    case 7: return 2.0 * k * b1 / (k * x);
+  }
+  return 0.0;
+}
+
+// The 14 moment (istyp 1) or 7 force (istyp 0) wavenumber integrals at epicentral distance xx (m) for each frequency.
 // gr, gi as in moment_sums (wavenumbers of frequency jf: j = 1..nwn[jf-1], k = (j-1) dwn).
 // Restates momentWavenumberIntegrals (f90/wavenumberIntegrals.f90:321-477) with
 // computeBesselFunctions (f90/besselFunctions.f90:33-51) in the frequency loop of
-// besselDisplacementSpectra (f90/displacementSpectra.f90:119-184).  J0, J1, Y0,
+// besselDisplacementSpectra (f90/displacementSpectra.f90:119-184); for a force,
+// forceWavenumberIntegrals (f90/wavenumberIntegrals.f90:122-250).  J0, J1, Y0,
 // Y1 come from the C library (the reference vendors Cody's SPECFUN routines).
+// wnint: [nf][nsum] complex.
 static const int kSpecWn[14] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 8, 9, 6, 7};
-void bessel_sums(int nf1, int nf2, const int* nwn, long nkf, const float* gr, const float* gi, double dwn, double re,
-                 double xx, double tapfrac, cd* wnint) {
+static const int kSpecWnForce[7] = {0, 1, 2, 3, 4, 4, 3};
+void bessel_sums(int istyp, int nf1, int nf2, const int* nwn, long nkf, const float* gr, const float* gi, double dwn,
+                 double re, double xx, double tapfrac, cd* wnint) {
+  const int nsum = istyp == 1 ? 14 : 7;
+  auto kernel = [&](int kint, double k, double x, double b0, double b1, double b2) {
+    return istyp == 1 ? wn_kernel(kint, k, x, re, b0, b1, b2) : wn_kernel_force(kint, k, x, re, b0, b1, b2);
+  };
   int nmax = 0;
   for (int jf = nf1; jf <= nf2; ++jf) nmax = std::max(nmax, nwn[jf - 1]);
   // computeBesselFunctions: besselj(j, 0..2) at k = (j-1) dwn, j = 1 .. nwnmax
@@ -273,19 +328,19 @@ void bessel_sums(int nf1, int nf2, const int* nwn, long nkf, const float* gr, co
     taper_wn(n, dwn, tapfrac, w.data());
     const int j33 = std::min(j3, n);
     const double wne = (n - 1) * dwn;
-    for (int kint = 1; kint <= 14; ++kint) {
-      const int s = kSpecWn[kint - 1];
+    for (int kint = 1; kint <= nsum; ++kint) {
+      const int s = istyp == 1 ? kSpecWn[kint - 1] : kSpecWnForce[kint - 1];
       auto G = [&](int j) { return cd(gr[(long)s * nkf + off + j - 1], gi[(long)s * nkf + off + j - 1]); };
-      for (int j = 1; j <= n; ++j) kern[j - 1] = wn_kernel(kint, (j - 1) * dwn, x, re, J0[j - 1], J1[j - 1], J2[j - 1]);
+      for (int j = 1; j <= n; ++j) kern[j - 1] = kernel(kint, (j - 1) * dwn, x, J0[j - 1], J1[j - 1], J2[j - 1]);
       cd z = 0.0;
       for (int j = 2; j <= j33 - 1; ++j) z += G(j) * kern[j - 1] * w[j - 1];
       z += 0.5 * G(j33) * kern[j33 - 1] * w[j33 - 1];
       z *= dwn;
       if (j33 != n) {
         double y0 = ::y0(wn3 * x), y1 = ::y1(wn3 * x), y2 = -y0 + 2.0 / (wn3 * x) * y1;
-        const double b3 = wn_kernel(kint, wn3, x, re, y0, y1, y2);
+        const double b3 = kernel(kint, wn3, x, y0, y1, y2);
         y0 = ::y0(wne * x); y1 = ::y1(wne * x); y2 = -y0 + 2.0 / (wne * x) * y1;
-        const double be = wn_kernel(kint, wne, x, re, y0, y1, y2);
+        const double be = kernel(kint, wne, x, y0, y1, y2);
         cd zs = G(j3) * kern[j3 - 1] * w[j3 - 1];
         for (int j = j3 + 1; j <= n - 1; ++j) zs += 2.0 * G(j) * kern[j - 1] * w[j - 1];
         zs += G(n) * kern[n - 1] * w[n - 1];
@@ -293,7 +348,7 @@ void bessel_sums(int nf1, int nf2, const int* nwn, long nkf, const float* gr, co
         z += (G(n) * be * w[n - 1] - G(j3) * b3 * w[j3 - 1]) / x * (1.0 - std::sin(xdk) / xdk);
       }
       z *= re * re / (2.0 * kPi);
-      wnint[(long)(jf - nf1) * 14 + kint - 1] = z;
+      wnint[(long)(jf - nf1) * nsum + kint - 1] = z;
     }
     off += n;
   }
@@ -338,15 +393,33 @@ int orc_moment_sums(int nf1, int nf2, const int* nwn, long nkf, const float* gr,
   return 0;
 }
 
-// wnint: [nrcv][nf][14] complex; x: epicentral distances in m
-int orc_bessel_sums(int nf1, int nf2, const int* nwn, long nkf, const float* gr, const float* gi, double dwn,
-                    double re, int nrcv, const double* x, double tapfrac, double* wnint, int nthreads) {
+// hdsum: [nrcv][nf][7] complex (interleaved)
+int orc_force_sums(int nf1, int nf2, const int* nwn, long nkf, const float* gr, const float* gi, int nrcv,
+                   const double* theta, double tapfrac, double* hdsum, int nthreads) {
   const int nf = nf2 - nf1 + 1;
 #pragma omp parallel for schedule(dynamic) num_threads(nthreads > 0 ? nthreads : 1)
   for (int r = 0; r < nrcv; ++r)
-    syn::bessel_sums(nf1, nf2, nwn, nkf, gr, gi, dwn, re, x[r], tapfrac,
-                     reinterpret_cast<syn::cd*>(wnint) + (long)r * nf * 14);
+    syn::force_sums(nf1, nf2, nwn, nkf, gr, gi, theta[r], tapfrac,
+                    reinterpret_cast<syn::cd*>(hdsum) + (long)r * nf * 7);
   return 0;
+}
+
+// wnint: [nrcv][nf][nsum] complex, nsum = 14 (istyp 1, moment) or 7 (istyp 0, force);
+// x: epicentral distances in m
+int orc_bessel_sums_src(int istyp, int nf1, int nf2, const int* nwn, long nkf, const float* gr, const float* gi,
+                        double dwn, double re, int nrcv, const double* x, double tapfrac, double* wnint,
+                        int nthreads) {
+  const int nf = nf2 - nf1 + 1, nsum = istyp == 1 ? 14 : 7;
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads > 0 ? nthreads : 1)
+  for (int r = 0; r < nrcv; ++r)
+    syn::bessel_sums(istyp, nf1, nf2, nwn, nkf, gr, gi, dwn, re, x[r], tapfrac,
+                     reinterpret_cast<syn::cd*>(wnint) + (long)r * nf * nsum);
+  return 0;
+}
+
+int orc_bessel_sums(int nf1, int nf2, const int* nwn, long nkf, const float* gr, const float* gi, double dwn,
+                    double re, int nrcv, const double* x, double tapfrac, double* wnint, int nthreads) {
+  return orc_bessel_sums_src(1, nf1, nf2, nwn, nkf, gr, gi, dwn, re, nrcv, x, tapfrac, wnint, nthreads);
 }
 
 double orc_besselj(int order, double x) { return order == 0 ? ::j0(x) : ::j1(x); }
@@ -357,24 +430,29 @@ double orc_bessely(int order, double x) { return order == 0 ? ::y0(x) : ::y1(x);
 // Restates the station loop of computeSyntheticSeismograms
 // (f90/computeSyntheticSeismograms.f90:311-388) with unit event and station
 // filters.  ts: [nrcv][3][nsamp]; zsp (optional): [nrcv][3][nf2] complex.
+// istyp 1: moment source, fomt = (Mrr, Mtt, Mff, Mrt, Mrf, Mtf); istyp 0: force,
+// fomt = (Fr, Ft, Ff) (evalSingleFrequencyDisplacementSpectra,
+// f90/displacementSpectra.f90:200-228).
 // dwn > 0: regional mode (besselDisplacementSpectra, f90/displacementSpectra.f90:119-184):
 // theta[r] is the epicentral distance in radians as in the global mode and the
 // wavenumber integrals are taken at x = theta[r] * re.
-int orc_synthetics_mode(int nf1, int nf2, double df, double sigma, const int* nwn, long nkf, const float* gr,
-                        const float* gi, double dwn, double re, int nrcv, const double* theta, const double* phi,
-                        const double* propdir, const double* tred, const double* mt, int velocity, int nsamp,
-                        double dt, float* ts, double* zsp, int nthreads) {
-  const int nf = nf2 - nf1 + 1;
+int orc_synthetics_src(int istyp, int nf1, int nf2, double df, double sigma, const int* nwn, long nkf,
+                       const float* gr, const float* gi, double dwn, double re, int nrcv, const double* theta,
+                       const double* phi, const double* propdir, const double* tred, const double* fomt, int velocity,
+                       int nsamp, double dt, float* ts, double* zsp, int nthreads) {
+  const int nf = nf2 - nf1 + 1, nsum = istyp == 1 ? 14 : 7;
   int bad = 0;
 #pragma omp parallel for schedule(dynamic) num_threads(nthreads > 0 ? nthreads : 1) reduction(| : bad)
   for (int r = 0; r < nrcv; ++r) {
-    std::vector<syn::cd> h((size_t)nf * 14), z((size_t)3 * nf2, 0.0);
-    if (dwn > 0.0) syn::bessel_sums(nf1, nf2, nwn, nkf, gr, gi, dwn, re, theta[r] * re, 0.2, h.data());
-    else syn::moment_sums(nf1, nf2, nwn, nkf, gr, gi, theta[r], 0.2, h.data());
+    std::vector<syn::cd> h((size_t)nf * nsum), z((size_t)3 * nf2, 0.0);
+    if (dwn > 0.0) syn::bessel_sums(istyp, nf1, nf2, nwn, nkf, gr, gi, dwn, re, theta[r] * re, 0.2, h.data());
+    else if (istyp == 1) syn::moment_sums(nf1, nf2, nwn, nkf, gr, gi, theta[r], 0.2, h.data());
+    else syn::force_sums(nf1, nf2, nwn, nkf, gr, gi, theta[r], 0.2, h.data());
     const double tr = tred ? tred[r] : 0.0;
     for (int jf = nf1; jf <= nf2; ++jf) {
       syn::cd u[3];
-      syn::moment_displacement(phi[r], mt, &h[(size_t)(jf - nf1) * 14], u);
+      if (istyp == 1) syn::moment_displacement(phi[r], fomt, &h[(size_t)(jf - nf1) * nsum], u);
+      else syn::force_displacement(phi[r], fomt, &h[(size_t)(jf - nf1) * nsum], u);
       // time shift filter, f90/computeSyntheticSeismograms.f90:361-365
       const double om = 2.0 * syn::kPi * (jf - 1) * df;
       const syn::cd tsf = std::exp(syn::cd(0.0, om * tr)) * std::exp(sigma * tr);
@@ -395,6 +473,14 @@ int orc_synthetics_mode(int nf1, int nf2, double df, double sigma, const int* nw
       bad |= syn::freq_time(nf1, nf2, df, sigma, &z[(size_t)c * nf2], nsamp, dt, ts + ((size_t)r * 3 + c) * nsamp);
   }
   return bad;
+}
+
+int orc_synthetics_mode(int nf1, int nf2, double df, double sigma, const int* nwn, long nkf, const float* gr,
+                        const float* gi, double dwn, double re, int nrcv, const double* theta, const double* phi,
+                        const double* propdir, const double* tred, const double* mt, int velocity, int nsamp,
+                        double dt, float* ts, double* zsp, int nthreads) {
+  return orc_synthetics_src(1, nf1, nf2, df, sigma, nwn, nkf, gr, gi, dwn, re, nrcv, theta, phi, propdir, tred, mt,
+                            velocity, nsamp, dt, ts, zsp, nthreads);
 }
 
 int orc_synthetics(int nf1, int nf2, double df, double sigma, const int* nwn, long nkf, const float* gr,

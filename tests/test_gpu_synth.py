@@ -113,10 +113,16 @@ def test_synthetics_match_oracle(syn):
 def test_synthetics_errors(syn):
     from gemini_unified_amd._lib import GeminiFKError
     nwn = np.array([10, 10], np.int32)
-    g = np.zeros((5, 20), np.float32)
+    g = np.zeros((3, 20), np.float32)
     with pytest.raises(GeminiFKError) as e:
         syn.synthetics(1, 2, 1 / 64, 0.0, nwn, g, g, [1.0], [0.0], np.ones(6), 64, 1.0)
-    assert e.value.code == 8
+    assert e.value.code == 8                        # neither 10 (moment) nor 5 (force) spectra
+    with pytest.raises(GeminiFKError) as e:
+        syn.moment_sums(1, 2, nwn, np.zeros((5, 20), np.float32), np.zeros((5, 20), np.float32), [1.0])
+    assert e.value.code == 9                        # force spectra given to the moment sums
+    with pytest.raises(GeminiFKError) as e:
+        syn.force_sums(1, 2, nwn, np.zeros((10, 20), np.float32), np.zeros((10, 20), np.float32), [1.0])
+    assert e.value.code == 9
     g = np.zeros((10, 20), np.float32)
     with pytest.raises(GeminiFKError) as e:
         syn.synthetics(1, 2, 1 / 64, 0.0, nwn, g, g, [1.0], [0.0], np.ones(6), 128, 1.0)
@@ -210,3 +216,130 @@ def test_regional_gfk_columns_match_oracle():
         print(f"regional f={f} iwn={iw0}..{iw0 + n - 1}: max rel-L2 {d.max():.2e}", flush=True)
         assert d.max() < 1e-12
     eng.close()
+
+
+# ---- single-force source (5 spectra U1 U2 V1 V2 W1, 7 sums) ----
+
+def _random_force_spectra(rng, nwn):
+    nkf = int(nwn.sum())
+    l = np.concatenate([np.arange(n) for n in nwn])
+    env = np.exp(-l / 80.0) + 0.05
+    gr = (rng.normal(size=(5, nkf)) * env).astype(np.float32)
+    gi = (rng.normal(size=(5, nkf)) * env).astype(np.float32)
+    return gr, gi
+
+
+def test_force_sums_match_oracle(syn):
+    rng = np.random.default_rng(31)
+    nf1, nf2 = 3, 10
+    nwn_all = np.zeros(nf2, np.int32)
+    nwn_all[nf1 - 1:] = np.linspace(130, 420, nf2 - nf1 + 1).astype(np.int32)
+    gr, gi = _random_force_spectra(rng, nwn_all[nf1 - 1:])
+    # every sum but kint 1 starts at l = 1: l = 0 of spectra U2, V1, V2, W1 is never read
+    off = np.concatenate([[0], np.cumsum(nwn_all[nf1 - 1:])[:-1]])
+    gr[1:, off] = np.nan
+    theta = np.sort(rng.uniform(0.02, 3.12, size=41))
+    print(f"force sums: {theta.size} receivers, {nf2 - nf1 + 1} frequencies", flush=True)
+    g = syn.force_sums(nf1, nf2, nwn_all, gr, gi, theta)                      # [7, R, nf]
+    o = oracle.force_sums(nf1, nf2, nwn_all, gr, gi, theta).transpose(2, 0, 1)
+    assert g.shape == (7, theta.size, nf2 - nf1 + 1)
+    assert np.all(np.isfinite(g)) and np.all(np.isfinite(o))
+    err = np.abs(g - o) / (np.abs(o).max(axis=2, keepdims=True) + 1e-300)
+    print("max error relative to the largest sum of each (kint, receiver):", err.max(), flush=True)
+    assert err.max() < 1e-5
+
+
+def test_force_sums_generating_function(syn):
+    """kint 5 = -sum_l P_l^1/sin(theta) t^l = -t (1 - 2 x t + t^2)^(-3/2) through the GPU contraction."""
+    L, t = 400, 0.6
+    nwn = np.array([L], np.int32)
+    gr = np.zeros((5, L), np.float32)
+    gi = np.zeros((5, L), np.float32)
+    l = np.arange(L)
+    gi[4] = t ** l * 4 * float(np.float32(np.pi)) / (2 * l + 1) * l * (l + 1) / 2
+    theta = np.linspace(0.1, 3.0, 129)
+    h = syn.force_sums(1, 1, nwn, gr, gi, theta)[4, :, 0]
+    ref = -1j * t * (1 - 2 * np.cos(theta) * t + t * t) ** -1.5
+    err = np.abs(h - ref) / np.abs(ref)
+    print("generating function max rel error", err.max(), flush=True)
+    assert err.max() < 1e-6
+
+
+def test_force_synthetics_match_oracle(syn):
+    rng = np.random.default_rng(32)
+    nf1, nf2, nsamp = 2, 64, 256
+    T = 256.0
+    df, dt = 1.0 / T, T / nsamp
+    sigma = 5 * df
+    nwn_all = np.zeros(nf2, np.int32)
+    nwn_all[nf1 - 1:] = np.linspace(40, 120, nf2 - nf1 + 1).astype(np.int32)
+    gr, gi = _random_force_spectra(rng, nwn_all[nf1 - 1:])
+    R = 7
+    theta = rng.uniform(0.1, 3.0, R)
+    phi = rng.uniform(0, 2 * np.pi, R)
+    propdir = rng.uniform(0, 2 * np.pi, R)
+    tred = rng.uniform(0, 20, R)
+    force = rng.normal(size=3) * 1e15
+    print(f"force synthetics: {R} receivers, nf2 {nf2}, nsamp {nsamp}", flush=True)
+    for vel, pd, tr in [(False, None, None), (True, propdir, tred)]:
+        ts, zsp = syn.synthetics(nf1, nf2, df, sigma, nwn_all, gr, gi, theta, phi, force, nsamp, dt, propdir=pd,
+                                 tred=tr, velocity=vel, want_spectra=True)
+        ots, ozsp = oracle.synthetics(nf1, nf2, df, sigma, nwn_all, gr, gi, theta, phi, force, nsamp, dt,
+                                      propdir=pd, tred=tr, velocity=vel)
+        zerr = np.abs(zsp - ozsp).max() / np.abs(ozsp).max()
+        ts64, ots64 = ts.astype(np.float64), ots.astype(np.float64)
+        terr = np.linalg.norm(ts64 - ots64, axis=2) / np.linalg.norm(ots64, axis=2)
+        print(f"velocity={vel}: spectra max rel {zerr:.2e}, seismogram rel-L2 max {terr.max():.2e}", flush=True)
+        assert zerr < 1e-5
+        assert terr.max() < 1e-5
+
+
+def test_force_bessel_sums_match_oracle(syn):
+    """Regional force wavenumber integrals on both sides of the trapezoid/Filon switch."""
+    rng = np.random.default_rng(33)
+    nf1, nf2 = 2, 7
+    nwn_all = np.zeros(nf2, np.int32)
+    nwn_all[nf1 - 1:] = [800, 900, 1000, 1100, 1200, 1300]
+    nkf = int(nwn_all.sum())
+    dwn, re = 1e-5, 6371000.0
+    k = np.concatenate([np.arange(n) * dwn for n in nwn_all[nf1 - 1:]])
+    env = np.exp(-k / 4e-3) + 0.05
+    gr = (rng.normal(size=(5, nkf)) * env).astype(np.float32)
+    gi = (rng.normal(size=(5, nkf)) * env).astype(np.float32)
+    x = np.array([150.0, 250.0, 375.0, 3.0 / (998.5 * dwn), 700.0, 1500.0, 5e3, 2e4, 1e5, 2e5])  # 4th: j3 = 1000
+    print(f"force bessel sums: {x.size} receivers, {nf2 - nf1 + 1} frequencies", flush=True)
+    g = syn.force_bessel_sums(nf1, nf2, nwn_all, gr, gi, dwn, re, x)
+    o = oracle.bessel_sums(nf1, nf2, nwn_all, gr, gi, dwn, re, x, istyp=0).transpose(2, 0, 1)
+    assert g.shape == (7, x.size, nf2 - nf1 + 1) and np.all(np.isfinite(g))
+    err = np.abs(g - o) / (np.abs(o).max(axis=2, keepdims=True) + 1e-300)
+    print("max error relative to the largest integral of each (kint, receiver):", err.max(), flush=True)
+    assert err.max() < 1e-5
+
+
+def test_force_regional_synthetics_match_oracle(syn):
+    rng = np.random.default_rng(34)
+    nf1, nf2, nsamp = 2, 40, 128
+    T = 51.2
+    df, dt = 1.0 / T, T / nsamp
+    sigma = 5 * df
+    dwn, re = 2e-5, 6371000.0
+    nwn_all = np.zeros(nf2, np.int32)
+    nwn_all[nf1 - 1:] = np.linspace(300, 600, nf2 - nf1 + 1).astype(np.int32)
+    nkf = int(nwn_all.sum())
+    k = np.concatenate([np.arange(n) * dwn for n in nwn_all[nf1 - 1:]])
+    env = np.exp(-k / 3e-3) + 0.02
+    gr = (rng.normal(size=(5, nkf)) * env).astype(np.float32)
+    gi = (rng.normal(size=(5, nkf)) * env).astype(np.float32)
+    R = 5
+    x = rng.uniform(1e3, 2e5, R)
+    phi = rng.uniform(0, 2 * np.pi, R)
+    force = rng.normal(size=3) * 1e15
+    ts, zsp = syn.synthetics(nf1, nf2, df, sigma, nwn_all, gr, gi, x, phi, force, nsamp, dt, velocity=True,
+                             want_spectra=True, dwn=dwn, rearth=re)
+    ots, ozsp = oracle.synthetics(nf1, nf2, df, sigma, nwn_all, gr, gi, x / re, phi, force, nsamp, dt,
+                                  velocity=True, dwn=dwn, rearth=re)
+    zerr = np.abs(zsp - ozsp).max() / np.abs(ozsp).max()
+    ts64, ots64 = ts.astype(np.float64), ots.astype(np.float64)
+    terr = np.linalg.norm(ts64 - ots64, axis=2) / np.linalg.norm(ots64, axis=2)
+    print(f"regional force: spectra max rel {zerr:.2e}, seismogram rel-L2 max {terr.max():.2e}", flush=True)
+    assert zerr < 1e-5 and terr.max() < 1e-5

@@ -204,3 +204,99 @@ def test_wavenumber_integrals_converge_to_closed_form(kint):
     for a, b in zip(errs, errs[1:]):
         assert 3.5 < a / b < 4.5, errs          # O(dwn^2)
 
+
+
+# ---- single-force source ----
+
+@pytest.mark.parametrize("theta", [0.4, 1.3, 2.6])
+def test_force_sums_generating_functions(theta):
+    """Each of the 7 force sums against a closed-form generating function.  Spectrum s carries its own
+    ratio t_s, so a sum that reads the wrong spectrum, basis row or sign misses its closed form:
+      kint 1 (U1): sum P_l t^l = D^-1/2;  kint 2 (U2): sum P_l^1 t^l = s t D^-3/2;
+      kint 3 (V1): sum dP_l/dtheta t^l = -s t D^-3/2;
+      kint 4, 7 (V2): -sum dP_l^1/dtheta t^l, +sum P_l^1/sin t^l with d/dtheta (s t D^-3/2) = c t D^-3/2 - 3 t^2 s^2 D^-5/2;
+      kint 5, 6 (W1): -sum P_l^1/sin t^l = -t D^-3/2 and +sum dP_l^1/dtheta t^l,
+    with D = 1 - 2 t cos(theta) + t^2 (P_l^m without the Condon-Shortley sign)."""
+    L = 240
+    ts = [0.3, 0.4, 0.5, 0.55, 0.6]
+    nwn = np.array([L], np.int32)
+    gr = np.zeros((5, L), np.float32)
+    gi = np.zeros((5, L), np.float32)
+    for s_, t in enumerate(ts):
+        g = _gen_spectra(1, L, t, "m0" if s_ in (0, 2) else "m1")
+        if s_ in (1, 3):
+            g = -g                                   # the -2/(l(l+1)) of kint 2 and 4 cancels
+        (gi if s_ == 3 else gr)[s_] = g              # V2 in the imaginary part
+    h = oracle.force_sums(1, 1, nwn, gr, gi, [theta])[0, 0]
+    c, s = np.cos(theta), np.sin(theta)
+    D = lambda t: 1 - 2 * t * c + t * t              # noqa: E731
+    dP1 = lambda t: c * t * D(t) ** -1.5 - 3 * t * t * s * s * D(t) ** -2.5     # noqa: E731
+    t0, t1, t2, t3, t4 = ts
+    ref = [D(t0) ** -0.5, s * t1 * D(t1) ** -1.5, -s * t2 * D(t2) ** -1.5, 1j * dP1(t3),
+           -t4 * D(t4) ** -1.5, dP1(t4), 1j * (-t3 * D(t3) ** -1.5)]
+    for k in range(7):
+        assert abs(h[k] - ref[k]) < 1e-6 * max(abs(ref[k]), 1.0), (k + 1, h[k], ref[k])
+
+
+def _force_case(rng, L=60):
+    gr = rng.normal(size=(5, L)).astype(np.float32)
+    gi = rng.normal(size=(5, L)).astype(np.float32)
+    return np.array([L], np.int32), gr, gi
+
+
+def test_force_displacement_rotation_invariance():
+    """Turning the horizontal force and the receiver azimuth by the same angle leaves the
+    (r, theta, phi) displacement unchanged; a vertical force moves nothing transversely and does
+    not depend on azimuth."""
+    rng = np.random.default_rng(5)
+    nwn, gr, gi = _force_case(rng)
+    th = [0.7] * 4
+    phi = np.array([0.2, 1.1, 2.9, 5.0])
+    f = np.array([0.3, 1.2, -0.7])
+    _, z0 = oracle.synthetics(1, 1, 1.0 / 64, 0.0, nwn, gr, gi, th, phi, f, 4, 16.0)
+    for a in (0.4, 2.2):
+        fr = np.array([f[0], f[1] * np.cos(a) - f[2] * np.sin(a), f[1] * np.sin(a) + f[2] * np.cos(a)])
+        _, z1 = oracle.synthetics(1, 1, 1.0 / 64, 0.0, nwn, gr, gi, th, phi + a, fr, 4, 16.0)
+        np.testing.assert_allclose(z1[:, :, 0], z0[:, :, 0], rtol=1e-12, atol=1e-12 * np.abs(z0).max())
+    _, zv = oracle.synthetics(1, 1, 1.0 / 64, 0.0, nwn, gr, gi, th, phi, [1.0, 0.0, 0.0], 4, 16.0)
+    zv = zv[:, :, 0]
+    assert np.all(np.abs(zv[:, 2]) == 0.0)
+    np.testing.assert_allclose(zv, np.broadcast_to(zv[0], zv.shape), rtol=1e-13)
+    # the horizontal force's radial motion is linear in (Ft cos phi + Ff sin phi): it vanishes
+    # for the azimuth perpendicular to the force
+    ph0 = np.arctan2(f[1], -f[2])
+    _, zp = oracle.synthetics(1, 1, 1.0 / 64, 0.0, nwn, gr, gi, [0.7], [ph0], [0.0, f[1], f[2]], 4, 16.0)
+    assert abs(zp[0, 0, 0]) < 1e-13 * np.abs(z0).max() and abs(zp[0, 1, 0]) < 1e-13 * np.abs(z0).max()
+
+
+def _force_wn_closed_form(dwn, x, kint, a=1000.0, kmax=0.04):
+    """Force wavenumber integral of G(k) = exp(-a k) 2 pi / R^2 with closed forms of Laplace transforms:
+    kint 1: int k J0 e^{-ak} = a / (a^2+x^2)^{3/2};  kint 2: -(2/R) int J1 e^{-ak} = -(2/R)(1 - a/r)/x;
+    kint 3: -R int k^2 J1 e^{-ak} = -R 3 a x / r^5;  kint 5, 7: -+(2/x) int J1 e^{-ak} = -+(2/x)(1 - a/r)/x;
+    kint 4, 6: -+2 int k (J0 - J1/(kx)) e^{-ak} = -+2 (a / r^3 - (1 - a/r) / x^2);  r = sqrt(a^2 + x^2)."""
+    pi32, re = float(np.float32(np.pi)), 6371000.0
+    nwn = int(round(kmax / dwn))
+    k = np.arange(nwn) * dwn
+    gr = np.zeros((5, nwn), np.float32)
+    gi = np.zeros((5, nwn), np.float32)
+    spec = {1: 0, 2: 1, 3: 2, 4: 3, 5: 4, 6: 4, 7: 3}[kint]
+    scale = {2: 1e26, 3: 1e14}.get(kint, 1e20)               # keep the FP32 spectra and results in range
+    gr[spec] = (np.exp(-a * k) * 2 * pi32 / re ** 2 * scale).astype(np.float32)
+    h = oracle.bessel_sums(1, 1, np.array([nwn], np.int32), gr, gi, dwn, re, [x], istyp=0)[0, 0, kint - 1].real / scale
+    r = np.sqrt(a * a + x * x)
+    r4 = 2 * (a / r ** 3 - (1 - a / r) / x ** 2)
+    ref = {1: a / r ** 3, 2: -2 / re * (1 - a / r) / x, 3: -re * 3 * a * x / r ** 5, 4: -r4,
+           5: -2 / x * (1 - a / r) / x, 6: r4, 7: 2 / x * (1 - a / r) / x}[kint]
+    return h, ref
+
+
+@pytest.mark.parametrize("kint", [1, 2, 3, 4, 5, 6, 7])
+def test_force_wavenumber_integrals_converge_to_closed_form(kint):
+    x = 5000.0
+    errs = []
+    for dwn in (2e-5, 1e-5, 5e-6):
+        h, ref = _force_wn_closed_form(dwn, x, kint)
+        errs.append(abs(h - ref) / abs(ref))
+    assert errs[2] < 1e-3, errs
+    for a, b in zip(errs, errs[1:]):
+        assert 3.5 < a / b < 4.5, errs          # O(dwn^2)
