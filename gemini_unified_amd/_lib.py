@@ -20,7 +20,8 @@ EXPORTS = ["gm_ctx_create", "gm_ctx_destroy", "gm_set_option", "gm_last_error", 
            "gm_nodes_set", "gm_minors_batch", "gm_gfk_batch", "gm_last_timing", "gm_last_system_stats",
            "gm_mlp_nparams", "gm_mlp_forward", "gm_mlp_grad", "gm_mlp_train", "gm_set_proposer",
            "gm_collect_steplog", "gm_minors_batch_wn", "gm_gfk_batch_wn", "gm_legendre_batch", "gm_moment_sums", "gm_synthetics", "gm_last_synth_timing",
-           "gm_bessel_sums", "gm_synthetics_regional", "gm_force_sums", "gm_force_bessel_sums"]
+           "gm_bessel_sums", "gm_synthetics_regional", "gm_force_sums", "gm_force_bessel_sums",
+           "gm_event_filter"]
 
 
 class GeminiFKError(RuntimeError):
@@ -64,15 +65,16 @@ def lib():
         L.gm_legendre_batch.argtypes = [VP, C.c_int, D, C.c_int, D, D, D, D, D]
         L.gm_moment_sums.argtypes = [VP, C.c_int, C.c_int, I, C.c_int, C.c_longlong, VP, VP, C.c_int, D, C.c_int, VP]
         L.gm_synthetics.argtypes = [VP, C.c_int, C.c_int, C.c_double, C.c_double, I, C.c_int, C.c_longlong, VP, VP,
-                                    C.c_int, D, D, D, D, D, C.c_int, C.c_int, C.c_double, C.c_int, VP, VP]
+                                    C.c_int, D, D, D, D, D, D, D, C.c_int, C.c_int, C.c_double, C.c_int, VP, VP]
         L.gm_last_synth_timing.argtypes = [VP, D]
         L.gm_bessel_sums.argtypes = [VP, C.c_int, C.c_int, I, C.c_int, C.c_longlong, VP, VP, C.c_double, C.c_double,
                                      C.c_int, D, C.c_int, VP]
         L.gm_force_sums.argtypes = L.gm_moment_sums.argtypes
         L.gm_force_bessel_sums.argtypes = L.gm_bessel_sums.argtypes
         L.gm_synthetics_regional.argtypes = [VP, C.c_int, C.c_int, C.c_double, C.c_double, I, C.c_int, C.c_longlong,
-                                             VP, VP, C.c_double, C.c_double, C.c_int, D, D, D, D, D, C.c_int, C.c_int,
-                                             C.c_double, C.c_int, VP, VP]
+                                             VP, VP, C.c_double, C.c_double, C.c_int, D, D, D, D, D, D, D, C.c_int,
+                                             C.c_int, C.c_double, C.c_int, VP, VP]
+        L.gm_event_filter.argtypes = [C.c_int, D, C.c_int, C.c_int, C.c_double, C.c_double, D]
         _lib = L
     return _lib
 

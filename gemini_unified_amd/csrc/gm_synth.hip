@@ -517,12 +517,14 @@ __global__ void k_wnfix(int R, int nf, int Rp, int Np, const int* __restrict__ n
 // Displacement spectra Z[r][3][nf2] (zero below nf1).
 // forceSingleFrequencyDisplacement (istyp 0, fomt = Fr, Ft, Ff) or
 // momentSingleFrequencyDisplacement (istyp 1, fomt = Mrr, Mtt, Mff, Mrt, Mrf,
-// Mtf) (singleFrequencyDisplacement.f90:35-46, 108-125), time shift
-// (computeSyntheticSeismograms.f90:361-365), velocity (:376-378), ZNE rotation
-// (axesRotation.f90:560-568).
+// Mtf) (singleFrequencyDisplacement.f90:35-46, 108-125), time shift and the
+// event and station filters (computeSyntheticSeismograms.f90:361-371; evf:
+// complex [nf2], stf: complex [R][3][nf2], NULL: unity), velocity (:376-378),
+// ZNE rotation (axesRotation.f90:560-568).
 __global__ void k_combine(int R, int nf1, int nf2, int Rp, int Np, const double* __restrict__ H,
                           const double* __restrict__ phi, const double* __restrict__ propdir,
-                          const double* __restrict__ tred, const double* __restrict__ mt, int velocity, double df,
+                          const double* __restrict__ tred, const double* __restrict__ evf,
+                          const double* __restrict__ stf, const double* __restrict__ mt, int velocity, double df,
                           double sigma, int istyp, double* __restrict__ Z) {
   const int id = blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= R * nf2) return;
@@ -562,6 +564,18 @@ __global__ void k_combine(int R, int nf1, int nf2, int Rp, int Np, const double*
       const cd tsf{cs * ex, sn * ex};
 #pragma unroll
       for (int k = 0; k < 3; ++k) u[k] = cmul(u[k], tsf);
+    }
+    if (evf) {
+      const cd e{evf[2 * (jf - 1)], evf[2 * (jf - 1) + 1]};
+#pragma unroll
+      for (int k = 0; k < 3; ++k) u[k] = cmul(u[k], e);
+    }
+    if (stf) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const size_t o = (((size_t)r * 3 + k) * nf2 + jf - 1) * 2;
+        u[k] = cmul(u[k], cd{stf[o], stf[o + 1]});
+      }
     }
     if (velocity) {
       const cd iz{sigma, om};      // i * (om - i sigma)
@@ -821,8 +835,9 @@ static int sums_impl(gm_ctx* c, int istyp, int nf1, int nf2, const int* nwn, int
 
 static int synth_impl(gm_ctx* c, int nf1, int nf2, double df, double sigma, const int* nwn, int nsp, long long nkf,
                       const float* gfkspr, const float* gfkspi, int nrcv, const double* theta, const double* phi,
-                      const double* propdir, const double* tred, const double* mt, int velocity, int nsamp,
-                      double dt, int flags, float* ts, double* zsp, double dwn, double re) {
+                      const double* propdir, const double* tred, const double* evfilter, const double* stafilter,
+                      const double* mt, int velocity, int nsamp, double dt, int flags, float* ts, double* zsp,
+                      double dwn, double re) {
   if (!c) return GM_ERR_ARG;
   SynPlan P;
   int rc = plan_sums(c, nf1, nf2, nwn, nsp, nkf, nrcv, P);
@@ -850,6 +865,8 @@ static int synth_impl(gm_ctx* c, int nf1, int nf2, double df, double sigma, cons
   SumBufs o = carve_sums(P, dev, cv);
   const size_t o_phi = cv.take(sizeof(double) * P.R), o_pd = cv.take(sizeof(double) * P.R);
   const size_t o_tr = cv.take(sizeof(double) * P.R), o_mt = cv.take(sizeof(double) * 6);
+  const size_t o_ev = cv.take(sizeof(double) * 2 * (size_t)nf2);
+  const size_t o_st = stafilter ? cv.take(sizeof(double) * 2 * 3 * (size_t)P.R * nf2) : 0;
   const size_t o_z = cv.take(sizeof(double) * 2 * 3 * (size_t)P.R * nf2);
   const size_t o_ts = dev ? 0 : cv.take(sizeof(float) * 3 * (size_t)P.R * nsamp);
   if ((rc = ensure_sbuf(c, cv.off))) return rc;
@@ -858,12 +875,17 @@ static int synth_impl(gm_ctx* c, int nf1, int nf2, double df, double sigma, cons
   if (propdir) HIPCHK(c, hipMemcpyAsync(base + o_pd, propdir, sizeof(double) * P.R, hipMemcpyHostToDevice, s));
   if (tred) HIPCHK(c, hipMemcpyAsync(base + o_tr, tred, sizeof(double) * P.R, hipMemcpyHostToDevice, s));
   HIPCHK(c, hipMemcpyAsync(base + o_mt, mt, sizeof(double) * (P.istyp ? 6 : 3), hipMemcpyHostToDevice, s));
+  if (evfilter) HIPCHK(c, hipMemcpyAsync(base + o_ev, evfilter, sizeof(double) * 2 * nf2, hipMemcpyHostToDevice, s));
+  if (stafilter)
+    HIPCHK(c, hipMemcpyAsync(base + o_st, stafilter, sizeof(double) * 2 * 3 * (size_t)P.R * nf2, hipMemcpyHostToDevice, s));
   if ((rc = run_sums(c, P, theta, gfkspr, gfkspi, dev, base, o, s, dwn, re))) return rc;
   double* dZ = (double*)(base + o_z);
   hipLaunchKernelGGL(k_combine, dim3((unsigned)(((size_t)P.R * nf2 + 255) / 256)), dim3(256), 0, s, P.R, nf1, nf2,
                      P.Rp, P.Np, (const double*)(base + o.H), (const double*)(base + o_phi),
                      propdir ? (const double*)(base + o_pd) : nullptr, tred ? (const double*)(base + o_tr) : nullptr,
-                     (const double*)(base + o_mt), velocity ? 1 : 0, df, sigma, P.istyp, dZ);
+                     evfilter ? (const double*)(base + o_ev) : nullptr,
+                     stafilter ? (const double*)(base + o_st) : nullptr, (const double*)(base + o_mt),
+                     velocity ? 1 : 0, df, sigma, P.istyp, dZ);
   HIPCHK(c, hipEventRecord(c->sev[4], s));
   float* dts = dev ? ts : (float*)(base + o_ts);
   int logm = 0;
@@ -918,23 +940,66 @@ extern "C" int gm_force_bessel_sums(gm_ctx* c, int nf1, int nf2, const int* nwn,
 
 extern "C" int gm_synthetics(gm_ctx* c, int nf1, int nf2, double df, double sigma, const int* nwn, int nsp,
                              long long nkf, const float* gfkspr, const float* gfkspi, int nrcv, const double* theta,
-                             const double* phi, const double* propdir, const double* tred, const double* mt,
-                             int velocity, int nsamp, double dt, int flags, float* ts, double* zsp) {
-  return synth_impl(c, nf1, nf2, df, sigma, nwn, nsp, nkf, gfkspr, gfkspi, nrcv, theta, phi, propdir, tred, mt,
-                    velocity, nsamp, dt, flags, ts, zsp, 0.0, 0.0);
+                             const double* phi, const double* propdir, const double* tred, const double* evfilter,
+                             const double* stafilter, const double* fomt, int velocity, int nsamp, double dt, int flags,
+                             float* ts, double* zsp) {
+  return synth_impl(c, nf1, nf2, df, sigma, nwn, nsp, nkf, gfkspr, gfkspi, nrcv, theta, phi, propdir, tred, evfilter,
+                    stafilter, fomt, velocity, nsamp, dt, flags, ts, zsp, 0.0, 0.0);
 }
 
 extern "C" int gm_synthetics_regional(gm_ctx* c, int nf1, int nf2, double df, double sigma, const int* nwn, int nsp,
                                       long long nkf, const float* gfkspr, const float* gfkspi, double dwn,
                                       double rearth, int nrcv, const double* x, const double* phi,
-                                      const double* propdir, const double* tred, const double* mt, int velocity,
-                                      int nsamp, double dt, int flags, float* ts, double* zsp) {
+                                      const double* propdir, const double* tred, const double* evfilter,
+                                      const double* stafilter, const double* fomt, int velocity, int nsamp, double dt,
+                                      int flags, float* ts, double* zsp) {
   if (!c) return GM_ERR_ARG;
   if (!(dwn > 0.0) || !(rearth > 0.0)) return gm_fail(c, GM_ERR_ARG, "dwn and rearth must be positive");
   for (int r = 0; r < nrcv && x; ++r)
     if (!(x[r] > 0.0)) return gm_fail(c, GM_ERR_ARG, "epicentral distance %d must be positive", r);
-  return synth_impl(c, nf1, nf2, df, sigma, nwn, nsp, nkf, gfkspr, gfkspi, nrcv, x, phi, propdir, tred, mt, velocity,
-                    nsamp, dt, flags, ts, zsp, dwn, rearth);
+  return synth_impl(c, nf1, nf2, df, sigma, nwn, nsp, nkf, gfkspr, gfkspi, nrcv, x, phi, propdir, tred, evfilter,
+                    stafilter, fomt, velocity, nsamp, dt, flags, ts, zsp, dwn, rearth);
+}
+
+extern "C" int gm_event_filter(int filtype, const double* filspecs, int nf1, int nf2, double df, double sigma,
+                               double* hfil) {
+  if (filtype != GM_FILTER_BUTTERWORTH_BANDPASS)
+    return gm_fail(nullptr, GM_ERR_ARG, "Unknown event filter type specified (%d)", filtype);
+  if (!filspecs || !hfil || nf1 < 1 || nf2 < nf1 || !(df > 0.0))
+    return gm_fail(nullptr, GM_ERR_ARG, "event filter: filspecs and hfil required, 1 <= nf1 <= nf2, df > 0");
+  // high-pass (order filspecs[0], corner filspecs[1]) times low-pass (order
+  // filspecs[2], corner filspecs[3]) Butterworth responses at the damped
+  // frequencies z = (f - i sigma / 2 pi) / fc: each order-n factor is the
+  // product over the unit roots e_j = e^{i pi (j - 1/2) / n} of z / (z - e_j)
+  // (high-pass) or -i / (z - e_j) (low-pass).  The frequency is formed from
+  // its index, not by the reference's running sum.
+  const int nord[2] = {(int)std::lround(filspecs[0]), (int)std::lround(filspecs[2])};
+  const double fc[2] = {filspecs[1], filspecs[3]};
+  for (int k = 0; k < 2; ++k)
+    if (nord[k] < 0 || (nord[k] > 0 && !(fc[k] > 0.0)))
+      return gm_fail(nullptr, GM_ERR_ARG, "event filter: orders must be >= 0 and corner frequencies positive");
+  for (int jf = 1; jf <= nf2; ++jf) {
+    double hr = 0.0, hi = 0.0;
+    if (jf >= nf1) {
+      hr = 1.0;
+      for (int k = 0; k < 2; ++k) {
+        const double zr = (jf - 1) * df / fc[k], zi = -sigma / (2.0 * kPi) / fc[k];
+        for (int j = 1; j <= nord[k]; ++j) {
+          const double a = kPi * (j - 0.5) / nord[k];
+          const double dr = zr - std::cos(a), di = zi - std::sin(a);
+          const double nr = k == 0 ? zr : 0.0, ni = k == 0 ? zi : -1.0;     // numerator z or -i
+          const double q = dr * dr + di * di;
+          const double fr = (nr * dr + ni * di) / q, fi = (ni * dr - nr * di) / q;
+          const double tr = hr * fr - hi * fi;
+          hi = hr * fi + hi * fr;
+          hr = tr;
+        }
+      }
+    }
+    hfil[2 * (jf - 1)] = hr;
+    hfil[2 * (jf - 1) + 1] = hi;
+  }
+  return GM_OK;
 }
 
 extern "C" int gm_last_synth_timing(gm_ctx* c, double* ms6) {

@@ -106,11 +106,14 @@ class SyntheticsEngine:
         return self._wn_sums(_lib.lib().gm_force_bessel_sums, FORCE_SUMS, nf1, nf2, nwn, gr, gi, dwn, rearth, x)
 
     def synthetics(self, nf1, nf2, df, sigma, nwn, gr, gi, theta, phi, mt, nsamp, dt, propdir=None, tred=None,
-                   velocity=False, want_spectra=False, device_ptrs=None, dwn=None, rearth=None):
+                   velocity=False, want_spectra=False, device_ptrs=None, dwn=None, rearth=None, evfilter=None,
+                   stafilter=None):
         """Seismograms [nrcv, 3, nsamp] FP32 (and displacement spectra [nrcv, 3, nf2] if asked).
 
         The source type follows the number of spectra nsp: 10, moment tensor, mt = (Mrr, Mtt, Mff, Mrt,
         Mrf, Mtf); 5, single force, mt = (Fr, Ft, Ff) (getForceMomentSeismicEvent's fomt).
+        evfilter: complex [nf2] event filter (e.g. event_filter()), stafilter: complex [nrcv, 3, nf2]
+        station filters of the r, theta, phi components; None: unity.
         Global branch: theta = epicentral distances in radians.  Regional branch (dwn given):
         theta = epicentral distances in metres, wavenumber integrals with k = (j-1) dwn.
         device_ptrs = (gr_ptr, gi_ptr, nsp, nkf, ts_ptr): spectra and output already in HBM
@@ -123,6 +126,10 @@ class SyntheticsEngine:
         nsp_in = device_ptrs[2] if device_ptrs is not None else np.shape(gr)[0]
         if phi.size != theta.size or mt.size != {10: 6, 5: 3}.get(int(nsp_in), 6):
             raise ValueError("phi must match theta and mt must hold 6 (moment) or 3 (force, nsp = 5) entries")
+        ev = None if evfilter is None else np.ascontiguousarray(evfilter, dtype=np.complex128)
+        st = None if stafilter is None else np.ascontiguousarray(stafilter, dtype=np.complex128)
+        if (ev is not None and ev.shape != (nf2,)) or (st is not None and st.shape != (theta.size, 3, nf2)):
+            raise ValueError("evfilter must be [nf2] and stafilter [nrcv, 3, nf2]")
         pd = None if propdir is None else np.ascontiguousarray(propdir, dtype=np.float64)
         tr = None if tred is None else np.ascontiguousarray(tred, dtype=np.float64)
         zsp = np.zeros((theta.size, 3, nf2), np.complex128) if want_spectra else None
@@ -137,7 +144,8 @@ class SyntheticsEngine:
             pgr, pgi, nsp, nkf, pts = device_ptrs
             ts, flags = None, 1
         tail = (theta.size, theta.ctypes.data_as(D), phi.ctypes.data_as(D), None if pd is None else pd.ctypes.data_as(D),
-                None if tr is None else tr.ctypes.data_as(D), mt.ctypes.data_as(D), int(bool(velocity)), int(nsamp),
+                None if tr is None else tr.ctypes.data_as(D), None if ev is None else ev.ctypes.data_as(D),
+                None if st is None else st.ctypes.data_as(D), mt.ctypes.data_as(D), int(bool(velocity)), int(nsamp),
                 float(dt), flags, pts, None if zsp is None else zsp.ctypes.data)
         L = _lib.lib()
         if dwn:
@@ -148,3 +156,18 @@ class SyntheticsEngine:
                                  int(nkf), pgr, pgi, *tail)
         check(self.h, rc)
         return (ts, zsp) if want_spectra else ts
+
+
+GM_FILTER_BUTTERWORTH_BANDPASS = 1
+
+
+def event_filter(filspecs, nf1, nf2, df, sigma, filtype=GM_FILTER_BUTTERWORTH_BANDPASS):
+    """computeEventfilter (f90/eventFilter.f90:40-83): complex [nf2], zero below nf1.
+    filspecs = (high-pass order, corner Hz, low-pass order, corner Hz)."""
+    spec = np.ascontiguousarray(filspecs, dtype=np.float64)
+    if spec.size != 4:
+        raise ValueError("filspecs holds 4 numbers")
+    h = np.zeros(nf2, np.complex128)
+    check(None, _lib.lib().gm_event_filter(int(filtype), spec.ctypes.data_as(D), int(nf1), int(nf2), float(df),
+                                           float(sigma), h.ctypes.data_as(D)))
+    return h

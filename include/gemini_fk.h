@@ -51,6 +51,7 @@ typedef struct gm_ctx gm_ctx;
 
 /* flags of the batch calls */
 #define GM_FLAG_DEVICE_PTRS 1  /* output pointers are device (HBM) pointers */
+#define GM_FILTER_BUTTERWORTH_BANDPASS 1  /* event filter type 'BUTTERWORTH_BANDPASS' (f90/eventFilter.f90:56) */
 
 /* options (gm_set_option) */
 #define GM_OPT_VARIANT 1       /* 0: exact (no FMA contraction, bitwise operation order), 1: fast (FMA) */
@@ -213,18 +214,22 @@ int gm_force_sums(gm_ctx* ctx, int nf1, int nf2, const int* nwn, int nsp, long l
  * GFK file: nsp = 10, moment source, fomt = Mrr, Mtt, Mff, Mrt, Mrf, Mtf
  * (f90/singleFrequencyDisplacement.f90:108-125); nsp = 5, single force,
  * fomt = Fr, Ft, Ff (:35-46).  Then time shift by tred[r] (NULL: none),
- * velocity (velocity = 1) and
+ * event and station filters (evfilter: complex [nf2], stafilter: complex
+ * [nrcv][3][nf2] for the components r, theta, phi, i.e. the reference's
+ * stafilter(nf2, 3, nstat); NULL: unity; f90/computeSyntheticSeismograms.f90:
+ * 367-371), velocity (velocity = 1) and
  * rotation to Z, N, E by propdir[r] (NULL: components r, theta, phi), then the
  * frequency-to-time transform to nsamp FP32 samples (transformFrequencyTime,
  * f90/frequencyTime.f90:55-113; nsamp = 2^ceil(-log2(df dt)) <= 8192, no
- * decimation).  Event and station filters are unity.  Replaces the station
- * loop body of f90/computeSyntheticSeismograms.f90:311-388.
+ * decimation).  Replaces the station loop body of
+ * f90/computeSyntheticSeismograms.f90:311-388 (without the travel-time
+ * automatic shift: tred is given).
  * ts: [nrcv][3][nsamp]; zsp (optional): complex [nrcv][3][nf2].
  * GM_FLAG_DEVICE_PTRS: gfkspr, gfkspi, ts and zsp are device pointers. */
 int gm_synthetics(gm_ctx* ctx, int nf1, int nf2, double df, double sigma, const int* nwn, int nsp, long long nkf,
                   const float* gfkspr, const float* gfkspi, int nrcv, const double* theta, const double* phi,
-                  const double* propdir, const double* tred, const double* fomt, int velocity, int nsamp, double dt,
-                  int flags, float* ts, double* zsp);
+                  const double* propdir, const double* tred, const double* evfilter, const double* stafilter,
+                  const double* fomt, int velocity, int nsamp, double dt, int flags, float* ts, double* zsp);
 
 /* Regional (Bessel) branch: the 14 moment wavenumber integrals (kint 1..14) of
  * one source node at epicentral distances x[r] in metres, for spectra sampled
@@ -253,7 +258,17 @@ int gm_force_bessel_sums(gm_ctx* ctx, int nf1, int nf2, const int* nwn, int nsp,
 int gm_synthetics_regional(gm_ctx* ctx, int nf1, int nf2, double df, double sigma, const int* nwn, int nsp,
                            long long nkf, const float* gfkspr, const float* gfkspi, double dwn, double rearth,
                            int nrcv, const double* x, const double* phi, const double* propdir, const double* tred,
-                           const double* fomt, int velocity, int nsamp, double dt, int flags, float* ts, double* zsp);
+                           const double* evfilter, const double* stafilter, const double* fomt, int velocity, int nsamp,
+                           double dt, int flags, float* ts, double* zsp);
+
+/* Event filter of the synthetics stage: computeEventfilter
+ * (f90/eventFilter.f90:40-83), type GM_FILTER_BUTTERWORTH_BANDPASS with
+ * filspecs = (high-pass order, high-pass corner Hz, low-pass order, low-pass
+ * corner Hz): the product of highPassComplexOmegaButterworthFilter and
+ * lowPassComplexOmegaButterworthFilter (f90/butterworthFilter.f90:49-96) at the
+ * damped frequencies (jf-1) df - i sigma / (2 pi).  Order 0 is unity.
+ * hfil: complex [nf2], zero below nf1.  Host computation, no context. */
+int gm_event_filter(int filtype, const double* filspecs, int nf1, int nf2, double df, double sigma, double* hfil);
 
 /* Kernel time of the last gm_moment_sums / gm_synthetics call in milliseconds:
  * 0 Legendre basis, 1 spectra preparation, 2 contraction, 3 combination,

@@ -322,10 +322,11 @@ def bessel_sums(nf1, nf2, nwn, gr, gi, dwn, rearth, x, tapfrac=0.2, nthreads=1, 
 
 
 def synthetics(nf1, nf2, df, sigma, nwn, gr, gi, theta, phi, mt, nsamp, dt, propdir=None, tred=None, velocity=False,
-               nthreads=1, dwn=0.0, rearth=0.0):
+               nthreads=1, dwn=0.0, rearth=0.0, evfilter=None, stafilter=None):
     """(ts float32 [nrcv, 3, nsamp], zsp complex [nrcv, 3, nf2]) of one source node; dwn > 0 selects the
     regional (wavenumber integral) branch with distances theta * rearth.  The source type follows the
-    number of spectra: 10 (moment, mt = Mrr, Mtt, Mff, Mrt, Mrf, Mtf) or 5 (force, mt = Fr, Ft, Ff)."""
+    number of spectra: 10 (moment, mt = Mrr, Mtt, Mff, Mrt, Mrf, Mtf) or 5 (force, mt = Fr, Ft, Ff).
+    evfilter: complex [nf2], stafilter: complex [nrcv, 3, nf2] (None: unity)."""
     nwn = np.ascontiguousarray(nwn, dtype=np.int32)
     gr = np.ascontiguousarray(gr, dtype=np.float32)
     gi = np.ascontiguousarray(gi, dtype=np.float32)
@@ -338,16 +339,29 @@ def synthetics(nf1, nf2, df, sigma, nwn, gr, gi, theta, phi, mt, nsamp, dt, prop
     F = C.POINTER(C.c_float)
     pd = None if propdir is None else np.ascontiguousarray(propdir, dtype=np.float64)
     tr = None if tred is None else np.ascontiguousarray(tred, dtype=np.float64)
+    ev = None if evfilter is None else np.ascontiguousarray(evfilter, dtype=np.complex128)
+    st = None if stafilter is None else np.ascontiguousarray(stafilter, dtype=np.complex128)
     istyp = {10: 1, 5: 0}[gr.shape[0]]
     assert mt.size == (6 if istyp == 1 else 3)
     rc = lib().orc_synthetics_src(istyp, int(nf1), int(nf2), C.c_double(df), C.c_double(sigma), nwn.ctypes.data_as(I),
                               C.c_long(gr.shape[1]), gr.ctypes.data_as(F), gi.ctypes.data_as(F), C.c_double(dwn),
                               C.c_double(rearth), nr, _d(theta),
-                              _d(phi), None if pd is None else _d(pd), None if tr is None else _d(tr), _d(mt),
+                              _d(phi), None if pd is None else _d(pd), None if tr is None else _d(tr),
+                              None if ev is None else ev.ctypes.data_as(D), None if st is None else st.ctypes.data_as(D),
+                              _d(mt),
                               int(bool(velocity)), int(nsamp), C.c_double(dt), ts.ctypes.data_as(F),
                               zsp.ctypes.data_as(D), int(nthreads))
     assert rc == 0, rc
     return ts, zsp
+
+
+def butterworth_bandpass(filspecs, nf1, nf2, df, sigma):
+    """computeEventfilter('BUTTERWORTH_BANDPASS', filspecs, ...): complex [nf2], zero below nf1."""
+    spec = np.ascontiguousarray(filspecs, dtype=np.float64)
+    h = np.zeros(nf2, np.complex128)
+    lib().orc_butterworth_bandpass(_d(spec), int(nf1), int(nf2), C.c_double(df), C.c_double(sigma),
+                                   h.ctypes.data_as(D))
+    return h
 
 
 def freq_time(nf1, nf2, df, sigma, sp, nsamp, dt):

@@ -236,6 +236,34 @@ int freq_time(int nf1, int nf2, double df, double sigma, const cd* sp, int nsamp
   return 0;
 }
 
+// Band-pass event filter: a high-pass Butterworth filter of order n1 and corner
+// f1 times a low-pass one of order n2 and corner f2, at the complex frequencies
+// f - i sigma / (2 pi), f = (jf-1) df, for jf = nf1..nf2; h[jf-1] (zero below nf1).
+// Restates computeEventfilter's BUTTERWORTH_BANDPASS case (f90/eventFilter.f90:40-83)
+// with highPassComplexOmegaButterworthFilter and lowPassComplexOmegaButterworthFilter
+// (f90/butterworthFilter.f90:49-96), including their running sum of the frequency.
+void butterworth_bandpass(int n1, double f1, int n2, double f2, int nf1, int nf2, double df, double sigma, cd* h) {
+  const int nf = nf2 - nf1 + 1;
+  const double kpi = kPi, two_pi = 2.0 * kPi;        // mc_pid, mc_two_pid (FP32 pi, as everywhere here)
+  std::vector<cd> hh(nf, 1.0), hl(nf, 1.0);
+  const cd i1(0.0, 1.0);
+  for (int pass = 0; pass < 2; ++pass) {
+    const int nord = pass == 0 ? n1 : n2;
+    const double fc = pass == 0 ? f1 : f2;
+    std::vector<cd>& hv = pass == 0 ? hh : hl;
+    const cd zfna = (cd((nf1 - 1) * df, 0.0) - i1 * sigma / two_pi) / fc;
+    for (int j = 1; j <= nord; ++j) {
+      cd zfn = zfna;
+      const cd ep = std::exp(i1 * kpi * (double(j) - 0.5) / double(nord));
+      for (int i = 0; i < nf; ++i) {
+        hv[i] = pass == 0 ? hv[i] * zfn / (zfn - ep) : hv[i] * (-i1) / (zfn - ep);
+        zfn = zfn + df / fc;
+      }
+    }
+  }
+  for (int jf = 1; jf <= nf2; ++jf) h[jf - 1] = jf < nf1 ? cd(0.0) : hh[jf - nf1] * hl[jf - nf1];
+}
+
 // ---- regional (Bessel) branch ----
 
 // cos^2 taper over the last tapfrac of the wavenumbers 1..nwn (1-based j, k = (j-1) dwn).
@@ -422,6 +450,13 @@ int orc_bessel_sums(int nf1, int nf2, const int* nwn, long nkf, const float* gr,
   return orc_bessel_sums_src(1, nf1, nf2, nwn, nkf, gr, gi, dwn, re, nrcv, x, tapfrac, wnint, nthreads);
 }
 
+// hfil: complex [nf2]; filspecs = (high-pass order, corner, low-pass order, corner)
+int orc_butterworth_bandpass(const double* filspecs, int nf1, int nf2, double df, double sigma, double* hfil) {
+  syn::butterworth_bandpass((int)std::lround(filspecs[0]), filspecs[1], (int)std::lround(filspecs[2]), filspecs[3],
+                            nf1, nf2, df, sigma, reinterpret_cast<syn::cd*>(hfil));
+  return 0;
+}
+
 double orc_besselj(int order, double x) { return order == 0 ? ::j0(x) : ::j1(x); }
 double orc_bessely(int order, double x) { return order == 0 ? ::y0(x) : ::y1(x); }
 
@@ -436,10 +471,15 @@ double orc_bessely(int order, double x) { return order == 0 ? ::y0(x) : ::y1(x);
 // dwn > 0: regional mode (besselDisplacementSpectra, f90/displacementSpectra.f90:119-184):
 // theta[r] is the epicentral distance in radians as in the global mode and the
 // wavenumber integrals are taken at x = theta[r] * re.
+// evfilter: complex [nf2] or NULL; stafilter: complex [nrcv][3][nf2] or NULL
+// (the event and station filters of :367-371; NULL: unity).
 int orc_synthetics_src(int istyp, int nf1, int nf2, double df, double sigma, const int* nwn, long nkf,
                        const float* gr, const float* gi, double dwn, double re, int nrcv, const double* theta,
-                       const double* phi, const double* propdir, const double* tred, const double* fomt, int velocity,
-                       int nsamp, double dt, float* ts, double* zsp, int nthreads) {
+                       const double* phi, const double* propdir, const double* tred, const double* evfilter,
+                       const double* stafilter, const double* fomt, int velocity, int nsamp, double dt, float* ts,
+                       double* zsp, int nthreads) {
+  const syn::cd* evf = reinterpret_cast<const syn::cd*>(evfilter);
+  const syn::cd* stf = reinterpret_cast<const syn::cd*>(stafilter);
   const int nf = nf2 - nf1 + 1, nsum = istyp == 1 ? 14 : 7;
   int bad = 0;
 #pragma omp parallel for schedule(dynamic) num_threads(nthreads > 0 ? nthreads : 1) reduction(| : bad)
@@ -456,7 +496,13 @@ int orc_synthetics_src(int istyp, int nf1, int nf2, double df, double sigma, con
       // time shift filter, f90/computeSyntheticSeismograms.f90:361-365
       const double om = 2.0 * syn::kPi * (jf - 1) * df;
       const syn::cd tsf = std::exp(syn::cd(0.0, om * tr)) * std::exp(sigma * tr);
-      for (int c = 0; c < 3; ++c) u[c] *= tsf;
+      // event and station filters with the time shift, :367-371
+      for (int c = 0; c < 3; ++c) {
+        syn::cd fil = tsf;
+        if (evf) fil = evf[jf - 1] * fil;
+        if (stf) fil = stf[((size_t)r * 3 + c) * nf2 + jf - 1] * fil;
+        u[c] *= fil;
+      }
       if (velocity) {                                   // :376-378
         const syn::cd zom(om, -sigma);
         for (int c = 0; c < 3; ++c) u[c] = u[c] * syn::cd(0.0, 1.0) * zom;
@@ -479,8 +525,8 @@ int orc_synthetics_mode(int nf1, int nf2, double df, double sigma, const int* nw
                         const float* gi, double dwn, double re, int nrcv, const double* theta, const double* phi,
                         const double* propdir, const double* tred, const double* mt, int velocity, int nsamp,
                         double dt, float* ts, double* zsp, int nthreads) {
-  return orc_synthetics_src(1, nf1, nf2, df, sigma, nwn, nkf, gr, gi, dwn, re, nrcv, theta, phi, propdir, tred, mt,
-                            velocity, nsamp, dt, ts, zsp, nthreads);
+  return orc_synthetics_src(1, nf1, nf2, df, sigma, nwn, nkf, gr, gi, dwn, re, nrcv, theta, phi, propdir, tred,
+                            nullptr, nullptr, mt, velocity, nsamp, dt, ts, zsp, nthreads);
 }
 
 int orc_synthetics(int nf1, int nf2, double df, double sigma, const int* nwn, long nkf, const float* gr,

@@ -343,3 +343,48 @@ def test_force_regional_synthetics_match_oracle(syn):
     terr = np.linalg.norm(ts64 - ots64, axis=2) / np.linalg.norm(ots64, axis=2)
     print(f"regional force: spectra max rel {zerr:.2e}, seismogram rel-L2 max {terr.max():.2e}", flush=True)
     assert zerr < 1e-5 and terr.max() < 1e-5
+
+
+# ---- event and station filters ----
+
+def test_event_filter_matches_oracle():
+    from gemini_unified_amd.synthetics import event_filter
+    from gemini_unified_amd._lib import GeminiFKError
+    for spec in ([2, 0.01, 4, 0.08], [0, 0.01, 6, 0.15], [3, 0.004, 0, 1.0]):
+        h = event_filter(spec, 3, 300, 1.0 / 512, 5.0 / 512)
+        o = oracle.butterworth_bandpass(spec, 3, 300, 1.0 / 512, 5.0 / 512)
+        assert np.all(h[:2] == 0)
+        np.testing.assert_allclose(h, o, rtol=1e-12, atol=1e-15)
+    with pytest.raises(GeminiFKError) as e:
+        event_filter([2, 0.01, 4, 0.08], 3, 300, 1.0 / 512, 5.0 / 512, filtype=7)
+    assert e.value.code == 9
+
+
+def test_filtered_synthetics_match_oracle(syn):
+    from gemini_unified_amd.synthetics import event_filter
+    rng = np.random.default_rng(41)
+    nf1, nf2, nsamp = 2, 64, 256
+    T = 256.0
+    df, dt = 1.0 / T, T / nsamp
+    sigma = 5 * df
+    nwn_all = np.zeros(nf2, np.int32)
+    nwn_all[nf1 - 1:] = np.linspace(40, 120, nf2 - nf1 + 1).astype(np.int32)
+    gr, gi = _random_spectra(rng, nwn_all[nf1 - 1:])
+    R = 6
+    theta = rng.uniform(0.1, 3.0, R)
+    phi = rng.uniform(0, 2 * np.pi, R)
+    propdir = rng.uniform(0, 2 * np.pi, R)
+    tred = rng.uniform(0, 20, R)
+    mt = rng.normal(size=6) * 1e18
+    evf = event_filter([2, 0.01, 4, 0.15], nf1, nf2, df, sigma)
+    stf = rng.normal(size=(R, 3, nf2)) + 1j * rng.normal(size=(R, 3, nf2))
+    print(f"filtered synthetics: {R} receivers, nf2 {nf2}", flush=True)
+    ts, zsp = syn.synthetics(nf1, nf2, df, sigma, nwn_all, gr, gi, theta, phi, mt, nsamp, dt, propdir=propdir,
+                             tred=tred, velocity=True, want_spectra=True, evfilter=evf, stafilter=stf)
+    ots, ozsp = oracle.synthetics(nf1, nf2, df, sigma, nwn_all, gr, gi, theta, phi, mt, nsamp, dt, propdir=propdir,
+                                  tred=tred, velocity=True, evfilter=evf, stafilter=stf)
+    zerr = np.abs(zsp - ozsp).max() / np.abs(ozsp).max()
+    ts64, ots64 = ts.astype(np.float64), ots.astype(np.float64)
+    terr = np.linalg.norm(ts64 - ots64, axis=2) / np.linalg.norm(ots64, axis=2)
+    print(f"spectra max rel {zerr:.2e}, seismogram rel-L2 max {terr.max():.2e}", flush=True)
+    assert zerr < 1e-5 and terr.max() < 1e-5

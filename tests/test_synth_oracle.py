@@ -300,3 +300,53 @@ def test_force_wavenumber_integrals_converge_to_closed_form(kint):
     assert errs[2] < 1e-3, errs
     for a, b in zip(errs, errs[1:]):
         assert 3.5 < a / b < 4.5, errs          # O(dwn^2)
+
+
+# ---- event and station filters ----
+
+@pytest.mark.parametrize("n1,f1,n2,f2", [(2, 0.01, 4, 0.08), (1, 0.02, 3, 0.05), (0, 0.01, 5, 0.1), (3, 0.005, 0, 1.0)])
+def test_butterworth_bandpass_vs_scipy_zpk(n1, f1, n2, f2):
+    """The event filter equals scipy's analog Butterworth prototypes H(s) evaluated at
+    s = i (f - i sigma / 2 pi) / fc, the complex frequencies of the damped spectrum (order 0: unity)."""
+    from scipy import signal
+    df, sigma, nf1, nf2 = 1.0 / 512, 5.0 / 512, 3, 200
+    pi32 = float(np.float32(np.pi))
+    h = oracle.butterworth_bandpass([n1, f1, n2, f2], nf1, nf2, df, sigma)
+    assert np.all(h[:nf1 - 1] == 0)
+    f = (np.arange(nf1, nf2 + 1) - 1) * df - 1j * sigma / (2 * pi32)
+
+    def analog(n, fc, btype):
+        if n == 0:
+            return np.ones_like(f)
+        z, p, k = signal.butter(n, 1.0, btype=btype, analog=True, output="zpk")
+        s = 1j * f / fc
+        num = np.prod([s - zz for zz in z], axis=0) if len(z) else np.ones_like(s)
+        return k * num / np.prod([s - pp for pp in p], axis=0)
+
+    ref = analog(n1, f1, "high") * analog(n2, f2, "low")
+    # the reference's unit roots use its FP32 pi, scipy's the exact one: ~1e-7 apart near the corners
+    np.testing.assert_allclose(h[nf1 - 1:], ref, rtol=1e-6, atol=1e-9)
+
+
+def test_station_filter_equals_time_shift():
+    """A station filter e^{i omega tau} e^{sigma tau} gives the seismogram of the time shift tau, and a
+    constant event filter scales it (the filters multiply the spectrum as the time shift does)."""
+    rng = np.random.default_rng(6)
+    L, nf1, nf2, nsamp = 40, 2, 30, 64
+    df, dt = 1.0 / 64, 1.0
+    sigma = 5 * df
+    nwn = np.zeros(nf2, np.int32)
+    nwn[nf1 - 1:] = L
+    gr = rng.normal(size=(10, L * (nf2 - nf1 + 1))).astype(np.float32)
+    gi = rng.normal(size=(10, L * (nf2 - nf1 + 1))).astype(np.float32)
+    th, ph = [0.5, 1.9], [0.3, 4.0]
+    mt = rng.normal(size=6)
+    tau = np.array([3.0, 7.5])
+    pi32 = float(np.float32(np.pi))
+    om = 2 * pi32 * (np.arange(nf2)) * df
+    stf = np.exp(1j * om[None, None, :] * tau[:, None, None]) * np.exp(sigma * tau)[:, None, None] * np.ones((2, 3, 1))
+    ts_a, z_a = oracle.synthetics(nf1, nf2, df, sigma, nwn, gr, gi, th, ph, mt, nsamp, dt, tred=tau)
+    ts_b, z_b = oracle.synthetics(nf1, nf2, df, sigma, nwn, gr, gi, th, ph, mt, nsamp, dt, stafilter=stf,
+                                  evfilter=np.full(nf2, 2.0 - 0.5j))
+    np.testing.assert_allclose(z_b[:, :, nf1 - 1:], (2.0 - 0.5j) * z_a[:, :, nf1 - 1:], rtol=1e-12,
+                               atol=1e-12 * np.abs(z_a).max())
