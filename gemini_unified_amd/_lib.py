@@ -65,15 +65,16 @@ def lib():
         L.gm_legendre_batch.argtypes = [VP, C.c_int, D, C.c_int, D, D, D, D, D]
         L.gm_moment_sums.argtypes = [VP, C.c_int, C.c_int, I, C.c_int, C.c_longlong, VP, VP, C.c_int, D, C.c_int, VP]
         L.gm_synthetics.argtypes = [VP, C.c_int, C.c_int, C.c_double, C.c_double, I, C.c_int, C.c_longlong, VP, VP,
-                                    C.c_int, D, D, D, D, D, D, D, C.c_int, C.c_int, C.c_double, C.c_int, VP, VP]
+                                    C.c_int, D, D, D, D, D, D, C.c_int, D, C.c_int, C.c_int, C.c_double, C.c_int,
+                                    VP, VP]
         L.gm_last_synth_timing.argtypes = [VP, D]
         L.gm_bessel_sums.argtypes = [VP, C.c_int, C.c_int, I, C.c_int, C.c_longlong, VP, VP, C.c_double, C.c_double,
                                      C.c_int, D, C.c_int, VP]
         L.gm_force_sums.argtypes = L.gm_moment_sums.argtypes
         L.gm_force_bessel_sums.argtypes = L.gm_bessel_sums.argtypes
         L.gm_synthetics_regional.argtypes = [VP, C.c_int, C.c_int, C.c_double, C.c_double, I, C.c_int, C.c_longlong,
-                                             VP, VP, C.c_double, C.c_double, C.c_int, D, D, D, D, D, D, D, C.c_int,
-                                             C.c_int, C.c_double, C.c_int, VP, VP]
+                                             VP, VP, C.c_double, C.c_double, C.c_int, D, D, D, D, D, D, C.c_int, D,
+                                             C.c_int, C.c_int, C.c_double, C.c_int, VP, VP]
         L.gm_event_filter.argtypes = [C.c_int, D, C.c_int, C.c_int, C.c_double, C.c_double, D]
         _lib = L
     return _lib

@@ -1,8 +1,7 @@
 // Synthetics half of the Green-FK path on MI355X: Legendre basis, harmonic
-// degree sums of a moment-tensor (14 sums) or single-force (7 sums) source as
-// a split-bf16 MFMA contraction, the per-frequency displacement combination,
-// and the frequency-to-time transform.  The source type follows the number of
-// spectra per node: 10 (moment) or 5 (force).
+// degree sums of a moment-tensor (14 sums, 10 spectra per node) or single-force
+// (7 sums, 5 spectra per node) source as a split-bf16 MFMA contraction, the
+// per-frequency displacement combination, and the frequency-to-time transform.
 //
 // Pipeline of gm_synthetics (one source node, R receivers, nf frequencies):
 //   k_twiddle   e^{+2 pi i k / ns}, k < ns/2, FP64 (pi as in the reference)
@@ -836,12 +835,16 @@ static int sums_impl(gm_ctx* c, int istyp, int nf1, int nf2, const int* nwn, int
 static int synth_impl(gm_ctx* c, int nf1, int nf2, double df, double sigma, const int* nwn, int nsp, long long nkf,
                       const float* gfkspr, const float* gfkspi, int nrcv, const double* theta, const double* phi,
                       const double* propdir, const double* tred, const double* evfilter, const double* stafilter,
-                      const double* mt, int velocity, int nsamp, double dt, int flags, float* ts, double* zsp,
-                      double dwn, double re) {
+                      int istyp, const double* mt, int velocity, int nsamp, double dt, int flags, float* ts,
+                      double* zsp, double dwn, double re) {
   if (!c) return GM_ERR_ARG;
+  if (istyp != 0 && istyp != 1) return gm_fail(c, GM_ERR_ARG, "istyp must be 0 (FORCE) or 1 (MOMENT)");
   SynPlan P;
   int rc = plan_sums(c, nf1, nf2, nwn, nsp, nkf, nrcv, P);
   if (rc) return rc;
+  if (P.istyp != istyp)
+    return gm_fail(c, GM_ERR_UNSUPPORTED, "istyp %d needs nsp = %d spectra (U, V, W without stresses), got %d", istyp,
+                   istyp ? 10 : 5, nsp);
   if (!gfkspr || !gfkspi || !theta || !phi || !mt || !ts)
     return gm_fail(c, GM_ERR_ARG, "spectra, theta, phi, fomt and ts are required");
   if (!(df > 0.0) || !(dt > 0.0)) return gm_fail(c, GM_ERR_ARG, "df and dt must be positive");
@@ -941,24 +944,24 @@ extern "C" int gm_force_bessel_sums(gm_ctx* c, int nf1, int nf2, const int* nwn,
 extern "C" int gm_synthetics(gm_ctx* c, int nf1, int nf2, double df, double sigma, const int* nwn, int nsp,
                              long long nkf, const float* gfkspr, const float* gfkspi, int nrcv, const double* theta,
                              const double* phi, const double* propdir, const double* tred, const double* evfilter,
-                             const double* stafilter, const double* fomt, int velocity, int nsamp, double dt, int flags,
-                             float* ts, double* zsp) {
+                             const double* stafilter, int istyp, const double* fomt, int velocity, int nsamp,
+                             double dt, int flags, float* ts, double* zsp) {
   return synth_impl(c, nf1, nf2, df, sigma, nwn, nsp, nkf, gfkspr, gfkspi, nrcv, theta, phi, propdir, tred, evfilter,
-                    stafilter, fomt, velocity, nsamp, dt, flags, ts, zsp, 0.0, 0.0);
+                    stafilter, istyp, fomt, velocity, nsamp, dt, flags, ts, zsp, 0.0, 0.0);
 }
 
 extern "C" int gm_synthetics_regional(gm_ctx* c, int nf1, int nf2, double df, double sigma, const int* nwn, int nsp,
                                       long long nkf, const float* gfkspr, const float* gfkspi, double dwn,
                                       double rearth, int nrcv, const double* x, const double* phi,
                                       const double* propdir, const double* tred, const double* evfilter,
-                                      const double* stafilter, const double* fomt, int velocity, int nsamp, double dt,
-                                      int flags, float* ts, double* zsp) {
+                                      const double* stafilter, int istyp, const double* fomt, int velocity, int nsamp,
+                                      double dt, int flags, float* ts, double* zsp) {
   if (!c) return GM_ERR_ARG;
   if (!(dwn > 0.0) || !(rearth > 0.0)) return gm_fail(c, GM_ERR_ARG, "dwn and rearth must be positive");
   for (int r = 0; r < nrcv && x; ++r)
     if (!(x[r] > 0.0)) return gm_fail(c, GM_ERR_ARG, "epicentral distance %d must be positive", r);
   return synth_impl(c, nf1, nf2, df, sigma, nwn, nsp, nkf, gfkspr, gfkspi, nrcv, x, phi, propdir, tred, evfilter,
-                    stafilter, fomt, velocity, nsamp, dt, flags, ts, zsp, dwn, rearth);
+                    stafilter, istyp, fomt, velocity, nsamp, dt, flags, ts, zsp, dwn, rearth);
 }
 
 extern "C" int gm_event_filter(int filtype, const double* filspecs, int nf1, int nf2, double df, double sigma,

@@ -107,11 +107,12 @@ class SyntheticsEngine:
 
     def synthetics(self, nf1, nf2, df, sigma, nwn, gr, gi, theta, phi, mt, nsamp, dt, propdir=None, tred=None,
                    velocity=False, want_spectra=False, device_ptrs=None, dwn=None, rearth=None, evfilter=None,
-                   stafilter=None):
+                   stafilter=None, istyp=None):
         """Seismograms [nrcv, 3, nsamp] FP32 (and displacement spectra [nrcv, 3, nf2] if asked).
 
-        The source type follows the number of spectra nsp: 10, moment tensor, mt = (Mrr, Mtt, Mff, Mrt,
-        Mrf, Mtf); 5, single force, mt = (Fr, Ft, Ff) (getForceMomentSeismicEvent's fomt).
+        Source type istyp (the GFK file's): 1, moment tensor, nsp = 10, mt = (Mrr, Mtt, Mff, Mrt, Mrf, Mtf);
+        0, single force, nsp = 5, mt = (Fr, Ft, Ff) (getForceMomentSeismicEvent's fomt).  None: the type
+        those spectrum counts imply.
         evfilter: complex [nf2] event filter (e.g. event_filter()), stafilter: complex [nrcv, 3, nf2]
         station filters of the r, theta, phi components; None: unity.
         Global branch: theta = epicentral distances in radians.  Regional branch (dwn given):
@@ -124,8 +125,10 @@ class SyntheticsEngine:
         phi = np.ascontiguousarray(np.atleast_1d(phi), dtype=np.float64)
         mt = np.ascontiguousarray(mt, dtype=np.float64)
         nsp_in = device_ptrs[2] if device_ptrs is not None else np.shape(gr)[0]
-        if phi.size != theta.size or mt.size != {10: 6, 5: 3}.get(int(nsp_in), 6):
-            raise ValueError("phi must match theta and mt must hold 6 (moment) or 3 (force, nsp = 5) entries")
+        if istyp is None:
+            istyp = 0 if int(nsp_in) == NSP[0] else 1
+        if phi.size != theta.size or mt.size != (6 if istyp == 1 else 3):
+            raise ValueError("phi must match theta and mt must hold 6 (moment) or 3 (force) entries")
         ev = None if evfilter is None else np.ascontiguousarray(evfilter, dtype=np.complex128)
         st = None if stafilter is None else np.ascontiguousarray(stafilter, dtype=np.complex128)
         if (ev is not None and ev.shape != (nf2,)) or (st is not None and st.shape != (theta.size, 3, nf2)):
@@ -145,7 +148,8 @@ class SyntheticsEngine:
             ts, flags = None, 1
         tail = (theta.size, theta.ctypes.data_as(D), phi.ctypes.data_as(D), None if pd is None else pd.ctypes.data_as(D),
                 None if tr is None else tr.ctypes.data_as(D), None if ev is None else ev.ctypes.data_as(D),
-                None if st is None else st.ctypes.data_as(D), mt.ctypes.data_as(D), int(bool(velocity)), int(nsamp),
+                None if st is None else st.ctypes.data_as(D), int(istyp), mt.ctypes.data_as(D), int(bool(velocity)),
+                int(nsamp),
                 float(dt), flags, pts, None if zsp is None else zsp.ctypes.data)
         L = _lib.lib()
         if dwn:

@@ -210,10 +210,11 @@ int gm_force_sums(gm_ctx* ctx, int nf1, int nf2, const int* nwn, int nsp, long l
 
 /* Seismograms of one source node: degree sums, then the moment-tensor or force
  * combination (evalSingleFrequencyDisplacementSpectra,
- * f90/displacementSpectra.f90:200-228).  The source type follows nsp as in the
- * GFK file: nsp = 10, moment source, fomt = Mrr, Mtt, Mff, Mrt, Mrf, Mtf
- * (f90/singleFrequencyDisplacement.f90:108-125); nsp = 5, single force,
- * fomt = Fr, Ft, Ff (:35-46).  Then time shift by tred[r] (NULL: none),
+ * f90/displacementSpectra.f90:200-228) by the GFK file's source type istyp:
+ * istyp = 1, moment source, nsp = 10, fomt = Mrr, Mtt, Mff, Mrt, Mrf, Mtf
+ * (f90/singleFrequencyDisplacement.f90:108-125); istyp = 0, single force,
+ * nsp = 5, fomt = Fr, Ft, Ff (:35-46).  Spectra with stresses (STRESSFLAG:
+ * nsp = 20 or 10 with dsvmask U, R, V, S, W, T) are not supported.  Then time shift by tred[r] (NULL: none),
  * event and station filters (evfilter: complex [nf2], stafilter: complex
  * [nrcv][3][nf2] for the components r, theta, phi, i.e. the reference's
  * stafilter(nf2, 3, nstat); NULL: unity; f90/computeSyntheticSeismograms.f90:
@@ -229,7 +230,8 @@ int gm_force_sums(gm_ctx* ctx, int nf1, int nf2, const int* nwn, int nsp, long l
 int gm_synthetics(gm_ctx* ctx, int nf1, int nf2, double df, double sigma, const int* nwn, int nsp, long long nkf,
                   const float* gfkspr, const float* gfkspi, int nrcv, const double* theta, const double* phi,
                   const double* propdir, const double* tred, const double* evfilter, const double* stafilter,
-                  const double* fomt, int velocity, int nsamp, double dt, int flags, float* ts, double* zsp);
+                  int istyp, const double* fomt, int velocity, int nsamp, double dt, int flags, float* ts,
+                  double* zsp);
 
 /* Regional (Bessel) branch: the 14 moment wavenumber integrals (kint 1..14) of
  * one source node at epicentral distances x[r] in metres, for spectra sampled
@@ -258,8 +260,8 @@ int gm_force_bessel_sums(gm_ctx* ctx, int nf1, int nf2, const int* nwn, int nsp,
 int gm_synthetics_regional(gm_ctx* ctx, int nf1, int nf2, double df, double sigma, const int* nwn, int nsp,
                            long long nkf, const float* gfkspr, const float* gfkspi, double dwn, double rearth,
                            int nrcv, const double* x, const double* phi, const double* propdir, const double* tred,
-                           const double* evfilter, const double* stafilter, const double* fomt, int velocity, int nsamp,
-                           double dt, int flags, float* ts, double* zsp);
+                           const double* evfilter, const double* stafilter, int istyp, const double* fomt, int velocity,
+                           int nsamp, double dt, int flags, float* ts, double* zsp);
 
 /* Event filter of the synthetics stage: computeEventfilter
  * (f90/eventFilter.f90:40-83), type GM_FILTER_BUTTERWORTH_BANDPASS with

@@ -17,6 +17,20 @@ def test_header_matches_binding_list():
     assert declared_symbols() == sorted(_lib.EXPORTS)
 
 
+def test_binding_argument_counts_match_header():
+    """Every ctypes prototype has as many arguments as the header declares (a missing or extra
+    argument would shift every later one)."""
+    txt = open(os.path.join(ROOT, "include", "gemini_fk.h")).read()
+    L = _lib.lib()
+    for name, params in re.findall(r"^int (gm_\w+)\(([^;]*?)\);", txt, flags=re.M | re.S):
+        n = 0 if params.strip() in ("", "void") else params.count(",") + 1
+        argtypes = getattr(L, name).argtypes
+        if argtypes is None:
+            assert n == 0, name
+        else:
+            assert len(argtypes) == n, (name, len(argtypes), n)
+
+
 def test_library_exports_all_symbols():
     lib = ctypes.CDLL(_lib.LIB_PATH)
     for name in declared_symbols():

@@ -117,6 +117,10 @@ def test_synthetics_errors(syn):
     with pytest.raises(GeminiFKError) as e:
         syn.synthetics(1, 2, 1 / 64, 0.0, nwn, g, g, [1.0], [0.0], np.ones(6), 64, 1.0)
     assert e.value.code == 8                        # neither 10 (moment) nor 5 (force) spectra
+    g = np.zeros((10, 20), np.float32)
+    with pytest.raises(GeminiFKError) as e:         # a force with stresses (STRESSFLAG) also has 10 spectra
+        syn.synthetics(1, 2, 1 / 64, 0.0, nwn, g, g, [1.0], [0.0], np.ones(3), 64, 1.0, istyp=0)
+    assert e.value.code == 8
     with pytest.raises(GeminiFKError) as e:
         syn.moment_sums(1, 2, nwn, np.zeros((5, 20), np.float32), np.zeros((5, 20), np.float32), [1.0])
     assert e.value.code == 9                        # force spectra given to the moment sums
