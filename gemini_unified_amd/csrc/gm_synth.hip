@@ -4,7 +4,7 @@
 // per-frequency displacement combination, and the frequency-to-time transform.
 //
 // Pipeline of gm_synthetics (one source node, R receivers, nf frequencies):
-//   k_twiddle   e^{+2 pi i k / ns}, k < ns/2, FP64 (pi as in the reference)
+//   k_twiddle   e^{+2 pi i k / ns}, k < ns, FP64 (pi as in the reference)
 //   k_legendre  per (receiver, m): P_l^m, dP/dtheta, P/sin (m = 0..2) by the FP64
 //               upward recurrence, times (2l+1)/(4 pi) and 2/(l(l+1)), as
 //               8 basis rows; each value is split into three bf16 pieces
@@ -51,7 +51,7 @@ constexpr int kPieces = 3;     // bf16 pieces per value
 constexpr int TM = 128, TN = 64, TK = 32, LDSK = 40;   // tile sizes; LDS row pitch (bf16)
 constexpr int kXcd = 8;        // XCDs of the MI355X the contraction's tile order is laid out for
 constexpr int kStrip = 8;      // row tiles per strip of that order
-constexpr int kMaxFFT = 8192;  // longest time series (ns/2 complex FP64 = 64 KB of LDS)
+constexpr int kMaxFFT = 8192;  // longest time series (ns/2 complex FP64 = 64 KB of LDS, 68 KB padded)
 
 // Sum j of source type t (0 single force, kint 1..7; 1 moment tensor, kint
 // 1..14): basis row, spectrum index (offset of (component, jump) in the GFK
@@ -81,7 +81,7 @@ __device__ __forceinline__ void split3(double v, __bf16& a, __bf16& b, __bf16& c
 
 __global__ void k_twiddle(int ns, double* __restrict__ tw) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= ns / 2) return;
+  if (k >= ns) return;
   double s, c;
   sincos((2.0 * kPi / ns) * k, &s, &c);
   tw[2 * k] = c;
@@ -596,52 +596,141 @@ __global__ void k_combine(int R, int nf1, int nf2, int Rp, int Np, const double*
   }
 }
 
+// R-point DFT in registers, v_q <- sum_r v_r e^{+2 pi i q r / R}: radix-2
+// decimation in time over the bit-reversed inputs.  The twiddles are powers of
+// e^{2 pi i / 16}: w16[p] = e^{2 pi i p / 16}, p < 8, taken once per thread from
+// the transform's table so that every angle uses the same pi.  All indices are
+// compile-time constants once unrolled, so v stays in registers.
+template <int R>
+__device__ __forceinline__ void dftRegs(cd* v, const cd* w16) {
+  constexpr int LOG = R == 2 ? 1 : R == 4 ? 2 : R == 8 ? 3 : 4;
+  cd u[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    int br = 0;
+#pragma unroll
+    for (int b = 0; b < LOG; ++b) br |= ((r >> b) & 1) << (LOG - 1 - b);
+    u[br] = v[r];
+  }
+#pragma unroll
+  for (int st = 1; st <= LOG; ++st) {
+    const int half = 1 << (st - 1);
+#pragma unroll
+    for (int g = 0; g < R; g += 2 * half)
+#pragma unroll
+      for (int pos = 0; pos < half; ++pos) {
+        const int i0 = g + pos, i1 = i0 + half;
+        const cd x1 = pos == 0 ? u[i1] : cmul(w16[pos * (16 >> st)], u[i1]);   // e^{2 pi i pos / (2 half)}
+        const cd x0 = u[i0];
+        u[i0] = cadd(x0, x1);
+        u[i1] = csub(x0, x1);
+      }
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = u[r];
+}
+
+// LDS slot of transform point i: one 16-byte pad after every 16 points, so that
+// the first pass's writes at a stride of 16 points spread over the banks
+__device__ __forceinline__ int fftSlot(int i) { return i + (i >> 4); }
+constexpr int kFFTSlots = kMaxFFT / 2 + kMaxFFT / 32;
+
+// One pass of the Stockham auto-sort transform over sx (M points, sub-transforms
+// of length Ns done): work item j < M/R reads R points at stride M/R, turns them
+// by e^{2 pi i (j mod Ns) r / (Ns R)}, transforms them in registers and writes
+// them at stride Ns from (j / Ns) Ns R + j mod Ns.  Every point is read once and
+// written once per pass, so the pass runs in place: all reads, barrier, all
+// writes, barrier.  M / R <= blockDim for every pass of this kernel.  The
+// turning factors w^r, w = e^{2 pi i (j mod Ns) / (Ns R)} = tw[(j mod Ns) 2M / (Ns R)],
+// are formed as running products of w (relative error below 1e-15 for r < 16).
+template <int R>
+__device__ __forceinline__ void stockhamPass(cd* sx, int M, int Ns, const double* __restrict__ tw, const cd* w16) {
+  const int nw = M / R, j = threadIdx.x;
+  cd v[R];
+  if (j < nw) {
+    const int t = (j % Ns) * (2 * M / (Ns * R));
+    const cd w{tw[2 * t], tw[2 * t + 1]};
+    cd wr = w;
+    v[0] = sx[fftSlot(j)];
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      v[r] = cmul(sx[fftSlot(j + r * nw)], wr);
+      if (r + 1 < R) wr = cmul(wr, w);
+    }
+    dftRegs<R>(v, w16);
+  }
+  __syncthreads();
+  if (j < nw) {
+    const int base = (j / Ns) * Ns * R + j % Ns;
+#pragma unroll
+    for (int r = 0; r < R; ++r) sx[fftSlot(base + r * Ns)] = v[r];
+  }
+  __syncthreads();
+}
+
 // One trace per block: spectrum X_j (j = nf1-1 .. nf2-1, Hermitian mirror,
 // zero elsewhere) -> x_t = sum_j X_j e^{+2 pi i j t / ns}, t < ns, then
 // ts = x df e^{sigma t dt} (transformFrequencyTime, frequencyTime.f90:55-113).
 // Real output from an M = ns/2 point complex transform of
 // Z_k = (X_k + conj X_{M-k}) + i W^k (X_k - conj X_{M-k}), W = e^{2 pi i / ns}:
-// z_n = x_{2n} + i x_{2n+1}.
+// z_n = x_{2n} + i x_{2n+1}.  The M-point transform is a Stockham auto-sort
+// FFT in radix-16 passes (and one radix 2, 4 or 8 pass for the rest of log2 M),
+// each pass a 16-point DFT in registers between two LDS barriers: three passes
+// for M = 4096 where radix 2 took twelve.
 __global__ void __launch_bounds__(256) k_ifft(int nf1, int nf2, int ns, int logm, const double* __restrict__ Z,
                                               const double* __restrict__ tw, double df, double sigma, double dt,
                                               float* __restrict__ ts) {
-  __shared__ cd sx[kMaxFFT / 2];
+  __shared__ cd sx[kFFTSlots];
   const int M = ns >> 1;
   const double* X = Z + (size_t)blockIdx.x * nf2 * 2;
   auto xin = [&](int j) -> cd {
     if (j < nf1 - 1 || j > nf2 - 1) return {0.0, 0.0};
     return {X[2 * j], X[2 * j + 1]};
   };
+  // unrolled so that the loads of several k are in flight together; between
+  // nf2 - 1 and M - nf2 + 1 both X_k and X_{M-k} vanish and nothing is loaded
+#pragma unroll 4
   for (int k = threadIdx.x; k < M; k += blockDim.x) {
-    const cd a = xin(k);
-    const cd bc = k == 0 ? cd{0.0, 0.0} : xin(M - k);
-    const cd b{bc.re, -bc.im};
-    const cd wk{tw[2 * k], tw[2 * k + 1]};
-    const cd s = cadd(a, b), d = csub(a, b);
-    const cd wd = cmul(wk, d);
-    const cd zk{s.re - wd.im, s.im + wd.re};       // s + i * wd
-    sx[__builtin_bitreverse32((unsigned)k) >> (32 - logm)] = zk;
+    cd z{0.0, 0.0};
+    if (k <= nf2 - 1 || k >= M - nf2 + 1) {
+      // the reference transforms the complex zero-frequency bin and keeps the
+      // real part of the result, so only its real part reaches the samples
+      const cd a0 = xin(k);
+      const cd a = k == 0 ? cd{a0.re, 0.0} : a0;
+      const cd bc = k == 0 ? cd{0.0, 0.0} : xin(M - k);
+      const cd b{bc.re, -bc.im};
+      const cd wk{tw[2 * k], tw[2 * k + 1]};
+      const cd s = cadd(a, b), d = csub(a, b);
+      const cd wd = cmul(wk, d);
+      z = cd{s.re - wd.im, s.im + wd.re};            // s + i * wd
+    }
+    sx[fftSlot(k)] = z;
   }
   __syncthreads();
-  for (int st = 1; st <= logm; ++st) {
-    const int half = 1 << (st - 1);
-    const int tstep = ns >> st;                     // twiddle e^{2 pi i pos / (2 half)} = W^{pos ns / (2 half)}
-    for (int bf = threadIdx.x; bf < M / 2; bf += blockDim.x) {
-      const int pos = bf & (half - 1);
-      const int i0 = ((bf >> (st - 1)) << st) + pos, i1 = i0 + half;
-      const cd wv{tw[2 * pos * tstep], tw[2 * pos * tstep + 1]};
-      const cd x1 = cmul(wv, sx[i1]);
-      const cd x0 = sx[i0];
-      sx[i0] = cadd(x0, x1);
-      sx[i1] = csub(x0, x1);
-    }
-    __syncthreads();
+  // e^{2 pi i p / 16} = tw[p ns / 16]; for ns < 16 only the p that are
+  // multiples of 16 / ns occur (the DFTs are at most M = ns / 2 points)
+  cd w16[8];
+#pragma unroll
+  for (int p = 0; p < 8; ++p) {
+    const int q = p * ns;
+    w16[p] = (q & 15) == 0 ? cd{tw[2 * (q >> 4)], tw[2 * (q >> 4) + 1]} : cd{1.0, 0.0};
   }
+  int Ns = 1, left = logm;
+  for (; left >= 4; left -= 4, Ns *= 16) stockhamPass<16>(sx, M, Ns, tw, w16);
+  if (left == 3) stockhamPass<8>(sx, M, Ns, tw, w16);
+  else if (left == 2) stockhamPass<4>(sx, M, Ns, tw, w16);
+  else if (left == 1) stockhamPass<2>(sx, M, Ns, tw, w16);
+  // e^{sigma dt t} at t = 2n and 2n + 1, n = threadIdx.x + i blockDim.x: one
+  // exponential per thread, then a product per step of 2 blockDim samples
   float* out = ts + (size_t)blockIdx.x * ns;
+  const double a = sigma * dt;
+  double e0 = df * exp(a * (2 * (int)threadIdx.x));
+  const double e1 = exp(a), estep = exp(a * (2 * (int)blockDim.x));
   for (int n = threadIdx.x; n < M; n += blockDim.x) {
-    const cd z = sx[n];
-    out[2 * n] = (float)(z.re * df * exp(sigma * dt * (2 * n)));
-    out[2 * n + 1] = (float)(z.im * df * exp(sigma * dt * (2 * n + 1)));
+    const cd z = sx[fftSlot(n)];
+    out[2 * n] = (float)(z.re * e0);
+    out[2 * n + 1] = (float)(z.im * (e0 * e1));
+    e0 *= estep;
   }
 }
 
@@ -860,8 +949,9 @@ static int synth_impl(gm_ctx* c, int nf1, int nf2, double df, double sigma, cons
   hipStream_t s = c->stream;
   if (c->tw_n != nsamp) {
     if (c->dtw) { (void)hipFree(c->dtw); c->dtw = nullptr; c->tw_n = 0; }
-    if (hipMalloc(&c->dtw, sizeof(double) * nsamp) != hipSuccess) return gm_fail(c, GM_ERR_NOMEM, "twiddle allocation failed");
-    hipLaunchKernelGGL(k_twiddle, dim3((nsamp / 2 + 255) / 256), dim3(256), 0, s, nsamp, c->dtw);
+    if (hipMalloc(&c->dtw, sizeof(double) * 2 * nsamp) != hipSuccess)
+      return gm_fail(c, GM_ERR_NOMEM, "twiddle allocation failed");
+    hipLaunchKernelGGL(k_twiddle, dim3((nsamp + 255) / 256), dim3(256), 0, s, nsamp, c->dtw);
     c->tw_n = nsamp;
   }
   Carve cv;

@@ -392,3 +392,26 @@ def test_filtered_synthetics_match_oracle(syn):
     terr = np.linalg.norm(ts64 - ots64, axis=2) / np.linalg.norm(ots64, axis=2)
     print(f"spectra max rel {zerr:.2e}, seismogram rel-L2 max {terr.max():.2e}", flush=True)
     assert zerr < 1e-5 and terr.max() < 1e-5
+
+
+def test_frequency_time_every_length(syn):
+    """The transform for every nsamp = 2^2 .. 2^13 (every mix of radix-16 and radix-2/4/8 passes),
+    with nearly the most frequencies each length takes without decimation (transformFrequencyTime's
+    test ceil(log2 nf2) + 1 <= log2 nsamp, in FP32 logarithms), against the oracle."""
+    rng = np.random.default_rng(51)
+    for lg in range(2, 14):
+        nsamp = 1 << lg
+        nf2 = max(nsamp // 2 - 1, 1)
+        nf1 = min(2, nf2)
+        df = 1.0 / 64.0
+        dt = 1.0 / (df * nsamp)
+        nwn = np.zeros(nf2, np.int32)
+        nwn[nf1 - 1:] = 12
+        gr, gi = _random_spectra(rng, nwn[nf1 - 1:])
+        mt = rng.normal(size=6)
+        ts, zsp = syn.synthetics(nf1, nf2, df, 5 * df, nwn, gr, gi, [1.1], [0.4], mt, nsamp, dt, want_spectra=True)
+        ots, ozsp = oracle.synthetics(nf1, nf2, df, 5 * df, nwn, gr, gi, [1.1], [0.4], mt, nsamp, dt)
+        ts64, ots64 = ts.astype(np.float64), ots.astype(np.float64)
+        terr = (np.linalg.norm(ts64 - ots64, axis=2) / np.linalg.norm(ots64, axis=2)).max()
+        print(f"nsamp {nsamp}: seismogram rel-L2 {terr:.2e}", flush=True)
+        assert terr < 1e-5, (nsamp, terr)
