@@ -209,44 +209,55 @@ __device__ __forceinline__ double taperWn(int l, int n, double dwn, double tapfr
   return c * c;
 }
 
+// One thread per (frequency column pair f, degree l): the taper is computed
+// once and applied to the real and imaginary parts of every spectrum; columns
+// 2f and 2f + 1 of every spectrum plane are written, zero beyond the
+// frequency's degrees and in the padding columns (f >= nf).  Grid (Lp / 256
+// rounded up, Np / 2).
 __global__ void k_gprep(int nsp, int nf, int Np, int Lp, const int* __restrict__ nwn, const long long* __restrict__ koff,
                         long long nkf, const float* __restrict__ gr, const float* __restrict__ gi, double tapfrac,
                         double dwn, int istyp, __bf16* __restrict__ Gs) {
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t per_s = (size_t)Np * Lp;
-  if (idx >= per_s * nsp) return;
-  const int s = (int)(idx / per_s);
-  const int n = (int)((idx % per_s) / Lp);
-  const int l = (int)(idx % Lp);
-  const int f = n >> 1;
-  double v = 0.0;
-  if (dwn > 0.0) {
-    if (f < nf && l < nwn[f] && l >= 1) {
-      const long long k = (long long)s * nkf + koff[f] + l;
-      const double gv = (n & 1) ? (double)gi[k] : (double)gr[k];
-      v = gv * taperWn(l, nwn[f], dwn, tapfrac);
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = blockIdx.y;
+  if (l >= Lp) return;
+  const size_t per_s = (size_t)Np * Lp, plane = per_s * nsp;
+  const bool regional = dwn > 0.0;
+  const int n = f < nf ? nwn[f] : 0;
+  double w = 0.0;
+  long long k = 0;
+  if (l < n) {
+    k = koff[f] + l;
+    if (regional) {
+      w = taperWn(l, n, dwn, tapfrac);
+    } else {
+      const int lmax = n - 1;
+      long jt = lround(lmax * (1.0 - tapfrac));
+      if (jt > lmax) jt = lmax;
+      if (jt < 1) jt = 1;
+      w = 1.0;
+      if (l >= jt) {
+        const double c1 = lmax * tapfrac * 2.0 / kPi;
+        const double c2 = (1.0 - tapfrac) / tapfrac * 0.5 * kPi;
+        const double c = cos((double)l / c1 - c2);
+        w = c * c;
+      }
     }
-  } else if (f < nf && l < nwn[f] && l >= c_lmin[istyp][s]) {
-    const long long k = (long long)s * nkf + koff[f] + l;
-    const double gv = (n & 1) ? (double)gi[k] : (double)gr[k];
-    const int lmax = nwn[f] - 1;
-    long jt = lround(lmax * (1.0 - tapfrac));
-    if (jt > lmax) jt = lmax;
-    if (jt < 1) jt = 1;
-    double w = 1.0;
-    if (l >= jt) {
-      const double c1 = lmax * tapfrac * 2.0 / kPi;
-      const double c2 = (1.0 - tapfrac) / tapfrac * 0.5 * kPi;
-      const double c = cos((double)l / c1 - c2);
-      w = c * c;
-    }
-    v = gv * w;
   }
-  __bf16 a, b, c;
-  split3(v, a, b, c);
-  Gs[idx] = a;
-  Gs[idx + per_s * nsp] = b;
-  Gs[idx + 2 * per_s * nsp] = c;
+  for (int s = 0; s < nsp; ++s) {
+    const bool on = l < n && (regional ? l >= 1 : l >= c_lmin[istyp][s]);
+    const double vr = on ? (double)gr[(long long)s * nkf + k] * w : 0.0;
+    const double vi = on ? (double)gi[(long long)s * nkf + k] * w : 0.0;
+    const size_t o = (size_t)s * per_s + (size_t)(2 * f) * Lp + l;
+    __bf16 a, b, c;
+    split3(vr, a, b, c);
+    Gs[o] = a;
+    Gs[o + plane] = b;
+    Gs[o + 2 * plane] = c;
+    split3(vi, a, b, c);
+    Gs[o + Lp] = a;
+    Gs[o + Lp + plane] = b;
+    Gs[o + Lp + 2 * plane] = c;
+  }
 }
 
 // H[j][Rp][Np] (FP64) = c_fac[t][j] * sum over degrees of basis row c_row[t][j]
@@ -830,8 +841,8 @@ int run_sums(gm_ctx* c, const SynPlan& P, const double* theta, const float* gr, 
                        (__bf16*)(base + o.Bs), nullptr, nullptr, nullptr, nullptr, nullptr, 0);
   }
   HIPCHK(c, hipEventRecord(c->sev[1], s));
-  const size_t ng = (size_t)P.nsp * P.Np * P.Lp;
-  hipLaunchKernelGGL(k_gprep, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, s, P.nsp, P.nf, P.Np, P.Lp,
+  hipLaunchKernelGGL(k_gprep, dim3((unsigned)((P.Lp + 255) / 256), (unsigned)(P.Np / 2)), dim3(256), 0, s, P.nsp, P.nf,
+                     P.Np, P.Lp,
                      (const int*)(base + o.nwn), (const long long*)(base + o.koff), P.nkf, dgr, dgi, 0.2, dwn,
                      P.istyp, (__bf16*)(base + o.Gs));
   HIPCHK(c, hipEventRecord(c->sev[2], s));
