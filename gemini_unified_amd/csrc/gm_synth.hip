@@ -8,7 +8,10 @@
 //   k_legendre  per (receiver, m): P_l^m, dP/dtheta, P/sin (m = 0..2) by the FP64
 //               upward recurrence, times (2l+1)/(4 pi) and 2/(l(l+1)), as
 //               8 basis rows; each value is split into three bf16 pieces
-//               (hi, mid, lo: 24 significant bits).  Layout [piece][row][Rp][Lp].
+//               (hi, mid, lo: 24 significant bits).  Layout
+//               [piece][row][Lp/32][Rp][32]: runs of 32 degrees (one
+//               contraction chunk), receivers next to each other, so that a
+//               128 x 32 tile of the contraction is one contiguous 8 KB block.
 //   k_gprep     spectra G_s(l, f) * taper_f(l) in FP64, split into three bf16
 //               pieces.  Layout [piece][s][Np][Lp], column n = 2 f' (+1 imag).
 //   k_contract  the sums S_j[r][n] = sum_l B_b(j)[r][l] G_s(j)[n][l] on the
@@ -90,14 +93,57 @@ __global__ void k_twiddle(int ns, double* __restrict__ tw) {
 
 // One (receiver, order m) per thread, m = blockIdx.y: the three recurrences in
 // l (m = 0, 1, 2) are independent, so they run on three lanes.  mode 0: basis
-// pieces Bs[piece][row][Rp][Lp] of the rows of order m (rows 0, 3 for m = 0;
+// pieces Bs[piece][row][Lp/32][Rp][32] of the rows of order m (rows 0, 3 for m = 0;
 // 1, 4, 6 for m = 1; 2, 5, 7 for m = 2) for degrees 0..Lp-1 (zero beyond
 // lmax); mode 1: FP64 arrays [R][3][lmax+1] of plm, dpdt, pdst, d2pdt2, dtpdst
 // (computeLegendreFunctions, legendreFunctions.f90:48-157, same operation order).
+// P_l^m (m = 0..2, no Condon-Shortley sign) from P_{l-1}^m = p1 and P_{l-2}^m = p2
+// by the upward recurrence (legendreFunctions.f90:87-100); pmm = P_m^m.
+__device__ __forceinline__ double legendreNext(int l, int m, int lmax, double ct, double pmm, double p1, double p2) {
+  if (l > lmax || l < m) return 0.0;
+  if (l == m) return pmm;
+  if (l == m + 1) return ct * (double)(2 * m + 1) * pmm;
+  return (ct * (double)(2 * l - 1) * p1 - (double)(l + m - 1) * p2) / (double)(l - m);
+}
+__device__ __forceinline__ double legendrePmm(int m, double st) {
+  return m == 0 ? 1.0 : (m == 1 ? (1.0 * 1.0) * st : ((1.0 * 1.0) * st * 3.0) * st);
+}
+
+// Start values of the degree segments of k_legendre: for every (receiver r,
+// order m) and segment s = 1..nseg-1, (P_{l0-1}^m, P_{l0-2}^m) at l0 = s seglen,
+// by the same recurrence and arithmetic as k_legendre, so that each segment
+// continues it bit for bit.  seeds: [Rp][3][nseg][2].  Grid (Rp / 64, 3).
+__global__ void __launch_bounds__(64) k_legendre_seed(int R, int Rp, int lmax, const double* __restrict__ theta,
+                                                      int nseg, int seglen, double* __restrict__ seeds) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = blockIdx.y;
+  if (r >= Rp) return;
+  const double th = r >= R ? 0.5 * kPi : theta[r];
+  const double ct = cos(th), st = sin(th);
+  const double pmm = legendrePmm(m, st);
+  double* out = seeds + ((size_t)r * 3 + m) * nseg * 2;
+  double a1 = 0.0, a2 = 0.0;
+  const int last = (nseg - 1) * seglen;
+  for (int l = 0; l < last; ++l) {
+    const double p = legendreNext(l, m, lmax, ct, pmm, a1, a2);
+    a2 = a1;
+    a1 = p;
+    if ((l + 1) % seglen == 0) {
+      const int sg = (l + 1) / seglen;
+      out[2 * sg] = a1;
+      out[2 * sg + 1] = a2;
+    }
+  }
+}
+
+// mode 0: basis rows for the contraction; the degrees are cut into segments of
+// seglen (blockIdx.z), each continuing the recurrence from its seeds (above).
+// mode 1: FP64 outputs of computeLegendreFunctions, one segment.
 __global__ void __launch_bounds__(64) k_legendre(int R, int Rp, int lmax, int Lp, const double* __restrict__ theta,
                                                  __bf16* __restrict__ Bs, double* __restrict__ o_plm,
                                                  double* __restrict__ o_dpdt, double* __restrict__ o_pdst,
-                                                 double* __restrict__ o_d2, double* __restrict__ o_ds, int mode) {
+                                                 double* __restrict__ o_d2, double* __restrict__ o_ds, int mode,
+                                                 int nseg, int seglen, const double* __restrict__ seeds) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const int m = blockIdx.y;
   if (r >= (mode == 0 ? Rp : R)) return;
@@ -110,20 +156,23 @@ __global__ void __launch_bounds__(64) k_legendre(int R, int Rp, int lmax, int Lp
   const double rst = poles ? 0.0 : 1.0 / st;
   const double sg = ct < 0.0 ? -1.0 : 1.0;
   const double fourpi = 4.0 * kPi;
-  const double pmm = m == 0 ? 1.0 : (m == 1 ? (1.0 * 1.0) * st : ((1.0 * 1.0) * st * 3.0) * st);
+  const double pmm = legendrePmm(m, st);
   // basis rows of this order and how many
   const int nrow = m == 0 ? 2 : 3;
   const int row0 = m, row1 = m + 3, row2 = m + 5;
+  const int seg = blockIdx.z;
   double p1 = 0.0, p2 = 0.0;
+  if (seg > 0) {
+    const double* sd = seeds + (((size_t)r * 3 + m) * nseg + seg) * 2;
+    p1 = sd[0];
+    p2 = sd[1];
+  }
   // one degree of the recurrence (values of this order m at degree l)
   auto step = [&](int l, double& p, double& d, double& sv, double& d2, double& ds) {
     const double ll1 = (double)(l * (l + 1));
     const double sl = (l & 1) ? sg : 1.0, sl1 = (l & 1) ? 1.0 : sg;
     const double q = (double)(l + 2) * (l + 1) * l * (l - 1);
-    if (l > lmax || l < m) p = 0.0;
-    else if (l == m) p = pmm;
-    else if (l == m + 1) p = ct * (double)(2 * m + 1) * pmm;
-    else p = (ct * (double)(2 * l - 1) * p1 - (double)(l + m - 1) * p2) / (double)(l - m);
+    p = legendreNext(l, m, lmax, ct, pmm, p1, p2);
     d = 0.0; sv = 0.0; d2 = 0.0; ds = 0.0;
     if (l <= lmax) {
       if (!poles) {
@@ -152,13 +201,16 @@ __global__ void __launch_bounds__(64) k_legendre(int R, int Rp, int lmax, int Lp
     }
     return;
   }
-  // mode 0: groups of 8 degrees (Lp is a multiple of 8), each row's 8 values
-  // split into bf16 pieces in registers and stored as one 16-byte vector
-  for (int l0 = 0; l0 < Lp; l0 += 8) {
-    bf16x8 v[kPieces][3];
+  // mode 0: runs of 32 degrees (Lp and seglen are multiples of 32), each row's
+  // values split into bf16 pieces in registers, 8 to a 16-byte vector, and the
+  // run's four vectors stored together: one receiver's 64-byte run of the layout
+  const int lend = min(Lp, (seg + 1) * seglen);
+  for (int l32 = seg * seglen; l32 < lend; l32 += 32) {
+    bf16x8 v[4][kPieces][3];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int l = l0 + e;
+    for (int e32 = 0; e32 < 32; ++e32) {
+      const int gq = e32 >> 3, e = e32 & 7;
+      const int l = l32 + e32;
       double p, d, sv, d2, ds;
       step(l, p, d, sv, d2, ds);
       const double ll1 = (double)(l * (l + 1));
@@ -180,7 +232,7 @@ __global__ void __launch_bounds__(64) k_legendre(int R, int Rp, int lmax, int Lp
         const double x = (pad || l > lmax) ? 0.0 : b[k];
         __bf16 h0, h1, h2;
         split3(x, h0, h1, h2);
-        v[0][k][e] = h0; v[1][k][e] = h1; v[2][k][e] = h2;
+        v[gq][0][k][e] = h0; v[gq][1][k][e] = h1; v[gq][2][k][e] = h2;
       }
     }
 #pragma unroll
@@ -189,7 +241,9 @@ __global__ void __launch_bounds__(64) k_legendre(int R, int Rp, int lmax, int Lp
       for (int k = 0; k < 3; ++k) {
         if (k >= nrow) continue;
         const int row = k == 0 ? row0 : (k == 1 ? row1 : row2);
-        *(bf16x8*)(Bs + ((size_t)pc * kRows + row) * plane + (size_t)r * Lp + l0) = v[pc][k];
+        bf16x8* dst = (bf16x8*)(Bs + ((size_t)pc * kRows + row) * plane + ((size_t)(l32 >> 5) * Rp + r) * 32);
+#pragma unroll
+        for (int gq = 0; gq < 4; ++gq) dst[gq] = v[gq][pc][k];
       }
   }
 }
@@ -296,7 +350,7 @@ __global__ void __launch_bounds__(256, 2) k_contract(int Rp, int Np, int Lp, int
   uint4 ra0, ra1, ra2, ra3, ra4, ra5, rb0, rb1, rb2;
 #define GM_A_SRC(i, k0)                                                                                        \
   (Bs + ((size_t)((t + 256 * (i)) >> 9) * kRows + rowb) * planeA +                                           \
-   (size_t)(r0 + (((t + 256 * (i)) & 511) >> 2)) * Lp + (k0) + 8 * ((t + 256 * (i)) & 3))
+   ((size_t)((k0) >> 5) * Rp + r0 + (((t + 256 * (i)) & 511) >> 2)) * 32 + 8 * ((t + 256 * (i)) & 3))
 #define GM_B_SRC(i, k0)                                                                                        \
   (Gs + ((size_t)((t + 256 * (i)) >> 8) * nsp + spec) * planeG +                                             \
    (size_t)(n0 + (((t + 256 * (i)) & 255) >> 2)) * Lp + (k0) + 8 * ((t + 256 * (i)) & 3))
@@ -421,12 +475,14 @@ __device__ __forceinline__ WnGeom wnGeom(double xx, double dwn) {
 // 2 <= j < j3: dwn (trapezoid); j3: dwn/2 + cF (trapezoid end, Filon start);
 // j > j3: 2 cF (Filon interior).  They are those of a frequency whose last
 // wavenumber lies beyond j3; k_wnfix corrects each frequency's last point.
-// One thread per (receiver, wavenumber); layout [piece][row][Rp][Lp].
+// One thread per (receiver, wavenumber); layout [piece][row][Lp/32][Rp][32] as
+// for the Legendre basis, with the thread index running in that order.
 __global__ void k_bessel_basis(int R, int Rp, int Lp, const double* __restrict__ xr, double dwn, double re,
                                __bf16* __restrict__ Bs) {
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (size_t)Rp * Lp) return;
-  const int r = (int)(idx / Lp), l = (int)(idx % Lp);
+  const unsigned rest = (unsigned)(idx >> 5);
+  const int r = (int)(rest % (unsigned)Rp), l = (int)(rest / (unsigned)Rp) * 32 + (int)(idx & 31);
   const size_t plane = (size_t)Rp * Lp;
   double w[kRows] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (r < R && l >= 1) {
@@ -813,7 +869,15 @@ int ensure_sbuf(gm_ctx* c, size_t bytes) {
 }
 
 // basis, spectra and the contraction into H; offsets of the carved buffer in `o`
-struct SumBufs { size_t theta, Bs, Gs, H, nwn, koff, gr, gi; };
+struct SumBufs { size_t theta, Bs, Gs, H, nwn, koff, gr, gi, seeds; };
+
+// degree segments of the Legendre basis: about 8 (at least 64 degrees each),
+// seglen a multiple of 32 (Lp is one)
+inline void legendreSegments(int Lp, int& nseg, int& seglen) {
+  const int want = std::max(1, std::min(8, Lp / 64));
+  seglen = (int)rup((size_t)((Lp + want - 1) / want), 32);
+  nseg = (Lp + seglen - 1) / seglen;
+}
 
 // dwn == 0: harmonic degree sums at epicentral distances theta (radians);
 // dwn > 0: wavenumber integrals at epicentral distances theta (metres), rearth = re
@@ -837,8 +901,15 @@ int run_sums(gm_ctx* c, const SynPlan& P, const double* theta, const float* gr, 
     hipLaunchKernelGGL(k_bessel_basis, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, P.R, P.Rp, P.Lp, dth, dwn,
                        re, (__bf16*)(base + o.Bs));
   } else {
-    hipLaunchKernelGGL(k_legendre, dim3((P.Rp + 63) / 64, 3), dim3(64), 0, s, P.R, P.Rp, P.L1 - 1, P.Lp, dth,
-                       (__bf16*)(base + o.Bs), nullptr, nullptr, nullptr, nullptr, nullptr, 0);
+    int nseg, seglen;
+    legendreSegments(P.Lp, nseg, seglen);
+    double* seeds = (double*)(base + o.seeds);
+    if (nseg > 1)
+      hipLaunchKernelGGL(k_legendre_seed, dim3((P.Rp + 63) / 64, 3), dim3(64), 0, s, P.R, P.Rp, P.L1 - 1, dth, nseg,
+                         seglen, seeds);
+    hipLaunchKernelGGL(k_legendre, dim3((P.Rp + 63) / 64, 3, nseg), dim3(64), 0, s, P.R, P.Rp, P.L1 - 1, P.Lp, dth,
+                       (__bf16*)(base + o.Bs), nullptr, nullptr, nullptr, nullptr, nullptr, 0, nseg, seglen,
+                       (const double*)seeds);
   }
   HIPCHK(c, hipEventRecord(c->sev[1], s));
   hipLaunchKernelGGL(k_gprep, dim3((unsigned)((P.Lp + 255) / 256), (unsigned)(P.Np / 2)), dim3(256), 0, s, P.nsp, P.nf,
@@ -861,6 +932,9 @@ int run_sums(gm_ctx* c, const SynPlan& P, const double* theta, const float* gr, 
 SumBufs carve_sums(const SynPlan& P, bool dev_in, Carve& cv) {
   SumBufs o;
   o.theta = cv.take(sizeof(double) * P.R);
+  int nseg, seglen;
+  legendreSegments(P.Lp, nseg, seglen);
+  o.seeds = cv.take(sizeof(double) * 2 * 3 * (size_t)P.Rp * nseg);
   o.nwn = cv.take(sizeof(int) * P.nf);
   o.koff = cv.take(sizeof(long long) * P.nf);
   o.Bs = cv.take(sizeof(__bf16) * kPieces * kRows * (size_t)P.Rp * P.Lp);
@@ -892,7 +966,8 @@ extern "C" int gm_legendre_batch(gm_ctx* c, int nrcv, const double* theta, int l
   HIPCHK(c, hipMemcpyAsync(base + ot, theta, sizeof(double) * nrcv, hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_legendre, dim3((nrcv + 63) / 64, 3), dim3(64), 0, s, nrcv, nrcv, lmax, 0,
                      (const double*)(base + ot), nullptr, (double*)(base + oo[0]), (double*)(base + oo[1]),
-                     (double*)(base + oo[2]), (double*)(base + oo[3]), (double*)(base + oo[4]), 1);
+                     (double*)(base + oo[2]), (double*)(base + oo[3]), (double*)(base + oo[4]), 1, 1, 1 << 30,
+                     nullptr);
   HIPCHK(c, hipGetLastError());
   double* outs[5] = {plm, dpdt, pdst, d2pdt2, dtpdst};
   for (int i = 0; i < 5; ++i) HIPCHK(c, hipMemcpyAsync(outs[i], base + oo[i], sizeof(double) * n, hipMemcpyDeviceToHost, s));
