@@ -98,12 +98,16 @@ __global__ void k_twiddle(int ns, double* __restrict__ tw) {
 // lmax); mode 1: FP64 arrays [R][3][lmax+1] of plm, dpdt, pdst, d2pdt2, dtpdst
 // (computeLegendreFunctions, legendreFunctions.f90:48-157, same operation order).
 // P_l^m (m = 0..2, no Condon-Shortley sign) from P_{l-1}^m = p1 and P_{l-2}^m = p2
-// by the upward recurrence (legendreFunctions.f90:87-100); pmm = P_m^m.
+// by the upward recurrence (legendreFunctions.f90:87-100); pmm = P_m^m.  The
+// reference divides by l - m; here the numerator is multiplied by 1/(l - m),
+// which does not depend on p1, p2 and leaves the chain from one degree to the
+// next two multiplications and a subtraction long (within an ulp of the quotient).
 __device__ __forceinline__ double legendreNext(int l, int m, int lmax, double ct, double pmm, double p1, double p2) {
   if (l > lmax || l < m) return 0.0;
   if (l == m) return pmm;
   if (l == m + 1) return ct * (double)(2 * m + 1) * pmm;
-  return (ct * (double)(2 * l - 1) * p1 - (double)(l + m - 1) * p2) / (double)(l - m);
+  const double rl = 1.0 / (double)(l - m);
+  return (ct * (double)(2 * l - 1) * p1 - (double)(l + m - 1) * p2) * rl;
 }
 __device__ __forceinline__ double legendrePmm(int m, double st) {
   return m == 0 ? 1.0 : (m == 1 ? (1.0 * 1.0) * st : ((1.0 * 1.0) * st * 3.0) * st);
@@ -123,16 +127,14 @@ __global__ void __launch_bounds__(64) k_legendre_seed(int R, int Rp, int lmax, c
   const double pmm = legendrePmm(m, st);
   double* out = seeds + ((size_t)r * 3 + m) * nseg * 2;
   double a1 = 0.0, a2 = 0.0;
-  const int last = (nseg - 1) * seglen;
-  for (int l = 0; l < last; ++l) {
-    const double p = legendreNext(l, m, lmax, ct, pmm, a1, a2);
-    a2 = a1;
-    a1 = p;
-    if ((l + 1) % seglen == 0) {
-      const int sg = (l + 1) / seglen;
-      out[2 * sg] = a1;
-      out[2 * sg + 1] = a2;
+  for (int sg = 1; sg < nseg; ++sg) {
+    for (int l = (sg - 1) * seglen; l < sg * seglen; ++l) {
+      const double p = legendreNext(l, m, lmax, ct, pmm, a1, a2);
+      a2 = a1;
+      a1 = p;
     }
+    out[2 * sg] = a1;
+    out[2 * sg + 1] = a2;
   }
 }
 
