@@ -139,6 +139,7 @@ extern "C" int gm_model_set(gm_ctx* c, int nk, const double* rk, const double* r
                             double fref) {
   if (!c) return GM_ERR_ARG;
   if (nk < 2 || nk > GM_MAXK) return gm_fail(c, GM_ERR_ARG, "nk=%d outside [2,%d]", nk, GM_MAXK);
+  if (fsflag != 0 && fsflag != 1) return gm_fail(c, GM_ERR_ARG, "full_space flag must be 0 or 1");
   if (nlay < 2 || nlay > GM_MAXL) return gm_fail(c, GM_ERR_ARG, "nlay=%d outside [2,%d]", nlay, GM_MAXL);
   // the minors kernels stage one frequency's spline table image in LDS
   if (gmLdsBytes(nk, nlay) > (size_t)c->max_lds)
@@ -149,6 +150,7 @@ extern "C" int gm_model_set(gm_ctx* c, int nk, const double* rk, const double* r
   double rn[GM_MAXN + 1]; int stt[GM_MAXN + 1], lay[GM_MAXN + 1];
   memcpy(rn, m.rnod, sizeof rn); memcpy(stt, m.state, sizeof stt); memcpy(lay, m.layer, sizeof lay);
   memset(&m, 0, sizeof m);
+  c->has_model = false;                  // until the new model has passed every check
   m.nnod = nnod; memcpy(m.rnod, rn, sizeof rn); memcpy(m.state, stt, sizeof stt); memcpy(m.layer, lay, sizeof lay);
   m.nk = nk; m.nlay = nlay; m.nic = nic; m.noc = noc; m.fsflag = fsflag; m.equidis = equidis;
   m.rearth = rearth; m.ric = ric; m.roc = roc; m.fref = fref;
@@ -157,6 +159,8 @@ extern "C" int gm_model_set(gm_ctx* c, int nk, const double* rk, const double* r
     m.vsh[i + 1] = vsh[i]; m.eta[i + 1] = eta[i]; m.qkinv[i + 1] = qkinv[i]; m.qminv[i + 1] = qminv[i];
   }
   for (int i = 0; i < nlay; ++i) { m.iktop[i + 1] = iktop[i]; m.rb[i + 1] = rb[i]; m.iflso[i + 1] = iflso[i]; }
+  // nodeEarthmodel.f90:134-141
+  if (fsflag == 1 && m.iflso[nlay] == 1) return gm_fail(c, GM_ERR_ARG, "Full space and ocean is not allowed");
   if (m.iktop[nlay] != nk) return gm_fail(c, GM_ERR_ARG, "iktop(nlay)=%d differs from nk=%d", m.iktop[nlay], nk);
   m.ikbot[1] = 0;
   for (int nl = 2; nl <= nlay; ++nl) m.ikbot[nl] = m.iktop[nl - 1] + 1;
@@ -239,7 +243,6 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
       if (m.state[k] != 0 || !(m.rnod[k] > m.roc))
         return gm_fail(c, GM_ERR_UNSUPPORTED, "node %d lies in the core or a fluid layer; only mantle nodes are supported", k);
   }
-  if (m.fsflag == 1) return gm_fail(c, GM_ERR_UNSUPPORTED, "full-space models are not supported");
   // (omega, l) plan
   Plan P;
   P.nfreq = nfreq;

@@ -931,7 +931,50 @@ static int minorsBottom(const Env& e, double rstart, cx* ys, double& htry) {
   }
   return OK;
 }
-// minorsSurfaceInitialValues (initialValues.f90:287-367), free surface only
+// The full-space start vector of minorsSurfaceInitialValues
+// (initialValues.f90:312-343) at radius rs in layer nl: minors of the outgoing
+// (second-kind spherical Hankel) P and S solutions of the homogeneous space.
+static int minorsOutgoing(const Env& e, double rs, int nl, cx* ys) {
+  const Model& M = *e.m;
+  double elp1 = e.dll1;
+  double el = -0.5 + std::sqrt(0.25 + elp1);
+  bool liquid = isFluid(M, nl);
+  double ro; cx z[8];
+  evalElcon(M, *e.s, rs, nl, ro, z);
+  cx zom = C(e.omre, e.omim);
+  double rsq = rs * rs;
+  cx zxa2 = cdiv(mulr(mul(rmul(ro, zom), zom), rsq), add(z[6], divr(rmul(4.0, z[7]), 3.0)));
+  cx zeta, zetb = C(0.0), zxb2 = C(0.0);
+  int rc;
+  if ((rc = sphancf(csqrt_(zxa2), el, zeta, 2)) != OK) return rc;
+  if (!liquid) {
+    zxb2 = cdiv(mulr(mul(rmul(ro, zom), zom), rsq), z[7]);
+    if ((rc = sphancf(csqrt_(zxb2), el, zetb, 2)) != OK) return rc;
+  }
+  if (el < DEPS) {
+    ys[1] = neg(zeta);
+    ys[2] = add(neg(mul(rmul(ro * rs, zom), zom)), mul(divr(rmul(4.0, z[7]), rs), zeta));
+  } else if (liquid) {
+    ys[1] = neg(cdiv(rsub(1.0, divr(zeta, el)), mulr(mulr(mul(zom, zom), ro), rs)));
+    ys[2] = C(1.0 / el);
+  } else {
+    cx za = divr(zeta, el);
+    cx zb = divr(zetb, el + 1.0);
+    cx mr = divr(z[7], rs);
+    ys[2] = divr(add(neg(za), mul(zb, subr(za, 1.0))), el);
+    ys[1] = mul(mr, add(divr(zxb2, el), rmul(2.0 * elp1, ys[2])));
+    ys[3] = mul(mr, add(neg(rmul(2.0, ys[2])), mul(divr(zxb2, elp1), subr(za, 1.0))));
+    ys[4] = neg(mul(mr, add(mul(divr(zxb2, el * el), rsub(1.0, zb)), rmul(4.0, ys[2]))));
+    cx t1 = neg(rmul(4.0 * (el - 1.0) * (el + 2.0), ys[2]));
+    cx inner = sub(sub(subr(divr(zxb2, elp1), 2.0 * (el - 1.0) * (2.0 * el + 1.0) / elp1),
+                       rmul(4.0 / (el + 1.0), za)),
+                   rmul(2.0 / el, zb));
+    ys[5] = mul(divr(mul(z[7], z[7]), rsq), add(t1, mul(divr(zxb2, el), inner)));
+  }
+  return OK;
+}
+// minorsSurfaceInitialValues (initialValues.f90:287-367): free surface, or the
+// outgoing solution of a full space above the model (full_space = 1)
 static int minorsSurface(const Env& e, cx* ys, double& htry) {
   const Model& M = *e.m;
   int nlay = M.nlay;
@@ -943,7 +986,7 @@ static int minorsSurface(const Env& e, cx* ys, double& htry) {
   double v = std::sqrt(z[7].re / ro);
   if (liquid) v = std::sqrt(z[6].re / ro);
   htry = 0.25 * v * MC_TWO_PID / e.omre;
-  if (M.fsflag == 1) return ERR_UNSUPPORTED;
+  if (M.fsflag == 1) return minorsOutgoing(e, M.rearth, nlay, ys);
   if (el < FEPS || liquid) { ys[1] = C(1.0); ys[2] = C(0.0); }
   else { ys[1] = C(0.0); ys[2] = C(1.0); ys[3] = C(0.0); ys[4] = C(0.0); ys[5] = C(0.0); }
   return OK;
@@ -973,7 +1016,22 @@ static int toroidalBottom(const Env& e, double rstart, cx* ys, double& htry) {
   }
   return OK;
 }
-// toroidalSurfaceInitialValues (initialValues.f90:427-469), free surface only
+// The full-space start vector of toroidalSurfaceInitialValues
+// (initialValues.f90:458-463) at radius rs in layer nl: the outgoing S solution.
+static int toroidalOutgoing(const Env& e, double rs, int nl, cx* ys) {
+  double ro; cx z[8];
+  evalElcon(*e.m, *e.s, rs, nl, ro, z);
+  double el = -0.5 + std::sqrt(0.25 + e.dll1);
+  cx zom = C(e.omre, e.omim);
+  cx zxb2 = cdiv(mulr(mul(rmul(ro, zom), zom), rs * rs), z[7]);
+  cx zetb;
+  int rc = sphancf(csqrt_(zxb2), el, zetb, 2);
+  if (rc != OK) return rc;
+  ys[1] = C(1.0 / el);
+  ys[2] = mul(divr(z[7], rs * el), rsub(el - 1.0, zetb));
+  return OK;
+}
+// toroidalSurfaceInitialValues (initialValues.f90:427-469)
 static int toroidalSurface(const Env& e, cx* ys, double& htry, double& rsurf, int& nl) {
   const Model& M = *e.m;
   if (hasOcean(M)) { nl = M.nlay - 1; rsurf = ruc_(M); } else { nl = M.nlay; rsurf = M.rearth; }
@@ -981,7 +1039,7 @@ static int toroidalSurface(const Env& e, cx* ys, double& htry, double& rsurf, in
   evalElcon(M, *e.s, rsurf, nl, ro, z);
   double v = std::sqrt(z[7].re / ro);
   htry = 0.25 * v * MC_TWO_PID / e.omre;
-  if (M.fsflag == 1) return ERR_UNSUPPORTED;
+  if (M.fsflag == 1) return toroidalOutgoing(e, rsurf, nl, ys);
   ys[1] = C(1.0); ys[2] = C(0.0);
   return OK;
 }
@@ -1567,7 +1625,9 @@ int orc_initial_values(void* h, double omre, double omim, double dll1, double r,
   if (kind == 0) rc = minorsBottom(env, r, ys, *htry);
   else if (kind == 1) rc = minorsSurface(env, ys, *htry);
   else if (kind == 2) rc = toroidalBottom(env, r, ys, *htry);
-  else rc = toroidalSurface(env, ys, *htry, rs, nlo);
+  else if (kind == 3) rc = toroidalSurface(env, ys, *htry, rs, nlo);
+  else if (kind == 4) rc = minorsOutgoing(env, r, nl, ys);      // full-space start vector at (r, nl)
+  else rc = toroidalOutgoing(env, r, nl, ys);
   for (int v = 1; v <= 5; ++v) { y[2 * (v - 1)] = ys[v].re; y[2 * (v - 1) + 1] = ys[v].im; }
   return rc;
 }

@@ -123,3 +123,43 @@ def test_large_model_gfk_matches_oracle(variant, tol):
     assert rc == 0
     err = rel_l2(r["gf"], ref)
     assert err.max() < tol, err.max()
+
+
+@pytest.mark.parametrize("variant,tol", [("exact", 1e-12), ("fast", 1e-9)])
+def test_full_space_model_matches_oracle(variant, tol):
+    """A model under a homogeneous full space (full_space = 1): the downward legs start from the
+    outgoing solution of that space (second-kind spherical Hankel ratios) instead of the free
+    surface.  Minors and FK columns against the oracle."""
+    import dataclasses
+    from gemini_unified_amd import ExternalRadialNodes, global_model
+    nem = dataclasses.replace(global_model(1, 200), fsflag=1)
+    nodes = ExternalRadialNodes.from_blocks(nem, [3], [30000.0])
+    jre = nodes.node_index(nem.rearth)
+    eng = GreenFKEngine(nem, nodes, variant=variant)
+    orc = oracle.Oracle(nem, nodes)
+    for f, l0, n in [(20 / TLEN, 0, 12), (60 / TLEN, 200, 16)]:
+        print(f"full space {variant}: f={f:.5f} l={l0}..{l0 + n - 1}", flush=True)
+        g = eng.minors([f], l0, n, SIGMA, EPS, 1, nodes.nnod)
+        assert (g["status"] == 0).all()
+        orc.set_frequency(f, 3)
+        om = 2.0 * MC_PI * f
+        for il in range(n):
+            dll1 = dll1_of(l0 + il, nem.rearth)
+            rc, u, d, det = orc.toroidal_minors(om, -SIGMA, dll1, 1, nodes.nnod, EPS)
+            assert rc == 0
+            for a, b in ((g["d_tor"][:, :, il], d), (g["det_tor"][:, il], det)):
+                den = np.abs(b).max()
+                assert den == 0 or np.abs(a - b).max() / den < tol
+            rc, us, ds, ulg, dlg, dets = orc.spheroidal_minors(om, -SIGMA, dll1, 1, nodes.nnod, EPS)
+            assert rc == 0
+            a, b = g["d_sph"][:, :5, il], ds[:, :5]
+            assert np.abs(a - b).max() / np.abs(b).max() < tol
+            b = dets
+            assert np.abs(g["det_sph"][:, il] - b).max() / np.abs(b).max() < tol
+        r = eng.gfk([f], l0, n, SIGMA, EPS, jre, 1, want_fp32=False)
+        rc, ref, _ = orc.gfk_degrees(om, -SIGMA, l0, n, jre, 1, EPS)
+        assert rc == 0
+        err = rel_l2(r["gf"], ref)
+        print(f"  FK columns max rel-L2 {err.max():.2e}", flush=True)
+        assert err.max() < (1e-12 if variant == "exact" else 1e-8)
+    eng.close()

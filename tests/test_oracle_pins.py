@@ -203,13 +203,13 @@ def test_spheroidal_determinant_and_pluecker(mantle_nodes, f, l):
         assert np.abs(pl / scale).max() < 1e-9
 
 
-def homogeneous_model(vp=8000.0, vs=4500.0, rho=3300.0, n=60):
+def homogeneous_model(vp=8000.0, vs=4500.0, rho=3300.0, n=60, full_space=0):
     R = 6371000.0
     r2 = np.linspace(100000.0, R, n)
     sph = dict(r=np.array([100000.0]), fluid=False, vp=[vp], vs=[vs], density=[rho], qk=[57823.0], qm=[300.0])
     lay = dict(r=r2, fluid=False, vp=np.full(n, vp), vs=np.full(n, vs), density=np.full(n, rho),
                qk=np.full(n, 57823.0), qm=np.full(n, 300.0))
-    return NodeEarthmodel.from_json(_layered_json("homogeneous", R, [sph, lay]))
+    return NodeEarthmodel.from_json(_layered_json("homogeneous", R, [sph, lay], full_space=full_space))
 
 
 @pytest.mark.parametrize("f,l", [(0.02, 30), (0.035, 120)])
@@ -241,6 +241,52 @@ def test_homogeneous_sphere_known_answer(f, l):
         v = ut[k]
         c = np.vdot(v, yt[:2]) / np.vdot(v, v)
         assert np.linalg.norm(yt[:2] - c * v) / np.linalg.norm(yt[:2]) < 1e-4
+
+
+@pytest.mark.parametrize("f,l", [(0.02, 30), (0.035, 120)])
+def test_full_space_known_answer(f, l):
+    """A homogeneous model under a full space (full_space = 1): the downward legs start from the
+    outgoing (second-kind spherical Hankel) solution at the top, an exact solution of the homogeneous
+    medium.  So the integrated vectors stay parallel to the same start vector evaluated at each node,
+    which pins the kind-2 continued-fraction ratio and both full-space start formulas.
+    - Toroidal, with W and T compared in units of s = rho vs omega (T ~ s W): to 1e-3 within the two
+      shallowest nodes (871 km of integration), where the free-surface solution is off by > 0.1.
+      Deeper, downward integration through evanescent zones pulls any solution towards the dominant one.
+    - Minors, compared as raw vectors as in the homogeneous-sphere test (dominated by the largest
+      minors): to 1e-4 at every node.  The known answer carries the 1e-6 stopping tolerance of the
+      continued fractions."""
+    nem = homogeneous_model(full_space=1)
+    assert nem.fsflag == 1
+    nodes = ExternalRadialNodes.split_at_discos(nem, np.array([2.5e6, 3.5e6, 4.5e6, 5.5e6, nem.rearth]), nem.rearth)
+    orc = oracle.Oracle(nem, nodes)
+    orc.set_frequency(f, 3)
+    om, omim, dll1 = 2 * MC_PI * f, -0.0024, dll1_of(l, nem.rearth)
+    rc, u, d, ulg, dlg, det = orc.spheroidal_minors(om, omim, dll1, 1, nodes.nnod, 1e-10)
+    assert rc == 0
+    rc2, ut, dt, dett = orc.toroidal_minors(om, omim, dll1, 1, nodes.nnod, 1e-10)
+    assert rc2 == 0
+    free = homogeneous_model()
+    orf = oracle.Oracle(free, nodes)
+    orf.set_frequency(f, 3)
+    rc3, _, dtf, _ = orf.toroidal_minors(om, omim, dll1, 1, nodes.nnod, 1e-10)
+    assert rc3 == 0
+    w = np.array([1.0, 1.0 / (3300.0 * 4500.0 * om)])
+
+    def deviation(v, ya, w=None):
+        a, b = (ya, v) if w is None else (ya * w, v * w)
+        c = np.vdot(b, a) / np.vdot(b, b)
+        return np.linalg.norm(a - c * b) / np.linalg.norm(a)
+
+    for k in range(nodes.nnod):
+        r = nodes.rnod[k]
+        rc, ya, _ = orc.initial_values(om, omim, dll1, r, 2, 4)
+        assert rc == 0
+        assert deviation(d[k, :5], ya) < 1e-4
+        if k >= nodes.nnod - 2:
+            rc, yt, _ = orc.initial_values(om, omim, dll1, r, 2, 5)
+            assert rc == 0
+            assert deviation(dt[k], yt[:2], w) < 1e-3
+            assert deviation(dtf[k], yt[:2], w) > 0.1
 
 
 # ---------------------------------------------------------------------------
