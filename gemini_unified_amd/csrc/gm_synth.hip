@@ -746,9 +746,12 @@ __device__ __forceinline__ void stockhamPass(cd* sx, int M, int Ns, const double
 // FFT in radix-16 passes (and one radix 2, 4 or 8 pass for the rest of log2 M),
 // each pass a 16-point DFT in registers between two LDS barriers: three passes
 // for M = 4096 where radix 2 took twelve.
+// ns: transform length, nsamp: samples kept per trace, every ndec-th (ndec = 1
+// or, in the reference's decimation branch, ns / nsamp), dt: step of the
+// transform (dts there).
 __global__ void __launch_bounds__(256) k_ifft(int nf1, int nf2, int ns, int logm, const double* __restrict__ Z,
                                               const double* __restrict__ tw, double df, double sigma, double dt,
-                                              float* __restrict__ ts) {
+                                              int nsamp, int ndec, float* __restrict__ ts) {
   __shared__ cd sx[kFFTSlots];
   const int M = ns >> 1;
   const double* X = Z + (size_t)blockIdx.x * nf2 * 2;
@@ -789,9 +792,19 @@ __global__ void __launch_bounds__(256) k_ifft(int nf1, int nf2, int ns, int logm
   if (left == 3) stockhamPass<8>(sx, M, Ns, tw, w16);
   else if (left == 2) stockhamPass<4>(sx, M, Ns, tw, w16);
   else if (left == 1) stockhamPass<2>(sx, M, Ns, tw, w16);
+  float* out = ts + (size_t)blockIdx.x * nsamp;
+  if (ndec > 1) {
+    // decimation branch of transformFrequencyTime (frequencyTime.f90:104-109):
+    // sample i is x_{ndec i}, damped by exp(sigma dts i) as written there
+    for (int i = threadIdx.x; i < nsamp; i += blockDim.x) {
+      const int t = ndec * i;
+      const cd z = sx[fftSlot(t >> 1)];
+      out[i] = (float)(((t & 1) ? z.im : z.re) * df * exp(sigma * dt * i));
+    }
+    return;
+  }
   // e^{sigma dt t} at t = 2n and 2n + 1, n = threadIdx.x + i blockDim.x: one
   // exponential per thread, then a product per step of 2 blockDim samples
-  float* out = ts + (size_t)blockIdx.x * ns;
   const double a = sigma * dt;
   double e0 = df * exp(a * (2 * (int)threadIdx.x));
   const double e1 = exp(a), estep = exp(a * (2 * (int)blockDim.x));
@@ -1025,22 +1038,34 @@ static int synth_impl(gm_ctx* c, int nf1, int nf2, double df, double sigma, cons
   if (!gfkspr || !gfkspi || !theta || !phi || !mt || !ts)
     return gm_fail(c, GM_ERR_ARG, "spectra, theta, phi, fomt and ts are required");
   if (!(df > 0.0) || !(dt > 0.0)) return gm_fail(c, GM_ERR_ARG, "df and dt must be positive");
-  // transformFrequencyTime (frequencyTime.f90:68-85): only the case without decimation
+  // transformFrequencyTime (frequencyTime.f90:68-85): transform length ns and
+  // decimation ndec; the reference takes the first logarithm in single precision
   int nslog = (int)std::ceil((double)std::log((float)nf2) / std::log(2.0)) + 1;
-  if ((1 << nslog) > nsamp) return gm_fail(c, GM_ERR_UNSUPPORTED, "nsamp = %d needs decimation (minimum %d)", nsamp, 1 << nslog);
-  nslog = (int)std::ceil(-std::log(df * dt) / std::log(2.0));
-  if (nslog < 2 || (1 << nslog) != nsamp)
-    return gm_fail(c, GM_ERR_ARG, "nsamp = %d must equal 2^ceil(-log2(df dt)) = %d", nsamp, nslog >= 2 ? 1 << nslog : 0);
-  if (nsamp > kMaxFFT) return gm_fail(c, GM_ERR_UNSUPPORTED, "nsamp = %d exceeds %d", nsamp, kMaxFFT);
+  int ns = 1 << nslog, ndec = 1;
+  double dt_fft = dt;
+  if (ns <= nsamp) {
+    ns = nsamp;
+    nslog = (int)std::ceil(-std::log(df * dt) / std::log(2.0));
+    if (nslog < 2 || (1 << nslog) != nsamp)
+      return gm_fail(c, GM_ERR_ARG, "nsamp = %d must equal 2^ceil(-log2(df dt)) = %d", nsamp, nslog >= 2 ? 1 << nslog : 0);
+  } else {
+    dt_fft = 1.0 / (df * ns);
+    ndec = (int)std::lround(dt / dt_fft);
+    if (std::fabs(dt - ndec * dt_fft) > 1.e-5)
+      return gm_fail(c, GM_ERR_ARG, "Non-integer ratio of desired dt and dt defined from frequency spectrum");
+    if (ndec < 2 || (size_t)(nsamp - 1) * ndec >= (size_t)ns)
+      return gm_fail(c, GM_ERR_ARG, "nsamp = %d samples at every %d-th of %d do not fit the transform", nsamp, ndec, ns);
+  }
+  if (ns > kMaxFFT) return gm_fail(c, GM_ERR_UNSUPPORTED, "transform length %d exceeds %d", ns, kMaxFFT);
   const bool dev = (flags & GM_FLAG_DEVICE_PTRS) != 0;
   HIPCHK(c, hipSetDevice(c->device));
   hipStream_t s = c->stream;
-  if (c->tw_n != nsamp) {
+  if (c->tw_n != ns) {
     if (c->dtw) { (void)hipFree(c->dtw); c->dtw = nullptr; c->tw_n = 0; }
-    if (hipMalloc(&c->dtw, sizeof(double) * 2 * nsamp) != hipSuccess)
+    if (hipMalloc(&c->dtw, sizeof(double) * 2 * ns) != hipSuccess)
       return gm_fail(c, GM_ERR_NOMEM, "twiddle allocation failed");
-    hipLaunchKernelGGL(k_twiddle, dim3((nsamp + 255) / 256), dim3(256), 0, s, nsamp, c->dtw);
-    c->tw_n = nsamp;
+    hipLaunchKernelGGL(k_twiddle, dim3((ns + 255) / 256), dim3(256), 0, s, ns, c->dtw);
+    c->tw_n = ns;
   }
   Carve cv;
   SumBufs o = carve_sums(P, dev, cv);
@@ -1070,9 +1095,9 @@ static int synth_impl(gm_ctx* c, int nf1, int nf2, double df, double sigma, cons
   HIPCHK(c, hipEventRecord(c->sev[4], s));
   float* dts = dev ? ts : (float*)(base + o_ts);
   int logm = 0;
-  while ((1 << (logm + 1)) <= nsamp / 2) ++logm;
-  hipLaunchKernelGGL(k_ifft, dim3(3 * P.R), dim3(256), 0, s, nf1, nf2, nsamp, logm, (const double*)dZ, c->dtw, df, sigma,
-                     dt, dts);
+  while ((1 << (logm + 1)) <= ns / 2) ++logm;
+  hipLaunchKernelGGL(k_ifft, dim3(3 * P.R), dim3(256), 0, s, nf1, nf2, ns, logm, (const double*)dZ, c->dtw, df, sigma,
+                     dt_fft, nsamp, ndec, dts);
   HIPCHK(c, hipEventRecord(c->sev[5], s));
   HIPCHK(c, hipGetLastError());
   if (!dev) HIPCHK(c, hipMemcpyAsync(ts, dts, sizeof(float) * 3 * (size_t)P.R * nsamp, hipMemcpyDeviceToHost, s));

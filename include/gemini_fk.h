@@ -221,8 +221,9 @@ int gm_force_sums(gm_ctx* ctx, int nf1, int nf2, const int* nwn, int nsp, long l
  * 367-371), velocity (velocity = 1) and
  * rotation to Z, N, E by propdir[r] (NULL: components r, theta, phi), then the
  * frequency-to-time transform to nsamp FP32 samples (transformFrequencyTime,
- * f90/frequencyTime.f90:55-113; nsamp = 2^ceil(-log2(df dt)) <= 8192, no
- * decimation).  Replaces the station loop body of
+ * f90/frequencyTime.f90:55-113): nsamp = 2^ceil(-log2(df dt)), or, when nf2
+ * needs more samples (ns = 2^(ceil(log2 nf2) + 1) > nsamp), every ndec-th
+ * sample of the ns-sample transform, ndec = dt df ns an integer; ns <= 8192.  Replaces the station loop body of
  * f90/computeSyntheticSeismograms.f90:311-388 (without the travel-time
  * automatic shift: tred is given).
  * ts: [nrcv][3][nsamp]; zsp (optional): complex [nrcv][3][nf2].

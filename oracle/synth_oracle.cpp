@@ -216,23 +216,34 @@ void fft(std::vector<double>& x, std::vector<double>& y, int n, int is) {
 }
 
 // Spectrum sp[jf-1], jf = 1..nf2 (entries below nf1 ignored) -> nsamp real
-// samples.  Restates transformFrequencyTime (f90/frequencyTime.f90:55-113) for
-// the case ns <= nsamp (no decimation).  Returns 0, or 1 if decimation would be needed.
+// samples.  Restates transformFrequencyTime (f90/frequencyTime.f90:55-113),
+// including its decimation branch: when nf2 needs more than nsamp samples the
+// transform runs at ns = 2^(ceil(log2 nf2)+1) samples of dts = 1/(df ns) and
+// every ndec-th sample is kept, ndec = nint(dt/dts); the damping factor then
+// advances by exp(sigma dts) per kept sample, as written there (:104-109).
+// Returns 0, or 2 for a non-integer ratio dt/dts, or 3 if dt does not match nsamp.
 int freq_time(int nf1, int nf2, double df, double sigma, const cd* sp, int nsamp, double dt, float* ts) {
   // the reference takes this log in single precision (log of real(nf2))
   int nslog = (int)std::ceil((double)std::log((float)nf2) / std::log(2.0)) + 1;
   int ns = 1 << nslog;
-  if (ns > nsamp) return 1;
-  ns = nsamp;
-  nslog = (int)std::ceil(-std::log(df * dt) / std::log(2.0));
-  if ((1 << nslog) != ns) return 2;
+  double dts = dt;
+  int ndec = 1;
+  if (ns <= nsamp) {
+    ns = nsamp;
+    nslog = (int)std::ceil(-std::log(df * dt) / std::log(2.0));
+    if ((1 << nslog) != ns) return 3;
+  } else {
+    dts = 1.0 / (df * ns);
+    ndec = (int)std::lround(dt / dts);
+    if (std::fabs(dt - ndec * dts) > 1.e-5) return 2;
+  }
   std::vector<double> x(ns, 0.0), y(ns, 0.0);
   for (int jf = nf1; jf <= nf2; ++jf) { x[jf - 1] = sp[jf - 1].real(); y[jf - 1] = sp[jf - 1].imag(); }
   for (int i = ns / 2 + 2; i <= ns; ++i) { x[i - 1] = x[ns - i + 1]; y[i - 1] = -y[ns - i + 1]; }
   fft(x, y, nslog, +1);
   double fac = 1.0;
-  const double expo = std::exp(sigma * dt);
-  for (int i = 0; i < nsamp; ++i) { ts[i] = (float)(x[i] * df * fac); fac = fac * expo; }
+  const double expo = std::exp(sigma * dts);
+  for (int i = 0; i < nsamp; ++i) { ts[i] = (float)(x[(size_t)ndec * i] * df * fac); fac = fac * expo; }
   return 0;
 }
 

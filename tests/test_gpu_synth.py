@@ -415,3 +415,30 @@ def test_frequency_time_every_length(syn):
         terr = (np.linalg.norm(ts64 - ots64, axis=2) / np.linalg.norm(ots64, axis=2)).max()
         print(f"nsamp {nsamp}: seismogram rel-L2 {terr:.2e}", flush=True)
         assert terr < 1e-5, (nsamp, terr)
+
+
+@pytest.mark.parametrize("nsamp", [128, 64])
+def test_decimated_synthetics_match_oracle(syn, nsamp):
+    """Fewer samples than the frequencies need: the transform runs at the full length and keeps every
+    ndec-th sample, as transformFrequencyTime's decimation branch does; a dt that is no multiple of
+    the transform's step is an error."""
+    from gemini_unified_amd._lib import GeminiFKError
+    rng = np.random.default_rng(61)
+    nf1, nf2, T = 2, 200, 256.0                      # 200 frequencies need 512 samples
+    df, dt = 1.0 / T, T / nsamp
+    sigma = 5 * df
+    nwn = np.zeros(nf2, np.int32)
+    nwn[nf1 - 1:] = 20
+    gr, gi = _random_spectra(rng, nwn[nf1 - 1:])
+    theta, phi = rng.uniform(0.2, 3.0, 3), rng.uniform(0, 6.2, 3)
+    mt = rng.normal(size=6)
+    print(f"decimated synthetics: nsamp {nsamp} of 512", flush=True)
+    ts = syn.synthetics(nf1, nf2, df, sigma, nwn, gr, gi, theta, phi, mt, nsamp, dt, velocity=True)
+    ots, _ = oracle.synthetics(nf1, nf2, df, sigma, nwn, gr, gi, theta, phi, mt, nsamp, dt, velocity=True)
+    ts64, ots64 = ts.astype(np.float64), ots.astype(np.float64)
+    terr = (np.linalg.norm(ts64 - ots64, axis=2) / np.linalg.norm(ots64, axis=2)).max()
+    print(f"seismogram rel-L2 {terr:.2e}", flush=True)
+    assert ts.shape == (3, 3, nsamp) and terr < 1e-5
+    with pytest.raises(GeminiFKError) as e:
+        syn.synthetics(nf1, nf2, df, sigma, nwn, gr, gi, theta, phi, mt, nsamp, 1.3 * dt)
+    assert e.value.code == 9

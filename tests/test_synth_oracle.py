@@ -112,8 +112,32 @@ def test_freq_time_matches_irfft():
     X[:nf2] = sp
     ref = np.fft.irfft(X, nsamp) * nsamp * df * np.exp(sigma * dt * np.arange(nsamp))
     np.testing.assert_allclose(ts, ref, rtol=0, atol=1e-5 * np.abs(ref).max())
-    rc, _ = oracle.freq_time(nf1, 300, df, sigma, np.zeros(300, complex), 512, dt)
-    assert rc != 0                                  # would need decimation: not handled
+    # 300 frequencies need 1024 samples: dt = 0.3 T / 512 is no multiple of T / 1024
+    rc, _ = oracle.freq_time(nf1, 300, df, sigma, np.zeros(300, complex), 512, 0.3 * T / 512)
+    assert rc == 2
+
+
+@pytest.mark.parametrize("ndec", [2, 4])
+def test_freq_time_decimation(ndec):
+    """With fewer samples than the frequencies need, transformFrequencyTime transforms at the full
+    length ns (step dts) and keeps every ndec-th sample; its damping factor advances by
+    exp(sigma dts) per kept sample (frequencyTime.f90:104-109), so the kept samples are those of the
+    full transform times exp(-sigma dts (ndec - 1) i)."""
+    rng = np.random.default_rng(2)
+    nf1, nf2 = 3, 200                               # needs ns = 512
+    T = 256.0
+    df, sigma = 1.0 / T, 5.0 / T
+    sp = np.zeros(nf2, np.complex128)
+    sp[nf1 - 1:] = rng.normal(size=nf2 - nf1 + 1) + 1j * rng.normal(size=nf2 - nf1 + 1)
+    ns = 512
+    dts = T / ns
+    rc, full = oracle.freq_time(nf1, nf2, df, sigma, sp, ns, dts)
+    assert rc == 0
+    rc, dec = oracle.freq_time(nf1, nf2, df, sigma, sp, ns // ndec, ndec * dts)
+    assert rc == 0
+    i = np.arange(ns // ndec)
+    ref = full[::ndec].astype(np.float64) * np.exp(-sigma * dts * (ndec - 1) * i)
+    np.testing.assert_allclose(dec, ref, rtol=1e-6, atol=1e-6 * np.abs(ref).max())
 
 
 def _gen_spectra(nf, L, t, kind):
