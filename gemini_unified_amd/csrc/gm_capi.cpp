@@ -255,6 +255,7 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
     for (int i = 0; i < nl[f]; ++i) { P.q.push_back(f); P.l.push_back(lmin[f] + i); }
   }
   P.ninst = (int)P.q.size();
+  if (P.ninst == 0) return GM_OK;     // no (omega, l) instance: every output is empty, stats unchanged
   {
     // work units: degree ranges of one frequency, highest frequency first and
     // low degrees (the longest integrations) first within a frequency

@@ -163,3 +163,19 @@ def test_full_space_model_matches_oracle(variant, tol):
         print(f"  FK columns max rel-L2 {err.max():.2e}", flush=True)
         assert err.max() < (1e-12 if variant == "exact" else 1e-8)
     eng.close()
+
+
+def test_empty_and_ragged_batches(c1_setup):
+    """An empty batch returns at once; frequencies with no degrees (ragged nl) add nothing and leave the
+    other columns as a batch without them computes them."""
+    nem, nodes, jre = c1_setup
+    eng = GreenFKEngine(nem, nodes, variant="exact")
+    r0 = eng.gfk(np.array([]), np.array([], np.int32), np.array([], np.int32), SIGMA, EPS, jre, 1)
+    assert r0["gf"].shape[-1] == 0 and np.all(r0["stats"] == 0)
+    f = np.array([10 / TLEN, 30 / TLEN, 50 / TLEN])
+    r1 = eng.gfk(f, np.array([0, 5, 7], np.int32), np.array([6, 0, 9], np.int32), SIGMA, EPS, jre, 1)
+    r2 = eng.gfk(f[[0, 2]], np.array([0, 7], np.int32), np.array([6, 9], np.int32), SIGMA, EPS, jre, 1)
+    assert r1["gf"].shape[-1] == 15
+    np.testing.assert_array_equal(r1["gf"], r2["gf"])
+    np.testing.assert_array_equal(r1["stats"], r2["stats"])
+    eng.close()
