@@ -318,10 +318,13 @@ __global__ void k_gprep(int nsp, int nf, int Np, int Lp, const int* __restrict__
 
 // H[j][Rp][Np] (FP64) = c_fac[t][j] * sum over degrees of basis row c_row[t][j]
 // times spectrum c_spec[t][j] (the j-th sum of source type t = istyp); one
-// block per (sum, tile) of Np/TN x Rp/TM tiles, 256 threads: 4 waves of 64 x 32.
+// block per (degree split, sum, tile) of Np/TN x Rp/TM tiles, 256 threads: 4
+// waves of 64 x 32.  With nsplit > 1 split s covers chunks [s kper, (s+1) kper)
+// of TK degrees and writes its partial sums to plane s of H ([nsplit][nsum][Rp][Np]),
+// which k_sumsplits adds in split order.
 __global__ void __launch_bounds__(256, 2) k_contract(int Rp, int Np, int Lp, int nsp, const __bf16* __restrict__ Bs,
                                                   const __bf16* __restrict__ Gs, double* __restrict__ H, int regional,
-                                                  int istyp) {
+                                                  int istyp, int nsplit, int kper) {
   __shared__ __attribute__((aligned(16))) __bf16 sA[kPieces][TM][LDSK];
   __shared__ __attribute__((aligned(16))) __bf16 sB[kPieces][TN][LDSK];
   // XCD-aware tile order.  The hardware deals blocks to the kXcd XCDs
@@ -331,11 +334,13 @@ __global__ void __launch_bounds__(256, 2) k_contract(int Rp, int Np, int Lp, int
   // tile fastest: blocks that run together on one XCD then share column
   // tiles of the spectra and row tiles of the basis in that XCD's L2.
   const int nsum = istyp ? kSums : kForceSums;
-  const int gx = Np / TN, gy = Rp / TM, ntiles = gx * gy, nwork = nsum * ntiles;
+  const int gx = Np / TN, gy = Rp / TM, ntiles = gx * gy, nwork = nsum * ntiles * nsplit;
   const int b = blockIdx.x, xcd = b % kXcd, sq = b / kXcd;
   const int per = (nwork + kXcd - 1) / kXcd, rem = nwork - (per - 1) * kXcd;   // rem XCDs get per items
   const int wk = xcd < rem ? xcd * per + sq : rem * per + (xcd - rem) * (per - 1) + sq;
-  const int j = wk / ntiles, tile = wk % ntiles;
+  const int split = wk / (nsum * ntiles), wks = wk % (nsum * ntiles);
+  const int j = wks / ntiles, tile = wks % ntiles;
+  const int kbeg = split * kper * TK, kend = min(Lp, (split + 1) * kper * TK);
   const int strip = tile / (kStrip * gx), sidx = tile % (kStrip * gx);
   const int sh = min(kStrip, gy - strip * kStrip);
   const int ty = strip * kStrip + sidx % sh, tx = sidx / sh;
@@ -383,11 +388,11 @@ __global__ void __launch_bounds__(256, 2) k_contract(int Rp, int Np, int Lp, int
       for (int e = 0; e < 4; ++e) sum[a][b][e] = 0.0;
     }
   const int fr = lane & 15, fk = (lane >> 4) * 8;
-  GM_GLOAD(0);
-  for (int k0 = 0; k0 < Lp; k0 += TK) {
+  GM_GLOAD(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += TK) {
     GM_LSTORE();
     __syncthreads();
-    if (k0 + TK < Lp) GM_GLOAD(k0 + TK);
+    if (k0 + TK < kend) GM_GLOAD(k0 + TK);
     bf16x8 bq[2][kPieces];
 #pragma unroll
     for (int b = 0; b < 2; ++b)
@@ -429,7 +434,7 @@ __global__ void __launch_bounds__(256, 2) k_contract(int Rp, int Np, int Lp, int
       for (int e = 0; e < 4; ++e) {
         const int row = r0 + wr + a * 16 + (lane >> 4) * 4 + e;
         const int col = n0 + wc + b * 16 + (lane & 15);
-        H[((size_t)j * Rp + row) * Np + col] = sum[a][b][e] * fac;
+        H[(((size_t)split * nsum + j) * Rp + row) * Np + col] = sum[a][b][e] * fac;
       }
 }
 #undef GM_GLOAD
@@ -438,6 +443,15 @@ __global__ void __launch_bounds__(256, 2) k_contract(int Rp, int Np, int Lp, int
 #undef GM_B_SRC
 #undef GM_A_DST
 #undef GM_B_DST
+
+// H[i] = sum over s < nsplit of Hp[s][i], in split order, i < n (nsplit > 1)
+__global__ void k_sumsplits(size_t n, int nsplit, const double* __restrict__ Hp, double* __restrict__ H) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double a = Hp[i];
+  for (int sp = 1; sp < nsplit; ++sp) a += Hp[(size_t)sp * n + i];
+  H[i] = a;
+}
 
 struct cd { double re, im; };
 __device__ __forceinline__ cd cmul(cd a, cd b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
@@ -884,7 +898,19 @@ int ensure_sbuf(gm_ctx* c, size_t bytes) {
 }
 
 // basis, spectra and the contraction into H; offsets of the carved buffer in `o`
-struct SumBufs { size_t theta, Bs, Gs, H, nwn, koff, gr, gi, seeds; };
+struct SumBufs { size_t theta, Bs, Gs, H, nwn, koff, gr, gi, seeds, Hp; };
+
+// degree splits of the contraction: long degree ranges with few (sum, tile)
+// blocks (the regional wavenumber sums) are cut so that about 6000 blocks share
+// the chip, each split at least 16 chunks of TK degrees
+inline void contractSplits(const SynPlan& P, int& nsplit, int& kper) {
+  const int nchunks = P.Lp / TK;
+  const long blocks = (long)P.nsum * (P.Np / TN) * (P.Rp / TM);
+  int s = (int)std::min<long>(8, std::max<long>(1, (6000 + blocks - 1) / blocks));
+  s = std::max(1, std::min(s, nchunks / 16));
+  kper = (nchunks + s - 1) / s;
+  nsplit = (nchunks + kper - 1) / kper;
+}
 
 // degree segments of the Legendre basis: about 8 (at least 64 degrees each),
 // seglen a multiple of 32 (Lp is one)
@@ -932,9 +958,17 @@ int run_sums(gm_ctx* c, const SynPlan& P, const double* theta, const float* gr, 
                      (const int*)(base + o.nwn), (const long long*)(base + o.koff), P.nkf, dgr, dgi, 0.2, dwn,
                      P.istyp, (__bf16*)(base + o.Gs));
   HIPCHK(c, hipEventRecord(c->sev[2], s));
-  hipLaunchKernelGGL(k_contract, dim3((P.Np / TN) * (P.Rp / TM) * P.nsum), dim3(256), 0, s, P.Rp, P.Np, P.Lp,
-                     P.nsp, (const __bf16*)(base + o.Bs), (const __bf16*)(base + o.Gs), (double*)(base + o.H),
-                     regional ? 1 : 0, P.istyp);
+  int nsplit, kper;
+  contractSplits(P, nsplit, kper);
+  double* Hout = (double*)(base + (nsplit > 1 ? o.Hp : o.H));
+  hipLaunchKernelGGL(k_contract, dim3((P.Np / TN) * (P.Rp / TM) * P.nsum * nsplit), dim3(256), 0, s, P.Rp, P.Np, P.Lp,
+                     P.nsp, (const __bf16*)(base + o.Bs), (const __bf16*)(base + o.Gs), Hout, regional ? 1 : 0,
+                     P.istyp, nsplit, kper);
+  if (nsplit > 1) {
+    const size_t nh = (size_t)P.nsum * P.Rp * P.Np;
+    hipLaunchKernelGGL(k_sumsplits, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, s, nh, nsplit,
+                       (const double*)Hout, (double*)(base + o.H));
+  }
   if (regional)
     hipLaunchKernelGGL(P.istyp ? k_wnfix<1> : k_wnfix<0>, dim3((P.R * P.nf + 255) / 256), dim3(256), 0, s, P.R, P.nf,
                        P.Rp, P.Np, (const int*)(base + o.nwn), (const long long*)(base + o.koff), P.nkf, dgr, dgi, dth,
@@ -955,6 +989,9 @@ SumBufs carve_sums(const SynPlan& P, bool dev_in, Carve& cv) {
   o.Bs = cv.take(sizeof(__bf16) * kPieces * kRows * (size_t)P.Rp * P.Lp);
   o.Gs = cv.take(sizeof(__bf16) * kPieces * (size_t)P.nsp * P.Np * P.Lp);
   o.H = cv.take(sizeof(double) * P.nsum * (size_t)P.Rp * P.Np);
+  int nsplit, kper;
+  contractSplits(P, nsplit, kper);
+  o.Hp = nsplit > 1 ? cv.take(sizeof(double) * nsplit * P.nsum * (size_t)P.Rp * P.Np) : 0;
   o.gr = dev_in ? 0 : cv.take(sizeof(float) * P.nsp * P.nkf);
   o.gi = dev_in ? 0 : cv.take(sizeof(float) * P.nsp * P.nkf);
   return o;
