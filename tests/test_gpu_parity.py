@@ -179,3 +179,30 @@ def test_empty_and_ragged_batches(c1_setup):
     np.testing.assert_array_equal(r1["gf"], r2["gf"])
     np.testing.assert_array_equal(r1["stats"], r2["stats"])
     eng.close()
+
+
+def test_model_limits():
+    """The largest model (512 knots) matches the oracle; a full space above an ocean is rejected as
+    nodeEarthmodel.f90:134-141 rejects it."""
+    import dataclasses
+    from gemini_unified_amd import ExternalRadialNodes, global_model
+    from gemini_unified_amd._lib import GeminiFKError
+    nem = global_model(2, 512)
+    assert nem.rk.size == 512
+    nodes = ExternalRadialNodes.from_blocks(nem, [1], [30000.0])
+    jre = nodes.node_index(nem.rearth)
+    f = 30 / TLEN
+    eng = GreenFKEngine(nem, nodes, variant="exact")
+    r = eng.gfk([f], 10, 8, SIGMA, EPS, jre, 1, want_fp32=False)
+    eng.close()
+    orc = oracle.Oracle(nem, nodes)
+    orc.set_frequency(f, 3)
+    rc, ref, _ = orc.gfk_degrees(2.0 * MC_PI * f, -SIGMA, 10, 8, jre, 1, EPS)
+    assert rc == 0 and (r["status"] == 0).all()
+    assert rel_l2(r["gf"], ref).max() < 1e-12
+    iflso = np.array(nem.iflso).copy()
+    iflso[-1] = 1
+    bad = dataclasses.replace(nem, fsflag=1, iflso=iflso)
+    with pytest.raises(GeminiFKError) as e:
+        GreenFKEngine(bad, nodes, variant="exact")
+    assert e.value.code == 9
