@@ -206,3 +206,24 @@ def test_model_limits():
     with pytest.raises(GeminiFKError) as e:
         GreenFKEngine(bad, nodes, variant="exact")
     assert e.value.code == 9
+
+
+@pytest.mark.parametrize("f,lmin,nl", [(20 / TLEN, 0, 20), (70 / TLEN, 300, 24)])
+def test_force_source_columns_match_oracle(c1_setup, f, lmin, nl):
+    """FK columns of a single-force source (istyp = 0: U, V x 2 jumps and W x 1, sourceTerms.f90:98-115)
+    against the oracle, exact and fast variants; the FP32 file values are the rounded FP64 values."""
+    nem, nodes, jre = c1_setup
+    orc = oracle.Oracle(nem, nodes)
+    orc.set_frequency(f, 3)
+    rc, ref, _ = orc.gfk_degrees(2.0 * MC_PI * f, -SIGMA, lmin, nl, jre, 0, EPS)
+    assert rc == 0 and ref.shape[1] == 5
+    for variant, tol in (("exact", 1e-12), ("fast", 1e-8)):
+        eng = GreenFKEngine(nem, nodes, variant=variant)
+        r = eng.gfk([f], lmin, nl, SIGMA, EPS, jre, 0)
+        eng.close()
+        assert r["gf"].shape[1] == 5 and (r["status"] == 0).all()
+        err = rel_l2(r["gf"], ref)
+        print(f"force {variant} f={f:.5f} l={lmin}..{lmin + nl - 1}: max rel-L2 {err.max():.2e}", flush=True)
+        assert err.max() < tol
+        if variant == "exact":
+            np.testing.assert_array_equal(r["gfre"], ref.real.astype(np.float32))
