@@ -113,29 +113,51 @@ __device__ __forceinline__ double legendrePmm(int m, double st) {
   return m == 0 ? 1.0 : (m == 1 ? (1.0 * 1.0) * st : ((1.0 * 1.0) * st * 3.0) * st);
 }
 
-// Start values of the degree segments of k_legendre: for every (receiver r,
-// order m) and segment s = 1..nseg-1, (P_{l0-1}^m, P_{l0-2}^m) at l0 = s seglen,
-// by the same recurrence and arithmetic as k_legendre, so that each segment
-// continues it bit for bit.  seeds: [Rp][3][nseg][2].  Grid (Rp / 64, 3).
+// Start values of the degree segments of k_legendre.  Segment s covers degrees
+// [s seglen, (s+1) seglen).  Beyond degree m + 1 the recurrence is linear in the
+// pair v_l = (P_l^m, P_{l-1}^m): v_l = M_l v_{l-1}, M_l = [[c_l, -d_l], [1, 0]]
+// (M_l = [[0, 0], [1, 0]] beyond lmax).  So, per (receiver r, order m), one
+// thread runs segment 0 by the recurrence itself and stores v at its last
+// degree, and one thread per segment s = 1..nseg-2 forms the product T_s of its
+// M_l.  k_legendre's segment s then starts from T_{s-1} ... T_1 v.  Segments 0
+// and 1 continue the recurrence bit for bit, later ones to rounding.
+// seeds: [Rp][3][nseg][4] (slot 0: v, slot s: T_s).  Grid (Rp / 64, 3, nseg - 1).
 __global__ void __launch_bounds__(64) k_legendre_seed(int R, int Rp, int lmax, const double* __restrict__ theta,
                                                       int nseg, int seglen, double* __restrict__ seeds) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  const int m = blockIdx.y;
+  const int m = blockIdx.y, sg = blockIdx.z;
   if (r >= Rp) return;
   const double th = r >= R ? 0.5 * kPi : theta[r];
   const double ct = cos(th), st = sin(th);
-  const double pmm = legendrePmm(m, st);
-  double* out = seeds + ((size_t)r * 3 + m) * nseg * 2;
-  double a1 = 0.0, a2 = 0.0;
-  for (int sg = 1; sg < nseg; ++sg) {
-    for (int l = (sg - 1) * seglen; l < sg * seglen; ++l) {
+  double* out = seeds + (((size_t)r * 3 + m) * nseg + sg) * 4;
+  if (sg == 0) {
+    const double pmm = legendrePmm(m, st);
+    double a1 = 0.0, a2 = 0.0;
+    for (int l = 0; l < seglen; ++l) {
       const double p = legendreNext(l, m, lmax, ct, pmm, a1, a2);
       a2 = a1;
       a1 = p;
     }
-    out[2 * sg] = a1;
-    out[2 * sg + 1] = a2;
+    out[0] = a1;
+    out[1] = a2;
+    return;
   }
+  // T = M_{l1} ... M_{l0}, l0 = sg seglen >= m + 2
+  double t00 = 1.0, t01 = 0.0, t10 = 0.0, t11 = 1.0;
+  for (int l = sg * seglen; l < (sg + 1) * seglen; ++l) {
+    double c = 0.0, d = 0.0;
+    if (l <= lmax) {
+      const double rl = 1.0 / (double)(l - m);
+      c = ct * (double)(2 * l - 1) * rl;
+      d = (double)(l + m - 1) * rl;
+    }
+    const double n00 = c * t00 - d * t10, n01 = c * t01 - d * t11;
+    t10 = t00;
+    t11 = t01;
+    t00 = n00;
+    t01 = n01;
+  }
+  out[0] = t00; out[1] = t01; out[2] = t10; out[3] = t11;
 }
 
 // mode 0: basis rows for the contraction; the degrees are cut into segments of
@@ -165,9 +187,15 @@ __global__ void __launch_bounds__(64) k_legendre(int R, int Rp, int lmax, int Lp
   const int seg = blockIdx.z;
   double p1 = 0.0, p2 = 0.0;
   if (seg > 0) {
-    const double* sd = seeds + (((size_t)r * 3 + m) * nseg + seg) * 2;
+    const double* sd = seeds + ((size_t)r * 3 + m) * nseg * 4;
     p1 = sd[0];
     p2 = sd[1];
+    for (int t = 1; t < seg; ++t) {
+      const double* T = sd + 4 * t;
+      const double q1 = T[0] * p1 + T[1] * p2, q2 = T[2] * p1 + T[3] * p2;
+      p1 = q1;
+      p2 = q2;
+    }
   }
   // one degree of the recurrence (values of this order m at degree l)
   auto step = [&](int l, double& p, double& d, double& sv, double& d2, double& ds) {
@@ -946,8 +974,8 @@ int run_sums(gm_ctx* c, const SynPlan& P, const double* theta, const float* gr, 
     legendreSegments(P.Lp, nseg, seglen);
     double* seeds = (double*)(base + o.seeds);
     if (nseg > 1)
-      hipLaunchKernelGGL(k_legendre_seed, dim3((P.Rp + 63) / 64, 3), dim3(64), 0, s, P.R, P.Rp, P.L1 - 1, dth, nseg,
-                         seglen, seeds);
+      hipLaunchKernelGGL(k_legendre_seed, dim3((P.Rp + 63) / 64, 3, nseg - 1), dim3(64), 0, s, P.R, P.Rp, P.L1 - 1,
+                         dth, nseg, seglen, seeds);
     hipLaunchKernelGGL(k_legendre, dim3((P.Rp + 63) / 64, 3, nseg), dim3(64), 0, s, P.R, P.Rp, P.L1 - 1, P.Lp, dth,
                        (__bf16*)(base + o.Bs), nullptr, nullptr, nullptr, nullptr, nullptr, 0, nseg, seglen,
                        (const double*)seeds);
@@ -983,7 +1011,7 @@ SumBufs carve_sums(const SynPlan& P, bool dev_in, Carve& cv) {
   o.theta = cv.take(sizeof(double) * P.R);
   int nseg, seglen;
   legendreSegments(P.Lp, nseg, seglen);
-  o.seeds = cv.take(sizeof(double) * 2 * 3 * (size_t)P.Rp * nseg);
+  o.seeds = cv.take(sizeof(double) * 4 * 3 * (size_t)P.Rp * nseg);
   o.nwn = cv.take(sizeof(int) * P.nf);
   o.koff = cv.take(sizeof(long long) * P.nf);
   o.Bs = cv.take(sizeof(__bf16) * kPieces * kRows * (size_t)P.Rp * P.Lp);
