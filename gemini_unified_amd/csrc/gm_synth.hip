@@ -1278,3 +1278,199 @@ extern "C" int gm_last_synth_timing(gm_ctx* c, double* ms6) {
   for (int i = 0; i < 6; ++i) ms6[i] = c->stiming[i];
   return GM_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Travel times from a ray table for the automatic time shift
+// (getTravelTimes, f90/travelTimes.f90:128-388), one receiver per lane.  The
+// table keeps the reference's FP32 arrays in Fortran order: rnod[nnod],
+// rtp[ntp], slowness[ntp], delta and ttime (nnod, ntp) with element (jn, k) at
+// (jn-1) + (k-1) nnod.  Indices below are 1-based as in the reference.  The
+// arithmetic is left unfused so that each sum is formed in the reference's order.
+// ---------------------------------------------------------------------------
+#pragma clang fp contract(off)
+
+namespace {
+
+// bisection for j with x(j) < xx <= x(j+1) on an ascending x (0 below, n
+// above), comparisons turned for a descending x (f90/locatePoint.f90:43-111)
+template <class V, class F>
+__device__ int ttLocate(V xx, int n, F x) {
+  int lo = 0, hi = n + 1;
+  const bool asc = x(n) > x(1);
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (asc == (xx > x(mid))) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+struct TtTable {
+  int nnod, ntp, istele;
+  const float *rnod, *rtp, *slow, *delta, *ttime;
+};
+
+// node bracket and linear weights of a radius, extrapolating up to 1e-5 r
+// beyond the table: 0, or 1 / 2 for a radius below / above it
+__device__ int ttBracket(const TtTable& T, double r, int& j, double& a, double& b) {
+  const double tol = 1e-5;
+  j = ttLocate((float)r, T.nnod, [&](int i) { return T.rnod[i - 1]; });
+  if (j == 0) {
+    if (fabs(T.rnod[0] - r) > tol * r) return 1;
+    j = 1;
+  } else if (j == T.nnod) {
+    if (fabs(T.rnod[T.nnod - 1] - r) > tol * r) return 2;
+    j = T.nnod - 1;
+  }
+  const float hf = T.rnod[j] - T.rnod[j - 1];
+  a = ((double)T.rnod[j] - r) / (double)hf;
+  b = 1.0 - a;
+  return 0;
+}
+
+__global__ void __launch_bounds__(256) k_travel_times(TtTable T, double rs, int nrcv, const double* __restrict__ re,
+                                                      const double* __restrict__ din, double* __restrict__ tt,
+                                                      double* __restrict__ pout, int* __restrict__ status) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrcv) return;
+  const double x = din[r];
+  tt[r] = -1.0;
+  pout[r] = 0.0;
+  int js, je, st;
+  double as, bs, ae, be;
+  if ((st = ttBracket(T, rs, js, as, bs))) { status[r] = st; return; }
+  auto rtp = [&](int i) { return T.rtp[i - 1]; };
+  const int jtps = ttLocate((float)rs, T.ntp, rtp);
+  if (jtps == 0) { status[r] = 3; return; }
+  const int jtpr = ttLocate((float)re[r], T.ntp, rtp);
+  if (jtpr == 0) { status[r] = 4; return; }
+  if ((st = ttBracket(T, re[r], je, ae, be))) { status[r] = st + 4; return; }
+  const int jtpmax = min(jtps, jtpr);
+  if (jtpmax < 2) { status[r] = 7; return; }
+  const int nn = T.nnod;
+  auto dat = [&](const float* a, int jn, int k) { return (double)a[(jn - 1) + (size_t)(k - 1) * nn]; };
+  auto p = [&](int k) { return (double)T.slow[k - 1]; };
+  // distance and intercept time via turning point k: down from the source and
+  // up to the receiver (sgn = 1), or the steep ray without a turn (sgn = -1)
+  auto del = [&](int k, double sgn) {
+    return as * dat(T.delta, js, k) + bs * dat(T.delta, js + 1, k) + sgn * (ae * dat(T.delta, je, k)) +
+           sgn * (be * dat(T.delta, je + 1, k));
+  };
+  auto tau = [&](int k, double sgn) {
+    return as * dat(T.ttime, js, k) + bs * dat(T.ttime, js + 1, k) + sgn * (ae * dat(T.ttime, je, k)) +
+           sgn * (be * dat(T.ttime, je + 1, k)) - p(k) * del(k, sgn);
+  };
+  double dmax = del(1, 1.0), dmin = dmax;
+  for (int k = 2; k <= jtpmax; ++k) {
+    const double v = del(k, 1.0);
+    dmax = fmax(dmax, v);
+    dmin = fmin(dmin, v);
+  }
+  int jd;
+  double ad, bd, sgn = 1.0;
+  if (x >= dmin) {
+    if (x > dmax) {   // beyond the deepest turning point: diffracted along it
+      tt[r] = tau(1, 1.0) + p(1) * x;
+      pout[r] = p(1);
+      status[r] = 0;
+      return;
+    }
+    if (T.istele) {
+      jd = ttLocate(x, jtpmax, [&](int k) { return del(k, 1.0); });
+    } else {   // the last sign change of del - x, searched from the end
+      jd = jtpmax;
+      double prod = 1.0;
+      while (prod > 0.0 && jd > 1) {
+        prod = (x - del(jd - 1, 1.0)) * (x - del(jd, 1.0));
+        --jd;
+      }
+    }
+    jd = max(1, min(jd, jtpmax - 1));
+    const double h = del(jd + 1, 1.0) - del(jd, 1.0);
+    ad = (del(jd + 1, 1.0) - x) / h;
+    bd = 1.0 - ad;
+  } else {
+    if (T.istele) { status[r] = 8; return; }
+    sgn = -1.0;
+    jd = ttLocate(x, jtpmax, [&](int k) { return del(k, -1.0); });
+    const bool left = jd == 0, right = jd == jtpmax;
+    if (left) jd = 1;
+    if (right) jd = jtpmax - 1;
+    const double h = del(jd + 1, -1.0) - del(jd, -1.0);
+    if ((left || right) && fabs(h) < 1e-4) {   // flat end: take the end point
+      ad = left ? 1.0 : 0.0;
+      bd = left ? 0.0 : 1.0;
+    } else {
+      ad = (del(jd + 1, -1.0) - x) / h;
+      bd = 1.0 - ad;
+    }
+  }
+  const double ta = ad * tau(jd, sgn) + bd * tau(jd + 1, sgn);
+  const double pa = ad * p(jd) + bd * p(jd + 1);
+  tt[r] = ta + pa * x;
+  pout[r] = pa;
+  status[r] = 0;
+}
+
+const char* ttMessage(int st) {
+  switch (st) {
+    case 1: return "source radius below ray table range";
+    case 2: return "source radius above ray table range";
+    case 3: return "no turning point below source";
+    case 4: return "no turning point below receiver";
+    case 5: return "receiver radius below ray table range";
+    case 6: return "receiver radius above ray table range";
+    case 7: return "fewer than two turning points below source and receiver";
+    default: return "Delta out of turning point range";
+  }
+}
+
+}  // namespace
+
+extern "C" int gm_travel_times(gm_ctx* c, int nnod, int ntp, int istele, const float* rnod, const float* rtp,
+                               const float* slowness, const float* delta, const float* ttime, double rs, int nrcv,
+                               const double* re, const double* deltain, double* tt, double* slow, int* status) {
+  if (!c) return GM_ERR_ARG;
+  if (nnod < 2 || ntp < 2 || !rnod || !rtp || !slowness || !delta || !ttime)
+    return gm_fail(c, GM_ERR_ARG, "ray table: nnod >= 2, ntp >= 2 and all five arrays required");
+  if (nrcv < 0 || (nrcv > 0 && (!re || !deltain || !tt)))
+    return gm_fail(c, GM_ERR_ARG, "nrcv >= 0 with re, deltain and tt required");
+  if (!(rs > 0.0)) return gm_fail(c, GM_ERR_ARG, "source radius must be positive");
+  if (nrcv == 0) return GM_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t tn = (size_t)nnod * ntp;
+  Carve cv;
+  const size_t orn = cv.take(sizeof(float) * nnod), ortp = cv.take(sizeof(float) * ntp),
+               osl = cv.take(sizeof(float) * ntp), odl = cv.take(sizeof(float) * tn), otm = cv.take(sizeof(float) * tn),
+               ore = cv.take(sizeof(double) * nrcv), odn = cv.take(sizeof(double) * nrcv),
+               ott = cv.take(sizeof(double) * nrcv), op = cv.take(sizeof(double) * nrcv), ost = cv.take(sizeof(int) * nrcv);
+  int rc = ensure_sbuf(c, cv.off);
+  if (rc) return rc;
+  char* base = (char*)c->sbuf;
+  hipStream_t s = c->stream;
+  HIPCHK(c, hipMemcpyAsync(base + orn, rnod, sizeof(float) * nnod, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(base + ortp, rtp, sizeof(float) * ntp, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(base + osl, slowness, sizeof(float) * ntp, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(base + odl, delta, sizeof(float) * tn, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(base + otm, ttime, sizeof(float) * tn, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(base + ore, re, sizeof(double) * nrcv, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(base + odn, deltain, sizeof(double) * nrcv, hipMemcpyHostToDevice, s));
+  const TtTable T{nnod, ntp, istele ? 1 : 0, (const float*)(base + orn), (const float*)(base + ortp),
+                  (const float*)(base + osl), (const float*)(base + odl), (const float*)(base + otm)};
+  hipLaunchKernelGGL(k_travel_times, dim3((nrcv + 255) / 256), dim3(256), 0, s, T, rs, nrcv,
+                     (const double*)(base + ore), (const double*)(base + odn), (double*)(base + ott),
+                     (double*)(base + op), (int*)(base + ost));
+  HIPCHK(c, hipGetLastError());
+  std::vector<int> stv(nrcv);
+  HIPCHK(c, hipMemcpyAsync(tt, base + ott, sizeof(double) * nrcv, hipMemcpyDeviceToHost, s));
+  if (slow) HIPCHK(c, hipMemcpyAsync(slow, base + op, sizeof(double) * nrcv, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(stv.data(), base + ost, sizeof(int) * nrcv, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  if (status) {
+    std::copy(stv.begin(), stv.end(), status);
+    return GM_OK;
+  }
+  for (int r = 0; r < nrcv; ++r)
+    if (stv[r]) return gm_fail(c, GM_ERR_ARG, "travel time of receiver %d: %s", r, ttMessage(stv[r]));
+  return GM_OK;
+}

@@ -51,6 +51,28 @@ class SyntheticsEngine:
         check(self.h, _lib.lib().gm_last_synth_timing(self.h, ms.ctypes.data_as(D)))
         return ms
 
+    def travel_times(self, table, rs, re, delta, raise_on_error=False):
+        """getTravelTimes (f90/travelTimes.f90:373-388) on the GPU for a source at radius rs and
+        receivers at radii re[r] and epicentral distances delta[r] (radians).  table: dict with the
+        FP32 arrays of raytable_travel_times (rnod [nnod], rtp [ntp], slowness [ntp], delta and
+        ttime [nnod, ntp]) and istele.  Returns (tt, slowness, status); with raise_on_error a failing
+        receiver raises GeminiFKError with the reference's message instead."""
+        f32 = lambda a: np.asfortranarray(a, dtype=np.float32)
+        rnod, rtp, slow = f32(table["rnod"]), f32(table["rtp"]), f32(table["slowness"])
+        dl, tm = f32(table["delta"]), f32(table["ttime"])
+        if dl.shape != (rnod.size, rtp.size) or tm.shape != dl.shape or slow.size != rtp.size:
+            raise ValueError("delta and ttime must be [nnod, ntp] and slowness [ntp]")
+        re = np.ascontiguousarray(np.broadcast_to(re, np.shape(delta)), dtype=np.float64).ravel()
+        dn = np.ascontiguousarray(delta, dtype=np.float64).ravel()
+        tt, p, st = np.zeros(dn.size), np.zeros(dn.size), np.zeros(dn.size, np.int32)
+        Fp = C.POINTER(C.c_float)
+        check(self.h, _lib.lib().gm_travel_times(
+            self.h, rnod.size, rtp.size, int(bool(table.get("istele", False))), rnod.ctypes.data_as(Fp),
+            rtp.ctypes.data_as(Fp), slow.ctypes.data_as(Fp), dl.ctypes.data_as(Fp), tm.ctypes.data_as(Fp), float(rs),
+            dn.size, re.ctypes.data_as(D), dn.ctypes.data_as(D), tt.ctypes.data_as(D), p.ctypes.data_as(D),
+            None if raise_on_error else st.ctypes.data_as(I)))
+        return tt, p, st
+
     def legendre(self, lmax, theta):
         """computeLegendreFunctions (f90/legendreFunctions.f90:48) at each theta: dict of [nrcv, 3, lmax+1]."""
         theta = np.ascontiguousarray(np.atleast_1d(theta), dtype=np.float64)

@@ -273,6 +273,26 @@ int gm_synthetics_regional(gm_ctx* ctx, int nf1, int nf2, double df, double sigm
  * hfil: complex [nf2], zero below nf1.  Host computation, no context. */
 int gm_event_filter(int filtype, const double* filspecs, int nf1, int nf2, double df, double sigma, double* hfil);
 
+/* Travel times for the automatic time shift of the synthetics stage:
+ * getTravelTimes (f90/travelTimes.f90:373-388, with setSourceInfoTravelTimes
+ * :128-176 and getReceiverTravelTimes :180-367) for a source at radius rs and
+ * receivers at radii re[r] and epicentral distances deltain[r] (radians), one
+ * receiver per GPU lane.  The ray table is the reference's
+ * raytable_travel_times (read by readTableTravelTimes, :68-124): FP32 rnod[nnod],
+ * rtp[ntp], slowness[ntp] (angular slowness rtp / v) and delta, ttime (nnod, ntp)
+ * in Fortran order; istele restricts turning points to the lower mantle.
+ * Outputs tt[r] (seconds, -1 on failure) and, when slow is not NULL, the ray's
+ * slowness.  status (optional) gets 0 or the failure per receiver: 1 / 2
+ * source radius below / above the table, 3 no turning point below the source,
+ * 4 none below the receiver, 5 / 6 receiver radius below / above the table,
+ * 7 fewer than two usable turning points, 8 distance out of the turning-point
+ * range with istele.  Without status a failure returns GM_ERR_ARG with the
+ * reference's message.  The caller forms the reduction time tred = tt - tsbuf
+ * (f90/computeSyntheticSeismograms.f90:344-352).  Host pointers. */
+int gm_travel_times(gm_ctx* ctx, int nnod, int ntp, int istele, const float* rnod, const float* rtp,
+                    const float* slowness, const float* delta, const float* ttime, double rs, int nrcv,
+                    const double* re, const double* deltain, double* tt, double* slow, int* status);
+
 /* Kernel time of the last gm_moment_sums / gm_synthetics call in milliseconds:
  * 0 Legendre basis, 1 spectra preparation, 2 contraction, 3 combination,
  * 4 frequency-time transform, 5 total. */

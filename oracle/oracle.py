@@ -371,3 +371,38 @@ def freq_time(nf1, nf2, df, sigma, sp, nsamp, dt):
     rc = lib().orc_freq_time(int(nf1), int(nf2), C.c_double(df), C.c_double(sigma), sp.ctypes.data_as(D), int(nsamp),
                              C.c_double(dt), ts.ctypes.data_as(C.POINTER(C.c_float)))
     return rc, ts
+
+
+def travel_times(table, rs, re, delta):
+    """getTravelTimes (f90/travelTimes.f90:373-388) for receivers at radii re[r]
+    and epicentral distances delta[r] (radians) from a source at radius rs.
+    table: dict with rnod [nnod], rtp [ntp], slowness [ntp], delta and ttime [nnod, ntp]
+    (FP32, the reference's raytable_travel_times) and istele.  Returns (tt, slowness, status)."""
+    f32 = lambda a: np.asfortranarray(a, dtype=np.float32)
+    rnod, rtp, slow = f32(table["rnod"]), f32(table["rtp"]), f32(table["slowness"])
+    dl, tm = f32(table["delta"]), f32(table["ttime"])
+    re = np.ascontiguousarray(re, dtype=np.float64)
+    dn = np.ascontiguousarray(delta, dtype=np.float64)
+    n = re.size
+    tt = np.zeros(n)
+    p = np.zeros(n)
+    st = np.zeros(n, np.int32)
+    F = C.POINTER(C.c_float)
+    lib().orc_travel_times(int(rnod.size), int(rtp.size), int(bool(table.get("istele", False))), rnod.ctypes.data_as(F),
+                           rtp.ctypes.data_as(F), slow.ctypes.data_as(F), dl.ctypes.data_as(F), tm.ctypes.data_as(F),
+                           C.c_double(rs), int(n), _d(re), _d(dn), _d(tt), _d(p), st.ctypes.data_as(C.POINTER(C.c_int)))
+    return tt, p, st
+
+
+def homogeneous_ray_table(radius, v, nnod, rmin_frac=0.3, istele=False):
+    """Ray table of a homogeneous sphere (straight rays) on nnod radii from rmin_frac * radius to
+    radius, with turning points at all radii but the top one: a ray turning at rtp reaches radius
+    r >= rtp at angular distance acos(rtp / r) after sqrt(r^2 - rtp^2) / v seconds; its angular
+    slowness is rtp / v.  Entries with r < rtp are zero."""
+    rnod = np.linspace(rmin_frac * radius, radius, nnod)
+    rtp = rnod[:-1]
+    R, P = np.meshgrid(rnod, rtp, indexing="ij")
+    reach = R >= P
+    dl = np.where(reach, np.arccos(np.minimum(P / R, 1.0)), 0.0)
+    tm = np.where(reach, np.sqrt(np.maximum(R * R - P * P, 0.0)) / v, 0.0)
+    return {"rnod": rnod, "rtp": rtp, "slowness": rtp / v, "delta": dl, "ttime": tm, "istele": istele}

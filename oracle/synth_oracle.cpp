@@ -549,3 +549,150 @@ int orc_synthetics(int nf1, int nf2, double df, double sigma, const int* nwn, lo
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Travel time of the first arrival from a ray table (f90/travelTimes.f90).
+// The table is the reference's raytable_travel_times with its FP32 arrays in
+// Fortran order: rnod[nnod], rtp[ntp], slowness[ntp] (angular slowness rtp/v),
+// delta and ttime (nnod, ntp), element (jn, k) at (jn-1) + (k-1) nnod.  All
+// indices below are 1-based as in the reference.
+// ---------------------------------------------------------------------------
+namespace tt {
+
+// locate (f90/locatePoint.f90:43-75): bisection returning j with x(j) < xx <= x(j+1)
+// for an ascending x (0 below, n above); for a descending x the comparisons turn.
+template <class V, class F>
+static int locate(V xx, int n, F x) {
+  int lo = 0, hi = n + 1;
+  const bool asc = x(n) > x(1);
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) / 2;
+    if (asc == (xx > x(mid))) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+struct Table {
+  int nnod, ntp, istele;
+  const float *rnod, *rtp, *slow, *delta, *ttime;
+  double d(int jn, int k) const { return (double)delta[(jn - 1) + (size_t)(k - 1) * nnod]; }
+  double t(int jn, int k) const { return (double)ttime[(jn - 1) + (size_t)(k - 1) * nnod]; }
+};
+
+// node bracket and weights of a radius (setSourceInfoTravelTimes :128-176 and the
+// receiver half of getReceiverTravelTimes :208-236): 0 or an error code
+static int bracket(const Table& T, double r, int& j, double& a, double& b) {
+  const double tol = 1e-5;
+  j = locate((float)r, T.nnod, [&](int i) { return T.rnod[i - 1]; });
+  if (j == 0) {
+    if (std::fabs(T.rnod[0] - r) > tol * r) return 1;   // below ray table range
+    j = 1;
+  } else if (j == T.nnod) {
+    if (std::fabs(T.rnod[T.nnod - 1] - r) > tol * r) return 2;   // above ray table range
+    j = T.nnod - 1;
+  }
+  const double h = (double)(T.rnod[j] - T.rnod[j - 1]);   // FP32 difference, as REAL arrays give
+  a = ((double)T.rnod[j] - r) / h;
+  b = 1.0 - a;
+  return 0;
+}
+
+// getTravelTimes (:373-388) = setSourceInfoTravelTimes + getReceiverTravelTimes
+// (:180-367).  Status: 0 ok; 1/2 source radius below/above the table; 3 no
+// turning point below the source; 4 none below the receiver; 5/6 receiver
+// radius below/above the table; 7 fewer than two usable turning points (the
+// reference would read past its arrays); 8 distance out of the turning-point
+// range with istele set.
+static int travel_time(const Table& T, double rs, double re, double din, double& tt, double& pout) {
+  tt = -1.0;
+  pout = 0.0;
+  int js, je;
+  double as, bs, ae, be;
+  int st = bracket(T, rs, js, as, bs);
+  if (st) return st;
+  auto rtp = [&](int i) { return T.rtp[i - 1]; };
+  const int jtps = locate((float)rs, T.ntp, rtp);
+  if (jtps == 0) return 3;
+  const int jtpr = locate((float)re, T.ntp, rtp);
+  if (jtpr == 0) return 4;
+  st = bracket(T, re, je, ae, be);
+  if (st) return st + 4;
+  const int jtpmax = std::min(jtps, jtpr);
+  if (jtpmax < 2) return 7;
+  auto p = [&](int k) { return (double)T.slow[k - 1]; };
+  // distance and intercept time for turning point k, down-going from the
+  // source and up to the receiver (sgn = +1) or the steep ray without a turn (sgn = -1)
+  auto del = [&](int k, double sgn) {
+    return as * T.d(js, k) + bs * T.d(js + 1, k) + sgn * (ae * T.d(je, k)) + sgn * (be * T.d(je + 1, k));
+  };
+  auto tau = [&](int k, double sgn) {
+    return as * T.t(js, k) + bs * T.t(js + 1, k) + sgn * (ae * T.t(je, k)) + sgn * (be * T.t(je + 1, k)) -
+           p(k) * del(k, sgn);
+  };
+  double dmax = del(1, 1.0), dmin = dmax;
+  for (int k = 2; k <= jtpmax; ++k) {
+    const double v = del(k, 1.0);
+    dmax = std::max(dmax, v);
+    dmin = std::min(dmin, v);
+  }
+  int jd;
+  double ad, bd;
+  double sgn = 1.0;
+  if (din >= dmin) {
+    if (din > dmax) {   // beyond the last turning point: diffracted along it
+      tt = tau(1, 1.0) + p(1) * din;
+      pout = p(1);
+      return 0;
+    }
+    if (T.istele) {
+      jd = locate(din, jtpmax, [&](int k) { return del(k, 1.0); });
+    } else {   // findZeroPassTravelTimes(..., 'end') (:395-413): the last sign change
+      jd = jtpmax;
+      double prod = 1.0;
+      while (prod > 0.0 && jd > 1) {
+        prod = (din - del(jd - 1, 1.0)) * (din - del(jd, 1.0));
+        --jd;
+      }
+    }
+    jd = std::max(1, std::min(jd, jtpmax - 1));
+    const double h = del(jd + 1, 1.0) - del(jd, 1.0);
+    ad = (del(jd + 1, 1.0) - din) / h;
+    bd = 1.0 - ad;
+  } else {
+    if (T.istele) return 8;
+    sgn = -1.0;
+    jd = locate(din, jtpmax, [&](int k) { return del(k, -1.0); });
+    if (jd == 0 || jd == jtpmax) {   // extrapolate with the end slope (:319-342)
+      const bool left = jd == 0;
+      jd = left ? 1 : jtpmax - 1;
+      const double h = del(jd + 1, -1.0) - del(jd, -1.0);
+      if (std::fabs(h) < 1e-4) {
+        ad = left ? 1.0 : 0.0;
+        bd = left ? 0.0 : 1.0;
+      } else {
+        ad = (del(jd + 1, -1.0) - din) / h;
+        bd = 1.0 - ad;
+      }
+    } else {
+      const double h = del(jd + 1, -1.0) - del(jd, -1.0);
+      ad = (del(jd + 1, -1.0) - din) / h;
+      bd = 1.0 - ad;
+    }
+  }
+  const double ta = ad * tau(jd, sgn) + bd * tau(jd + 1, sgn);
+  const double pa = ad * p(jd) + bd * p(jd + 1);
+  tt = ta + pa * din;
+  pout = pa;
+  return 0;
+}
+
+}  // namespace tt
+
+extern "C" int orc_travel_times(int nnod, int ntp, int istele, const float* rnod, const float* rtp, const float* slowness,
+                                const float* delta, const float* ttime, double rs, int nrcv, const double* re,
+                                const double* din, double* tt, double* slow, int* status) {
+  const tt::Table T{nnod, ntp, istele, rnod, rtp, slowness, delta, ttime};
+  for (int r = 0; r < nrcv; ++r) status[r] = tt::travel_time(T, rs, re[r], din[r], tt[r], slow[r]);
+  return 0;
+}

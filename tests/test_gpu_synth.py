@@ -442,3 +442,49 @@ def test_decimated_synthetics_match_oracle(syn, nsamp):
     with pytest.raises(GeminiFKError) as e:
         syn.synthetics(nf1, nf2, df, sigma, nwn, gr, gi, theta, phi, mt, nsamp, 1.3 * dt)
     assert e.value.code == 9
+
+
+def _ray_tables():
+    rng = np.random.default_rng(11)
+    R, v = 6371e3, 8000.0
+    for n, tele in ((400, False), (400, True), (150, False)):
+        yield oracle.homogeneous_ray_table(R, v, n, istele=tele)
+    # a rough table: seeded perturbations make distance versus turning point
+    # non-monotonic, so the search from the end meets several sign changes
+    T = oracle.homogeneous_ray_table(R, v, 300)
+    T["delta"] = T["delta"] * (1 + 0.02 * rng.standard_normal(T["delta"].shape))
+    T["ttime"] = T["ttime"] * (1 + 0.02 * rng.standard_normal(T["ttime"].shape))
+    yield T
+
+
+def test_travel_times_match_oracle(syn):
+    """Every branch (steep, turning, diffracted, out of range) against the
+    oracle: same FP32 table, same unfused FP64 arithmetic in the same order,
+    so the times agree to rounding (1e-13 relative) and the statuses exactly."""
+    rng = np.random.default_rng(5)
+    R = 6371e3
+    for T in _ray_tables():
+        for rs in (6000e3, R, 5700e3, 0.999 * R):
+            nr = 3000
+            re = rng.choice([R, 6200e3, 5900e3, 0.25 * R, 1.001 * R, R * (1 + 4e-6)], nr)
+            d = rng.uniform(0.0, np.pi, nr)
+            tt, p, st = syn.travel_times(T, rs, re, d)
+            to, po, so = oracle.travel_times(T, rs, re, d)
+            print(f"travel times: nnod {T['rnod'].size} istele {T['istele']} rs {rs:.0f}", flush=True)
+            np.testing.assert_array_equal(st, so)
+            assert (st == 0).sum() > nr // 2 and (st != 0).any()
+            np.testing.assert_allclose(tt, to, rtol=1e-13, atol=1e-9)
+            np.testing.assert_allclose(p, po, rtol=1e-13, atol=1e-12)
+            assert (tt[st != 0] == -1.0).all()
+
+
+def test_travel_times_errors_and_empty(syn):
+    from gemini_unified_amd._lib import GeminiFKError
+    T = oracle.homogeneous_ray_table(6371e3, 8000.0, 200)
+    tt, p, st = syn.travel_times(T, 6000e3, np.zeros(0), np.zeros(0))
+    assert tt.size == 0
+    with pytest.raises(GeminiFKError) as e:
+        syn.travel_times(T, 6000e3, np.array([6371e3, 1e6]), np.array([0.5, 0.5]), raise_on_error=True)
+    assert e.value.code == 9 and "no turning point below receiver" in str(e.value)
+    with pytest.raises(GeminiFKError):
+        syn.travel_times(T, -1.0, np.array([6371e3]), np.array([0.5]))

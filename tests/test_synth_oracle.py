@@ -401,3 +401,79 @@ def test_moment_displacement_rotation_invariance():
     for a in (0.3, 2.0, 4.4):
         _, z1 = oracle.synthetics(1, 1, 1.0 / 64, 0.0, nwn, gr, gi, th, phi + a, _rotate_mt(mt, a), 4, 16.0)
         np.testing.assert_allclose(z1[:, :, 0], z0[:, :, 0], rtol=1e-12, atol=1e-12 * np.abs(z0).max())
+
+
+# ---- travel times from a ray table (f90/travelTimes.f90) ----
+# In a homogeneous sphere rays are straight: the first arrival between radii
+# rs and re at angular distance d takes chord / v with
+# chord^2 = rs^2 + re^2 - 2 rs re cos d, and its slowness is dT/dd =
+# rs re sin d / (v chord).  The table method interpolates linearly in the
+# turning-point index, so its error must fall at second order as the table is
+# refined; an index or sign slip in the interpolation leaves an O(1) error.
+_R, _V = 6371e3, 8000.0
+
+
+def _chord(rs, re, d):
+    return np.sqrt(rs * rs + re * re - 2 * rs * re * np.cos(d)) / _V
+
+
+@pytest.mark.parametrize("rs,re", [(6000e3, 6371e3), (6371e3, 6000e3), (5800e3, 6300e3)])
+def test_travel_times_turning_rays_converge_to_chord(rs, re):
+    dmin = np.arccos(min(rs, re) / max(rs, re))      # the ray grazing the shallower radius
+    d = np.linspace(dmin + 0.15, 2.0, 400)
+    errs = []
+    for n in (400, 800):
+        tt, p, st = oracle.travel_times(oracle.homogeneous_ray_table(_R, _V, n), rs, np.full(d.size, re), d)
+        assert (st == 0).all()
+        ex = _chord(rs, re, d)
+        pe = rs * re * np.sin(d) / (_V * _V * ex)
+        errs.append(np.max(np.abs(tt - ex) / ex))
+        assert np.max(np.abs(p - pe) / pe) < 5e-4
+    assert errs[0] < 3e-5 and errs[1] < 6e-6
+    assert errs[0] / errs[1] > 2.5                    # second order
+
+
+def test_travel_times_steep_rays():
+    """Below the smallest turning distance the reference takes the steep ray,
+    distance and time of the source leg minus those of the receiver leg
+    (f90/travelTimes.f90:300-354).  That is the geometric ray for a source above
+    the receiver; for a source below it the sign makes both negative, and the
+    restatement keeps the reference's result (an O(1) departure from the chord)."""
+    rs, re = 6371e3, 6000e3
+    d = np.linspace(0.03, np.arccos(re / rs) - 0.05, 200)
+    errs = []
+    for n in (400, 800):
+        tt, _, st = oracle.travel_times(oracle.homogeneous_ray_table(_R, _V, n), rs, np.full(d.size, re), d)
+        assert (st == 0).all()
+        errs.append(np.max(np.abs(tt - _chord(rs, re, d)) / _chord(rs, re, d)))
+    assert errs[0] < 5e-5 and errs[0] / errs[1] > 2.5
+    tt, _, st = oracle.travel_times(oracle.homogeneous_ray_table(_R, _V, 400), re, np.full(d.size, rs), d)
+    assert (st == 0).all() and np.max(np.abs(tt - _chord(rs, re, d)) / _chord(rs, re, d)) > 0.5
+
+
+def test_travel_times_beyond_deepest_turning_point():
+    """Past the largest tabulated distance the arrival travels along the deepest
+    turning point: continuous there, with that ray's slowness as slope."""
+    T = oracle.homogeneous_ray_table(_R, _V, 400, rmin_frac=0.55)
+    rs, re = 6100e3, 6371e3
+    d = np.linspace(1.5, 3.1, 500)
+    tt, p, st = oracle.travel_times(T, rs, np.full(d.size, re), d)
+    assert (st == 0).all()
+    far = p == np.float32(T["slowness"][0])
+    assert far.any() and (~far).any()
+    k = np.argmax(far)
+    assert abs(tt[k] - tt[k - 1]) < 2 * p[k] * (d[k] - d[k - 1])         # no jump
+    np.testing.assert_allclose(np.diff(tt[far]), p[far][1:] * np.diff(d[far]), rtol=1e-9)
+
+
+def test_travel_times_errors():
+    T = oracle.homogeneous_ray_table(_R, _V, 200)
+    d = np.array([0.5])
+    for rs, re, code in ((0.2 * _R, _R, 1), (1.01 * _R, _R, 2), (_R, 0.2 * _R, 4), (_R, 1.01 * _R, 6)):
+        tt, _, st = oracle.travel_times(T, rs, np.array([re]), d)
+        assert st[0] == code and tt[0] == -1.0
+    # within 1e-5 r of the top radius the table is extrapolated
+    assert oracle.travel_times(T, _R * (1 + 5e-6), np.array([_R]), d)[2][0] == 0
+    # teleseismic tables have no steep rays
+    tt, _, st = oracle.travel_times(dict(T, istele=True), 6000e3, np.array([6371e3]), np.array([0.05]))
+    assert st[0] == 8
