@@ -73,6 +73,17 @@ class SyntheticsEngine:
             None if raise_on_error else st.ctypes.data_as(I)))
         return tt, p, st
 
+    def auto_timeshift(self, table, rs, re, delta, tsbuf, twinlen, tlen, dt, nsamp):
+        """Reduction time and true sample count per station for autoTimeshift
+        (f90/computeSyntheticSeismograms.f90:344-352): tred = tt - tsbuf, the window ends at
+        tt + twinlen but at most tlen after tred, nstrue = min(int((tend - tred) / dt) + 1, nsamp).
+        A failing station raises GeminiFKError with the reference's message, as there."""
+        tt, _, _ = self.travel_times(table, rs, re, delta, raise_on_error=True)
+        tred = tt - tsbuf
+        tend = np.minimum(tt + twinlen, tred + tlen)
+        nstrue = np.minimum(((tend - tred) / dt).astype(np.int64) + 1, nsamp)
+        return tred, nstrue
+
     def legendre(self, lmax, theta):
         """computeLegendreFunctions (f90/legendreFunctions.f90:48) at each theta: dict of [nrcv, 3, lmax+1]."""
         theta = np.ascontiguousarray(np.atleast_1d(theta), dtype=np.float64)

@@ -488,3 +488,36 @@ def test_travel_times_errors_and_empty(syn):
     assert e.value.code == 9 and "no turning point below receiver" in str(e.value)
     with pytest.raises(GeminiFKError):
         syn.travel_times(T, -1.0, np.array([6371e3]), np.array([0.5]))
+
+
+def test_auto_timeshift_synthetics_match_oracle(syn):
+    """autoTimeshift end to end: GPU travel times give the reduction times and true sample
+    counts of f90/computeSyntheticSeismograms.f90:344-352, which then shift the GPU
+    seismograms; the oracle does the same from its own travel times."""
+    rng = np.random.default_rng(43)
+    nf1, nf2, nsamp = 2, 64, 256
+    T = 256.0
+    df, dt = 1.0 / T, T / nsamp
+    sigma = 5 * df
+    nwn_all = np.zeros(nf2, np.int32)
+    nwn_all[nf1 - 1:] = np.linspace(40, 120, nf2 - nf1 + 1).astype(np.int32)
+    gr, gi = _random_spectra(rng, nwn_all[nf1 - 1:])
+    R, rearth, rs = 8, 6371e3, 6371e3 - 35e3
+    theta = rng.uniform(0.1, 3.0, R)
+    phi = rng.uniform(0, 2 * np.pi, R)
+    mt = rng.normal(size=6) * 1e18
+    table = oracle.homogeneous_ray_table(rearth, 8000.0, 400)
+    tsbuf, twinlen, tlen = 30.0, 120.0, T
+    tred, nstrue = syn.auto_timeshift(table, rs, rearth, theta, tsbuf, twinlen, tlen, dt, nsamp)
+    tto, _, so = oracle.travel_times(table, rs, np.full(R, rearth), theta)
+    assert (so == 0).all()
+    np.testing.assert_allclose(tred, tto - tsbuf, rtol=1e-13)
+    assert (nstrue == np.minimum((np.minimum(tto + twinlen, tto - tsbuf + tlen) - (tto - tsbuf)) / dt + 1,
+                                 nsamp).astype(np.int64)).all()
+    assert (np.abs(nstrue - (int(150.0 / dt) + 1)) <= 1).all()          # window of tsbuf + twinlen
+    print(f"auto timeshift: {R} receivers, tred {tred.min():.1f}..{tred.max():.1f} s", flush=True)
+    ts = syn.synthetics(nf1, nf2, df, sigma, nwn_all, gr, gi, theta, phi, mt, nsamp, dt, tred=tred)
+    ots, _ = oracle.synthetics(nf1, nf2, df, sigma, nwn_all, gr, gi, theta, phi, mt, nsamp, dt, tred=tto - tsbuf)
+    ts64, ots64 = ts.astype(np.float64), ots.astype(np.float64)
+    terr = np.linalg.norm(ts64 - ots64, axis=2) / np.linalg.norm(ots64, axis=2)
+    assert terr.max() < 1e-5
