@@ -132,6 +132,50 @@ extern "C" int gm_last_timing(gm_ctx* c, double* ms) {
   return GM_OK;
 }
 
+// Structural checks of a node earth model before anything indexes it: the
+// spline kernel and the knot search of every evaluation trust iktop, the
+// layer flags and the knot radii.  Layer 1 (the central sphere or halfspace,
+// constant) needs one knot, every other layer at least two with strictly
+// increasing radii (splineEarthmodelCoefficients.f90:59-94).
+static int check_model(gm_ctx* c, int nk, const double* rk, const double* ro, const double* vpv, const double* vph,
+                       const double* vsv, const double* vsh, const double* eta, const double* qkinv,
+                       const double* qminv, int nlay, const int* iktop, const double* rb, const int* iflso, int nic,
+                       int noc, int equidis, double rearth) {
+  if (!rk || !ro || !vpv || !vph || !vsv || !vsh || !eta || !qkinv || !qminv || !iktop || !rb || !iflso)
+    return gm_fail(c, GM_ERR_ARG, "model arrays must not be NULL");
+  if (equidis != 0 && equidis != 1) return gm_fail(c, GM_ERR_ARG, "equidistant flag must be 0 or 1");
+  if (!(rearth > 0.0) || !std::isfinite(rearth)) return gm_fail(c, GM_ERR_ARG, "earth radius must be positive");
+  if (nic < 0 || noc < nic || noc > nlay)
+    return gm_fail(c, GM_ERR_ARG, "layer indices must satisfy 0 <= nic=%d <= noc=%d <= nlay=%d", nic, noc, nlay);
+  for (int i = 0; i < nk; ++i) {
+    const double v[9] = {rk[i], ro[i], vpv[i], vph[i], vsv[i], vsh[i], eta[i], qkinv[i], qminv[i]};
+    for (double x : v)
+      if (!std::isfinite(x)) return gm_fail(c, GM_ERR_ARG, "knot %d holds a non-finite value", i + 1);
+    if (!(ro[i] > 0.0)) return gm_fail(c, GM_ERR_ARG, "knot %d: density must be positive", i + 1);
+  }
+  if (iktop[0] < 1) return gm_fail(c, GM_ERR_ARG, "iktop(1)=%d must be >= 1", iktop[0]);
+  for (int nl = 1; nl < nlay; ++nl)
+    if (iktop[nl] - iktop[nl - 1] < 2)
+      return gm_fail(c, GM_ERR_ARG, "layer %d needs at least two knots (iktop %d -> %d)", nl + 1, iktop[nl - 1],
+                     iktop[nl]);
+  if (iktop[nlay - 1] != nk) return gm_fail(c, GM_ERR_ARG, "iktop(nlay)=%d differs from nk=%d", iktop[nlay - 1], nk);
+  for (int nl = 1; nl < nlay; ++nl)
+    for (int j = iktop[nl - 1] + 1; j < iktop[nl]; ++j)      // 1-based knots of layer nl + 1
+      if (!(rk[j] > rk[j - 1]))
+        return gm_fail(c, GM_ERR_ARG, "knot radii of layer %d must increase (knot %d)", nl + 1, j + 1);
+  for (int nl = 0; nl < nlay; ++nl) {
+    if (iflso[nl] != 0 && iflso[nl] != 1) return gm_fail(c, GM_ERR_ARG, "isfluid of layer %d must be 0 or 1", nl + 1);
+    if (!(rb[nl] > 0.0) || (nl > 0 && !(rb[nl] > rb[nl - 1])))
+      return gm_fail(c, GM_ERR_ARG, "layer top radii must be positive and increase (layer %d)", nl + 1);
+    const double rt = rk[iktop[nl] - 1];
+    if (std::fabs(rb[nl] - rt) > 1e-9 * rearth)
+      return gm_fail(c, GM_ERR_ARG, "top radius of layer %d (%g) differs from its top knot (%g)", nl + 1, rb[nl], rt);
+  }
+  if (noc > 0 && iflso[noc - 1] == 1 && noc == nlay)
+    return gm_fail(c, GM_ERR_ARG, "the fluid outer core (layer %d) must lie below another layer", noc);
+  return GM_OK;
+}
+
 extern "C" int gm_model_set(gm_ctx* c, int nk, const double* rk, const double* ro, const double* vpv, const double* vph,
                             const double* vsv, const double* vsh, const double* eta, const double* qkinv,
                             const double* qminv, int nlay, const int* iktop, const double* rb, const int* iflso,
@@ -141,6 +185,11 @@ extern "C" int gm_model_set(gm_ctx* c, int nk, const double* rk, const double* r
   if (nk < 2 || nk > GM_MAXK) return gm_fail(c, GM_ERR_ARG, "nk=%d outside [2,%d]", nk, GM_MAXK);
   if (fsflag != 0 && fsflag != 1) return gm_fail(c, GM_ERR_ARG, "full_space flag must be 0 or 1");
   if (nlay < 2 || nlay > GM_MAXL) return gm_fail(c, GM_ERR_ARG, "nlay=%d outside [2,%d]", nlay, GM_MAXL);
+  {
+    const int rc = check_model(c, nk, rk, ro, vpv, vph, vsv, vsh, eta, qkinv, qminv, nlay, iktop, rb, iflso, nic, noc,
+                               equidis, rearth);
+    if (rc) { c->has_model = false; return rc; }
+  }
   // the minors kernels stage one frequency's spline table image in LDS
   if (gmLdsBytes(nk, nlay) > (size_t)c->max_lds)
     return gm_fail(c, GM_ERR_UNSUPPORTED, "nk=%d: spline table image of %zu B exceeds the %d B of LDS per block", nk,
@@ -178,6 +227,13 @@ extern "C" int gm_model_set(gm_ctx* c, int nk, const double* rk, const double* r
 extern "C" int gm_nodes_set(gm_ctx* c, int nnod, const double* rnod, const int* state, const int* layer) {
   if (!c) return GM_ERR_ARG;
   if (nnod < 1 || nnod > GM_MAXN) return gm_fail(c, GM_ERR_ARG, "nnod=%d outside [1,%d]", nnod, GM_MAXN);
+  if (!rnod || !state || !layer) return gm_fail(c, GM_ERR_ARG, "node arrays must not be NULL");
+  for (int i = 0; i < nnod; ++i) {
+    if (!std::isfinite(rnod[i]) || !(rnod[i] > 0.0) || (i > 0 && !(rnod[i] > rnod[i - 1])))
+      return gm_fail(c, GM_ERR_ARG, "node radii must be positive and increase (node %d)", i + 1);
+    if (state[i] != 0 && state[i] != 1) return gm_fail(c, GM_ERR_ARG, "state of node %d must be 0 or 1", i + 1);
+    if (layer[i] < 1 || layer[i] > GM_MAXL) return gm_fail(c, GM_ERR_ARG, "layer of node %d outside [1,%d]", i + 1, GM_MAXL);
+  }
   GmModel& m = c->hmodel;
   m.nnod = nnod;
   for (int i = 0; i < nnod; ++i) { m.rnod[i + 1] = rnod[i]; m.state[i + 1] = state[i]; m.layer[i + 1] = layer[i]; }
@@ -236,6 +292,13 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   GmModel& m = c->hmodel;
   const int nnod = m.nnod;
   if (jsl < 1 || jsl > nnod || jsr < 1 || jsr > nnod) return gm_fail(c, GM_ERR_NODEIDX, "Invalid index for source node");
+  // nodes and model may be set in either order: check the nodes against the model here
+  for (int k = 1; k <= nnod; ++k) {
+    if (m.layer[k] < 1 || m.layer[k] > m.nlay)
+      return gm_fail(c, GM_ERR_ARG, "layer %d of node %d outside the model's %d layers", m.layer[k], k, m.nlay);
+    if (m.rnod[k] > m.rb[m.nlay] * (1.0 + 1e-12))
+      return gm_fail(c, GM_ERR_ARG, "node %d at r=%g lies above the model top %g", k, m.rnod[k], m.rb[m.nlay]);
+  }
   if (full) {
     if (istyp != 0 && istyp != 1) return gm_fail(c, GM_ERR_ARG, "istyp must be 0 (FORCE) or 1 (MOMENT)");
     if (m.state[jre] != 0) return gm_fail(c, GM_ERR_UNSUPPORTED, "receiver node in a fluid layer is not supported");

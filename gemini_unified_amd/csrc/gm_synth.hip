@@ -835,8 +835,9 @@ __global__ void __launch_bounds__(256) k_ifft(int nf1, int nf2, int ns, int logm
   else if (left == 2) stockhamPass<4>(sx, M, Ns, tw, w16);
   else if (left == 1) stockhamPass<2>(sx, M, Ns, tw, w16);
   float* out = ts + (size_t)blockIdx.x * nsamp;
-  if (ndec > 1) {
-    // decimation branch of transformFrequencyTime (frequencyTime.f90:104-109):
+  if (ndec > 1 || nsamp != ns) {
+    // decimation branch of transformFrequencyTime (frequencyTime.f90:104-109),
+    // taken whenever the transform is longer than the trace (ndec may be 1):
     // sample i is x_{ndec i}, damped by exp(sigma dts i) as written there
     for (int i = threadIdx.x; i < nsamp; i += blockDim.x) {
       const int t = ndec * i;
@@ -1118,7 +1119,8 @@ static int synth_impl(gm_ctx* c, int nf1, int nf2, double df, double sigma, cons
     ndec = (int)std::lround(dt / dt_fft);
     if (std::fabs(dt - ndec * dt_fft) > 1.e-5)
       return gm_fail(c, GM_ERR_ARG, "Non-integer ratio of desired dt and dt defined from frequency spectrum");
-    if (ndec < 2 || (size_t)(nsamp - 1) * ndec >= (size_t)ns)
+    // ndec = 1 is allowed: nsamp < ns samples at the transform's own step, the first nsamp kept
+    if (ndec < 1 || (size_t)(nsamp - 1) * ndec >= (size_t)ns)
       return gm_fail(c, GM_ERR_ARG, "nsamp = %d samples at every %d-th of %d do not fit the transform", nsamp, ndec, ns);
   }
   if (ns > kMaxFFT) return gm_fail(c, GM_ERR_UNSUPPORTED, "transform length %d exceeds %d", ns, kMaxFFT);

@@ -166,8 +166,11 @@ class SyntheticsEngine:
         st = None if stafilter is None else np.ascontiguousarray(stafilter, dtype=np.complex128)
         if (ev is not None and ev.shape != (nf2,)) or (st is not None and st.shape != (theta.size, 3, nf2)):
             raise ValueError("evfilter must be [nf2] and stafilter [nrcv, 3, nf2]")
-        pd = None if propdir is None else np.ascontiguousarray(propdir, dtype=np.float64)
-        tr = None if tred is None else np.ascontiguousarray(tred, dtype=np.float64)
+        # one value per receiver: a scalar (e.g. the event's reduction time of the reference's non-auto
+        # branch, getTimeshiftSeismicEvent) applies to every receiver; any other size is an error, since
+        # the library reads nrcv values
+        pd = None if propdir is None else per_receiver(propdir, theta.size, "propdir")
+        tr = None if tred is None else per_receiver(tred, theta.size, "tred")
         zsp = np.zeros((theta.size, 3, nf2), np.complex128) if want_spectra else None
         if device_ptrs is None:
             gr, gi = self._spectra(gr, gi)
@@ -196,6 +199,16 @@ class SyntheticsEngine:
 
 
 GM_FILTER_BUTTERWORTH_BANDPASS = 1
+
+
+def per_receiver(a, nrcv, name):
+    """a as nrcv contiguous doubles: a scalar is broadcast, an array must hold exactly nrcv values."""
+    a = np.asarray(a, dtype=np.float64)
+    if a.size == 1:
+        return np.full(nrcv, float(a.reshape(-1)[0]))
+    if a.size != nrcv:
+        raise ValueError(f"{name} holds {a.size} values for {nrcv} receivers")
+    return np.ascontiguousarray(a.reshape(-1))
 
 
 def event_filter(filspecs, nf1, nf2, df, sigma, filtype=GM_FILTER_BUTTERWORTH_BANDPASS):

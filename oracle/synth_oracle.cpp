@@ -221,7 +221,8 @@ void fft(std::vector<double>& x, std::vector<double>& y, int n, int is) {
 // transform runs at ns = 2^(ceil(log2 nf2)+1) samples of dts = 1/(df ns) and
 // every ndec-th sample is kept, ndec = nint(dt/dts); the damping factor then
 // advances by exp(sigma dts) per kept sample, as written there (:104-109).
-// Returns 0, or 2 for a non-integer ratio dt/dts, or 3 if dt does not match nsamp.
+// Returns 0, or 2 for a non-integer ratio dt/dts, 3 if dt does not match nsamp,
+// 4 if the kept samples run past the transform.
 int freq_time(int nf1, int nf2, double df, double sigma, const cd* sp, int nsamp, double dt, float* ts) {
   // the reference takes this log in single precision (log of real(nf2))
   int nslog = (int)std::ceil((double)std::log((float)nf2) / std::log(2.0)) + 1;
@@ -236,6 +237,8 @@ int freq_time(int nf1, int nf2, double df, double sigma, const cd* sp, int nsamp
     dts = 1.0 / (df * ns);
     ndec = (int)std::lround(dt / dts);
     if (std::fabs(dt - ndec * dts) > 1.e-5) return 2;
+    // the reference indexes x(ndec*(i-1)+1) up to i = nsamp without a check
+    if (ndec < 1 || (size_t)ndec * (nsamp - 1) >= (size_t)ns) return 4;
   }
   std::vector<double> x(ns, 0.0), y(ns, 0.0);
   for (int jf = nf1; jf <= nf2; ++jf) { x[jf - 1] = sp[jf - 1].real(); y[jf - 1] = sp[jf - 1].imag(); }

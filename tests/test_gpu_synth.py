@@ -97,11 +97,15 @@ def test_synthetics_match_oracle(syn):
     tred = rng.uniform(0, 20, R)
     mt = rng.normal(size=6) * 1e18
     print(f"synthetics: {R} receivers, nf2 {nf2}, nsamp {nsamp}", flush=True)
-    for vel, pd, tr in [(False, None, None), (True, propdir, tred)]:
+    # the third case is the reference's non-automatic time shift: one reduction time for the event
+    # (getTimeshiftSeismicEvent), given as a scalar and applied to every receiver
+    for vel, pd, tr in [(False, None, None), (True, propdir, tred), (False, 0.0, 7.5)]:
         ts, zsp = syn.synthetics(nf1, nf2, df, sigma, nwn_all, gr, gi, theta, phi, mt, nsamp, dt, propdir=pd,
                                  tred=tr, velocity=vel, want_spectra=True)
-        ots, ozsp = oracle.synthetics(nf1, nf2, df, sigma, nwn_all, gr, gi, theta, phi, mt, nsamp, dt, propdir=pd,
-                                      tred=tr, velocity=vel)
+        opd = None if pd is None else np.broadcast_to(np.asarray(pd, float), (R,)).copy()
+        otr = None if tr is None else np.broadcast_to(np.asarray(tr, float), (R,)).copy()
+        ots, ozsp = oracle.synthetics(nf1, nf2, df, sigma, nwn_all, gr, gi, theta, phi, mt, nsamp, dt, propdir=opd,
+                                      tred=otr, velocity=vel)
         zerr = np.abs(zsp - ozsp).max() / np.abs(ozsp).max()
         ts64, ots64 = ts.astype(np.float64), ots.astype(np.float64)       # FP32 squares overflow at 1e18 N m
         terr = np.linalg.norm(ts64 - ots64, axis=2) / np.linalg.norm(ots64, axis=2)
@@ -131,6 +135,8 @@ def test_synthetics_errors(syn):
     with pytest.raises(GeminiFKError) as e:
         syn.synthetics(1, 2, 1 / 64, 0.0, nwn, g, g, [1.0], [0.0], np.ones(6), 128, 1.0)
     assert e.value.code == 9
+    with pytest.raises(ValueError):                 # one reduction time per receiver, or one for all
+        syn.synthetics(1, 2, 1 / 64, 0.0, nwn, g, g, [1.0, 2.0], [0.0, 0.0], np.ones(6), 64, 1.0, tred=[1.0, 2.0, 3.0])
 
 
 def test_bessel_sums_match_oracle(syn):
@@ -417,22 +423,23 @@ def test_frequency_time_every_length(syn):
         assert terr < 1e-5, (nsamp, terr)
 
 
-@pytest.mark.parametrize("nsamp", [128, 64])
-def test_decimated_synthetics_match_oracle(syn, nsamp):
+@pytest.mark.parametrize("nsamp,ndec", [(128, 4), (64, 8), (300, 1), (100, 2)])
+def test_decimated_synthetics_match_oracle(syn, nsamp, ndec):
     """Fewer samples than the frequencies need: the transform runs at the full length and keeps every
-    ndec-th sample, as transformFrequencyTime's decimation branch does; a dt that is no multiple of
-    the transform's step is an error."""
+    ndec-th sample, as transformFrequencyTime's decimation branch does, also for ndec = 1 (the first
+    nsamp samples at the transform's own step) and for nsamp that is no power of two; a dt that is no
+    multiple of the transform's step is an error."""
     from gemini_unified_amd._lib import GeminiFKError
     rng = np.random.default_rng(61)
     nf1, nf2, T = 2, 200, 256.0                      # 200 frequencies need 512 samples
-    df, dt = 1.0 / T, T / nsamp
+    df, dt = 1.0 / T, ndec * T / 512
     sigma = 5 * df
     nwn = np.zeros(nf2, np.int32)
     nwn[nf1 - 1:] = 20
     gr, gi = _random_spectra(rng, nwn[nf1 - 1:])
     theta, phi = rng.uniform(0.2, 3.0, 3), rng.uniform(0, 6.2, 3)
     mt = rng.normal(size=6)
-    print(f"decimated synthetics: nsamp {nsamp} of 512", flush=True)
+    print(f"decimated synthetics: nsamp {nsamp} of 512, every {ndec}", flush=True)
     ts = syn.synthetics(nf1, nf2, df, sigma, nwn, gr, gi, theta, phi, mt, nsamp, dt, velocity=True)
     ots, _ = oracle.synthetics(nf1, nf2, df, sigma, nwn, gr, gi, theta, phi, mt, nsamp, dt, velocity=True)
     ts64, ots64 = ts.astype(np.float64), ots.astype(np.float64)

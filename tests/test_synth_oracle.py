@@ -117,8 +117,8 @@ def test_freq_time_matches_irfft():
     assert rc == 2
 
 
-@pytest.mark.parametrize("ndec", [2, 4])
-def test_freq_time_decimation(ndec):
+@pytest.mark.parametrize("ndec,nkeep", [(2, None), (4, None), (1, 300), (2, 100)])
+def test_freq_time_decimation(ndec, nkeep):
     """With fewer samples than the frequencies need, transformFrequencyTime transforms at the full
     length ns (step dts) and keeps every ndec-th sample; its damping factor advances by
     exp(sigma dts) per kept sample (frequencyTime.f90:104-109), so the kept samples are those of the
@@ -133,11 +133,16 @@ def test_freq_time_decimation(ndec):
     dts = T / ns
     rc, full = oracle.freq_time(nf1, nf2, df, sigma, sp, ns, dts)
     assert rc == 0
-    rc, dec = oracle.freq_time(nf1, nf2, df, sigma, sp, ns // ndec, ndec * dts)
+    nsamp = ns // ndec if nkeep is None else nkeep   # ndec = 1 with nsamp < ns keeps the first nsamp
+    rc, dec = oracle.freq_time(nf1, nf2, df, sigma, sp, nsamp, ndec * dts)
     assert rc == 0
-    i = np.arange(ns // ndec)
-    ref = full[::ndec].astype(np.float64) * np.exp(-sigma * dts * (ndec - 1) * i)
+    i = np.arange(nsamp)
+    ref = full[::ndec][:nsamp].astype(np.float64) * np.exp(-sigma * dts * (ndec - 1) * i)
     np.testing.assert_allclose(dec, ref, rtol=1e-6, atol=1e-6 * np.abs(ref).max())
+    if ndec > 1:
+        # more kept samples than the transform holds: an error, not a read past the transform
+        rc, _ = oracle.freq_time(nf1, nf2, df, sigma, sp, ns // ndec + 1, ndec * dts)
+        assert rc == 4
 
 
 def _gen_spectra(nf, L, t, kind):
