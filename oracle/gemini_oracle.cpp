@@ -1680,6 +1680,28 @@ int orc_spheroidal_minors(void* h, double omre, double omim, double dll1, int js
   return rc;
 }
 
+// computeSpheroidalAdjoint on its own: the solidt system integrated from ra to
+// re, starting from ys[4] (complex); adj[nnod][4] complex at the nodes passed
+// (zero elsewhere), yend[4] the end value.  Exposed for the pins of the adjoint
+// system.
+int orc_spheroidal_adjoint(void* h, double omre, double omim, double dll1, const double* ys, double ra, double re,
+                           double eps, double* adj, double* yend, int64_t* stats) {
+  Ctx* c = (Ctx*)h;
+  Env env; env.m = &c->m; env.nod = &c->nod; env.s = &c->s; env.omre = omre; env.omim = omim; env.dll1 = dll1; env.nl = 1;
+  cx y[6];
+  for (int i = 0; i < 6; ++i) y[i] = C(0.0);
+  for (int i = 1; i <= 4; ++i) y[i] = C(ys[2 * (i - 1)], ys[2 * (i - 1) + 1]);
+  std::vector<cx> a;
+  Stats st;
+  int rc = spheroidalAdjoint(env, y, ra, re, eps, a, st);
+  if (rc == OK) {
+    put(adj, a);
+    for (int i = 1; i <= 4; ++i) { yend[2 * (i - 1)] = y[i].re; yend[2 * (i - 1) + 1] = y[i].im; }
+  }
+  if (stats) for (int i = 0; i < 10; ++i) stats[i] += st.v[i];
+  return rc;
+}
+
 // one column: gf (nsp, nnod) complex, ujgfsph (4,4,nnod), ujgftor (2,2,nnod); returns detflag in *flag
 int orc_gfk_column(void* h, double omre, double omim, double dll1, int jre, int istyp, double eps, double* gf,
                    double* ujs, double* ujt, int* flag, int64_t* stats) {

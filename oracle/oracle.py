@@ -47,6 +47,8 @@ def lib():
         _lib.orc_initial_values.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
                                             C.c_int, D, D]
         _lib.orc_sphancf.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, D]
+        _lib.orc_spheroidal_adjoint.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, D, C.c_double,
+                                                C.c_double, C.c_double, D, D, L]
         _lib.orc_zetl.argtypes = [C.c_double, C.c_double, C.c_double, D]
         _lib.orc_toroidal_minors.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                                              C.c_double, D, D, D, L]
@@ -157,6 +159,16 @@ class Oracle:
                                          ulg.ctypes.data_as(D), dlg.ctypes.data_as(D), _c(det),
                                          self.stats.ctypes.data_as(L))
         return rc, u, d, ulg, dlg, det
+
+    def spheroidal_adjoint(self, omre, omim, dll1, ys, ra, re, eps):
+        """computeSpheroidalAdjoint from ra to re starting at ys[4]; returns (rc, adj[nnod, 4], yend[4])."""
+        n = self.nodes.nnod
+        y0 = np.ascontiguousarray(ys, dtype=np.complex128)
+        adj, yend = np.zeros((n, 4), np.complex128), np.zeros(4, np.complex128)
+        rc = lib().orc_spheroidal_adjoint(self.h, C.c_double(omre), C.c_double(omim), C.c_double(dll1), _c(y0),
+                                          C.c_double(ra), C.c_double(re), C.c_double(eps), _c(adj), _c(yend),
+                                          self.stats.ctypes.data_as(L))
+        return rc, adj, yend
 
     def gfk_column(self, omre, omim, dll1, jre, istyp, eps):
         n = self.nodes.nnod
@@ -337,8 +349,9 @@ def synthetics(nf1, nf2, df, sigma, nwn, gr, gi, theta, phi, mt, nsamp, dt, prop
     ts = np.zeros((nr, 3, nsamp), np.float32)
     zsp = np.zeros((nr, 3, nf2), np.complex128)
     F = C.POINTER(C.c_float)
-    pd = None if propdir is None else np.ascontiguousarray(propdir, dtype=np.float64)
-    tr = None if tred is None else np.ascontiguousarray(tred, dtype=np.float64)
+    # one value per receiver (a scalar applies to all): the library reads nr values
+    pd = None if propdir is None else np.ascontiguousarray(np.broadcast_to(np.asarray(propdir, np.float64), (nr,)))
+    tr = None if tred is None else np.ascontiguousarray(np.broadcast_to(np.asarray(tred, np.float64), (nr,)))
     ev = None if evfilter is None else np.ascontiguousarray(evfilter, dtype=np.complex128)
     st = None if stafilter is None else np.ascontiguousarray(stafilter, dtype=np.complex128)
     istyp = {10: 1, 5: 0}[gr.shape[0]]

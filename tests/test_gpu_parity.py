@@ -100,8 +100,9 @@ def test_gfk_columns_match_oracle(engines, orc, c1_setup, f, lmin, nl):
     rf = fa.gfk([f], lmin, nl, SIGMA, EPS, jre, 1)
     errf = rel_l2(rf["gf"], ref)
     print(f"fast variant  f={f:.5f}: max rel-L2 {errf.max():.2e}, p99 {np.quantile(errf, 0.99):.2e}")
-    assert np.quantile(errf, 0.99) < 1e-8
-    assert errf.max() < 1e-5
+    # the fast variant changes rounding only (FMA, reciprocals); its columns must still meet the 1e-8
+    # relative FP64 bound of the reference parity, not merely the step tolerance
+    assert errf.max() < 1e-8
 
 
 @pytest.mark.parametrize("variant,tol", [("exact", 1e-12), ("fast", 1e-9)])

@@ -149,7 +149,8 @@ def test_zetl_bessel_ratio(x, el):
 @pytest.mark.parametrize("x,el,kind", [(3.0 + 0.1j, 2.0, 1), (25.0 - 0.5j, 7.0, 1), (12.0 + 0.3j, 20.0, 1),
                                        (60.0 - 1j, 3.0, 1)])
 def test_sphancf_hankel_ratio(x, el, kind):
-    # kind 1 only: kind 2 serves the full-space start values, a branch outside this build
+    # kind 1 against the first-kind Hankel ratio; kind 2 is the reference's own formula, which is the
+    # second-kind ratio only asymptotically (test_oracle_closed_forms.test_sphancf_kind2_is_the_reference_formula)
     rc, z = oracle.sphancf(complex(x), el, kind)
     assert rc == 0
     n = int(el)
@@ -254,7 +255,9 @@ def test_full_space_known_answer(f, l):
       Deeper, downward integration through evanescent zones pulls any solution towards the dominant one.
     - Minors, compared as raw vectors as in the homogeneous-sphere test (dominated by the largest
       minors): to 1e-4 at every node.  The known answer carries the 1e-6 stopping tolerance of the
-      continued fractions."""
+      continued fractions.  Componentwise, m1 differs by about 1 %: the reference's kind-2 ratio is the
+      outgoing one only asymptotically, so its start vector is not exactly an outgoing solution
+      (test_oracle_closed_forms pins the start formula and that ratio separately)."""
     nem = homogeneous_model(full_space=1)
     assert nem.fsflag == 1
     nodes = ExternalRadialNodes.split_at_discos(nem, np.array([2.5e6, 3.5e6, 4.5e6, 5.5e6, nem.rearth]), nem.rearth)

@@ -408,6 +408,74 @@ def test_moment_displacement_rotation_invariance():
         np.testing.assert_allclose(z1[:, :, 0], z0[:, :, 0], rtol=1e-12, atol=1e-12 * np.abs(z0).max())
 
 
+def _small_source(rng, L=60, nf1=2, nf2=24):
+    """Random FP32 spectra of a moment source at frequencies nf1..nf2, L degrees each."""
+    nwn = np.zeros(nf2, np.int32)
+    nwn[nf1 - 1:] = L
+    nkf = int(nwn.sum())
+    gr = (rng.normal(size=(10, nkf)) * np.exp(-np.tile(np.arange(L), nf2 - nf1 + 1) / 20.0)).astype(np.float32)
+    gi = (rng.normal(size=(10, nkf)) * np.exp(-np.tile(np.arange(L), nf2 - nf1 + 1) / 20.0)).astype(np.float32)
+    return nwn, gr, gi
+
+
+def test_zne_rotation_follows_the_propagation_direction():
+    """vectorDcmplxZNEfromRLTAxesRotation (axesRotation.f90:560-568) with propdir measured at the
+    receiver from south over east (computeSyntheticSeismograms.f90:315-317): built here from the
+    geometry instead, the longitudinal unit vector is L = (E, N) = (sin a, -cos a) (a = 0 points
+    south) and the transverse one T = up x L.  Z = R, (N, E) = ul L + ut T for random directions,
+    the horizontal norm is kept, a = 0 gives (N, E) = (-L, T) and a = pi/2 gives (T, L)."""
+    rng = np.random.default_rng(21)
+    nwn, gr, gi = _small_source(rng)
+    R = 5
+    theta, phi = rng.uniform(0.3, 2.8, R), rng.uniform(0, 2 * np.pi, R)
+    mt = rng.normal(size=6)
+    T = 64.0
+    args = (2, 24, 1.0 / T, 5.0 / T, nwn, gr, gi, theta, phi, mt, 64, T / 64)
+    _, rlt = oracle.synthetics(*args)
+    for a in (rng.uniform(0, 2 * np.pi, R), np.zeros(R), np.full(R, np.pi / 2)):
+        _, zne = oracle.synthetics(*args, propdir=a)
+        for r in range(R):
+            Lv = np.array([np.sin(a[r]), -np.cos(a[r]), 0.0])             # (E, N, U)
+            Tv = np.cross([0.0, 0.0, 1.0], Lv)
+            ul, ut = rlt[r, 1], rlt[r, 2]
+            np.testing.assert_allclose(zne[r, 0], rlt[r, 0], rtol=1e-14, atol=0)
+            np.testing.assert_allclose(zne[r, 1], ul * Lv[1] + ut * Tv[1], rtol=1e-12, atol=1e-14 * np.abs(ul).max())
+            np.testing.assert_allclose(zne[r, 2], ul * Lv[0] + ut * Tv[0], rtol=1e-12, atol=1e-14 * np.abs(ul).max())
+            h0 = np.abs(ul) ** 2 + np.abs(ut) ** 2
+            h1 = np.abs(zne[r, 1]) ** 2 + np.abs(zne[r, 2]) ** 2
+            np.testing.assert_allclose(h1, h0, rtol=1e-12, atol=1e-24 * h0.max())
+    _, z0 = oracle.synthetics(*args, propdir=np.zeros(R))
+    np.testing.assert_allclose(z0[:, 1], -rlt[:, 1], rtol=1e-14)
+    np.testing.assert_allclose(z0[:, 2], rlt[:, 2], rtol=1e-14)
+
+
+def test_velocity_is_the_time_derivative_of_displacement():
+    """The velocity branch multiplies by i (omega - i sigma) (computeSyntheticSeismograms.f90:376-378),
+    which after the transform and the exp(sigma t) undamping is exactly d/dt of the displacement
+    trace.  Checked in the time domain by central differences of densely sampled displacement traces:
+    the residual falls at second order as dt halves (a real omega, or the damping term with the wrong
+    sign, leaves an O(sigma/omega) error that does not shrink)."""
+    rng = np.random.default_rng(22)
+    nwn, gr, gi = _small_source(rng)
+    theta, phi, mt = np.array([0.7, 2.1]), np.array([0.4, 3.9]), rng.normal(size=6)
+    T = 64.0
+    errs = []
+    for nsamp in (512, 1024, 2048):
+        dt = T / nsamp
+        args = (2, 24, 1.0 / T, 5.0 / T, nwn, gr, gi, theta, phi, mt, nsamp, dt)
+        u, _ = oracle.synthetics(*args)
+        v, _ = oracle.synthetics(*args, velocity=True)
+        u, v = u.astype(np.float64), v.astype(np.float64)
+        du = (u[:, :, 2:] - u[:, :, :-2]) / (2 * dt)
+        vv = v[:, :, 1:-1]
+        # the first and last 10 % of the window hold the wrap-around of the periodic transform
+        k0, k1 = nsamp // 10, nsamp - nsamp // 10
+        errs.append(np.linalg.norm(du[:, :, k0:k1] - vv[:, :, k0:k1]) / np.linalg.norm(vv[:, :, k0:k1]))
+    print(f"velocity vs central difference of displacement: {errs}")
+    assert errs[-1] < 3e-3
+    assert errs[0] / errs[1] > 3.0 and errs[1] / errs[2] > 3.0
+
+
 # ---- travel times from a ray table (f90/travelTimes.f90) ----
 # In a homogeneous sphere rays are straight: the first arrival between radii
 # rs and re at angular distance d takes chord / v with
