@@ -13,19 +13,27 @@ there.
 Procedure of the default run:
   1. error sample: 64 seeded frequencies x 16 consecutive degrees, truth = the
      CPU oracle at eps = 1e-10 (FK columns of all 10 spectra at all nodes);
-  2. plain control: the loosest eps of a ladder whose sample error is <= 1e-6;
-     W warmup + K timed steps of the whole workload at that eps;
-  3. learned control: step logs of the plain integrator on a seeded 0.4 % of the
-     (omega, l) grid at that eps, GPU training of the proposer network, then the
-     loosest ladder eps whose sample error is <= 1e-6, timed the same way;
-  4. the learned proposer on the 10 % / 90 % split of the 102-frequency,
+  2. plain control (the reference's step rule, target try count 6): the
+     loosest eps of a ladder whose sample error is <= 1e-6; W warmup + K timed
+     steps of the whole workload at that eps;
+  3. learned control: step logs of the plain integrator on a seeded 0.2 % of
+     the (omega, l) grid at that eps (every record kept, in instance order),
+     GPU training of the proposer network on one sample per instance, then its
+     per-instance policy (target try count, first step) at the loosest ladder
+     eps whose sample error is <= 1e-6, timed the same way;
+  4. the ablation of the policy: the fixed target try count 7 without a
+     network, at its own equal-error eps; then a second back-to-back pair of
+     plain and learned timings;
+  5. the learned policy on the 10 % / 90 % split of the 102-frequency,
      l <= 600 workload at eps = 1e-6 (rejected-step ratio, solves/s);
-  5. the degree-sum contraction and frequency-time transform of 4096
+  6. the degree-sum contraction and frequency-time transform of 4096
      receivers on spectra of the same shape as the main workload's;
-  6. the regional branch: minors over 512 frequencies x 19,999 wavenumbers of a
-     6-layer crustal model and the wavenumber-integral synthetics of 1024
+  7. the regional branch: minors over 512 frequencies x 19,999 wavenumbers of
+     a 6-layer crustal model, plain and learned, each at its equal-error eps on
+     a seeded error sample, and the wavenumber-integral synthetics of 1024
      receivers;
-  7. the CPU oracle on a bounded sample of the workload at the plain eps.
+  8. the CPU oracle on a bounded sample of the workload at the plain eps, on
+     every host core this process may use (at most 16) and on one.
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -79,14 +87,16 @@ def oracle_mod():
     return oracle
 
 
-def error_sample(grid, seed=5, nfreq=64, per_f=16):
+def error_sample(grid, seed=5, nfreq=64, per_f=16, first=0):
+    """Seeded (frequency, degree-run) pairs; first = the lowest index the grid's runs start from (1-based
+    wavenumber indices of the regional grid start at 2)."""
     rng = np.random.default_rng(seed)
     qs = np.sort(rng.choice(grid.freqs.size, size=min(nfreq, grid.freqs.size), replace=False))
-    l0 = np.array([int(rng.integers(1, max(2, int(grid.nwn[q]) - per_f))) for q in qs], np.int32)
+    l0 = np.array([first + int(rng.integers(1, max(2, int(grid.nwn[q]) - per_f - first))) for q in qs], np.int32)
     return grid.freqs[qs], l0, np.full(qs.size, per_f, np.int32)
 
 
-def truth_columns(nem, nodes, grid, jre, sample, threads):
+def truth_columns(nem, nodes, grid, jre, sample, threads, dwn=None):
     """FK columns of the sample from the CPU oracle at eps = 1e-10: complex [nnod, 10, k]."""
     oracle = oracle_mod()
     from gemini_unified_amd import MC_PI
@@ -94,14 +104,18 @@ def truth_columns(nem, nodes, grid, jre, sample, threads):
     cols = []
     for f, l0, n in zip(*sample):
         orc.set_frequency(f, 3)
-        rc, ref, status = orc.gfk_degrees(2.0 * MC_PI * f, -grid.sigma, int(l0), int(n), jre, 1, 1e-10, threads)
+        if dwn:
+            rc, ref, status = orc.gfk_wavenumbers(2.0 * MC_PI * f, -grid.sigma, int(l0), int(n), dwn, jre, 1, 1e-10,
+                                                  threads)
+        else:
+            rc, ref, status = orc.gfk_degrees(2.0 * MC_PI * f, -grid.sigma, int(l0), int(n), jre, 1, 1e-10, threads)
         assert rc == 0 and np.all(status == 0)
         cols.append(ref)
     return np.concatenate(cols, axis=2)
 
 
-def sample_error(eng, grid, jre, sample, eps, truth):
-    r = eng.gfk(sample[0], sample[1], sample[2], grid.sigma, eps, jre, 1, want_fp32=False)
+def sample_error(eng, grid, jre, sample, eps, truth, dwn=None):
+    r = eng.gfk(sample[0], sample[1], sample[2], grid.sigma, eps, jre, 1, want_fp32=False, dwn=dwn)
     g = r["gf"]
     k = g.shape[-1]
     num = np.linalg.norm((g - truth).reshape(-1, k), axis=0)
@@ -110,67 +124,81 @@ def sample_error(eng, grid, jre, sample, eps, truth):
     return float(err.max()), float(np.quantile(err, 0.99))
 
 
-def pick_eps(eng, grid, jre, sample, truth, tag):
+def pick_eps(eng, grid, jre, sample, truth, tag, dwn=None):
     """Loosest ladder eps whose sample error is <= the target; returns (eps, max, p99, table)."""
     table = []
     for eps in EPS_LADDER:
-        emax, ep99 = sample_error(eng, grid, jre, sample, eps, truth)
+        emax, ep99 = sample_error(eng, grid, jre, sample, eps, truth, dwn)
         table.append({"eps": eps, "max": emax, "p99": ep99})
-        log(f"{tag}: eps {eps:.2g} -> sample error max {emax:.2e} p99 {ep99:.2e}")
+        log(f"{tag}: eps {eps:.3g} -> sample error max {emax:.2e} p99 {ep99:.2e}")
         if emax <= ERR_TARGET:
             return eps, emax, ep99, table
     return None, None, None, table
 
 
-def time_minors(eng, grid, jre, eps, steps, warmup, tag):
+def time_minors(eng, grid, jre, eps, steps, warmup, tag, lmin=0, dwn=None):
     freqs, nwn = grid.freqs, grid.nwn
     for w in range(warmup):
         t0 = time.perf_counter()
-        eng.run_minors_resident(freqs, 0, nwn, grid.sigma, eps, jre, jre)
+        eng.run_minors_resident(freqs, lmin, nwn, grid.sigma, eps, jre, jre, dwn=dwn)
         log(f"{tag} warmup {w + 1}/{warmup}: {time.perf_counter() - t0:.2f} s")
     ts, stats = [], None
     for k in range(steps):
         t0 = time.perf_counter()
-        stats = eng.run_minors_resident(freqs, 0, nwn, grid.sigma, eps, jre, jre)
+        stats = eng.run_minors_resident(freqs, lmin, nwn, grid.sigma, eps, jre, jre, dwn=dwn)
         ts.append(time.perf_counter() - t0)
         log(f"{tag} step {k + 1}/{steps}: {ts[-1]:.3f} s")
     return 1e3 * float(np.mean(ts)), stats, eng.timing()
 
 
-def train_proposer(eng, grid, jre, eps, frac, seed, maxrec=3_000_000):
+def train_policy(eng, grid, jre, eps, frac, seed, first=0, dwn=None, maxrec=5_000_000):
+    """Step logs of the plain integrator on a seeded fraction of the grid, per-instance policy targets
+    (proposer.policy_targets_from_log) and GPU training of the proposer network.  Every record of the
+    log is kept and the records are put in (instance, step) order, so the trained network is the same
+    from run to run."""
     from gemini_unified_amd import proposer
     train, _ = proposer.split_grid(grid, frac, seed)
     fr, lm, nl = proposer.as_batch(train)
-    log(f"proposer: logging the steps of {int(nl.sum())} solves at eps {eps:g}")
+    lm = lm + first
+    log(f"policy: logging the steps of {int(nl.sum())} solves at eps {eps:g}")
     t0 = time.perf_counter()
-    rec, nrec, _ = eng.collect_steplog(fr, lm, nl, grid.sigma, eps, jre, jre, maxrec=maxrec)
-    X, T, _ = proposer.targets_from_log(rec)
+    rec, nrec, _ = eng.collect_steplog(fr, lm, nl, grid.sigma, eps, jre, jre, maxrec=maxrec, dwn=dwn)
+    if nrec > maxrec:
+        raise RuntimeError(f"step log holds {nrec} records, more than the {maxrec} kept")
+    X, T, _ = proposer.policy_targets_from_log(rec)
     t1 = time.perf_counter()
-    log(f"proposer: {nrec} step records in {t1 - t0:.1f} s; training")
-    params, norm, losses = proposer.train(X, T, epochs=12, batch=2048, lr=2e-3)
+    params, norm, losses = proposer.train(X, T, epochs=40, batch=64, lr=2e-3, seed=1)
     t2 = time.perf_counter()
-    log(f"proposer: {nrec} step records ({X.shape[0]} used) in {t1 - t0:.1f} s, trained in {t2 - t1:.1f} s, "
-        f"loss {losses[0]:.3f} -> {losses[-1]:.3f}")
-    return params, norm, {"records": int(nrec), "samples": int(X.shape[0]), "log_s": round(t1 - t0, 2),
+    kvals, kcnt = np.unique(T[:, 1], return_counts=True)
+    log(f"policy: {nrec} step records -> {X.shape[0]} instances in {t1 - t0:.1f} s, trained in {t2 - t1:.1f} s, "
+        f"loss {losses[0]:.3f} -> {losses[-1]:.3f}, target try counts {dict(zip(kvals.tolist(), kcnt.tolist()))}")
+    return params, norm, {"records": int(nrec), "instances": int(X.shape[0]), "log_s": round(t1 - t0, 2),
                           "train_s": round(t2 - t1, 2), "loss_first": float(losses[0]),
-                          "loss_last": float(losses[-1])}
+                          "loss_last": float(losses[-1]),
+                          "target_try_counts": {str(int(k)): int(c) for k, c in zip(kvals, kcnt)}}
 
 
-def c3_report(variant, safety, every):
-    """Learned proposer on the 102-frequency, l <= 600 workload: 10 % train, 90 % held out, eps = 1e-6."""
+def controller(eng, name, params=None, norm=None, safety=0.8):
+    """Configure one of the step controllers: the reference rule (target try 6), the trained network's
+    per-instance policy, or the fixed target try 7 without a network (the ablation of the policy)."""
+    eng.set_proposer(enable=False)
+    eng.configure(order_target=7 if name == "order7" else 6)
+    if name == "policy":
+        eng.set_proposer(params, norm, safety=safety, mode="policy")
+
+
+def c3_report(variant, safety):
+    """Learned policy on the 102-frequency, l <= 600 workload: 10 % train, 90 % held out, eps = 1e-6."""
     from gemini_unified_amd import GreenFKEngine, proposer
     nem, nodes, grid, jre, desc = workload("c1")
     eng = GreenFKEngine(nem, nodes, device=0, variant=variant)
-    eng.configure(persistent=0, propose_every=every)
-    params, norm, info = train_proposer(eng, grid, jre, 1e-6, 0.1, 4)
+    eng.configure(persistent=0)
+    params, norm, info = train_policy(eng, grid, jre, 1e-6, 0.1, 4)
     _, test = proposer.split_grid(grid, 0.1, 4)
     tf, tl, tn = proposer.as_batch(test)
     out = {"workload": desc + ", held-out 90 %", "eps": 1e-6, "train": info}
-    for name, on in (("plain", False), ("learned", True)):
-        if on:
-            eng.set_proposer(params, norm, safety=safety)
-        else:
-            eng.set_proposer(enable=False)
+    for name in ("plain", "policy", "order7"):
+        controller(eng, "reference" if name == "plain" else name, params, norm, safety)
         eng.run_minors_resident(tf, tl, tn, grid.sigma, 1e-6, jre, jre)
         st = eng.run_minors_resident(tf, tl, tn, grid.sigma, 1e-6, jre, jre)
         tm = eng.timing()
@@ -179,7 +207,9 @@ def c3_report(variant, safety, every):
                      "rejected_ratio": round(float(st[2]) / max(1, float(st[1] + st[2])), 5),
                      "accepted_steps": int(st[1]), "matvecs": int(st[4])}
         log(f"C3 {name}: {out[name]}")
+    controller(eng, "reference")
     eng.close()
+    out["policy_over_plain"] = round(out["policy"]["solves_per_s"] / out["plain"]["solves_per_s"], 4)
     return out
 
 
@@ -244,38 +274,55 @@ def gfk_report(eng, grid, jre, eps, ninst):
     return out
 
 
-def c5_report(variant, nrcv=1024, seed=7):
-    """Regional (Bessel) branch: minors over the whole regional grid and the wavenumber-integral
-    synthetics of nrcv receivers at 1-200 km on spectra of the same shape."""
+def c5_report(variant, steps, warmup, threads, safety, nrcv=1024, seed=7):
+    """Regional (Bessel) branch: minors over the whole regional grid with the reference rule and with
+    the learned policy, each at its equal-error eps on a seeded error sample, and the
+    wavenumber-integral synthetics of nrcv receivers at 1-200 km on spectra of the same shape."""
     from gemini_unified_amd import ExternalRadialNodes, FrequencyGrid, GreenFKEngine, regional_model
     from gemini_unified_amd.synthetics import SyntheticsEngine
     nem = regional_model(6)
     nodes = ExternalRadialNodes.from_blocks(nem, [32], [30000.0 / 32])
     jre = nodes.node_index(nem.rearth)
     grid = FrequencyGrid.regional_fixed(102.4, 5.0, 20000, 8.0e6)
-    eps = 1e-6
+    dwn = grid.dwn
     nf = grid.freqs.size
     # the reference skips k = 0 (iwn = 1): wavenumber indices 2..20000
-    iw = np.full(nf, 2, np.int32)
-    nw = np.full(nf, 19999, np.int32)
-    ninst = int(nw.sum())
+    rgrid = FrequencyGrid(df=grid.df, nf1=grid.nf1, nf2=grid.nf2, sigma=grid.sigma,
+                          nwn=np.full(nf, 19999, dtype=np.int64))
+    ninst = int(rgrid.nwn.sum())
     eng = GreenFKEngine(nem, nodes, device=0, variant=variant)
     eng.configure(persistent=0)
-    eng.run_minors_resident(grid.freqs, iw, nw, grid.sigma, eps, jre, jre, dwn=grid.dwn)
+    sample = error_sample(rgrid, seed=6, nfreq=32, per_f=8, first=2)
     t0 = time.perf_counter()
-    st = eng.run_minors_resident(grid.freqs, iw, nw, grid.sigma, eps, jre, jre, dwn=grid.dwn)
-    wall = time.perf_counter() - t0
-    tm = eng.timing()
-    flops = sum(int(eng.system_stats(s_)[7]) for s_ in (0, 2, 3))
-    eng.close()
+    truth = truth_columns(nem, nodes, rgrid, jre, sample, threads, dwn=dwn)
+    log(f"C5 truth: {truth.shape[-1]} FK columns in {time.perf_counter() - t0:.1f} s")
     out = {"workload": ("seeded regional model (6 layers of 2-40 km over a halfspace), T=102.4 s, f<=5 Hz "
                         f"(512 frequencies) x 19,999 wavenumbers (dwn = 2 pi / 8000 km) = {ninst:,} solves, "
-                        f"32 source depths 0.9-30 km + surface receiver, eps {eps:g}, plain control"),
-           "solves_per_s": round(ninst / wall, 1), "ms_per_pass": round(wall * 1e3, 1),
-           "stage_ms": {"toroidal": round(tm[2], 1), "spheroidal": round(tm[3], 1)},
-           "k_spheroidal_tflops": round(flops / (tm[3] * 1e-3) / 1e12, 3),
-           "rejected_ratio": round(float(st[2]) / max(1, float(st[1] + st[2])), 5)}
-    log(f"C5 minors: {out['solves_per_s']:.0f} solves/s ({wall:.2f} s per pass)")
+                        "32 source depths 0.9-30 km + surface receiver"),
+           "error_sample": f"{truth.shape[-1]} FK columns (32 seeded frequencies x 8 wavenumbers), truth = CPU "
+                           f"oracle at eps=1e-10"}
+    params = norm = None
+    for name in ("plain", "policy"):
+        if name == "policy":
+            params, norm, info = train_policy(eng, rgrid, jre, out["plain"]["eps"], 0.0005, 4, first=2, dwn=dwn)
+            out["train"] = info
+        controller(eng, "reference" if name == "plain" else name, params, norm, safety)
+        eps, emax, ep99, ladder = pick_eps(eng, rgrid, jre, sample, truth, f"C5 {name}", dwn=dwn)
+        if eps is None:
+            eps, emax, ep99 = EPS_LADDER[-1], ladder[-1]["max"], ladder[-1]["p99"]
+        ms, st, tm = time_minors(eng, rgrid, jre, eps, steps, warmup, f"C5 {name}", lmin=2, dwn=dwn)
+        flops = sum(int(eng.system_stats(s_)[7]) for s_ in (0, 2, 3))
+        out[name] = {"solves_per_s": round(ninst / (ms * 1e-3), 1), "ms_per_pass": round(ms, 1), "eps": eps,
+                     "spectrum_rel_err": {"max": emax, "p99": ep99},
+                     "stage_ms": {"policy_and_setup": round(tm[1], 1), "toroidal": round(tm[2], 1),
+                                  "spheroidal": round(tm[3], 1)},
+                     "k_spheroidal_tflops": round(flops / (tm[3] * 1e-3) / 1e12, 3),
+                     "rejected_ratio": round(float(st[2]) / max(1, float(st[1] + st[2])), 5),
+                     "accepted_steps": int(st[1])}
+        log(f"C5 {name}: {out[name]['solves_per_s']:.0f} solves/s at eps {eps:g}")
+    controller(eng, "reference")
+    eng.close()
+    out["policy_over_plain"] = round(out["policy"]["solves_per_s"] / out["plain"]["solves_per_s"], 4)
     rng = np.random.default_rng(seed)
     nwn = np.zeros(grid.nf2, np.int32)
     nwn[grid.nf1 - 1:] = 20000
@@ -332,6 +379,26 @@ def cpu_baseline(nem, nodes, grid, jre, eps, seconds, threads, seed=4):
     return done / t_used, done, nf
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_cores():
+    """Cores this process may use: its affinity set, at most 16 (one GPU's share of the host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
 def read_traffic():
     """HBM bytes per k_spheroidal launch from the committed rocprofv3 counter summary, if present."""
     try:
@@ -366,10 +433,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["c1", "c2"], default="c2")
     ap.add_argument("--variant", choices=["exact", "fast"], default="fast")
-    ap.add_argument("--eps", type=float, default=None, help="skip the error ladder: time plain (and learned) at eps")
+    ap.add_argument("--eps", type=float, default=None, help="skip the error ladders: time every controller at eps")
     ap.add_argument("--safety", type=float, default=0.8)
-    ap.add_argument("--propose-every", type=int, default=256)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
@@ -387,10 +453,10 @@ def main():
     from gemini_unified_amd import GreenFKEngine
     nem, nodes, grid, jre, desc = workload(args.workload)
     ninst = grid.nkf
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = host_cores()
     log(f"workload {args.workload}: {ninst} (omega,l) solves per step, {nodes.nnod} nodes, receiver node {jre}")
     eng = GreenFKEngine(nem, nodes, device=0, variant=args.variant)
-    eng.configure(persistent=0, propose_every=args.propose_every)
+    eng.configure(persistent=0)
 
     # 1. error sample and truth
     sample = error_sample(grid)
@@ -398,79 +464,93 @@ def main():
     truth = truth_columns(nem, nodes, grid, jre, sample, threads)
     log(f"truth: {truth.shape[-1]} FK columns from the CPU oracle at eps=1e-10 in {time.perf_counter() - t0:.1f} s")
 
-    # 2. plain control
-    if args.eps:
-        eps_p = args.eps
-        emax, ep99 = sample_error(eng, grid, jre, sample, eps_p, truth)
-        ladder_p = [{"eps": eps_p, "max": emax, "p99": ep99}]
-    else:
-        eps_p, emax, ep99, ladder_p = pick_eps(eng, grid, jre, sample, truth, "plain")
-        if eps_p is None:
-            eps_p, emax, ep99 = EPS_LADDER[-1], ladder_p[-1]["max"], ladder_p[-1]["p99"]
-    ms_p, stats_p, tm_p = time_minors(eng, grid, jre, eps_p, args.steps, args.warmup, "plain")
-    value_p = ninst / (ms_p / 1e3)
-    sph_flops = sum(int(eng.system_stats(s)[7]) for s in (0, 2, 3))
+    def operating_point(tag):
+        if args.eps:
+            emax, ep99 = sample_error(eng, grid, jre, sample, args.eps, truth)
+            return args.eps, emax, ep99, [{"eps": args.eps, "max": emax, "p99": ep99}]
+        eps, emax, ep99, ladder = pick_eps(eng, grid, jre, sample, truth, tag)
+        if eps is None:
+            eps, emax, ep99 = EPS_LADDER[-1], ladder[-1]["max"], ladder[-1]["p99"]
+        return eps, emax, ep99, ladder
+
+    def arm(tag, eps, emax, ep99, ladder, steps, warmup):
+        ms, st, tm = time_minors(eng, grid, jre, eps, steps, warmup, tag)
+        sph = sum(int(eng.system_stats(s_)[7]) for s_ in (0, 2, 3))
+        rec = {"value": round(ninst / (ms / 1e3), 3), "ms_per_step": round(ms, 3), "eps": eps,
+               "spectrum_rel_err": {"max": emax, "p99": ep99}, "eps_ladder": ladder,
+               "rejected_ratio": round(float(st[2]) / max(1, float(st[1] + st[2])), 5),
+               "stage_ms": {"tables": round(tm[0], 3), "setup_and_policy": round(tm[1], 3),
+                            "toroidal": round(tm[2], 3), "spheroidal": round(tm[3], 3)},
+               "k_spheroidal_tflops": round(sph / (tm[3] * 1e-3) / 1e12, 4),
+               "work": {k: int(v) for k, v in zip(["propagations", "accepted_steps", "rejected_steps",
+                                                   "mmp_sequences", "matvecs", "sysmat_evals", "neville_cells",
+                                                   "flops"], st[:8])}}
+        log(f"{tag}: eps {eps:g}, {rec['value']:.1f} solves/s")
+        return rec, tm, sph
+
+    # 2. plain control: the reference's step rule (target try count 6)
+    controller(eng, "reference")
+    op_p = operating_point("plain")
+    plain, tm_p, sph_flops = arm("plain", *op_p, args.steps, args.warmup)
     tor_flops = int(eng.system_stats(1)[7])
     achieved = sph_flops / (tm_p[3] * 1e-3) / 1e12
-    traffic = read_traffic()
     roofline = {"bound": "fp64-valu", "kernel": "k_spheroidal", "achieved": round(achieved, 4),
                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
-                "traffic": traffic, "kernel_ms": round(tm_p[3], 3), "algorithmic_flops_per_launch": sph_flops,
+                "traffic": read_traffic(), "kernel_ms": round(tm_p[3], 3), "algorithmic_flops_per_launch": sph_flops,
                 "toroidal": {"kernel": "k_toroidal", "kernel_ms": round(tm_p[2], 3),
                              "achieved": round(tor_flops / (tm_p[2] * 1e-3) / 1e12, 4)}}
-    log(f"plain: eps {eps_p:g}, {value_p:.1f} solves/s; k_spheroidal {tm_p[3]:.1f} ms, {achieved:.3f} TFLOP/s")
-    plain = {"value": round(value_p, 3), "ms_per_step": round(ms_p, 3), "eps": eps_p,
-             "spectrum_rel_err": {"max": emax, "p99": ep99}, "eps_ladder": ladder_p,
-             "rejected_ratio": round(float(stats_p[2]) / max(1, float(stats_p[1] + stats_p[2])), 5),
-             "stage_ms": {"tables": round(tm_p[0], 3), "setup": round(tm_p[1], 3), "toroidal": round(tm_p[2], 3),
-                          "spheroidal": round(tm_p[3], 3)},
-             "work": {k: int(v) for k, v in zip(["propagations", "accepted_steps", "rejected_steps", "mmp_sequences",
-                                                 "matvecs", "sysmat_evals", "neville_cells", "flops"],
-                                                stats_p[:8])}}
+    gfk = None if args.no_gfk else gfk_report(eng, grid, jre, op_p[0], ninst)
 
-    gfk = None if args.no_gfk else gfk_report(eng, grid, jre, eps_p, ninst)
-
-    # 3. learned control
-    learned = None
+    # 3. learned control: the trained network's per-instance policy; 4. its ablation, the fixed target
+    # try count 7 without a network; 5. a second back-to-back pair of plain and learned timings
+    learned = ablation = None
     if not args.no_learned:
-        params, norm, info = train_proposer(eng, grid, jre, eps_p, 0.004, 4)
-        eng.set_proposer(params, norm, safety=args.safety)
-        if args.eps:
-            eps_l = args.eps
-            lmax_, lp99 = sample_error(eng, grid, jre, sample, eps_l, truth)
-            ladder_l = [{"eps": eps_l, "max": lmax_, "p99": lp99}]
-        else:
-            eps_l, lmax_, lp99, ladder_l = pick_eps(eng, grid, jre, sample, truth, "learned")
-        if eps_l is not None:
-            ms_l, stats_l, tm_l = time_minors(eng, grid, jre, eps_l, args.steps, args.warmup, "learned")
-            value_l = ninst / (ms_l / 1e3)
-            learned = {"value": round(value_l, 3), "ms_per_step": round(ms_l, 3), "eps": eps_l,
-                       "spectrum_rel_err": {"max": lmax_, "p99": lp99}, "eps_ladder": ladder_l,
-                       "rejected_ratio": round(float(stats_l[2]) / max(1, float(stats_l[1] + stats_l[2])), 5),
-                       "matvecs": int(stats_l[4]), "accepted_steps": int(stats_l[1]),
-                       "safety": args.safety, "propose_every": args.propose_every, "train": info,
-                       "trained_on": "step logs of the plain integrator on a seeded 0.4 % of the (omega,l) grid",
-                       "learned_over_plain": round(value_l / value_p, 4)}
-            log(f"learned: eps {eps_l:g}, {value_l:.1f} solves/s ({value_l / value_p:.3f} x plain)")
-        eng.set_proposer(enable=False)
+        params, norm, info = train_policy(eng, grid, jre, op_p[0], 0.002, 4)
+        controller(eng, "policy", params, norm, args.safety)
+        op_l = operating_point("learned")
+        learned, tm_l, _ = arm("learned", *op_l, args.steps, args.warmup)
+        learned.update({"mode": "per-instance policy of the proposer network (target try count, first step)",
+                        "safety": args.safety, "train": info,
+                        "trained_on": "step logs of the plain integrator on a seeded 0.2 % of the (omega,l) grid"})
+        controller(eng, "order7")
+        op_a = operating_point("order7")
+        ablation, _, _ = arm("order7", *op_a, max(1, args.steps // 4), 1)
+        short = max(1, args.steps // 4)
+        controller(eng, "reference")
+        p2, _, _ = arm("plain (second)", op_p[0], op_p[1], op_p[2], [], short, 0)
+        controller(eng, "policy", params, norm, args.safety)
+        l2, _, _ = arm("learned (second)", op_l[0], op_l[1], op_l[2], [], short, 0)
+        controller(eng, "reference")
+        ratios = [round(learned["value"] / plain["value"], 4), round(l2["value"] / p2["value"], 4)]
+        learned["learned_over_plain"] = ratios[0]
+        learned["learned_over_plain_back_to_back"] = ratios
+        learned["target_over_plain"] = 2.0
+        learned["order7_over_plain"] = round(ablation["value"] / plain["value"], 4)
+        learned["learned_over_order7"] = round(learned["value"] / ablation["value"], 4)
     eng.close()
 
-    c3 = None if (args.no_learned or args.no_c3) else c3_report(args.variant, args.safety, args.propose_every)
+    c3 = None if (args.no_learned or args.no_c3) else c3_report(args.variant, args.safety)
     synth = None if args.no_synth else synth_report(grid)
-    c5 = None if args.no_c5 else c5_report(args.variant)
+    c5 = None if args.no_c5 else c5_report(args.variant, max(1, args.steps // 4), 1, threads, args.safety)
 
     cpu = None
     if not args.no_cpu:
-        log(f"cpu baseline: oracle on {threads} threads for ~{args.cpu_seconds:.0f} s at eps {eps_p:g}")
-        rate, nsolves, nfreq = cpu_baseline(nem, nodes, grid, jre, eps_p, args.cpu_seconds, threads)
+        log(f"cpu baseline: oracle on {threads} threads for ~{args.cpu_seconds:.0f} s at eps {op_p[0]:g}")
+        rate, nsolves, nfreq = cpu_baseline(nem, nodes, grid, jre, op_p[0], args.cpu_seconds, threads)
+        log(f"cpu baseline: {rate:.2f} solves/s; now 1 thread")
+        rate1, nsolves1, nfreq1 = cpu_baseline(nem, nodes, grid, jre, op_p[0], args.cpu_seconds / 2, 1, seed=8)
         cpu = {"value": round(rate, 3), "unit": "solves/s", "cores": threads, "kind": "port",
                "sample": f"{nsolves} solves: {nfreq} seeded random frequencies x 32 consecutive degrees at a "
-                         f"seeded random start, same workload and eps {eps_p:g}"}
-        log(f"cpu baseline: {rate:.2f} solves/s")
+                         f"seeded random start, same workload and eps {op_p[0]:g}",
+               "cores_all": threads, "value_1core": round(rate1, 3), "cores_1": 1,
+               "sample_1core": f"{nsolves1} solves ({nfreq1} frequencies x 32 degrees)",
+               "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+               "gpu_plain_over_cpu_all": round(plain["value"] / rate, 1)}
+        log(f"cpu baseline: {rate:.2f} solves/s on {threads} threads, {rate1:.2f} on 1")
 
-    # the metric's value: solves/s of the learned controller at the equal-error operating point
-    # (the plain controller's when the learned part was skipped)
-    value = learned["value"] if learned else value_p
+    # the metric's value: solves/s of the learned controller at its equal-error operating point (the plain
+    # controller's when the learned part was skipped)
+    value = learned["value"] if learned else plain["value"]
     out = {
         "metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": learned["ms_per_step"] if learned else plain["ms_per_step"],
@@ -479,9 +559,10 @@ def main():
                    "error_target": ERR_TARGET,
                    "error_sample": f"{truth.shape[-1]} FK columns (64 seeded frequencies x 16 degrees), truth = "
                                    f"CPU oracle at eps=1e-10, relative L2 over 10 spectra x {nodes.nnod} nodes",
-                   "step_control": "learned" if learned else "plain"},
+                   "step_control": "learned policy" if learned else "plain"},
         "plain": plain,
         "learned": learned,
+        "order7_ablation": ablation,
         "c3_learned_proposer": c3,
         "synthetics": synth,
         "gfk_columns": gfk,

@@ -1,10 +1,12 @@
 """Schedule autotuner for the minors kernels.
 
 Picks, by measurement on the GPU, the kernel schedule of a GreenFKEngine:
-one lane per (omega, l) job or the persistent substep-scheduled kernels, the
-waves per SIMD those kernels are compiled for (1: no register spills, 2: more
-latency hiding) and the work-unit size (instances of one frequency handed to
-a block).  Every candidate performs the same arithmetic in every lane, so the
+one lane per (omega, l) job, with the 256-instance work unit split into 1, 2
+or 4 blocks (smaller blocks free their SIMD sooner; larger ones restage the
+frequency's table image less often), or the persistent substep-scheduled
+kernels, the waves per SIMD those kernels are compiled for (1: no register
+spills, 2: more latency hiding) and their work-unit size (instances of one
+frequency handed to a block, at most 256).  Every candidate performs the same arithmetic in every lane, so the
 choice never changes a result, only the time.
 
 The sample is frequency-stratified: every k-th frequency of the grid with all
@@ -21,11 +23,12 @@ import time
 import numpy as np
 
 DEFAULT_CANDIDATES = (
-    {"persistent": 0, "waves": 1, "unit": 1024},
-    {"persistent": 1, "waves": 1, "unit": 512},
-    {"persistent": 1, "waves": 1, "unit": 1024},
-    {"persistent": 1, "waves": 1, "unit": 4096},
-    {"persistent": 1, "waves": 2, "unit": 1024},
+    {"persistent": 0, "sph_split": 4},
+    {"persistent": 0, "sph_split": 2},
+    {"persistent": 0, "sph_split": 1},
+    {"persistent": 1, "waves": 1, "unit": 128},
+    {"persistent": 1, "waves": 1, "unit": 256},
+    {"persistent": 1, "waves": 2, "unit": 256},
 )
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_cache.json")
 

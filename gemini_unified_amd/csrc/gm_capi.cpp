@@ -14,10 +14,12 @@
 namespace gm_exact {
 void launch_stage(int, const GmModel*, const GmBatch&, int, const double*, int, GmKnot*, hipStream_t, int, size_t, int);
 int persist_blocks_per_cu(size_t, int);
+void launch_policy(const GmModel*, const GmBatch&, hipStream_t, size_t, float*, const float*, float*, float*, double, int);
 }
 namespace gm_fast {
 void launch_stage(int, const GmModel*, const GmBatch&, int, const double*, int, GmKnot*, hipStream_t, int, size_t, int);
 int persist_blocks_per_cu(size_t, int);
+void launch_policy(const GmModel*, const GmBatch&, hipStream_t, size_t, float*, const float*, float*, float*, double, int);
 }
 
 // per-system constants of the algorithmic work model
@@ -111,8 +113,14 @@ extern "C" int gm_set_option(gm_ctx* c, int option, int value) {
     c->prop.every = value;
     return GM_OK;
   }
+  if (option == GM_OPT_ORDER_TARGET) {
+    if (value < 3 || value > 7) return gm_fail(c, GM_ERR_ARG, "target try count must be in [3, 7]");
+    c->order_target = value;
+    return GM_OK;
+  }
   if (option == GM_OPT_UNIT) {
-    if (value < 64 || value > (1 << 20)) return gm_fail(c, GM_ERR_ARG, "unit size must be in [64, 2^20]");
+    // the set-up, adjoint and assembly kernels cover one work unit with one 256-thread block
+    if (value < 64 || value > 256) return gm_fail(c, GM_ERR_ARG, "unit size must be in [64, 256]");
     c->unit = value;
     return GM_OK;
   }
@@ -360,7 +368,13 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   const int nunits = (int)P.uq.size();
   size_t o_uq = take(sizeof(int) * (nunits + 1)), o_ust = take(sizeof(int) * (nunits + 1));
   size_t o_ue = take(sizeof(int) * (nunits + 1)), o_uc = take(sizeof(int) * GM_NCOUNTERS);
-  const int ctl_mode = c->log_max > 0 ? 1 : (c->prop_enabled ? 2 : 0);
+  const int ctl_mode = c->log_max > 0 ? 1 : (c->prop_enabled == 1 ? 2 : 0);
+  // per-instance policy: the proposer network's (enable 2) or a fixed target try count
+  const bool pol_net = ctl_mode == 0 && c->prop_enabled == 2;
+  const bool pol = pol_net || (ctl_mode == 0 && c->order_target != 6);
+  size_t o_pol = pol ? take(sizeof(float) * 2 * n) : 0;
+  size_t o_px = pol_net ? take(sizeof(float) * GM_MLP_IN * n) : 0;
+  size_t o_pw = pol_net ? take(sizeof(float) * gm_mlp_forward_work((int)std::min(n, (size_t)GM_MLP_CHUNK))) : 0;
   size_t o_log = ctl_mode == 1 ? take(sizeof(float) * GM_LOGREC * (size_t)c->log_max) : 0;
   size_t o_logc = take(sizeof(int) * 4);
   if (off > c->dbuf_size) {
@@ -413,6 +427,7 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   B.logbuf = ctl_mode == 1 ? (float*)(base + o_log) : nullptr;
   B.logcount = (int*)(base + o_logc);
   B.logmax = c->log_max;
+  B.policy = pol ? (const float*)(base + o_pol) : nullptr;
   {
     int z4[4] = {0, 0, 0, 0};
     HIPCHK(c, hipMemcpyAsync(base + o_logc, z4, sizeof z4, hipMemcpyHostToDevice, s));
@@ -424,10 +439,15 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   GmKnot* dT = (GmKnot*)(base + o_tables);
   const double* dF = (const double*)(base + o_freqs);
   HIPCHK(c, hipEventRecord(c->ev[0], s));
-  const bool pers = c->persistent != 0 && ctl_mode == 0;   // step logging / learned control: one lane per job
+  const bool pers = c->persistent != 0 && ctl_mode == 0 && !pol;   // step logging / learned control: one lane per job
   launch(0, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
   HIPCHK(c, hipEventRecord(c->ev[1], s));
   launch(1, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
+  if (pol) {
+    auto lp = c->variant == 0 ? gm_exact::launch_policy : gm_fast::launch_policy;
+    lp(dM, B, s, lds, pol_net ? (float*)(base + o_px) : nullptr, c->dmlp, pol_net ? (float*)(base + o_pw) : nullptr,
+       (float*)(base + o_pol), m.rearth, pol_net ? 0 : c->order_target);
+  }
   HIPCHK(c, hipEventRecord(c->ev[2], s));
   launch(pers ? 6 : 2, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
   HIPCHK(c, hipEventRecord(c->ev[3], s));
@@ -529,6 +549,7 @@ extern "C" int gm_last_system_stats(gm_ctx* c, int sys, long long* stats) {
 extern "C" int gm_set_proposer(gm_ctx* c, const float* params, const float* norm, float safety, int kcheck, int enable) {
   if (!c) return GM_ERR_ARG;
   if (!enable) { c->prop_enabled = 0; return GM_OK; }
+  if (enable != 1 && enable != 2) return gm_fail(c, GM_ERR_ARG, "enable must be 0, 1 (every step) or 2 (per instance)");
   if (!params || !norm || !(safety > 0.0f)) return gm_fail(c, GM_ERR_ARG, "proposer needs parameters, normalisation and safety > 0");
   HIPCHK(c, hipSetDevice(c->device));
   if (!c->dmlp && hipMalloc(&c->dmlp, sizeof(float) * GM_MLP_NPARAM) != hipSuccess)
@@ -543,7 +564,7 @@ extern "C" int gm_set_proposer(gm_ctx* c, const float* params, const float* norm
   c->prop.safety = safety;
   c->prop.kcheck = kcheck ? 1 : 0;
   c->prop.every = c->prop_every;
-  c->prop_enabled = 1;
+  c->prop_enabled = enable;
   return GM_OK;
 }
 

@@ -131,6 +131,11 @@ struct GmBatch {
   int* __restrict__ logcount;
   int logmax;
   GmProposer prop;
+  // per-instance step policy [n][2] (null: the reference's control): the
+  // target try count of the x1.25 / x0.8 rule (complexBulirschStep.f90:346-359
+  // targets 6) and the first step of the spheroidal up pass in metres (<= 0:
+  // the start value's lambda/4); read by the toroidal and spheroidal kernels
+  const float* __restrict__ policy;
 };
 #define GM_LOGREC 20
 // work counters (GmBatch::unit_counter): persistent toroidal / spheroidal

@@ -8,7 +8,7 @@ struct gm_ctx {
   int device = 0;
   int variant = 1;
   int persistent = 0;      // 1: substep-scheduled persistent kernels, 0: one lane per job (default, faster here)
-  int unit = 1024;         // instances per work unit of the persistent kernels
+  int unit = 256;          // instances per work unit of the persistent kernels (at most one 256-thread block)
   int waves = 1;           // waves per SIMD the persistent kernels are compiled for (1 or 2)
   int sph_split = 4;       // blocks per work unit of the one-lane-per-job spheroidal kernel
   int ncu = 0;
@@ -29,6 +29,7 @@ struct gm_ctx {
   GmProposer prop;
   int prop_enabled = 0;
   int prop_every = 1;      // network consulted every prop_every accepted steps
+  int order_target = 6;    // target try count of the step rule without a network (6: the reference's)
   // step logging of the next run (gm_collect_steplog)
   int log_max = 0;
   float* log_host = nullptr;

@@ -3,7 +3,7 @@
 #include "gm_common.h"
 #pragma clang fp contract(fast)
 #define GM_NS gm_fast
-namespace gm_fast { static constexpr bool kFast = true; }
+namespace gm_fast { static constexpr bool kFast = true; static constexpr bool kLearnedFused = false; }
 #include "gm_device.inc"
 #include "gm_kernels.inc"
 #include "gm_persist.inc"

@@ -7,6 +7,11 @@
 #define GM_MLP_OUT 2
 #define GM_MLP_NPARAM (GM_MLP_IN * GM_MLP_H + GM_MLP_H + 2 * (GM_MLP_H * GM_MLP_H + GM_MLP_H) + GM_MLP_H * GM_MLP_OUT + GM_MLP_OUT)
 
+// host-side entry of the batched forward pass on device buffers (gm_mlp.hip)
+#define GM_MLP_CHUNK (1 << 19)
+size_t gm_mlp_forward_work(int n);
+const float* gm_mlp_forward_dev(hipStream_t s, const float* params, const float* X, int n, float* work);
+
 // feature / target normalisation and control of the learned proposer
 struct GmProposer {
   float mu[GM_MLP_IN], isd[GM_MLP_IN];   // x_norm = (x - mu) * isd

@@ -179,6 +179,18 @@ static void carve(float* base, int Bn, Work& w) {
 
 using namespace gm_mlp;
 
+// forward of n device-resident samples on stream s (inference inside a batch
+// call): X[n][9] -> the returned Y[n][2] inside work (gm_mlp_forward_work(n)
+// floats).  The GEMM grid's second dimension is n/16 blocks, so callers pass
+// at most GM_MLP_CHUNK samples at a time.
+size_t gm_mlp_forward_work(int n) { return work_floats(n); }
+const float* gm_mlp_forward_dev(hipStream_t s, const float* params, const float* X, int n, float* work) {
+  Work w;
+  carve(work, n, w);
+  forward(s, params, X, n, w);
+  return w.y4;
+}
+
 // ---------------------------------------------------------------------------
 // C ABI (declared in include/gemini_fk.h)
 // ---------------------------------------------------------------------------

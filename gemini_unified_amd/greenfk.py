@@ -72,11 +72,16 @@ class GreenFKEngine:
         check(self.h, _lib.lib().gm_set_option(self.h, 1, VARIANTS[variant]))
         self.variant = variant
 
-    def configure(self, persistent=None, unit=None, waves=None, propose_every=None, sph_split=None):
+    def configure(self, persistent=None, unit=None, waves=None, propose_every=None, sph_split=None,
+                  order_target=None):
         """Kernel schedule options (all result-preserving): persistent substep-scheduled kernels or
-        one lane per job, work-unit size, waves per SIMD of the persistent kernels; and how often the
-        learned proposer is consulted (every N accepted steps, plus sub-interval starts)."""
+        one lane per job, work-unit size, waves per SIMD of the persistent kernels; how often the
+        learned proposer is consulted (every N accepted steps, plus sub-interval starts); and the
+        target try count of the step rule without a network (6: the reference's; changes results
+        at the tolerance level)."""
         L = _lib.lib()
+        if order_target is not None:
+            check(self.h, L.gm_set_option(self.h, 7, int(order_target)))
         if propose_every is not None:
             check(self.h, L.gm_set_option(self.h, 5, int(propose_every)))
         if sph_split is not None:
@@ -132,15 +137,17 @@ class GreenFKEngine:
         check(self.h, rc)
         return st
 
-    def set_proposer(self, params=None, norm=None, safety=0.9, kcheck=False, enable=True):
-        """Install the learned step proposer (or remove it with enable=False)."""
+    def set_proposer(self, params=None, norm=None, safety=0.9, kcheck=False, enable=True, mode="step"):
+        """Install the learned step proposer (or remove it with enable=False).  mode "step": the network
+        proposes during the integration (sequential step engine); "policy": it chooses, once per
+        (omega, l), the target try count and first step, and the kernels keep the fused sweep."""
         if not enable:
             check(self.h, _lib.lib().gm_set_proposer(self.h, None, None, 1.0, 0, 0))
             return
         p = np.ascontiguousarray(params, dtype=np.float32)
         nv = np.ascontiguousarray(norm, dtype=np.float32)
         check(self.h, _lib.lib().gm_set_proposer(self.h, p.ctypes.data_as(F), nv.ctypes.data_as(F), float(safety),
-                                                 int(bool(kcheck)), 1))
+                                                 int(bool(kcheck)), {"step": 1, "policy": 2}[mode]))
 
     def collect_steplog(self, freqs, lmin, nl, sigma, eps, jsl, jsr, maxrec=2_000_000,
                         attmode="ATTENUATION_AND_DISPERSION", dwn=None):

@@ -18,8 +18,12 @@ between the last two tests.  With the previous step size it tells the network
 how far below or above the tolerance the previous step was.
 
 The network is trained on the GPU by the library's own forward / backward /
-Adam kernels and then proposes every step; the reference error test still
-decides, and a rejected step is halved as in the reference.
+Adam kernels.  In the "step" mode it proposes during the integration; the
+reference error test still decides, and a rejected step is halved as in the
+reference.  In the "policy" mode (policy_targets_from_log) it is trained on one
+sample per (omega, l) and chooses, before the integration, each instance's
+target try count and first step; the kernels then run the reference's step
+rule around that target on the fused seven-sequence sweep.
 """
 from __future__ import annotations
 
@@ -61,6 +65,48 @@ def targets_from_log(rec, kmin=5):
     kbest = np.argmin(np.where(np.isfinite(cost), cost, np.inf), axis=1)
     T = np.stack([lhk[np.arange(len(lh)), kbest], kbest + 3.0], axis=1)
     return X[ok].astype(np.float32), T[ok].astype(np.float32), ok
+
+
+def canonical(rec):
+    """Step records in (instance, step) order.  Lanes claim log slots with an atomic counter, so the
+    order in which records arrive depends on which lane wins each slot; sorted, a log is the same from
+    run to run."""
+    rec = np.asarray(rec)
+    return rec[np.lexsort((rec[:, 17], rec[:, 16]))]
+
+
+def subsample(rec, nmax, seed=4):
+    """A seeded subset of at most nmax records, kept in their order."""
+    if len(rec) <= nmax:
+        return rec
+    idx = np.sort(np.random.default_rng(seed).choice(len(rec), size=nmax, replace=False))
+    return rec[idx]
+
+
+def policy_targets_from_log(rec, kmin=5):
+    """Per-instance targets of the policy mode (one network evaluation per (omega, l)).
+
+    Features: those of each instance's first logged step, at its spheroidal start radius (no step
+    history, as the policy kernel sees them).  Targets: the try count k* in kmin..7 that would cover the
+    instance's logged steps in the fewest steps of the fused sweep, where every step computes all seven
+    sequences and so costs the same: a logged step of size h covers h / h_k = (err_k / eps)^(1/q_k)
+    steps of the largest step try k accepts there; and log10(h*/R) of the first step that k* accepts.
+    Returns (X[n, 9], T[n, 2], instance ids)."""
+    rec = canonical(rec)
+    job = rec[:, 16].astype(np.int64)
+    e = rec[:, 10:15].astype(np.float64)
+    starts = np.flatnonzero(np.r_[True, job[1:] != job[:-1]])
+    ok = np.all(np.isfinite(e), axis=1)
+    terms = np.where(ok[:, None], 10.0 ** (np.where(ok[:, None], e, 0.0) / Q_ORDER[None, :]), 0.0)
+    cost = np.add.reduceat(terms, starts, axis=0)
+    cost[:, :kmin - 3] = np.inf
+    kbest = np.argmin(cost, axis=1)
+    first = rec[starts]
+    e0 = first[:, 10:15].astype(np.float64)
+    lh = first[:, 9].astype(np.float64) - e0[np.arange(len(first)), kbest] / Q_ORDER[kbest]
+    T = np.stack([lh, kbest + 3.0], axis=1)
+    good = (first[:, 17] == 0) & np.all(np.isfinite(first[:, :NIN]), axis=1) & np.all(np.isfinite(T), axis=1)
+    return first[good, :NIN].astype(np.float32), T[good].astype(np.float32), job[starts][good]
 
 
 def normalisation(X, T):

@@ -56,13 +56,19 @@ typedef struct gm_ctx gm_ctx;
 /* options (gm_set_option) */
 #define GM_OPT_VARIANT 1       /* 0: exact (no FMA contraction, bitwise operation order), 1: fast (FMA) */
 #define GM_OPT_PERSISTENT 2    /* 0 (default): one lane per job, 1: persistent substep-scheduled kernels */
-#define GM_OPT_UNIT 3          /* (omega, l) instances per work unit of the persistent kernels */
+#define GM_OPT_UNIT 3          /* (omega, l) instances per work unit of the persistent kernels, 64..256
+                                  (default 256) */
 #define GM_OPT_WAVES 4         /* waves per SIMD the persistent kernels are built for: 1 or 2 */
 #define GM_OPT_PROPOSE_EVERY 5 /* learned proposer: consult the network after every N-th accepted step,
                                   and at a sub-interval start once N accepted steps have passed since the
                                   last consultation (default 1: every step) */
 #define GM_OPT_SPH_SPLIT 6     /* blocks per 256-instance work unit of the spheroidal kernel: 1, 2
                                   or 4 (default) */
+#define GM_OPT_ORDER_TARGET 7  /* target try count of the step rule of the toroidal and spheroidal
+                                  kernels without a network: 6 (default) is the reference's
+                                  (complexBulirschStep.f90:346-359), 3..7 allowed; on the fused
+                                  seven-sequence sweep every step costs the same, so 7 takes the
+                                  largest steps */
 
 /* work counters returned in stats[10] (same meaning as the CPU oracle's) */
 #define GM_NSTAT 10            /* propagations, accepted steps, rejected steps, MMP sequences,
@@ -160,8 +166,18 @@ int gm_mlp_grad(int device, const float* params, int n, const float* X, const fl
 /* Adam training on the GPU; params in/out; losses[epochs] = mean loss per epoch */
 int gm_mlp_train(int device, float* params, int n, const float* X, const float* T, float w0, float w1, int epochs,
                  int batch, float lr, unsigned seed, float* losses);
-/* install (enable = 1) or remove (enable = 0) the proposer of a context;
- * norm = mu[9], sd[9] of the features and mu[2], sd[2] of the outputs */
+/* install or remove (enable = 0) the proposer of a context;
+ * norm = mu[9], sd[9] of the features and mu[2], sd[2] of the outputs.
+ * enable = 1: the network proposes step size and try count during the
+ *   spheroidal integration (every GM_OPT_PROPOSE_EVERY accepted steps), on the
+ *   sequential step engine;
+ * enable = 2: per-instance policy: before the minors kernels the network is
+ *   evaluated once per (omega, l) on its features at the spheroidal start
+ *   radius (no step history), and its try count (5..7) becomes that
+ *   instance's target of the reference step rule in the toroidal and
+ *   spheroidal kernels, its step size the first step of the spheroidal up
+ *   pass; the kernels stay on the fused sweep.
+ * In both modes the reference error test accepts or rejects every step. */
 int gm_set_proposer(gm_ctx* ctx, const float* params, const float* norm, float safety, int kcheck, int enable);
 /* run the plain minors integration over the given (f, l) (or, with dwn > 0,
  * the wavenumber indices of gm_minors_batch_wn) and record every

@@ -104,3 +104,83 @@ def test_learned_control_stays_accurate(small_case):
         den = np.linalg.norm(o.reshape(-1, n), axis=0)
         err = num[den > 0] / den[den > 0]
         assert err.max() < 1e-4, (fj, err.max())
+
+
+def _rel_l2_cols(a, b):
+    k = a.shape[-1]
+    both = ~np.isfinite(a) & ~np.isfinite(b)
+    a, b = np.where(both, 0, a), np.where(both, 0, b)
+    num = np.linalg.norm((a - b).reshape(-1, k), axis=0)
+    den = np.linalg.norm(b.reshape(-1, k), axis=0)
+    return np.where(den > 0, num / np.where(den > 0, den, 1), num)
+
+
+def test_fixed_target_matches_host_build(small_case):
+    """The step rule around a fixed target try count (GM_OPT_ORDER_TARGET 7) on the GPU is the CPU
+    build of the same kernel source bit for bit (exact variant), work counters included."""
+    import sys, os
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "hostsim"))
+    from hostsim import HostSim
+    nem, nodes, jre, f, lmin, nl, sigma = small_case
+    eng = GreenFKEngine(nem, nodes, variant="exact")
+    eng.configure(persistent=0, order_target=7)
+    g = eng.gfk(f, lmin, nl, sigma, 1e-7, jre, 1, want_fp32=False)
+    eng.close()
+    sim = HostSim(nem, nodes, "exact")
+    sim.order_target = 7
+    r = sim.gfk(f, lmin, nl, sigma, 1e-7, jre, 1)
+    both = np.isnan(g["gf"]) & np.isnan(r["gf"])
+    assert np.array_equal(np.where(both, 0, g["gf"]), np.where(both, 0, r["gf"]))
+    np.testing.assert_array_equal(g["stats"], r["stats"])
+
+
+def test_learned_policy_needs_fewer_steps_at_equal_error():
+    """What the trained network gains over the method without it: on a held-out sample of the 16-node
+    global workload, the per-instance policy (network trained on the plain integrator's step logs of a
+    disjoint sample) and the reference rule each run at the loosest eps of a ladder that keeps the
+    FK-column error <= 1e-6 against the oracle at 1e-10.  The policy needs far fewer matvecs (it targets
+    try 7, where the fused sweep takes its largest steps); its gain over the fixed target 7 without a
+    network is printed beside it."""
+    from gemini_unified_amd import ExternalRadialNodes, global_model
+    nem = global_model(1, 200)
+    nodes = ExternalRadialNodes.from_blocks(nem, [15], [700000.0 / 15])
+    jre = nodes.node_index(nem.rearth)
+    sigma = 5 / 4096.0
+    rng = np.random.default_rng(5)
+    fr = np.sort(rng.choice(np.arange(2, 820), 16, replace=False)) / 4096.0
+    l0 = np.array([int(rng.integers(0, 2484)) for _ in fr], np.int32)
+    nl = np.full(fr.size, 16, np.int32)
+    orc = oracle.Oracle(nem, nodes)
+    truth = []
+    for fj, a in zip(fr, l0):
+        orc.set_frequency(fj, 3)
+        rc, ref, _ = orc.gfk_degrees(2.0 * MC_PI * fj, -sigma, int(a), 16, jre, 1, 1e-10, oracle.num_threads())
+        assert rc == 0
+        truth.append(ref)
+    truth = np.concatenate(truth, axis=2)
+    eng = GreenFKEngine(nem, nodes, variant="fast")
+    eng.configure(persistent=0)
+    trng = np.random.default_rng(9)
+    tf = np.sort(trng.choice(np.arange(2, 820), 32, replace=False)) / 4096.0
+    tl = np.array([int(trng.integers(0, 2476)) for _ in tf], np.int32)
+    rec, nrec, _ = eng.collect_steplog(tf, tl, np.full(tf.size, 24, np.int32), sigma, 4e-9, jre, jre,
+                                       maxrec=3_000_000)
+    assert nrec <= 3_000_000
+    X, T, _ = proposer.policy_targets_from_log(rec)
+    params, norm, losses = proposer.train(X, T, epochs=40, batch=64, lr=2e-3, seed=1)
+    ladder = [2e-8, 1e-8, 6e-9, 4e-9, 2.5e-9, 1.5e-9, 1e-9]
+    result = {}
+    for name in ("plain", "order7", "policy"):
+        eng.set_proposer(enable=False)
+        eng.configure(order_target=7 if name == "order7" else 6)
+        if name == "policy":
+            eng.set_proposer(params, norm, safety=0.8, mode="policy")
+        for eps in ladder:
+            r = eng.gfk(fr, l0, nl, sigma, eps, jre, 1, want_fp32=False)
+            if _rel_l2_cols(r["gf"], truth).max() <= 1e-6:
+                result[name] = (eps, int(r["stats"][4]))
+                break
+    eng.close()
+    print(f"equal-error operating points (eps, matvecs): {result}")
+    assert set(result) == {"plain", "order7", "policy"}
+    assert result["policy"][1] < 0.7 * result["plain"][1]

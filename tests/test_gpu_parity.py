@@ -27,7 +27,7 @@ def rel_l2(a, b):
     return np.where(den > 0, num / np.where(den > 0, den, 1), num)
 
 
-SCHEDULES = [dict(persistent=1, waves=1, unit=64), dict(persistent=1, waves=2, unit=1024),
+SCHEDULES = [dict(persistent=1, waves=1, unit=64), dict(persistent=1, waves=2, unit=256),
              dict(persistent=0)]
 
 
