@@ -260,7 +260,8 @@ def synth_report(grid, nrcv=4096, nsamp=8192, seed=3):
 
 def gfk_report(eng, grid, jre, eps, ninst):
     """The whole drop-in batch gm_gfk_batch on the main workload: minors, adjoint, unit-jump Green
-    functions, reciprocity, source terms and FP32 packing, outputs left in HBM (plain control)."""
+    functions, reciprocity, source terms and FP32 packing, outputs left in HBM (with the step control
+    the engine is configured for)."""
     eng.run_gfk_resident(grid.freqs, 0, grid.nwn, grid.sigma, eps, jre, 1)
     t0 = time.perf_counter()
     st = eng.run_gfk_resident(grid.freqs, 0, grid.nwn, grid.sigma, eps, jre, 1)
@@ -304,7 +305,7 @@ def c5_report(variant, steps, warmup, threads, safety, nrcv=1024, seed=7):
     params = norm = None
     for name in ("plain", "policy"):
         if name == "policy":
-            params, norm, info = train_policy(eng, rgrid, jre, out["plain"]["eps"], 0.0005, 4, first=2, dwn=dwn)
+            params, norm, info = train_policy(eng, rgrid, jre, out["plain"]["eps"], 0.0002, 4, first=2, dwn=dwn)
             out["train"] = info
         controller(eng, "reference" if name == "plain" else name, params, norm, safety)
         eps, emax, ep99, ladder = pick_eps(eng, rgrid, jre, sample, truth, f"C5 {name}", dwn=dwn)
@@ -520,6 +521,9 @@ def main():
         p2, _, _ = arm("plain (second)", op_p[0], op_p[1], op_p[2], [], short, 0)
         controller(eng, "policy", params, norm, args.safety)
         l2, _, _ = arm("learned (second)", op_l[0], op_l[1], op_l[2], [], short, 0)
+        controller(eng, "reference")
+        controller(eng, "policy", params, norm, args.safety)
+        learned["gfk_columns"] = None if args.no_gfk else gfk_report(eng, grid, jre, op_l[0], ninst)
         controller(eng, "reference")
         ratios = [round(learned["value"] / plain["value"], 4), round(l2["value"] / p2["value"], 4)]
         learned["learned_over_plain"] = ratios[0]

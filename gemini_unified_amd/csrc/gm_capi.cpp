@@ -562,7 +562,8 @@ extern "C" int gm_set_proposer(gm_ctx* c, const float* params, const float* norm
   }
   for (int o = 0; o < GM_MLP_OUT; ++o) { c->prop.omu[o] = norm[2 * GM_MLP_IN + o]; c->prop.osd[o] = norm[2 * GM_MLP_IN + 2 + o]; }
   c->prop.safety = safety;
-  c->prop.kcheck = kcheck ? 1 : 0;
+  c->prop.kcheck = (enable == 1 && kcheck) ? 1 : 0;
+  c->prop.first_step = (enable == 2 && kcheck) ? 1 : 0;
   c->prop.every = c->prop_every;
   c->prop_enabled = enable;
   return GM_OK;

@@ -19,7 +19,7 @@ struct GmProposer {
   float safety;                           // proposed step = safety * predicted step
   int kcheck;                             // 1: first error test at the predicted try - 1
   int every;                              // consult the network every `every` accepted steps
-  int pad;
+  int first_step;                         // per-instance policy: also apply the network's first step
 };
 
 // One forward pass for one lane.  Every lane reads the same weight at the same

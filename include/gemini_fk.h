@@ -174,9 +174,11 @@ int gm_mlp_train(int device, float* params, int n, const float* X, const float* 
  * enable = 2: per-instance policy: before the minors kernels the network is
  *   evaluated once per (omega, l) on its features at the spheroidal start
  *   radius (no step history), and its try count (5..7) becomes that
- *   instance's target of the reference step rule in the toroidal and
- *   spheroidal kernels, its step size the first step of the spheroidal up
- *   pass; the kernels stay on the fused sweep.
+ *   instance's target of the reference step rule in the toroidal,
+ *   spheroidal and adjoint kernels, which stay on the fused sweep; with
+ *   kcheck = 1 its step size also becomes the first step of the spheroidal
+ *   up pass (kcheck = 1 in the every-step mode: the first error test is at
+ *   the proposed try count - 1).
  * In both modes the reference error test accepts or rejects every step. */
 int gm_set_proposer(gm_ctx* ctx, const float* params, const float* norm, float safety, int kcheck, int enable);
 /* run the plain minors integration over the given (f, l) (or, with dwn > 0,

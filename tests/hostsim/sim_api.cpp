@@ -48,6 +48,7 @@ extern "C" GmProposer* sim_proposer_new(const float* norm, float safety, int kch
   for (int o = 0; o < GM_MLP_OUT; ++o) { p->omu[o] = norm[2 * GM_MLP_IN + o]; p->osd[o] = norm[2 * GM_MLP_IN + 2 + o]; }
   p->safety = safety;
   p->kcheck = kcheck ? 1 : 0;
+  p->first_step = kcheck ? 1 : 0;
   p->every = every;
   return p;
 }
