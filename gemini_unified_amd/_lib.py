@@ -21,7 +21,7 @@ EXPORTS = ["gm_ctx_create", "gm_ctx_destroy", "gm_set_option", "gm_last_error", 
            "gm_mlp_nparams", "gm_mlp_forward", "gm_mlp_grad", "gm_mlp_train", "gm_set_proposer",
            "gm_collect_steplog", "gm_minors_batch_wn", "gm_gfk_batch_wn", "gm_legendre_batch", "gm_moment_sums", "gm_synthetics", "gm_last_synth_timing",
            "gm_bessel_sums", "gm_synthetics_regional", "gm_force_sums", "gm_force_bessel_sums",
-           "gm_event_filter", "gm_travel_times"]
+           "gm_event_filter", "gm_travel_times", "gm_gfk_batch_noalloc"]
 
 
 class GeminiFKError(RuntimeError):
@@ -76,6 +76,8 @@ def lib():
                                              VP, VP, C.c_double, C.c_double, C.c_int, D, D, D, D, D, D, C.c_int, D,
                                              C.c_int, C.c_int, C.c_double, C.c_int, VP, VP]
         L.gm_event_filter.argtypes = [C.c_int, D, C.c_int, C.c_int, C.c_double, C.c_double, D]
+        L.gm_gfk_batch_noalloc.argtypes = [VP, C.c_int, D, I, I, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                                           VP, VP, VP, VP, LL]
         L.gm_travel_times.argtypes = [VP, C.c_int, C.c_int, C.c_int, F, F, F, F, F, C.c_double, C.c_int, D, D, D, D, I]
         _lib = L
     return _lib

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgemini_fk.so")
 OBJDIR = os.path.join(HERE, "_build")
-SOURCES = ["gm_exact.hip", "gm_fast.hip", "gm_mlp.hip", "gm_synth.hip", "gm_capi.cpp"]
+SOURCES = ["gm_exact.hip", "gm_fast.hip", "gm_mlp.hip", "gm_synth.hip", "gm_capi.cpp", "gm_noalloc.hip"]
 DEPS = SOURCES + ["gm_common.h", "gm_ctx.h", "gm_mlp.h", "gm_device.inc", "gm_kernels.inc", "gm_launch.inc",
                   "gm_persist.inc"]
 

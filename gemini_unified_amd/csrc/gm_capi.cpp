@@ -63,6 +63,11 @@ extern "C" int gm_ctx_create(int device, gm_ctx** out) {
     return gm_fail(nullptr, GM_ERR_HIP, "context creation failed on device %d", device);
   }
   if (c->ncu <= 0) c->ncu = 1;
+  // the address of the batch-without-allocation array on this device (once per device)
+  if (gm_noalloc_init(device) != GM_OK) {
+    gm_ctx_destroy(c);
+    return gm_fail(nullptr, GM_ERR_HIP, "no address of the scratch array on device %d", device);
+  }
   memset(&c->hmodel, 0, sizeof(GmModel));
   *out = c;
   return GM_OK;
@@ -287,12 +292,18 @@ struct Plan {
 };
 }
 
-// common driver of gm_minors_batch / gm_gfk_batch
+// common driver of gm_minors_batch / gm_gfk_batch; with na (gm_gfk_batch_noalloc)
+// every buffer lives in na's array, the plan is built on the device, outputs
+// (status included) are device pointers, and nothing is allocated
 static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const int* nl, double dwn, double sigma,
                double eps, int attmode, int jsl, int jsr, int jre, int istyp, bool full, int flags,
                double* u_tor, double* d_tor, double* dettor, double* u_sph, double* d_sph, double* ulg, double* dlg,
-               double* detsph, double* gf, float* gfre, float* gfim, int* status, long long* stats) {
+               double* detsph, double* gf, float* gfre, float* gfim, int* status, long long* stats,
+               const GmNoAlloc* na = nullptr) {
   if (!c) return GM_ERR_ARG;
+  if (na && nfreq > GM_NOALLOC_MAXF)
+    return gm_fail(c, GM_ERR_ARG, "%d frequencies: the batch without allocation takes at most %d", nfreq,
+                   GM_NOALLOC_MAXF);
   if (!c->has_model || !c->has_nodes) return gm_fail(c, GM_ERR_ARG, "model and nodes must be set first");
   if (nfreq < 0 || (nfreq > 0 && (!freqs || !lmin || !nl))) return gm_fail(c, GM_ERR_ARG, "invalid frequency arguments");
   if (attmode < 0 || attmode > 3) return gm_fail(c, GM_ERR_ARG, "invalid attenuation mode %d", attmode);
@@ -315,19 +326,27 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
         return gm_fail(c, GM_ERR_UNSUPPORTED, "node %d lies in the core or a fluid layer; only mantle nodes are supported", k);
   }
   // (omega, l) plan
-  Plan P;
-  P.nfreq = nfreq;
-  P.omre.resize(nfreq);
+  const int usz = (c->persistent && !na) ? c->unit : 256;   // one-lane-per-job kernels: one unit per block
+  size_t ninst = 0;
+  int nunits = 0;
   for (int f = 0; f < nfreq; ++f) {
     if (nl[f] < 0 || lmin[f] < 0) return gm_fail(c, GM_ERR_ARG, "negative degree range at frequency %d", f);
     if (dwn > 0.0 && nl[f] > 0 && lmin[f] < 1) return gm_fail(c, GM_ERR_ARG, "wavenumber indices are 1-based (frequency %d)", f);
     if (!(freqs[f] > 0.0)) return gm_fail(c, GM_ERR_ARG, "frequency %d must be positive", f);
-    P.omre[f] = 2.0 * (double)3.1415927410125732 * freqs[f];
-    for (int i = 0; i < nl[f]; ++i) { P.q.push_back(f); P.l.push_back(lmin[f] + i); }
+    ninst += (size_t)nl[f];
+    nunits += (nl[f] + usz - 1) / usz;
   }
-  P.ninst = (int)P.q.size();
-  if (P.ninst == 0) return GM_OK;     // no (omega, l) instance: every output is empty, stats unchanged
-  {
+  if (ninst == 0) return GM_OK;     // no (omega, l) instance: every output is empty, stats unchanged
+  if (ninst > (size_t)INT32_MAX) return gm_fail(c, GM_ERR_ARG, "more than 2^31 - 1 instances in one batch");
+  Plan P;
+  P.nfreq = nfreq;
+  P.ninst = (int)ninst;
+  if (!na) {
+    P.omre.resize(nfreq);
+    for (int f = 0; f < nfreq; ++f) {
+      P.omre[f] = 2.0 * (double)3.1415927410125732 * freqs[f];
+      for (int i = 0; i < nl[f]; ++i) { P.q.push_back(f); P.l.push_back(lmin[f] + i); }
+    }
     // work units: degree ranges of one frequency, highest frequency first and
     // low degrees (the longest integrations) first within a frequency
     std::vector<int> off(nfreq + 1, 0);
@@ -335,7 +354,6 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
     std::vector<int> order(nfreq);
     for (int f = 0; f < nfreq; ++f) order[f] = f;
     std::sort(order.begin(), order.end(), [&](int a, int b) { return freqs[a] > freqs[b]; });
-    const int usz = c->persistent ? c->unit : 256;   // one-lane-per-job kernels: one unit per block
     for (int f : order)
       for (int a = off[f]; a < off[f + 1]; a += usz) {
         P.uq.push_back(f); P.us.push_back(a); P.ue.push_back(std::min(a + usz, off[f + 1]));
@@ -365,7 +383,6 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   size_t o_gfre = (full && (gfre && !dev)) ? take(4 * nsp * nnod * n + 4) : 0;
   size_t o_gfim = (full && (gfim && !dev)) ? take(4 * nsp * nnod * n + 4) : 0;
   size_t o_hist = take(sizeof(unsigned long long) * 6 * 8);
-  const int nunits = (int)P.uq.size();
   size_t o_uq = take(sizeof(int) * (nunits + 1)), o_ust = take(sizeof(int) * (nunits + 1));
   size_t o_ue = take(sizeof(int) * (nunits + 1)), o_uc = take(sizeof(int) * GM_NCOUNTERS);
   const int ctl_mode = c->log_max > 0 ? 1 : (c->prop_enabled == 1 ? 2 : 0);
@@ -377,24 +394,43 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   size_t o_pw = pol_net ? take(sizeof(float) * gm_mlp_forward_work((int)std::min(n, (size_t)GM_MLP_CHUNK))) : 0;
   size_t o_log = ctl_mode == 1 ? take(sizeof(float) * GM_LOGREC * (size_t)c->log_max) : 0;
   size_t o_logc = take(sizeof(int) * 4);
-  if (off > c->dbuf_size) {
-    if (c->dbuf) { HIPCHK(c, hipFree(c->dbuf)); c->dbuf = nullptr; c->dbuf_size = 0; }
-    if (hipMalloc(&c->dbuf, off) != hipSuccess) return gm_fail(c, GM_ERR_NOMEM, "device allocation of %zu bytes failed", off);
-    c->dbuf_size = off;
+  size_t o_lmin = na ? take(sizeof(int) * (nfreq + 1)) : 0, o_nl = na ? take(sizeof(int) * (nfreq + 1)) : 0;
+  size_t o_smax = na ? take(sizeof(int)) : 0;
+  char* base;
+  if (na) {
+    if (off > na->bytes)
+      return gm_fail(c, GM_ERR_ARG, "the batch needs %zu bytes, more than the %zu bytes of the batch without allocation",
+                     off, na->bytes);
+    base = (char*)na->base;
+  } else {
+    if (off > c->dbuf_size) {
+      if (c->dbuf) { HIPCHK(c, hipFree(c->dbuf)); c->dbuf = nullptr; c->dbuf_size = 0; }
+      if (hipMalloc(&c->dbuf, off) != hipSuccess) return gm_fail(c, GM_ERR_NOMEM, "device allocation of %zu bytes failed", off);
+      c->dbuf_size = off;
+    }
+    base = (char*)c->dbuf;
   }
-  char* base = (char*)c->dbuf;
   hipStream_t s = c->stream;
   GmModel* dM = (GmModel*)(base + o_model);
   HIPCHK(c, hipMemcpyAsync(dM, &m, sizeof(GmModel), hipMemcpyHostToDevice, s));
   HIPCHK(c, hipMemcpyAsync(base + o_freqs, freqs, sizeof(double) * nfreq, hipMemcpyHostToDevice, s));
-  HIPCHK(c, hipMemcpyAsync(base + o_omre, P.omre.data(), sizeof(double) * nfreq, hipMemcpyHostToDevice, s));
-  HIPCHK(c, hipMemcpyAsync(base + o_q, P.q.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
-  HIPCHK(c, hipMemcpyAsync(base + o_l, P.l.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
-  HIPCHK(c, hipMemcpyAsync(base + o_uq, P.uq.data(), sizeof(int) * nunits, hipMemcpyHostToDevice, s));
-  HIPCHK(c, hipMemcpyAsync(base + o_ust, P.us.data(), sizeof(int) * nunits, hipMemcpyHostToDevice, s));
-  HIPCHK(c, hipMemcpyAsync(base + o_ue, P.ue.data(), sizeof(int) * nunits, hipMemcpyHostToDevice, s));
-  std::vector<unsigned long long> zero(48, 0ULL);
-  HIPCHK(c, hipMemcpyAsync(base + o_hist, zero.data(), sizeof(unsigned long long) * 48, hipMemcpyHostToDevice, s));
+  if (na) {
+    HIPCHK(c, hipMemcpyAsync(base + o_lmin, lmin, sizeof(int) * nfreq, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(base + o_nl, nl, sizeof(int) * nfreq, hipMemcpyHostToDevice, s));
+    gm_noalloc_plan(s, nfreq, (const double*)(base + o_freqs), (const int*)(base + o_lmin), (const int*)(base + o_nl),
+                    (int*)(base + o_q), (int*)(base + o_l), (double*)(base + o_omre), (int*)(base + o_uq),
+                    (int*)(base + o_ust), (int*)(base + o_ue), (unsigned long long*)(base + o_hist),
+                    (int*)(base + o_logc), (int*)(base + o_smax));
+  } else {
+    HIPCHK(c, hipMemcpyAsync(base + o_omre, P.omre.data(), sizeof(double) * nfreq, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(base + o_q, P.q.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(base + o_l, P.l.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(base + o_uq, P.uq.data(), sizeof(int) * nunits, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(base + o_ust, P.us.data(), sizeof(int) * nunits, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(base + o_ue, P.ue.data(), sizeof(int) * nunits, hipMemcpyHostToDevice, s));
+    std::vector<unsigned long long> zero(48, 0ULL);
+    HIPCHK(c, hipMemcpyAsync(base + o_hist, zero.data(), sizeof(unsigned long long) * 48, hipMemcpyHostToDevice, s));
+  }
   GmBatch B;
   memset(&B, 0, sizeof B);
   B.n = (int)n; B.nnod = nnod; B.jsl = jsl; B.jsr = jsr; B.jre = jre; B.istyp = istyp;
@@ -405,7 +441,7 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   B.ltab = (const unsigned char*)(base + o_ltab); B.lstride = (int)lstride;
   B.ttab = (const unsigned char*)(base + o_ttab); B.tstride = (int)tstride;
   B.tor_split = 4;
-  B.status = (int*)(base + o_status); B.rs = (double*)(base + o_rs);
+  B.status = (na && status) ? status : (int*)(base + o_status); B.rs = (double*)(base + o_rs);
   B.ut = (double*)(base + o_ut); B.dt = (double*)(base + o_dt); B.dett = (double*)(base + o_dett);
   B.us = (double*)(base + o_us); B.ds = (double*)(base + o_ds); B.ulg = (double*)(base + o_ulg);
   B.dlg = (double*)(base + o_dlg); B.dets = (double*)(base + o_dets);
@@ -428,7 +464,7 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   B.logcount = (int*)(base + o_logc);
   B.logmax = c->log_max;
   B.policy = pol ? (const float*)(base + o_pol) : nullptr;
-  {
+  if (!na) {
     int z4[4] = {0, 0, 0, 0};
     HIPCHK(c, hipMemcpyAsync(base + o_logc, z4, sizeof z4, hipMemcpyHostToDevice, s));
   }
@@ -439,7 +475,8 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   GmKnot* dT = (GmKnot*)(base + o_tables);
   const double* dF = (const double*)(base + o_freqs);
   HIPCHK(c, hipEventRecord(c->ev[0], s));
-  const bool pers = c->persistent != 0 && ctl_mode == 0 && !pol;   // step logging / learned control: one lane per job
+  // step logging / learned control / a policy / the batch without allocation: one lane per job
+  const bool pers = c->persistent != 0 && ctl_mode == 0 && !pol && !na;
   launch(0, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
   HIPCHK(c, hipEventRecord(c->ev[1], s));
   launch(1, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
@@ -478,8 +515,15 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
     if (gfre) HIPCHK(c, hipMemcpyAsync(gfre, base + o_gfre, 4 * nsp * nnod * n, hipMemcpyDeviceToHost, s));
     if (gfim) HIPCHK(c, hipMemcpyAsync(gfim, base + o_gfim, 4 * nsp * nnod * n, hipMemcpyDeviceToHost, s));
   }
-  std::vector<int> hstatus(n + 1);
-  HIPCHK(c, hipMemcpyAsync(hstatus.data(), base + o_status, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+  int first = 0;
+  std::vector<int> hstatus;
+  if (na) {
+    gm_noalloc_status(s, (int)n, B.status, (int*)(base + o_smax));
+    HIPCHK(c, hipMemcpyAsync(&first, base + o_smax, sizeof(int), hipMemcpyDeviceToHost, s));
+  } else {
+    hstatus.resize(n + 1);
+    HIPCHK(c, hipMemcpyAsync(hstatus.data(), base + o_status, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+  }
   unsigned long long hist[48];
   HIPCHK(c, hipMemcpyAsync(hist, base + o_hist, sizeof hist, hipMemcpyDeviceToHost, s));
   if (ctl_mode == 1) {
@@ -497,12 +541,19 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[5]));
   c->timing[5] = ms;
   memcpy(c->last_hist, hist, sizeof hist);
-  if (status) memcpy(status, hstatus.data(), sizeof(int) * n);
+  if (status && !na) memcpy(status, hstatus.data(), sizeof(int) * n);
   if (stats) derive_stats(hist, stats);
-  int first = 0;
-  for (size_t i = 0; i < n; ++i) if (hstatus[i]) { first = hstatus[i]; break; }
+  if (!na)
+    for (size_t i = 0; i < n; ++i) if (hstatus[i]) { first = hstatus[i]; break; }
   if (first) return gm_fail(c, first, "at least one (omega,l) instance failed with error %d", first);
   return GM_OK;
+}
+
+int gm_run_batch(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const int* nl, double dwn, double sigma,
+                 double eps, int attmode, int jre, int istyp, double* gf, float* gfre, float* gfim, int* status,
+                 long long* stats, const GmNoAlloc* na) {
+  return run(c, nfreq, freqs, lmin, nl, dwn, sigma, eps, attmode, jre, jre, jre, istyp, true, GM_FLAG_DEVICE_PTRS,
+             nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, gf, gfre, gfim, status, stats, na);
 }
 
 extern "C" int gm_minors_batch(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const int* nl, double sigma,

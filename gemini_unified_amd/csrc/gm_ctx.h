@@ -46,3 +46,22 @@ struct gm_ctx {
 // records the message of a failure on c (or of a failed context creation) and returns code
 int gm_fail(gm_ctx* c, int code, const char* fmt, ...);
 #define HIPCHK(c, x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return gm_fail((c), GM_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); } while (0)
+
+// Batch without allocation (gm_noalloc.hip): every buffer lives in the
+// per-device file-scope array of GM_NOALLOC_BYTES; at most GM_NOALLOC_MAXF
+// frequencies per call (the plan is built by one block).
+#define GM_NOALLOC_BYTES (128u << 20)
+#define GM_NOALLOC_MAXF 1024
+struct GmNoAlloc {
+  void* base;       // the array's address on the context's device
+  size_t bytes;     // its size
+};
+int gm_noalloc_init(int device);               // takes the array's address on the current device (once per device)
+void* gm_noalloc_scratch(int device);
+void gm_noalloc_plan(hipStream_t s, int nfreq, const double* freqs, const int* lmin, const int* nl, int* q, int* l,
+                     double* omre, int* uq, int* us, int* ue, unsigned long long* hist, int* logc, int* smax);
+void gm_noalloc_status(hipStream_t s, int n, const int* status, int* smax);
+// the Green-FK column batch of gm_gfk_batch (device outputs) in the buffers of na
+int gm_run_batch(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const int* nl, double dwn, double sigma,
+                 double eps, int attmode, int jre, int istyp, double* gf, float* gfre, float* gfim, int* status,
+                 long long* stats, const GmNoAlloc* na);

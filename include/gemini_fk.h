@@ -128,6 +128,21 @@ int gm_gfk_batch(gm_ctx* ctx, int nfreq, const double* freqs, const int* lmin, c
                  double eps, int attmode, int jre, int istyp, int flags, double* gf, float* gfre, float* gfim,
                  int* status, long long* stats);
 
+/* The Green-FK columns of gm_gfk_batch with GM_FLAG_DEVICE_PTRS, allocating
+ * nothing.  No counterpart in the reference: it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Every buffer of the batch lives in one per-device array
+ * of 128 MiB (GM_NOALLOC_BYTES) that the library holds until the process ends;
+ * a batch that needs more (about 560 B x nodes per (omega, l) plus 330 B x
+ * knots per frequency) or has more than 1024 frequencies is rejected with
+ * GM_ERR_ARG before any launch.  gf, gfre, gfim and status[k] are device
+ * pointers (status may be NULL), freqs, lmin, nl and stats host pointers.  All
+ * calls on one GPU share the array and it keeps the last call's contents:
+ * calls that overlap on one GPU race. */
+int gm_gfk_batch_noalloc(gm_ctx* ctx, int nfreq, const double* freqs, const int* lmin, const int* nl, double sigma,
+                         double eps, int attmode, int jre, int istyp, double* gf, float* gfre, float* gfim,
+                         int* status, long long* stats);
+
 /* Regional (wavenumber) mode of gm_minors_batch / gm_gfk_batch: the instances
  * of frequency q are the 1-based wavenumber indices iwn = iwnmin[q] ..
  * iwnmin[q]+nwn[q]-1 with k = (iwn-1) dwn and dll1 = (k R)^2
