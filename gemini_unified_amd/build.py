@@ -59,5 +59,31 @@ def build(force: bool = True, verbose: bool = False) -> str:
     return LIB
 
 
+# GFK file library: host C++ over the HDF5 C library (no GPU code)
+FILE_LIB = os.path.join(HERE, "libgemini_gfkfile.so")
+HDF5_DIR = os.environ.get("HDF5_DIR", "/opt/conda")
+
+
+def hdf5_available() -> bool:
+    return os.path.exists(os.path.join(HDF5_DIR, "include", "hdf5.h")) and \
+        os.path.exists(os.path.join(HDF5_DIR, "lib", "libhdf5.so"))
+
+
+def build_file_lib(verbose: bool = False):
+    """libgemini_gfkfile.so, or None when the HDF5 C library is not installed."""
+    if not hdf5_available():
+        return None
+    inc, lib = os.path.join(HDF5_DIR, "include"), os.path.join(HDF5_DIR, "lib")
+    tmp = FILE_LIB + ".tmp"
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall",
+           os.path.join(CSRC, "gm_gfkfile.cpp"), f"-I{inc}", f"-L{lib}", "-lhdf5", f"-Wl,-rpath,{lib}", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), file=sys.stderr)
+    subprocess.check_call(cmd, cwd=CSRC)
+    os.replace(tmp, FILE_LIB)
+    return FILE_LIB
+
+
 if __name__ == "__main__":
     print(build(force=True, verbose=True))
+    print(build_file_lib(verbose=True))

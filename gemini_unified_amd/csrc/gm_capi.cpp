@@ -12,11 +12,15 @@
 #include "../../include/gemini_fk.h"
 
 namespace gm_exact {
+void launch_node_props(const GmModel*, const double*, int, GmKnot*, unsigned char*, int, unsigned char*, int, double*,
+                       double*, double*, hipStream_t);
 void launch_stage(int, const GmModel*, const GmBatch&, int, const double*, int, GmKnot*, hipStream_t, int, size_t, int);
 int persist_blocks_per_cu(size_t, int);
 void launch_policy(const GmModel*, const GmBatch&, hipStream_t, size_t, float*, const float*, float*, float*, double, int);
 }
 namespace gm_fast {
+void launch_node_props(const GmModel*, const double*, int, GmKnot*, unsigned char*, int, unsigned char*, int, double*,
+                       double*, double*, hipStream_t);
 void launch_stage(int, const GmModel*, const GmBatch&, int, const double*, int, GmKnot*, hipStream_t, int, size_t, int);
 int persist_blocks_per_cu(size_t, int);
 void launch_policy(const GmModel*, const GmBatch&, hipStream_t, size_t, float*, const float*, float*, float*, double, int);
@@ -254,6 +258,28 @@ extern "C" int gm_nodes_set(gm_ctx* c, int nnod, const double* rnod, const int* 
   return GM_OK;
 }
 
+// nodes and model may be set in either order: check the nodes against the
+// model before a launch
+static int check_nodes(gm_ctx* c) {
+  const GmModel& m = c->hmodel;
+  for (int k = 1; k <= m.nnod; ++k) {
+    if (m.layer[k] < 1 || m.layer[k] > m.nlay)
+      return gm_fail(c, GM_ERR_ARG, "layer %d of node %d outside the model's %d layers", m.layer[k], k, m.nlay);
+    if (m.rnod[k] > m.rb[m.nlay] * (1.0 + 1e-12))
+      return gm_fail(c, GM_ERR_ARG, "node %d at r=%g lies above the model top %g", k, m.rnod[k], m.rb[m.nlay]);
+  }
+  return GM_OK;
+}
+
+// grows the context's device scratch to at least `bytes`
+static int grow_dbuf(gm_ctx* c, size_t bytes) {
+  if (bytes <= c->dbuf_size) return GM_OK;
+  if (c->dbuf) { HIPCHK(c, hipFree(c->dbuf)); c->dbuf = nullptr; c->dbuf_size = 0; }
+  if (hipMalloc(&c->dbuf, bytes) != hipSuccess) return gm_fail(c, GM_ERR_NOMEM, "device allocation of %zu bytes failed", bytes);
+  c->dbuf_size = bytes;
+  return GM_OK;
+}
+
 static void derive_stats(const unsigned long long* hist, long long* st, int only_sys = -1) {
   static const int seq[8] = {0, 2, 4, 6, 8, 12, 16, 24};
   static const int nx[8] = {0, 3, 2, 4, 4, 4, 8, 8};
@@ -311,13 +337,7 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   GmModel& m = c->hmodel;
   const int nnod = m.nnod;
   if (jsl < 1 || jsl > nnod || jsr < 1 || jsr > nnod) return gm_fail(c, GM_ERR_NODEIDX, "Invalid index for source node");
-  // nodes and model may be set in either order: check the nodes against the model here
-  for (int k = 1; k <= nnod; ++k) {
-    if (m.layer[k] < 1 || m.layer[k] > m.nlay)
-      return gm_fail(c, GM_ERR_ARG, "layer %d of node %d outside the model's %d layers", m.layer[k], k, m.nlay);
-    if (m.rnod[k] > m.rb[m.nlay] * (1.0 + 1e-12))
-      return gm_fail(c, GM_ERR_ARG, "node %d at r=%g lies above the model top %g", k, m.rnod[k], m.rb[m.nlay]);
-  }
+  if (const int rc = check_nodes(c)) return rc;
   if (full) {
     if (istyp != 0 && istyp != 1) return gm_fail(c, GM_ERR_ARG, "istyp must be 0 (FORCE) or 1 (MOMENT)");
     if (m.state[jre] != 0) return gm_fail(c, GM_ERR_UNSUPPORTED, "receiver node in a fluid layer is not supported");
@@ -403,11 +423,7 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
                      off, na->bytes);
     base = (char*)na->base;
   } else {
-    if (off > c->dbuf_size) {
-      if (c->dbuf) { HIPCHK(c, hipFree(c->dbuf)); c->dbuf = nullptr; c->dbuf_size = 0; }
-      if (hipMalloc(&c->dbuf, off) != hipSuccess) return gm_fail(c, GM_ERR_NOMEM, "device allocation of %zu bytes failed", off);
-      c->dbuf_size = off;
-    }
+    if (const int rc = grow_dbuf(c, off)) return rc;
     base = (char*)c->dbuf;
   }
   hipStream_t s = c->stream;
@@ -588,6 +604,44 @@ extern "C" int gm_gfk_batch_wn(gm_ctx* c, int nfreq, const double* freqs, const 
   if (!(dwn > 0.0)) return gm_fail(c, GM_ERR_ARG, "dwn must be positive");
   return run(c, nfreq, freqs, iwnmin, nwn, dwn, sigma, eps, attmode, jre, jre, jre, istyp, true, flags, nullptr,
              nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, gf, gfre, gfim, status, stats);
+}
+
+// elasticConstantsExternalRadialNodes (externalRadialNodes.f90:422-433) for
+// any frequency and attenuation mode: spline table of f on the device, then
+// its evaluation at every node.
+extern "C" int gm_node_properties(gm_ctx* c, double f, int attmode, double* ro, double* elcon, double* qinv) {
+  if (!c) return GM_ERR_ARG;
+  if (!c->has_model || !c->has_nodes) return gm_fail(c, GM_ERR_ARG, "model and nodes must be set first");
+  if (!ro || !elcon || !qinv) return gm_fail(c, GM_ERR_ARG, "output arrays must not be NULL");
+  if (attmode < 0 || attmode > 3) return gm_fail(c, GM_ERR_ARG, "invalid attenuation mode %d", attmode);
+  if (!(f > 0.0) || !std::isfinite(f)) return gm_fail(c, GM_ERR_ARG, "frequency must be positive");
+  if (const int rc = check_nodes(c)) return rc;
+  const GmModel& m = c->hmodel;
+  const int nnod = m.nnod;
+  size_t off = 0;
+  auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~size_t(255); return o; };
+  const size_t lstride = gmImageBytes<GmKnotL>(m.nk, m.nlay), tstride = gmImageBytes<GmKnotT>(m.nk, m.nlay);
+  const size_t o_model = take(sizeof(GmModel)), o_f = take(sizeof(double));
+  const size_t o_tab = take(sizeof(GmKnot) * (size_t)(m.nk + 1));
+  const size_t o_lt = take(lstride), o_tt = take(tstride);
+  const size_t o_ro = take(sizeof(double) * nnod), o_el = take(sizeof(double) * 14 * nnod);
+  const size_t o_q = take(sizeof(double) * 2 * nnod);
+  HIPCHK(c, hipSetDevice(c->device));
+  if (const int rc = grow_dbuf(c, off)) return rc;
+  char* base = (char*)c->dbuf;
+  hipStream_t s = c->stream;
+  HIPCHK(c, hipMemcpyAsync(base + o_model, &m, sizeof(GmModel), hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(base + o_f, &f, sizeof(double), hipMemcpyHostToDevice, s));
+  auto launch = c->variant == 0 ? gm_exact::launch_node_props : gm_fast::launch_node_props;
+  launch((const GmModel*)(base + o_model), (const double*)(base + o_f), attmode, (GmKnot*)(base + o_tab),
+         (unsigned char*)(base + o_lt), (int)lstride, (unsigned char*)(base + o_tt), (int)tstride,
+         (double*)(base + o_ro), (double*)(base + o_el), (double*)(base + o_q), s);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(ro, base + o_ro, sizeof(double) * nnod, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(elcon, base + o_el, sizeof(double) * 14 * nnod, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(qinv, base + o_q, sizeof(double) * 2 * nnod, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  return GM_OK;
 }
 
 extern "C" int gm_last_system_stats(gm_ctx* c, int sys, long long* stats) {

@@ -87,6 +87,16 @@ module geminiFKc
          type(c_ptr), value :: state
          type(c_ptr), value :: layer
       end function gm_nodes_set
+      integer(c_int) function gm_node_properties(ctx, f, attmode, ro, elcon, qinv) bind(C, &
+          name='gm_node_properties')
+         import
+         type(c_ptr), value :: ctx
+         real(c_double), value :: f
+         integer(c_int), value :: attmode
+         type(c_ptr), value :: ro
+         type(c_ptr), value :: elcon
+         type(c_ptr), value :: qinv
+      end function gm_node_properties
       integer(c_int) function gm_minors_batch(ctx, nfreq, freqs, lmin, nl, sigma, eps, attmode, jsl, jsr, &
           flags, u_tor, d_tor, dettor, u_sph, d_sph, ulg, dlg, detsph, status, stats) bind(C, &
           name='gm_minors_batch')

@@ -202,12 +202,12 @@ class GreenFKEngine:
         return out
 
     def gfk(self, freqs, lmin, nl, sigma, eps, jre, istyp=1, attmode="ATTENUATION_AND_DISPERSION",
-            want_fp32=True, raise_on_error=True, dwn=None):
+            want_fp32=True, raise_on_error=True, dwn=None, want_gf=True):
         """Green-FK columns; gf[node, isp, k] complex128 (and FP32 re/im as written to the file)."""
         freqs, lmin, nl, nk = self._plan(freqs, lmin, nl)
         nn = self.nodes.nnod
         nsp = 10 if istyp == 1 else 5
-        gf = np.zeros((nn, nsp, nk), np.complex128)
+        gf = np.zeros((nn, nsp, nk), np.complex128) if want_gf else None
         gfre = np.zeros((nn, nsp, nk), np.float32) if want_fp32 else None
         gfim = np.zeros((nn, nsp, nk), np.float32) if want_fp32 else None
         status = np.zeros(nk, np.int32)
@@ -218,17 +218,30 @@ class GreenFKEngine:
         if dwn:
             rc = _lib.lib().gm_gfk_batch_wn(self.h, freqs.size, freqs.ctypes.data_as(D), lmin.ctypes.data_as(I),
                                             nl.ctypes.data_as(I), float(dwn), float(sigma), float(eps), am, int(jre),
-                                            int(istyp), 0, gf.ctypes.data_as(D), fre, fim, status.ctypes.data_as(I),
+                                            int(istyp), 0, _dp(gf), fre, fim, status.ctypes.data_as(I),
                                             st.ctypes.data_as(LL))
         else:
             rc = _lib.lib().gm_gfk_batch(self.h, freqs.size, freqs.ctypes.data_as(D), lmin.ctypes.data_as(I),
                                          nl.ctypes.data_as(I), float(sigma), float(eps), am, int(jre), int(istyp), 0,
-                                         gf.ctypes.data_as(D), fre, fim, status.ctypes.data_as(I),
+                                         _dp(gf), fre, fim, status.ctypes.data_as(I),
                                          st.ctypes.data_as(LL))
         self.stats += st
         if raise_on_error:
             check(self.h, rc)
         return dict(gf=gf, gfre=gfre, gfim=gfim, status=status, stats=st)
+
+    def node_properties(self, f=None, attmode="ELASTIC"):
+        """elasticConstantsExternalRadialNodes (externalRadialNodes.f90:422-433): density ro[nnod],
+        complex moduli elcon[nnod, 7] (A, C, F, L, N, kappa, mu) and inverse Q qinv[nnod, 2] at the
+        nodes, for frequency f (default: the model's reference frequency) and attenuation mode."""
+        n = self.nodes.nnod
+        ro, el, q = np.zeros(n), np.zeros((n, 7), np.complex128), np.zeros((n, 2))
+        f = self.nem.fref if f is None else f
+        rc = _lib.lib().gm_node_properties(self.h, float(f), ATTMODES[attmode] if isinstance(attmode, str) else
+                                           int(attmode), ro.ctypes.data_as(D), el.ctypes.data_as(D),
+                                           q.ctypes.data_as(D))
+        check(self.h, rc)
+        return ro, el, q
 
     def compute_green_fk_spectra_for_synthetics(self, grid: FrequencyGrid, jre: int, eps: float, istyp: int = 1,
                                                 attmode="ATTENUATION_AND_DISPERSION", freq_block: int = 64):

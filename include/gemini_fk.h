@@ -99,6 +99,14 @@ int gm_model_set(gm_ctx* ctx, int nk, const double* rk, const double* ro, const 
  * Replaces the external_radial_nodes object (f90/externalRadialNodes.f90:62-72). */
 int gm_nodes_set(gm_ctx* ctx, int nnod, const double* rnod, const int* state, const int* layer);
 
+/* Density ro[nnod], complex moduli elcon[nnod][7] (A, C, F, L, N, kappa, mu)
+ * and inverse Q qinv[nnod][2] (QK, QM) at the nodes for frequency f and
+ * attenuation mode attmode, host arrays.  Replaces
+ * elasticConstantsExternalRadialNodes (f90/externalRadialNodes.f90:422-433),
+ * which the reference calls with f = fref, 'ELASTIC' for the GFK file's
+ * externalNodes group (:208-265). */
+int gm_node_properties(gm_ctx* ctx, double f, int attmode, double* ro, double* elcon, double* qinv);
+
 /* Toroidal and spheroidal minors for all (f_q, l), l = lmin[q] .. lmin[q]+nl[q]-1,
  * with omega = 2*pi32*f_q - i*sigma and dll1 = (sqrt(l(l+1))/R*R)^2.
  * Replaces computeToroidalMinors (f90/toroidalMinors.f90:71) and

@@ -41,6 +41,7 @@ def lib():
             fn.argtypes = [C.c_void_p, C.c_int, D, I, I, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                            C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, F, C.c_void_p, C.c_int, C.c_int,
                            C.c_int] + [D] * 9 + [I, U, F, I, F]
+            getattr(L, "sim_run_" + v + "_node_props").argtypes = [C.c_void_p, C.c_double, C.c_int, D, D, D]
         _lib = L
     return _lib
 
@@ -70,6 +71,15 @@ class HostSim:
                 lib().sim_proposer_free(self.prop)
         except Exception:
             pass
+
+    def node_properties(self, f, attmode=0):
+        """k_spline + k_node_props on the CPU: ro[nnod], elcon[nnod, 7] complex, qinv[nnod, 2]."""
+        n = self.nodes.nnod
+        ro, el, q = np.zeros(n), np.zeros((n, 7), np.complex128), np.zeros((n, 2))
+        rc = getattr(lib(), "sim_run_" + self.variant + "_node_props")(
+            self.m, float(f), int(attmode), ro.ctypes.data_as(D), el.ctypes.data_as(D), q.ctypes.data_as(D))
+        assert rc == 0
+        return ro, el, q
 
     def set_proposer(self, params, norm, safety=0.8, kcheck=False, every=1, mode="step"):
         """mode "step": learned control during the integration (runs with learned=True); "policy": the

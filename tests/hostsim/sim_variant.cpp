@@ -37,3 +37,18 @@ extern "C" int SIM_ENTRY(const GmModel* m, int nfreq, const double* freqs, const
   }
   return 0;
 }
+
+// node properties for frequency f (gm_node_properties): the spline kernel on
+// one frequency, then the node evaluation kernel
+#define SIM_CAT2(a, b) a##b
+#define SIM_CAT(a, b) SIM_CAT2(a, b)
+extern "C" int SIM_CAT(SIM_ENTRY, _node_props)(const GmModel* m, double f, int attmode, double* ro, double* elcon,
+                                               double* qinv) {
+  using namespace SIM_NS;
+  const size_t lstride = gmImageBytes<GmKnotL>(m->nk, m->nlay), tstride = gmImageBytes<GmKnotT>(m->nk, m->nlay);
+  std::vector<GmKnot> tables((size_t)(m->nk + 1));
+  std::vector<unsigned char> ltab(lstride), ttab(tstride);
+  grid(1, [&] { k_spline(m, &f, attmode, tables.data(), ltab.data(), (int)lstride, ttab.data(), (int)tstride); });
+  grid(1, [&] { k_node_props(m, tables.data(), ro, elcon, qinv); });
+  return 0;
+}

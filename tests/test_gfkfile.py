@@ -1,0 +1,252 @@
+"""GFK file writer and reader (libgemini_gfkfile.so), on the CPU.
+
+* The library exports what include/gemini_gfkfile.h declares, with matching
+  ctypes prototypes.
+* Round trip: spectra written in frequency blocks of any order read back per
+  node as the reference's reader returns them, (nkfmax, nsp) in Fortran order;
+  meta data, strings and node arrays come back as written (reals as float32).
+* Layout pinned independently of this library: h5dump (the HDF5 command line
+  tool) shows the datasets, types and string padding the Fortran writer makes,
+  and single values at (node, spectrum, k) in C order sit where the Fortran
+  index (k, spectrum, node) puts them.
+* The node properties written are the kernel source's (CPU build), which equal
+  the oracle's spline evaluation.
+* Errors: missing file, wrong sizes and ranges come back as codes.
+"""
+import ctypes
+import os
+import re
+import shutil
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from gemini_unified_amd import ExternalRadialNodes, FrequencyGrid, global_model
+from gemini_unified_amd import build as gbuild
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+pytestmark = pytest.mark.skipif(not os.path.exists(gbuild.FILE_LIB), reason="libgemini_gfkfile.so not built (no HDF5)")
+
+from gemini_unified_amd import gfkfile  # noqa: E402
+
+H5DUMP = shutil.which("h5dump") or ("/opt/conda/bin/h5dump" if os.path.exists("/opt/conda/bin/h5dump") else None)
+
+
+def declared():
+    txt = open(os.path.join(ROOT, "include", "gemini_gfkfile.h")).read()
+    return re.findall(r"^int (gm_\w+)\(([^;]*?)\);", txt, flags=re.M | re.S)
+
+
+def test_header_matches_binding():
+    names = sorted(n for n, _ in declared())
+    assert names == sorted(gfkfile.EXPORTS)
+    L = gfkfile.lib()
+    raw = ctypes.CDLL(gbuild.FILE_LIB)
+    for name, params in declared():
+        assert hasattr(raw, name), name
+        assert len(getattr(L, name).argtypes) == params.count(",") + 1, name
+
+
+def _setup(nf1=3, nf2=9, seed=0):
+    nem = global_model(1, 200)
+    nodes = ExternalRadialNodes.from_blocks(nem, [5], [40000.0])
+    rng = np.random.default_rng(seed)
+    nwn = np.zeros(nf2, np.int32)
+    nwn[nf1 - 1:] = rng.integers(3, 17, nf2 - nf1 + 1)
+    nsp = 10
+    koff = np.concatenate(([0], np.cumsum(nwn)))
+    full = rng.standard_normal((nodes.nnod, nsp, int(nwn.sum()))).astype(np.float32)
+    fulli = rng.standard_normal(full.shape).astype(np.float32)
+    return nem, nodes, nwn, nsp, koff, full, fulli
+
+
+def _write(path, nem, nodes, nwn, nsp, koff, full, fulli, nf1, blocks, identity="Thu Dec  4 10:00:00 2026"):
+    nf2 = nwn.size
+    ro = np.linspace(3000.0, 5000.0, nodes.nnod)
+    q = np.stack([np.full(nodes.nnod, 1e-3), np.linspace(1e-3, 1e-2, nodes.nnod)], axis=1)
+    el = (np.arange(7 * nodes.nnod).reshape(nodes.nnod, 7) * 1.1e9 + 1j * 3.3e7)
+    with gfkfile.GfkFileWriter(path) as w:
+        w.write_identity(identity)
+        w.write_meta("PREM-like test model", "ATTENUATION_AND_DISPERSION",
+                     (nem.rearth, nodes.rnod[-1], 5 / 2048.0, 1 / 2048.0, 2 * np.pi / 8e6, nem.fref),
+                     (nf1, nf2, int(nwn.sum()), int(nwn.max()), 2, 0, 1, 1), nwn)
+        w.write_nodes(nodes, ro, q, el)
+        w.create_spectra((1, 4, 2, 4, 2), nsp, nodes.nnod, nf1, nf2, nwn)
+        for jf1, jf2 in blocks:
+            a, b = koff[jf1 - 1], koff[jf2]
+            w.write_frequencies(jf1, jf2, full[:, :, a:b], fulli[:, :, a:b])
+    return ro, q, el
+
+
+def test_round_trip(tmp_path):
+    nf1 = 3
+    nem, nodes, nwn, nsp, koff, full, fulli = _setup(nf1)
+    path = str(tmp_path / "gfk.MOMENT")
+    blocks = [(7, 9), (3, 3), (5, 6), (4, 4)]          # any order, like the interleaved frequency split
+    ro, q, el = _write(path, nem, nodes, nwn, nsp, koff, full, fulli, nf1, blocks)
+    g = gfkfile.GreenFKSpectra.open(path)
+    for p in ("earthModelName", "attenuationMode"):
+        g.read_meta(p)
+    assert g.identity == "Thu Dec  4 10:00:00 2026"
+    assert g.earth_model_name == "PREM-like test model"
+    assert g.attnmode == "ATTENUATION_AND_DISPERSION"
+    f32 = lambda x: float(np.float32(x))  # noqa: E731
+    assert (g.rearth, g.rse, g.sigma, g.df, g.fref) == (f32(nem.rearth), f32(nodes.rnod[-1]), f32(5 / 2048.0),
+                                                        f32(1 / 2048.0), f32(nem.fref))
+    assert g.dwn == f32(2 * np.pi / 8e6)
+    assert (g.nf1, g.nf2, g.nkfmax, g.nwnmax, g.numtasks, g.global_) == (nf1, nwn.size, int(nwn.sum()),
+                                                                        int(nwn.max()), 1, 1)
+    assert (g.istyp, g.ncsph, g.nctor, g.njsph, g.njtor) == (1, 4, 2, 4, 2)
+    np.testing.assert_array_equal(g.dsvmask, [1, 0, 1, 0, 0, 0, 1, 0, 0])
+    assert g.number_of_spectra() == nsp == gfkfile.number_of_spectra((nf1, 9, 0, 0, 2, 0, 1, 1), (1, 4, 2, 4, 2))
+    np.testing.assert_array_equal(g.nwn, nwn)
+    np.testing.assert_array_equal(g.exnod.rnod, nodes.rnod.astype(np.float32))
+    np.testing.assert_array_equal(g.exnod.state, nodes.state)
+    np.testing.assert_array_equal(g.exnod.layer, nodes.layer)
+    assert g.exnod.rearth == f32(nodes.rearth)
+    np.testing.assert_array_equal(g.node_ro, ro.astype(np.float32))
+    np.testing.assert_array_equal(g.node_qinv, q.astype(np.float32))
+    np.testing.assert_array_equal(g.node_elcon.real, el.real.astype(np.float32))
+    np.testing.assert_array_equal(g.node_elcon.imag, el.imag.astype(np.float32))
+    for jr in range(1, nodes.nnod + 1):
+        gr, gi = g.read_data_node(jr)
+        np.testing.assert_array_equal(gr, full[jr - 1])
+        np.testing.assert_array_equal(gi, fulli[jr - 1])
+    # a wavenumber range: the spectra of one frequency
+    jf = 6
+    gr, gi = g.read_data_node(2, int(koff[jf - 1]), int(nwn[jf - 1]))
+    np.testing.assert_array_equal(gr, full[1][:, koff[jf - 1]:koff[jf]])
+    # the node closest to a radius among the file's single precision radii
+    assert g.node_index(nodes.rnod[2] + 1.0) == 3
+    assert g.node_index(0.0) == 1
+    assert g.node_index(1e9) == nodes.nnod
+    g2 = gfkfile.GreenFKSpectra(path)
+    assert g.check_identity(g2)
+    g.close()
+    g2.close()
+
+
+def test_unwritten_frequencies_read_zero(tmp_path):
+    nf1 = 3
+    nem, nodes, nwn, nsp, koff, full, fulli = _setup(nf1)
+    path = str(tmp_path / "gfk.MOMENT")
+    _write(path, nem, nodes, nwn, nsp, koff, full, fulli, nf1, [(5, 5)])
+    with gfkfile.GreenFKSpectra.open(path) as g:
+        gr, _ = g.read_data_node(1)
+    a, b = koff[4], koff[5]
+    np.testing.assert_array_equal(gr[:, a:b], full[0][:, a:b])
+    assert not gr[:, :a].any() and not gr[:, b:].any()
+
+
+@pytest.mark.skipif(H5DUMP is None, reason="h5dump not installed")
+def test_layout_seen_by_h5dump(tmp_path):
+    """Dataset shapes, element types and string padding as the HDF5 tool reports them, and values
+    addressed in C order (node, spectrum, k) = Fortran (k, spectrum, node)."""
+    nf1 = 3
+    nem, nodes, nwn, nsp, koff, full, fulli = _setup(nf1)
+    path = str(tmp_path / "gfk.MOMENT")
+    _write(path, nem, nodes, nwn, nsp, koff, full, fulli, nf1, [(3, 9)])
+    hdr = subprocess.run([H5DUMP, "-H", path], capture_output=True, text=True, check=True).stdout
+    nk = int(nwn.sum())
+
+    def block(name):
+        i = hdr.index(f'"{name}"')
+        return hdr[i:hdr.index("}", hdr.index("DATASPACE", i)) + 1]
+    for name in ("GreenFKSpectraReal", "GreenFKSpectraImag"):
+        b = block(name)
+        assert "H5T_IEEE_F32LE" in b
+        assert f"( {nodes.nnod}, {nsp}, {nk} )" in b
+    assert "H5T_IEEE_F32LE" in block("reals") and "( 6 )" in block("reals")
+    assert "H5T_STD_I32LE" in block("integers") and "( 8 )" in block("integers")
+    assert f"( {nwn.size} )" in block("numberOfWavenumbersForFrequency")
+    assert "( 5 )" in block("dataSpecificIntegers")
+    for name in ("inverseQ", "realElcon"):
+        assert f"( {nodes.nnod}, {2 if name == 'inverseQ' else 7} )" in block(name)
+    ident = hdr[hdr.index('ATTRIBUTE "identity"'):]
+    assert "H5T_STR_SPACEPAD" in ident[:400] and "STRSIZE 24;" in ident[:400]
+    em = block("earthModelName")
+    assert "H5T_STR_SPACEPAD" in em and f"STRSIZE {len('PREM-like test model')};" in em
+    rng = np.random.default_rng(3)
+    for _ in range(4):
+        jr, isp, k = int(rng.integers(nodes.nnod)), int(rng.integers(nsp)), int(rng.integers(nk))
+        out = subprocess.run([H5DUMP, "-d", "GreenFKSpectraReal", "-s", f"{jr},{isp},{k}", "-c", "1,1,1", "-m",
+                              "%.9g", path], capture_output=True, text=True, check=True).stdout
+        val = float(re.search(r"\(\d+,\d+,\d+\):\s*([-0-9.eE+]+)", out).group(1))
+        assert np.float32(val) == full[jr, isp, k]
+
+
+def test_node_properties_of_the_kernels_round_trip(tmp_path):
+    """Node properties computed by the kernel source on the CPU (k_spline + k_node_props, equal to
+    the oracle's evaluation bit for bit) written and read back as float32."""
+    sys.path.insert(0, os.path.join(ROOT, "tests", "hostsim"))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from hostsim import HostSim
+    import oracle
+    nem = global_model(1, 200)
+    nodes = ExternalRadialNodes.from_blocks(nem, [10], [200000.0])
+    ro, el, q = HostSim(nem, nodes, "exact").node_properties(nem.fref, 0)
+    orc = oracle.Oracle(nem, nodes)
+    orc.set_frequency(nem.fref, 0)
+    ref = [orc.eval_elcon(r, nl) for r, nl in zip(nodes.rnod, nodes.layer)]
+    np.testing.assert_array_equal(ro, [x[0] for x in ref])
+    path = str(tmp_path / "nodes.h5")
+    with gfkfile.GfkFileWriter(path) as w:
+        w.write_nodes(nodes, ro, q, el)
+    g = gfkfile.GreenFKSpectra(path)
+    g.read_meta("externalNodes")
+    np.testing.assert_array_equal(g.node_elcon.real, np.array([x[1] for x in ref]).real.astype(np.float32))
+    np.testing.assert_array_equal(g.node_qinv, np.array([x[2] for x in ref]).astype(np.float32))
+    g.close()
+
+
+def test_aski_meta_file(tmp_path):
+    """The ASKI variant's meta file: sourceNodeRadii and integers with derivflag = 1 (UP, VP, WP)."""
+    path = str(tmp_path / "gfk.meta")
+    nwn = np.array([0, 4, 5], np.int32)
+    with gfkfile.GfkFileWriter(path) as w:
+        w.write_identity("id")
+        w.write_meta("m", "ELASTIC", (6371e3, 6371e3, 0.1, 0.02, 0.0, 1.0), (2, 3, 9, 5, 2, 1, 4, 1), nwn)
+        w.write_source_nodes([6.0e6, 6.1e6, 6.2e6])
+    g = gfkfile.GreenFKSpectra(path)
+    for p in ("integers", "sourceNodeRadii", "numberOfWavenumbersForFrequency"):
+        g.read_meta(p)
+    np.testing.assert_array_equal(g.rsnod, np.float32([6.0e6, 6.1e6, 6.2e6]))
+    np.testing.assert_array_equal(g.dsvmask, [1, 0, 1, 0, 1, 1, 1, 0, 1])
+    np.testing.assert_array_equal(g.nwn, nwn)
+    # ASKI data file of a moment source: U, V, UP, VP x 4 jumps + W, WP x 2 jumps
+    assert gfkfile.number_of_spectra((2, 3, 9, 5, 2, 1, 4, 1), (1, 4, 2, 4, 2)) == 20
+    # positions of the spectra (positionDataGreenFKSpectra)
+    g.njsph, g.njtor = 4, 2
+    assert g.position(1, 1) == 1 and g.position(3, 2) == 6 and g.position(5, 1) == 9 and g.position(7, 1) == 17
+    g.close()
+
+
+def test_errors(tmp_path):
+    with pytest.raises(gfkfile.GfkFileError) as e:
+        gfkfile.GreenFKSpectra(str(tmp_path / "missing.h5"))
+    assert e.value.code == 12
+    nf1 = 3
+    nem, nodes, nwn, nsp, koff, full, fulli = _setup(nf1)
+    w = gfkfile.GfkFileWriter(str(tmp_path / "bad.h5"))
+    with pytest.raises(gfkfile.GfkFileError) as e:      # nkfmax inconsistent with nwn
+        w.write_meta("m", "a", np.zeros(6), (nf1, nwn.size, int(nwn.sum()) + 1, 1, 2, 0, 1, 1), nwn)
+    assert e.value.code == 9
+    with pytest.raises(gfkfile.GfkFileError) as e:      # spectra before their datasets
+        w.write_frequencies(3, 3, full[:, :, :nwn[2]], fulli[:, :, :nwn[2]])
+    w.create_spectra((1, 4, 2, 4, 2), nsp, nodes.nnod, nf1, nwn.size, nwn)
+    with pytest.raises(gfkfile.GfkFileError) as e:      # below nf1
+        w.write_frequencies(2, 3, full[:, :, :nwn[2]], fulli[:, :, :nwn[2]])
+    assert e.value.code == 9
+    with pytest.raises(ValueError):
+        w.write_frequencies(3, 4, full[:, :, :nwn[2]], fulli[:, :, :nwn[2]])
+    w.close()
+    g = gfkfile.GreenFKSpectra.open(str(tmp_path / "bad.h5"), paths=("dataSpecificIntegers",))
+    g.nkfmax = int(nwn.sum())
+    g.dsvmask = np.array([1, 0, 1, 0, 0, 0, 1, 0, 0])
+    with pytest.raises(gfkfile.GfkFileError):
+        g.read_data_node(nodes.nnod + 1)
+    with pytest.raises(gfkfile.GfkFileError):
+        g.read_meta("sourceNodeRadii")
+    g.close()

@@ -1,0 +1,102 @@
+"""GFK file written from the GPU batches (write_green_fk_spectra_for_synthetics)
+and read back per node as computeSyntheticSeismograms reads it, against the
+oracle: exact variant, so the FP32 file values are the oracle's FP64 columns
+rounded, bit for bit; node properties at the reference frequency equal the
+oracle's spline evaluation bit for bit."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from gemini_unified_amd import ExternalRadialNodes, FrequencyGrid, GreenFKEngine, MC_PI, global_model, regional_model
+from gemini_unified_amd import build as gbuild
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not os.path.exists(gbuild.FILE_LIB), reason="libgemini_gfkfile.so not built")]
+
+from gemini_unified_amd import gfkfile  # noqa: E402
+
+
+def test_node_properties_match_oracle():
+    nem = global_model(1, 200)
+    prn = np.concatenate(([50000.0], nem.rearth - 150000.0 * np.arange(40, -1, -1)))
+    nodes = ExternalRadialNodes.split_at_discos(nem, prn, nem.rearth)
+    orc = oracle.Oracle(nem, nodes)
+    for variant in ("exact", "fast"):
+        eng = GreenFKEngine(nem, nodes, variant=variant)
+        for f, am in ((nem.fref, 0), (0.037, 3)):
+            print(f"node properties variant={variant} f={f} attmode={am}", flush=True)
+            ro, el, q = eng.node_properties(f, am)
+            orc.set_frequency(f, am)
+            ref = [orc.eval_elcon(r, nl) for r, nl in zip(nodes.rnod, nodes.layer)]
+            if variant == "exact":
+                np.testing.assert_array_equal(ro, [x[0] for x in ref])
+                np.testing.assert_array_equal(el, [x[1] for x in ref])
+                np.testing.assert_array_equal(q, [x[2] for x in ref])
+            else:
+                np.testing.assert_allclose(ro, [x[0] for x in ref], rtol=1e-13)
+                np.testing.assert_allclose(q, [x[2] for x in ref], rtol=1e-12, atol=1e-15)
+        eng.close()
+
+
+def test_global_file_matches_oracle(tmp_path):
+    nem = global_model(1, 200)
+    nodes = ExternalRadialNodes.from_blocks(nem, [2], [30000.0])
+    jre = nodes.node_index(nem.rearth)
+    tlen = 2048.0
+    grid = FrequencyGrid(df=1 / tlen, nf1=20, nf2=24, sigma=5 / tlen, nwn=np.array([30, 31, 32, 33, 34]))
+    eng = GreenFKEngine(nem, nodes, variant="exact")
+    path = str(tmp_path / "gfk_glob.MOMENT")
+    print("global GFK file, 5 frequencies", flush=True)
+    ident = gfkfile.write_green_fk_spectra_for_synthetics(eng, path, grid, jre, 1e-6, istyp=1,
+                                                          earth_model_name="test", freq_block=2)
+    g = gfkfile.GreenFKSpectra.open(path)
+    assert g.identity == ident and g.nkfmax == int(grid.nwn.sum()) and g.number_of_spectra() == 10
+    assert g.rse == float(np.float32(nodes.rnod[jre - 1]))
+    orc = oracle.Oracle(nem, nodes)
+    ref = {}
+    for i, f in enumerate(grid.freqs):
+        orc.set_frequency(f, 3)
+        rc, gf, _ = orc.gfk_degrees(2.0 * MC_PI * f, -grid.sigma, 0, int(grid.nwn[i]), jre, 1, 1e-6)
+        assert rc == 0
+        ref[i] = gf
+    for jr in range(1, nodes.nnod + 1):
+        gr, gi = g.read_data_node(jr)
+        want_r = np.concatenate([ref[i][jr - 1].real for i in range(grid.freqs.size)], axis=1).astype(np.float32)
+        want_i = np.concatenate([ref[i][jr - 1].imag for i in range(grid.freqs.size)], axis=1).astype(np.float32)
+        np.testing.assert_array_equal(gr, want_r)
+        np.testing.assert_array_equal(gi, want_i)
+    # the external nodes' properties at the reference frequency, elastic
+    orc.set_frequency(nem.fref, 0)
+    ro_ref = np.array([orc.eval_elcon(r, nl)[0] for r, nl in zip(nodes.rnod, nodes.layer)])
+    np.testing.assert_array_equal(g.node_ro, ro_ref.astype(np.float32))
+    g.close()
+    eng.close()
+
+
+def test_regional_file_matches_oracle(tmp_path):
+    nem = regional_model(6)
+    nodes = ExternalRadialNodes.from_blocks(nem, [4], [5000.0])
+    jre = nodes.node_index(nem.rearth)
+    grid = FrequencyGrid.regional_fixed(tlen=64.0, fmax=2.0, nwn=14, xlen=8.0e6, fmin=1.5)
+    eng = GreenFKEngine(nem, nodes, variant="exact")
+    path = str(tmp_path / "gfk_reg.MOMENT")
+    print(f"regional GFK file, {grid.freqs.size} frequencies", flush=True)
+    gfkfile.write_green_fk_spectra_for_synthetics(eng, path, grid, jre, 1e-6, global_=0, freq_block=8)
+    g = gfkfile.GreenFKSpectra.open(path)
+    assert g.global_ == 0 and g.dwn == float(np.float32(grid.dwn))
+    orc = oracle.Oracle(nem, nodes)
+    gr, gi = g.read_data_node(1)
+    off = 0
+    for i, f in enumerate(grid.freqs):
+        n = int(grid.nwn[i])
+        assert not gr[:, off].any() and not gi[:, off].any()          # k = 0 written as zero
+        orc.set_frequency(f, 3)
+        rc, gf, _ = orc.gfk_wavenumbers(2.0 * MC_PI * f, -grid.sigma, 2, n - 1, grid.dwn, jre, 1, 1e-6)
+        assert rc == 0
+        np.testing.assert_array_equal(gr[:, off + 1:off + n], gf[0].real.astype(np.float32))
+        np.testing.assert_array_equal(gi[:, off + 1:off + n], gf[0].imag.astype(np.float32))
+        off += n
+    g.close()
+    eng.close()

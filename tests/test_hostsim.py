@@ -180,3 +180,37 @@ def test_policy_and_order_target_beat_the_reference_rule_at_equal_error():
     assert set(steps) == {"reference", "order7", "policy"}
     assert steps["order7"] < 0.65 * steps["reference"]
     assert steps["policy"] < 0.65 * steps["reference"]
+
+
+@pytest.mark.parametrize("model,blocks,f,attmode", [
+    ("global", ([40], [150000.0]), None, 0),
+    ("global", ([40], [150000.0]), 0.037, 3),
+    ("regional", ([12], [4000.0]), 1.3, 1),
+])
+def test_node_properties_match_oracle(model, blocks, f, attmode):
+    """k_node_props (the GFK file's externalNodes group) against the oracle's spline evaluation:
+    bitwise in the exact variant, nodes in every layer including the central sphere."""
+    nem = global_model(1, 200) if model == "global" else regional_model(6)
+    nodes = ExternalRadialNodes.from_blocks(nem, *blocks)
+    if model == "global":        # one node in the central sphere (layer 1, below 100 km)
+        prn = np.concatenate(([50000.0], nem.rearth - 150000.0 * np.arange(40, -1, -1)))
+        nodes = ExternalRadialNodes.split_at_discos(nem, prn, nem.rearth)
+    f = nem.fref if f is None else f
+    orc = oracle.Oracle(nem, nodes)
+    orc.set_frequency(f, attmode)
+    ref = [orc.eval_elcon(r, nl) for r, nl in zip(nodes.rnod, nodes.layer)]
+    ro_ref = np.array([x[0] for x in ref])
+    el_ref = np.array([x[1] for x in ref])
+    q_ref = np.array([x[2] for x in ref])
+    if model == "global":
+        assert set(nodes.layer.tolist()) == set(range(1, nem.nlay + 1))
+    for variant in ("exact", "fast"):
+        ro, el, q = HostSim(nem, nodes, variant).node_properties(f, attmode)
+        if variant == "exact":
+            np.testing.assert_array_equal(ro, ro_ref)
+            np.testing.assert_array_equal(el, el_ref)
+            np.testing.assert_array_equal(q, q_ref)
+        else:
+            np.testing.assert_allclose(ro, ro_ref, rtol=1e-13)
+            np.testing.assert_allclose(el, el_ref, rtol=1e-12, atol=1e-12 * np.abs(el_ref).max())
+            np.testing.assert_allclose(q, q_ref, rtol=1e-12, atol=1e-15)
