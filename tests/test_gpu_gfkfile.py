@@ -30,9 +30,16 @@ def test_node_properties_match_oracle():
             ro, el, q = eng.node_properties(f, am)
             orc.set_frequency(f, am)
             ref = [orc.eval_elcon(r, nl) for r, nl in zip(nodes.rnod, nodes.layer)]
-            if variant == "exact":
+            if variant == "exact" and am == 0:
+                # elastic moduli at fref (the file's case) take no transcendental function
                 np.testing.assert_array_equal(ro, [x[0] for x in ref])
                 np.testing.assert_array_equal(el, [x[1] for x in ref])
+                np.testing.assert_array_equal(q, [x[2] for x in ref])
+            elif variant == "exact":
+                # attenuation and dispersion go through the device's pow/atan/sin/cos, which may differ
+                # from the host's libm in the last bit; the spline arithmetic after them is the oracle's
+                np.testing.assert_array_equal(ro, [x[0] for x in ref])
+                np.testing.assert_allclose(el, [x[1] for x in ref], rtol=1e-15)
                 np.testing.assert_array_equal(q, [x[2] for x in ref])
             else:
                 np.testing.assert_allclose(ro, [x[0] for x in ref], rtol=1e-13)
