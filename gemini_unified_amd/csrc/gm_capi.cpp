@@ -340,10 +340,17 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   if (const int rc = check_nodes(c)) return rc;
   if (full) {
     if (istyp != 0 && istyp != 1) return gm_fail(c, GM_ERR_ARG, "istyp must be 0 (FORCE) or 1 (MOMENT)");
+    // unitJumpGreenFunctions.f90:241-298 (receiver in the ocean) is not built; the receiver must lie in
+    // the solid mantle, the other nodes anywhere (inner core, fluid core, mantle, ocean)
     if (m.state[jre] != 0) return gm_fail(c, GM_ERR_UNSUPPORTED, "receiver node in a fluid layer is not supported");
-    for (int k = 1; k <= nnod; ++k)
-      if (m.state[k] != 0 || !(m.rnod[k] > m.roc))
-        return gm_fail(c, GM_ERR_UNSUPPORTED, "node %d lies in the core or a fluid layer; only mantle nodes are supported", k);
+    if (m.hascore && !(m.rnod[jre] > m.roc))
+      return gm_fail(c, GM_ERR_UNSUPPORTED, "receiver node below the core-mantle boundary is not supported");
+  }
+  // nodes in a fluid layer or the inner core take the adjoint's end values, inner-core nodes a fifth job
+  bool core_nodes = false, ic_nodes = false;
+  for (int k = 1; k <= nnod; ++k) {
+    if (m.state[k] == 1 || m.rnod[k] < m.ric) core_nodes = true;
+    if (m.state[k] == 0 && m.rnod[k] < m.ric) ic_nodes = true;
   }
   // (omega, l) plan
   const int usz = (c->persistent && !na) ? c->unit : 256;   // one-lane-per-job kernels: one unit per block
@@ -398,6 +405,7 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   size_t o_us = take(16 * 6 * nnod * n + 16), o_ds = take(16 * 6 * nnod * n + 16);
   size_t o_ulg = take(8 * nnod * n + 8), o_dlg = take(8 * nnod * n + 8), o_dets = take(16 * nnod * n + 16);
   size_t o_ujs = full ? take(16 * 16 * nnod * n + 16) : 0;
+  size_t o_adje = (full && core_nodes) ? take(16 * 8 * n + 16) : 0;
   bool dev = (flags & GM_FLAG_DEVICE_PTRS) != 0;
   size_t o_gf = (full && (gf && !dev)) ? take(16 * nsp * nnod * n + 16) : 0;
   size_t o_gfre = (full && (gfre && !dev)) ? take(4 * nsp * nnod * n + 4) : 0;
@@ -462,6 +470,8 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   B.us = (double*)(base + o_us); B.ds = (double*)(base + o_ds); B.ulg = (double*)(base + o_ulg);
   B.dlg = (double*)(base + o_dlg); B.dets = (double*)(base + o_dets);
   B.ujs = full ? (double*)(base + o_ujs) : nullptr;
+  B.adjend = (full && core_nodes) ? (double*)(base + o_adje) : nullptr;
+  B.adj_jobs = ic_nodes ? 5 : 4;
   if (full) {
     B.gf = gf ? (dev ? gf : (double*)(base + o_gf)) : nullptr;
     B.gfre = gfre ? (dev ? gfre : (float*)(base + o_gfre)) : nullptr;

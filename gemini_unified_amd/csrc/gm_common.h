@@ -112,6 +112,11 @@ struct GmBatch {
   double* __restrict__ dlg;        // [nnod][n]
   double* __restrict__ dets;       // [nnod][n]
   double* __restrict__ ujs;        // spheroidal UJGF [nnod][4 jumps][4 comps][n] complex
+  // third component of the adjoint at the end of each leg [2 legs][4 jumps][n]
+  // complex (leg 0: at the ocean bottom or surface, leg 1: at the CMB, start
+  // radius or lowest node; null when no node lies in a fluid layer or the inner core)
+  double* __restrict__ adjend;
+  int adj_jobs;                    // adjoint jobs per instance: 4 jumps, + 1 inner-core propagator
   double* __restrict__ gf;         // FK column, complex [nnod][nsp][n] (may be null)
   float* __restrict__ gfre;        // FP32 real part [nnod][nsp][n] (may be null)
   float* __restrict__ gfim;

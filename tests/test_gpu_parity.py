@@ -228,3 +228,31 @@ def test_force_source_columns_match_oracle(c1_setup, f, lmin, nl):
         assert err.max() < tol
         if variant == "exact":
             np.testing.assert_array_equal(r["gfre"], ref.real.astype(np.float32))
+
+
+@pytest.mark.parametrize("istyp", [1, 0])
+@pytest.mark.parametrize("f,lmin,nl", [(40 / TLEN, 0, 24), (10 / TLEN, 1, 12), (30 / TLEN, 100, 10)])
+def test_core_nodes_match_oracle(f, lmin, nl, istyp):
+    """Nodes in the inner core, the fluid outer core (both sides of the ICB and the CMB) and the
+    mantle: the inner-core propagator job of k_adjoint and the fluid-node branches of k_assemble
+    against the oracle, exact (FP32 file values bit for bit) and fast variants."""
+    from gemini_unified_amd import ExternalRadialNodes, global_model
+    nem = global_model(1, 200)
+    prn = np.array([600e3, 1000e3, 2000e3, 3000e3, 4000e3, 5500e3, nem.rearth])
+    nodes = ExternalRadialNodes.split_at_discos(nem, prn, nem.rearth)
+    jre = nodes.nnod
+    orc = oracle.Oracle(nem, nodes)
+    orc.set_frequency(f, 3)
+    rc, ref, _ = orc.gfk_degrees(2.0 * MC_PI * f, -SIGMA, lmin, nl, jre, istyp, EPS)
+    assert rc == 0
+    for variant, tol in (("exact", 1e-12), ("fast", 1e-8)):
+        eng = GreenFKEngine(nem, nodes, variant=variant)
+        r = eng.gfk([f], lmin, nl, SIGMA, EPS, jre, istyp)
+        eng.close()
+        assert (r["status"] == 0).all()
+        err = rel_l2(r["gf"], ref)
+        print(f"core nodes {variant} istyp={istyp} f={f:.5f} l={lmin}..{lmin + nl - 1}: max rel-L2 {err.max():.2e}",
+              flush=True)
+        assert err.max() < tol
+        if variant == "exact":
+            np.testing.assert_array_equal(r["gfre"], ref.real.astype(np.float32))

@@ -214,3 +214,27 @@ def test_node_properties_match_oracle(model, blocks, f, attmode):
             np.testing.assert_allclose(ro, ro_ref, rtol=1e-13)
             np.testing.assert_allclose(el, el_ref, rtol=1e-12, atol=1e-12 * np.abs(el_ref).max())
             np.testing.assert_allclose(q, q_ref, rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize("istyp", [1, 0])
+def test_kernel_source_core_nodes_match_oracle(istyp):
+    """Nodes in the inner core, the fluid outer core (both sides of the ICB and the CMB) and the
+    mantle: the inner-core propagator job, the fluid-node reciprocity and source terms, bit for bit
+    with the oracle, work counters included."""
+    nem = global_model(1, 200)
+    prn = np.array([600e3, 1000e3, 2000e3, 3000e3, 4000e3, 5500e3, nem.rearth])
+    nodes = ExternalRadialNodes.split_at_discos(nem, prn, nem.rearth)
+    assert set(nodes.state.tolist()) == {0, 1} and nodes.rnod[0] < nem.ric
+    jre = nodes.nnod
+    for f, lmin, nl in [(40 / 2048.0, 0, 12), (10 / 2048.0, 1, 6), (30 / 2048.0, 100, 5)]:
+        sim = HostSim(nem, nodes, "exact")
+        r = sim.gfk([f], lmin, nl, 5 / 2048.0, 1e-6, jre, istyp)
+        orc = oracle.Oracle(nem, nodes)
+        orc.set_frequency(f, 3)
+        orc.stats[:] = 0
+        rc, ref, st = orc.gfk_degrees(2 * MC_PI * f, -5 / 2048.0, lmin, nl, jre, istyp, 1e-6)
+        assert rc == 0 and np.all(r["status"] == 0)
+        g = r["gf"]
+        both = np.isnan(g) & np.isnan(ref)
+        assert np.array_equal(np.where(both, 0, g), np.where(both, 0, ref))
+        np.testing.assert_array_equal(r["stats"], orc.stats)

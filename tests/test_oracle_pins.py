@@ -343,6 +343,41 @@ def test_ujgf_jump_conditions(c1_setup):
         np.testing.assert_allclose(above - below, s, atol=1e-7 * scale + 1e-12)
 
 
+def _core_nodes(nem, d=1.0):
+    """Nodes on both sides of the ICB and the CMB, d metres away, in the inner core, the fluid outer
+    core and the mantle; the receiver just below the surface."""
+    rn = np.array([700e3, nem.ric - d, nem.ric + d, 2500e3, nem.roc - d, nem.roc + d, 5000e3, nem.rearth - 1000.0])
+    st = np.array([0, 0, 1, 1, 1, 0, 0, 0], np.int32)
+    la = np.array([2, 2, 3, 3, 3, 4, 4, 4], np.int32)
+    return ExternalRadialNodes(rearth=nem.rearth, rnod=rn, state=st, layer=la)
+
+
+@pytest.mark.parametrize("f,l", [(10 / 2048.0, 2), (40 / 2048.0, 5), (40 / 2048.0, 30), (90 / 2048.0, 60)])
+def test_ujgf_continuous_across_core_boundaries(f, l):
+    """Below a mantle receiver the unit-jump Green functions are one solution of the homogeneous
+    equations through the whole Earth: U and R are continuous across the ICB and the CMB, and the
+    tangential stress S vanishes on the solid side of a solid-fluid boundary.  Inner-core nodes take
+    the inner-core propagator scaled by the mantle adjoint at the CMB, outer-core nodes the fluid
+    basis solution scaled by the same value, mantle nodes the mantle adjoint (four different
+    branches of the oracle, unitJumpGreenFunctions.f90:355-367); nodes 1 m either side of each
+    boundary agree to the first-order change over 2 m."""
+    nem = global_model(1, 200)
+    nodes = _core_nodes(nem)
+    orc = oracle.Oracle(nem, nodes)
+    orc.set_frequency(f, 3)
+    rc, gf, ujs, ujt, flag = orc.gfk_column(2 * MC_PI * f, -5 / 2048.0, dll1_of(l, nem.rearth), nodes.nnod, 1, 1e-8)
+    assert rc == 0 and flag == 0
+    for a, b in ((1, 2), (4, 5)):                  # (IC, OC) at the ICB, (OC, mantle) at the CMB
+        for c in (0, 1):                           # U, R per jump
+            np.testing.assert_allclose(ujs[a][:, c], ujs[b][:, c], rtol=1e-3)
+        solid = a if nodes.state[a] == 0 else b
+        assert np.all(np.abs(ujs[solid][:, 3]) < 1e-4 * np.abs(ujs[solid][:, 1]))
+        assert np.all(np.abs(ujs[a][:, :2]) > 0)
+    # the two inner-core nodes are nonzero and the fluid nodes carry no tangential components
+    assert np.all(ujs[[0, 1]][:, :, :2] != 0)
+    assert not ujs[[2, 3, 4]][:, :, 2:].any()
+
+
 def test_reciprocity_general_relation():
     """G(a,l)(re,rs) = s_a (rs/re)^2 sum_i G(i, c_a)(rs,re) T(i,l) / (elp1 for a>2),
     c = (2,1,4,3), s = (+,-,+,-), T(2,1)=-1, T(1,2)=1, T(4,3)=-elp1, T(3,4)=elp1."""
