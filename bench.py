@@ -489,9 +489,13 @@ def main():
         log(f"{tag}: eps {eps:g}, {rec['value']:.1f} solves/s")
         return rec, tm, sph
 
-    # 2. plain control: the reference's step rule (target try count 6)
+    # 2. plain control: the reference's step rule (target try count 6), on the kernel schedule measured
+    # fastest on a frequency-stratified sample (every candidate does the same arithmetic per lane)
     controller(eng, "reference")
     op_p = operating_point("plain")
+    from gemini_unified_amd import autotune
+    sched, sched_table = autotune.tune(eng, grid, jre, op_p[0], use_cache=False, log=log)
+    log(f"schedule: {sched}")
     plain, tm_p, sph_flops = arm("plain", *op_p, args.steps, args.warmup)
     tor_flops = int(eng.system_stats(1)[7])
     achieved = sph_flops / (tm_p[3] * 1e-3) / 1e12
@@ -564,6 +568,8 @@ def main():
                    "error_sample": f"{truth.shape[-1]} FK columns (64 seeded frequencies x 16 degrees), truth = "
                                    f"CPU oracle at eps=1e-10, relative L2 over 10 spectra x {nodes.nnod} nodes",
                    "step_control": "learned policy" if learned else "plain"},
+        "schedule": {"chosen": sched, "candidates_ms": [[c, round(ms, 2)] for c, ms in sched_table],
+                     "sample": "every 48th frequency with all its degrees (autotune.py)"},
         "plain": plain,
         "learned": learned,
         "order7_ablation": ablation,

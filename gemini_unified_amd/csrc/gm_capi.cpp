@@ -89,6 +89,8 @@ extern "C" int gm_ctx_destroy(gm_ctx* c) {
   for (int i = 0; i < 6; ++i) if (c->sev[i]) (void)hipEventDestroy(c->sev[i]);
   if (c->sbuf) (void)hipFree(c->sbuf);
   if (c->dtw) (void)hipFree(c->dtw);
+  if (c->g_exec) (void)hipGraphExecDestroy(c->g_exec);
+  if (c->g_graph) (void)hipGraphDestroy(c->g_graph);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return GM_OK;
@@ -125,6 +127,11 @@ extern "C" int gm_set_option(gm_ctx* c, int option, int value) {
   if (option == GM_OPT_ORDER_TARGET) {
     if (value < 3 || value > 7) return gm_fail(c, GM_ERR_ARG, "target try count must be in [3, 7]");
     c->order_target = value;
+    return GM_OK;
+  }
+  if (option == GM_OPT_GRAPH) {
+    if (value != 0 && value != 1) return gm_fail(c, GM_ERR_ARG, "graph must be 0 or 1");
+    c->graph = value;
     return GM_OK;
   }
   if (option == GM_OPT_UNIT) {
@@ -500,27 +507,68 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   const int pgrid = std::max(1, std::min(nunits, c->ncu * bpc(lds, c->waves)));
   GmKnot* dT = (GmKnot*)(base + o_tables);
   const double* dF = (const double*)(base + o_freqs);
-  HIPCHK(c, hipEventRecord(c->ev[0], s));
   // step logging / learned control / a policy / the batch without allocation: one lane per job
   const bool pers = c->persistent != 0 && ctl_mode == 0 && !pol && !na;
-  launch(0, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
-  HIPCHK(c, hipEventRecord(c->ev[1], s));
-  launch(1, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
-  if (pol) {
-    auto lp = c->variant == 0 ? gm_exact::launch_policy : gm_fast::launch_policy;
-    lp(dM, B, s, lds, pol_net ? (float*)(base + o_px) : nullptr, c->dmlp, pol_net ? (float*)(base + o_pw) : nullptr,
-       (float*)(base + o_pol), m.rearth, pol_net ? 0 : c->order_target);
+  // the kernel stages as one hipGraph (plain control, no policy): the uploads above and the copies
+  // below stay on the stream; every stage clears what it accumulates with kernels of its own, so a
+  // replay needs no memset node
+  const bool use_graph = c->graph && ctl_mode == 0 && !pol && !na;
+  if (use_graph) {
+    struct Shape { int nfreq, attmode, pgrid, waves, variant, pers, full; size_t lds; const void* dM; const void* dF;
+                   const void* dT; } sh;
+    memset(&sh, 0, sizeof sh);
+    sh.nfreq = nfreq; sh.attmode = attmode; sh.pgrid = pgrid; sh.waves = c->waves; sh.variant = c->variant;
+    sh.pers = pers; sh.full = full; sh.lds = lds; sh.dM = dM; sh.dF = dF; sh.dT = dT;
+    std::vector<unsigned char> key(sizeof B + sizeof sh);
+    memcpy(key.data(), &B, sizeof B);
+    memcpy(key.data() + sizeof B, &sh, sizeof sh);
+    if (!c->g_exec || key != c->g_key) {
+      if (c->g_exec) { HIPCHK(c, hipGraphExecDestroy(c->g_exec)); c->g_exec = nullptr; }
+      if (c->g_graph) { HIPCHK(c, hipGraphDestroy(c->g_graph)); c->g_graph = nullptr; }
+      c->g_key.clear();
+      HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      launch(0, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
+      launch(1, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
+      launch(pers ? 6 : 2, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
+      launch(pers ? 7 : 3, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
+      if (full) {
+        launch(4, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
+        launch(5, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
+      }
+      hipGraph_t g = nullptr;
+      const hipError_t ec = hipStreamEndCapture(s, &g);
+      if (ec != hipSuccess) return gm_fail(c, GM_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ec));
+      c->g_graph = g;
+      HIPCHK(c, hipGraphInstantiate(&c->g_exec, c->g_graph, nullptr, nullptr, 0));
+      c->g_key.swap(key);
+      ++c->graph_captures;
+    } else {
+      ++c->graph_replays;
+    }
+    HIPCHK(c, hipEventRecord(c->ev[0], s));
+    HIPCHK(c, hipGraphLaunch(c->g_exec, s));
+    for (int k = 1; k <= 5; ++k) HIPCHK(c, hipEventRecord(c->ev[k], s));
+  } else {
+    HIPCHK(c, hipEventRecord(c->ev[0], s));
+    launch(0, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
+    HIPCHK(c, hipEventRecord(c->ev[1], s));
+    launch(1, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
+    if (pol) {
+      auto lp = c->variant == 0 ? gm_exact::launch_policy : gm_fast::launch_policy;
+      lp(dM, B, s, lds, pol_net ? (float*)(base + o_px) : nullptr, c->dmlp, pol_net ? (float*)(base + o_pw) : nullptr,
+         (float*)(base + o_pol), m.rearth, pol_net ? 0 : c->order_target);
+    }
+    HIPCHK(c, hipEventRecord(c->ev[2], s));
+    launch(pers ? 6 : 2, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
+    HIPCHK(c, hipEventRecord(c->ev[3], s));
+    launch(pers ? 7 : 3, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
+    HIPCHK(c, hipEventRecord(c->ev[4], s));
+    if (full) {
+      launch(4, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
+      launch(5, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
+    }
+    HIPCHK(c, hipEventRecord(c->ev[5], s));
   }
-  HIPCHK(c, hipEventRecord(c->ev[2], s));
-  launch(pers ? 6 : 2, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
-  HIPCHK(c, hipEventRecord(c->ev[3], s));
-  launch(pers ? 7 : 3, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
-  HIPCHK(c, hipEventRecord(c->ev[4], s));
-  if (full) {
-    launch(4, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
-    launch(5, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
-  }
-  HIPCHK(c, hipEventRecord(c->ev[5], s));
   HIPCHK(c, hipGetLastError());
   // results
   auto cp = [&](void* dst, size_t o, size_t bytes) -> hipError_t {

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <string>
+#include <vector>
 #include "gm_common.h"
 
 struct gm_ctx {
@@ -30,6 +31,13 @@ struct gm_ctx {
   int prop_enabled = 0;
   int prop_every = 1;      // network consulted every prop_every accepted steps
   int order_target = 6;    // target try count of the step rule without a network (6: the reference's)
+  // hipGraph of the kernel stages of the last batch (GM_OPT_GRAPH): replayed while the batch's
+  // kernel arguments (GmBatch by value and the launch shape) are unchanged
+  int graph = 0;
+  hipGraph_t g_graph = nullptr;
+  hipGraphExec_t g_exec = nullptr;
+  std::vector<unsigned char> g_key;
+  long long graph_captures = 0, graph_replays = 0;
   // step logging of the next run (gm_collect_steplog)
   int log_max = 0;
   float* log_host = nullptr;

@@ -29,6 +29,7 @@ module geminiFKc
    integer(c_int), parameter :: GM_OPT_PROPOSE_EVERY = 5
    integer(c_int), parameter :: GM_OPT_SPH_SPLIT = 6
    integer(c_int), parameter :: GM_OPT_ORDER_TARGET = 7
+   integer(c_int), parameter :: GM_OPT_GRAPH = 8
    integer(c_int), parameter :: GM_NSTAT = 10
    interface
       integer(c_int) function gm_ctx_create(device, out) bind(C, name='gm_ctx_create')

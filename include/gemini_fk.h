@@ -70,6 +70,13 @@ typedef struct gm_ctx gm_ctx;
                                   seven-sequence sweep every step costs the same, so 7 takes the
                                   largest steps */
 
+#define GM_OPT_GRAPH 8         /* 1: the kernel stages of a batch with the plain step rule (no network,
+                                  no step log) run as one hipGraph, captured on the first call and
+                                  replayed while the next call's shapes, buffers and parameters are
+                                  the same (results unchanged; gm_last_timing then reports the graph's
+                                  time as its first and last entries); 0 (default): stage-by-stage
+                                  launches */
+
 /* work counters returned in stats[10] (same meaning as the CPU oracle's) */
 #define GM_NSTAT 10            /* propagations, accepted steps, rejected steps, MMP sequences,
                                   matvecs, sysmat evaluations, Neville cells, algorithmic flops,

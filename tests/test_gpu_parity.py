@@ -256,3 +256,26 @@ def test_core_nodes_match_oracle(f, lmin, nl, istyp):
         assert err.max() < tol
         if variant == "exact":
             np.testing.assert_array_equal(r["gfre"], ref.real.astype(np.float32))
+
+
+@pytest.mark.parametrize("f,lmin,nl", [(40 / TLEN, 0, 16), (30 / TLEN, 50, 10)])
+def test_ocean_nodes_match_oracle(f, lmin, nl):
+    """Ocean-bottom receiver, nodes in the ocean above it and in the outer core below it."""
+    from ocean_model import ocean_model, ocean_nodes
+    nem = ocean_model()
+    nodes, jre = ocean_nodes(nem)
+    orc = oracle.Oracle(nem, nodes)
+    orc.set_frequency(f, 3)
+    for istyp in (1, 0):
+        rc, ref, _ = orc.gfk_degrees(2.0 * MC_PI * f, -SIGMA, lmin, nl, jre, istyp, EPS)
+        assert rc == 0
+        for variant, tol in (("exact", 1e-12), ("fast", 1e-8)):
+            eng = GreenFKEngine(nem, nodes, variant=variant)
+            r = eng.gfk([f], lmin, nl, SIGMA, EPS, jre, istyp)
+            eng.close()
+            assert (r["status"] == 0).all()
+            err = rel_l2(r["gf"], ref)
+            print(f"ocean nodes {variant} istyp={istyp} f={f:.5f}: max rel-L2 {err.max():.2e}", flush=True)
+            assert err.max() < tol
+            if variant == "exact":
+                np.testing.assert_array_equal(r["gfre"], ref.real.astype(np.float32))

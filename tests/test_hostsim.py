@@ -238,3 +238,23 @@ def test_kernel_source_core_nodes_match_oracle(istyp):
         both = np.isnan(g) & np.isnan(ref)
         assert np.array_equal(np.where(both, 0, g), np.where(both, 0, ref))
         np.testing.assert_array_equal(r["stats"], orc.stats)
+
+
+def test_kernel_source_ocean_nodes_match_oracle():
+    """Ocean-bottom receiver with nodes in the ocean above it (fluid branch of the up leg) and in the
+    outer core below it: bit for bit with the oracle, moment and force sources."""
+    from ocean_model import ocean_model, ocean_nodes
+    nem = ocean_model()
+    nodes, jre = ocean_nodes(nem)
+    for istyp in (1, 0):
+        for f, lmin, nl in [(40 / 2048.0, 0, 8), (30 / 2048.0, 50, 5)]:
+            r = HostSim(nem, nodes, "exact").gfk([f], lmin, nl, 5 / 2048.0, 1e-6, jre, istyp)
+            orc = oracle.Oracle(nem, nodes)
+            orc.set_frequency(f, 3)
+            orc.stats[:] = 0
+            rc, ref, st = orc.gfk_degrees(2 * MC_PI * f, -5 / 2048.0, lmin, nl, jre, istyp, 1e-6)
+            assert rc == 0 and np.all(r["status"] == 0)
+            g = r["gf"]
+            both = np.isnan(g) & np.isnan(ref)
+            assert np.array_equal(np.where(both, 0, g), np.where(both, 0, ref))
+            np.testing.assert_array_equal(r["stats"], orc.stats)

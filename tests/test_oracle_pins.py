@@ -378,6 +378,26 @@ def test_ujgf_continuous_across_core_boundaries(f, l):
     assert not ujs[[2, 3, 4]][:, :, 2:].any()
 
 
+@pytest.mark.parametrize("f,l", [(10 / 2048.0, 2), (40 / 2048.0, 5), (40 / 2048.0, 30)])
+def test_ujgf_continuous_across_ocean_bottom(f, l):
+    """Above an ocean-bottom receiver the Green functions continue into the ocean as the fluid basis
+    solution scaled by the up-going adjoint at the ocean bottom (unitJumpGreenFunctions.f90:339-342):
+    U and R continuous across the ocean bottom, S = 0 on its solid side."""
+    from ocean_model import ocean_model, ocean_nodes
+    nem = ocean_model()
+    nodes, jre = ocean_nodes(nem, d=0.01)          # 500 m above the receiver R varies by ~1e-3 per metre
+    orc = oracle.Oracle(nem, nodes)
+    orc.set_frequency(f, 3)
+    rc, gf, ujs, ujt, flag = orc.gfk_column(2 * MC_PI * f, -5 / 2048.0, dll1_of(l, nem.rearth), jre, 1, 1e-8)
+    assert rc == 0 and flag == 0
+    a, b = 3, 4                                    # mantle node 1 cm below, ocean node 1 cm above
+    for c in (0, 1):
+        np.testing.assert_allclose(ujs[a][:, c], ujs[b][:, c], rtol=1e-3)
+    # S falls linearly to zero at the boundary: 1 cm away it is below 1e-4 of its value at the receiver
+    assert np.all(np.abs(ujs[a][:, 3]) < 1e-4 * np.abs(ujs[jre - 1][:, 3]))
+    assert np.all(np.abs(ujs[b][:, :2]) > 0) and not ujs[b][:, 2:].any()
+
+
 def test_reciprocity_general_relation():
     """G(a,l)(re,rs) = s_a (rs/re)^2 sum_i G(i, c_a)(rs,re) T(i,l) / (elp1 for a>2),
     c = (2,1,4,3), s = (+,-,+,-), T(2,1)=-1, T(1,2)=1, T(4,3)=-elp1, T(3,4)=elp1."""
