@@ -258,20 +258,35 @@ def synth_report(grid, nrcv=4096, nsamp=8192, seed=3):
     return out
 
 
-def gfk_report(eng, grid, jre, eps, ninst):
+def gfk_report(eng, grid, jre, eps, ninst, sample=None, truth=None):
     """The whole drop-in batch gm_gfk_batch on the main workload: minors, adjoint, unit-jump Green
     functions, reciprocity, source terms and FP32 packing, outputs left in HBM (with the step control
-    the engine is configured for)."""
-    eng.run_gfk_resident(grid.freqs, 0, grid.nwn, grid.sigma, eps, jre, 1)
-    t0 = time.perf_counter()
-    st = eng.run_gfk_resident(grid.freqs, 0, grid.nwn, grid.sigma, eps, jre, 1)
-    dt = time.perf_counter() - t0
-    tm = eng.timing()
-    out = {"columns_per_s": round(ninst / dt, 1), "ms_per_batch": round(dt * 1e3, 1), "eps": eps,
-           "stage_ms": {"toroidal": round(tm[2], 1), "spheroidal": round(tm[3], 1),
-                        "adjoint_and_assembly": round(tm[4], 1)},
-           "propagations": int(st[0])}
-    log(f"gfk columns: {out['columns_per_s']:.0f} columns/s ({out['ms_per_batch']} ms per batch), {out['stage_ms']}")
+    the engine is configured for); then the same batch with two adjoint integrations per leg instead
+    of four (GM_OPT_ADJOINT_PAIR), its time and its error on the sample."""
+    def one(tag):
+        eng.run_gfk_resident(grid.freqs, 0, grid.nwn, grid.sigma, eps, jre, 1)
+        t0 = time.perf_counter()
+        st = eng.run_gfk_resident(grid.freqs, 0, grid.nwn, grid.sigma, eps, jre, 1)
+        dt = time.perf_counter() - t0
+        tm = eng.timing()
+        o = {"columns_per_s": round(ninst / dt, 1), "ms_per_batch": round(dt * 1e3, 1), "eps": eps,
+             "stage_ms": {"toroidal": round(tm[2], 1), "spheroidal": round(tm[3], 1),
+                          "adjoint_and_assembly": round(tm[4], 1)},
+             "propagations": int(st[0])}
+        if truth is not None:
+            emax, ep99 = sample_error(eng, grid, jre, sample, eps, truth)
+            o["spectrum_rel_err"] = {"max": emax, "p99": ep99}
+        log(f"{tag}: {o['columns_per_s']:.0f} columns/s ({o['ms_per_batch']} ms per batch), {o['stage_ms']}, "
+            f"error {o.get('spectrum_rel_err')}")
+        return o
+    out = one("gfk columns")
+    eng.configure(adjoint_pair=True)
+    try:
+        out["adjoint_pair"] = one("gfk columns, adjoint pairs")
+    finally:
+        eng.configure(adjoint_pair=False)
+    out["adjoint_pair"]["over_reference_adjoints"] = round(out["adjoint_pair"]["columns_per_s"] /
+                                                           out["columns_per_s"], 4)
     return out
 
 
@@ -443,6 +458,7 @@ def main():
     ap.add_argument("--no-synth", action="store_true")
     ap.add_argument("--no-c5", action="store_true")
     ap.add_argument("--no-gfk", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default schedule (for profiling runs)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -494,7 +510,15 @@ def main():
     controller(eng, "reference")
     op_p = operating_point("plain")
     from gemini_unified_amd import autotune
-    sched, sched_table = autotune.tune(eng, grid, jre, op_p[0], use_cache=False, log=log)
+    # the full grid: the blocks-per-unit split trades table staging against SIMD release at the end of a
+    # unit, which a sample of a few hundred units does not reproduce (the persistent kernels measured
+    # 10x slower on the sample and are left out here)
+    if args.no_autotune:
+        sched, sched_table = {"persistent": 0, "sph_split": 4}, []
+        eng.configure(**sched)
+    else:
+        sched, sched_table = autotune.tune(eng, grid, jre, op_p[0], candidates=autotune.DEFAULT_CANDIDATES[:3],
+                                           stride=1, use_cache=False, log=log)
     log(f"schedule: {sched}")
     plain, tm_p, sph_flops = arm("plain", *op_p, args.steps, args.warmup)
     tor_flops = int(eng.system_stats(1)[7])
@@ -504,7 +528,7 @@ def main():
                 "traffic": read_traffic(), "kernel_ms": round(tm_p[3], 3), "algorithmic_flops_per_launch": sph_flops,
                 "toroidal": {"kernel": "k_toroidal", "kernel_ms": round(tm_p[2], 3),
                              "achieved": round(tor_flops / (tm_p[2] * 1e-3) / 1e12, 4)}}
-    gfk = None if args.no_gfk else gfk_report(eng, grid, jre, op_p[0], ninst)
+    gfk = None if args.no_gfk else gfk_report(eng, grid, jre, op_p[0], ninst, sample, truth)
 
     # 3. learned control: the trained network's per-instance policy; 4. its ablation, the fixed target
     # try count 7 without a network; 5. a second back-to-back pair of plain and learned timings
@@ -527,7 +551,7 @@ def main():
         l2, _, _ = arm("learned (second)", op_l[0], op_l[1], op_l[2], [], short, 0)
         controller(eng, "reference")
         controller(eng, "policy", params, norm, args.safety)
-        learned["gfk_columns"] = None if args.no_gfk else gfk_report(eng, grid, jre, op_l[0], ninst)
+        learned["gfk_columns"] = None if args.no_gfk else gfk_report(eng, grid, jre, op_l[0], ninst, sample, truth)
         controller(eng, "reference")
         ratios = [round(learned["value"] / plain["value"], 4), round(l2["value"] / p2["value"], 4)]
         learned["learned_over_plain"] = ratios[0]
@@ -569,7 +593,7 @@ def main():
                                    f"CPU oracle at eps=1e-10, relative L2 over 10 spectra x {nodes.nnod} nodes",
                    "step_control": "learned policy" if learned else "plain"},
         "schedule": {"chosen": sched, "candidates_ms": [[c, round(ms, 2)] for c, ms in sched_table],
-                     "sample": "every 48th frequency with all its degrees (autotune.py)"},
+                     "sample": "the full grid, one warm-up and one timed run per candidate (autotune.py)"},
         "plain": plain,
         "learned": learned,
         "order7_ablation": ablation,

@@ -129,6 +129,11 @@ extern "C" int gm_set_option(gm_ctx* c, int option, int value) {
     c->order_target = value;
     return GM_OK;
   }
+  if (option == GM_OPT_ADJOINT_PAIR) {
+    if (value != 0 && value != 1) return gm_fail(c, GM_ERR_ARG, "adjoint pair must be 0 or 1");
+    c->adj_pair = value;
+    return GM_OK;
+  }
   if (option == GM_OPT_GRAPH) {
     if (value != 0 && value != 1) return gm_fail(c, GM_ERR_ARG, "graph must be 0 or 1");
     c->graph = value;
@@ -478,7 +483,8 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   B.dlg = (double*)(base + o_dlg); B.dets = (double*)(base + o_dets);
   B.ujs = full ? (double*)(base + o_ujs) : nullptr;
   B.adjend = (full && core_nodes) ? (double*)(base + o_adje) : nullptr;
-  B.adj_jobs = ic_nodes ? 5 : 4;
+  B.adj_jumps = c->adj_pair ? 2 : 4;
+  B.adj_jobs = B.adj_jumps + (ic_nodes ? 1 : 0);
   if (full) {
     B.gf = gf ? (dev ? gf : (double*)(base + o_gf)) : nullptr;
     B.gfre = gfre ? (dev ? gfre : (float*)(base + o_gfre)) : nullptr;

@@ -116,7 +116,9 @@ struct GmBatch {
   // complex (leg 0: at the ocean bottom or surface, leg 1: at the CMB, start
   // radius or lowest node; null when no node lies in a fluid layer or the inner core)
   double* __restrict__ adjend;
-  int adj_jobs;                    // adjoint jobs per instance: 4 jumps, + 1 inner-core propagator
+  int adj_jobs;                    // adjoint jobs per instance: adj_jumps, + 1 inner-core propagator
+  int adj_jumps;                   // 4: one adjoint per unit jump (the reference's); 2: the two
+                                   // adjoints whose combinations give the four (GM_OPT_ADJOINT_PAIR)
   double* __restrict__ gf;         // FK column, complex [nnod][nsp][n] (may be null)
   float* __restrict__ gfre;        // FP32 real part [nnod][nsp][n] (may be null)
   float* __restrict__ gfim;

@@ -31,6 +31,7 @@ struct gm_ctx {
   int prop_enabled = 0;
   int prop_every = 1;      // network consulted every prop_every accepted steps
   int order_target = 6;    // target try count of the step rule without a network (6: the reference's)
+  int adj_pair = 0;        // 1: two adjoint integrations per leg give the four jumps (GM_OPT_ADJOINT_PAIR)
   // hipGraph of the kernel stages of the last batch (GM_OPT_GRAPH): replayed while the batch's
   // kernel arguments (GmBatch by value and the launch shape) are unchanged
   int graph = 0;

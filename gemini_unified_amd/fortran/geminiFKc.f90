@@ -30,6 +30,7 @@ module geminiFKc
    integer(c_int), parameter :: GM_OPT_SPH_SPLIT = 6
    integer(c_int), parameter :: GM_OPT_ORDER_TARGET = 7
    integer(c_int), parameter :: GM_OPT_GRAPH = 8
+   integer(c_int), parameter :: GM_OPT_ADJOINT_PAIR = 9
    integer(c_int), parameter :: GM_NSTAT = 10
    interface
       integer(c_int) function gm_ctx_create(device, out) bind(C, name='gm_ctx_create')

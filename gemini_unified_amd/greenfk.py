@@ -73,16 +73,19 @@ class GreenFKEngine:
         self.variant = variant
 
     def configure(self, persistent=None, unit=None, waves=None, propose_every=None, sph_split=None,
-                  order_target=None, graph=None):
+                  order_target=None, graph=None, adjoint_pair=None):
         """Kernel schedule options (all result-preserving): persistent substep-scheduled kernels or
         one lane per job, work-unit size, waves per SIMD of the persistent kernels; how often the
         learned proposer is consulted (every N accepted steps, plus sub-interval starts); and the
         target try count of the step rule without a network (6: the reference's; changes results
         at the tolerance level); graph: the kernel stages of plain-control batches as one hipGraph,
-        captured once and replayed while shapes and buffers are unchanged."""
+        captured once and replayed while shapes and buffers are unchanged; adjoint_pair: two adjoint
+        integrations per leg give the four unit jumps (half the adjoint work, eps-level differences)."""
         L = _lib.lib()
         if graph is not None:
             check(self.h, L.gm_set_option(self.h, 8, int(bool(graph))))
+        if adjoint_pair is not None:
+            check(self.h, L.gm_set_option(self.h, 9, int(bool(adjoint_pair))))
         if order_target is not None:
             check(self.h, L.gm_set_option(self.h, 7, int(order_target)))
         if propose_every is not None:

@@ -77,6 +77,13 @@ typedef struct gm_ctx gm_ctx;
                                   time as its first and last entries); 0 (default): stage-by-stage
                                   launches */
 
+#define GM_OPT_ADJOINT_PAIR 9  /* 1: the unit-jump Green functions of a (omega, l) from two adjoint
+                                  integrations per leg instead of four: the four start values
+                                  (unitJumpGreenFunctions.f90:306-330) span a plane, so the four
+                                  adjoints are combinations of two (half the adjoint work; results
+                                  differ from the reference's at the level of eps, not bitwise);
+                                  0 (default): one adjoint per unit jump as the reference */
+
 /* work counters returned in stats[10] (same meaning as the CPU oracle's) */
 #define GM_NSTAT 10            /* propagations, accepted steps, rejected steps, MMP sequences,
                                   matvecs, sysmat evaluations, Neville cells, algorithmic flops,

@@ -36,6 +36,7 @@ def lib():
         L.sim_proposer_new.restype = C.c_void_p
         L.sim_proposer_new.argtypes = [F, C.c_float, C.c_int, C.c_int]
         L.sim_proposer_free.argtypes = [C.c_void_p]
+        L.sim_set_adjoint_pair.argtypes = [C.c_int]
         for v in VARIANTS:
             fn = getattr(L, "sim_run_" + v)
             fn.argtypes = [C.c_void_p, C.c_int, D, I, I, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,

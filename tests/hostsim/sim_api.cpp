@@ -54,3 +54,8 @@ extern "C" GmProposer* sim_proposer_new(const float* norm, float safety, int kch
 }
 
 extern "C" void sim_proposer_free(GmProposer* p) { delete p; }
+
+extern "C" {
+int g_sim_adj_pair = 0;
+void sim_set_adjoint_pair(int v) { g_sim_adj_pair = v; }
+}

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <vector>
 #include "gm_common.h"
+extern "C" int g_sim_adj_pair;   // sim_api.cpp: GM_OPT_ADJOINT_PAIR of the CPU build
 #define GM_NS SIM_NS
 namespace SIM_NS {
 static constexpr bool kFast = SIM_FAST != 0;

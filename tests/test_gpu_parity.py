@@ -279,3 +279,26 @@ def test_ocean_nodes_match_oracle(f, lmin, nl):
             assert err.max() < tol
             if variant == "exact":
                 np.testing.assert_array_equal(r["gfre"], ref.real.astype(np.float32))
+
+
+@pytest.mark.parametrize("f,lmin,nl", [(40 / TLEN, 1, 24), (90 / TLEN, 300, 16)])
+def test_adjoint_pairs_keep_the_error(f, lmin, nl):
+    """GM_OPT_ADJOINT_PAIR on the GPU: the Green functions from two adjoint integrations per leg carry
+    the error of the four separate ones (against the oracle at eps 1e-10), within 1e-4 of them."""
+    from gemini_unified_amd import ExternalRadialNodes, global_model
+    nem = global_model(1, 200)
+    nodes = ExternalRadialNodes.from_blocks(nem, [15], [700000.0 / 15])
+    jre = nodes.node_index(nem.rearth)
+    orc = oracle.Oracle(nem, nodes)
+    orc.set_frequency(f, 3)
+    rc, truth, _ = orc.gfk_degrees(2.0 * MC_PI * f, -SIGMA, lmin, nl, jre, 1, 1e-10)
+    assert rc == 0
+    eng = GreenFKEngine(nem, nodes, variant="fast")
+    r0 = eng.gfk([f], lmin, nl, SIGMA, EPS, jre, 1)
+    eng.configure(adjoint_pair=True)
+    r1 = eng.gfk([f], lmin, nl, SIGMA, EPS, jre, 1)
+    eng.close()
+    e0, e1 = rel_l2(r0["gf"], truth), rel_l2(r1["gf"], truth)
+    print(f"adjoint pairs f={f:.5f}: error {e1.max():.2e} against {e0.max():.2e} with four adjoints", flush=True)
+    assert e1.max() <= 1.1 * e0.max() + 1e-9
+    assert rel_l2(r1["gf"], r0["gf"]).max() < 1e-4

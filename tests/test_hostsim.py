@@ -258,3 +258,35 @@ def test_kernel_source_ocean_nodes_match_oracle():
             both = np.isnan(g) & np.isnan(ref)
             assert np.array_equal(np.where(both, 0, g), np.where(both, 0, ref))
             np.testing.assert_array_equal(r["stats"], orc.stats)
+
+
+@pytest.mark.parametrize("case", ["mantle", "core"])
+def test_adjoint_pairs_keep_the_error(case):
+    """Two adjoint integrations per leg (GM_OPT_ADJOINT_PAIR) in the kernel source: the four unit-jump
+    start values span a plane, so combining two integrated adjoints gives the Green functions with the
+    error of the four separate integrations (against the oracle at eps 1e-10), using fewer propagations."""
+    import hostsim
+    nem = global_model(1, 200)
+    if case == "mantle":
+        nodes = ExternalRadialNodes.from_blocks(nem, [15], [700000.0 / 15])
+    else:
+        prn = np.array([600e3, 1000e3, 2000e3, 3000e3, 4000e3, 5500e3, nem.rearth])
+        nodes = ExternalRadialNodes.split_at_discos(nem, prn, nem.rearth)
+    jre = nodes.node_index(nem.rearth)
+    try:
+        for f, lmin, nl in [(40 / 2048.0, 1, 24), (90 / 2048.0, 300, 16)]:
+            orc = oracle.Oracle(nem, nodes)
+            orc.set_frequency(f, 3)
+            rc, truth, _ = orc.gfk_degrees(2 * MC_PI * f, -5 / 2048.0, lmin, nl, jre, 1, 1e-10)
+            assert rc == 0
+            res = {}
+            for pair in (0, 1):
+                hostsim.lib().sim_set_adjoint_pair(pair)
+                res[pair] = HostSim(nem, nodes, "exact").gfk([f], lmin, nl, 5 / 2048.0, 1e-6, jre, 1)
+                assert np.all(res[pair]["status"] == 0)
+            e0, e1 = _rel_l2(res[0]["gf"], truth), _rel_l2(res[1]["gf"], truth)
+            assert e1.max() <= 1.1 * e0.max() + 1e-9
+            assert _rel_l2(res[1]["gf"], res[0]["gf"]).max() < 1e-4
+            assert res[1]["stats"][0] < res[0]["stats"][0]
+    finally:
+        hostsim.lib().sim_set_adjoint_pair(0)
