@@ -8,6 +8,7 @@ calls, with outputs in the same [node][component][k] layouts.
 from __future__ import annotations
 
 import ctypes as C
+import fcntl
 import os
 import subprocess
 
@@ -26,7 +27,11 @@ _lib = None
 def lib():
     global _lib
     if _lib is None:
-        subprocess.run(["make", "-s", "-C", HERE, "-j4"], check=True)
+        # one build at a time when several test processes start together
+        with open(os.path.join(HERE, ".build.lock"), "w") as lk:
+            fcntl.flock(lk, fcntl.LOCK_EX)
+            subprocess.run(["make", "-s", "-C", HERE, "-j4"], check=True)
+            fcntl.flock(lk, fcntl.LOCK_UN)
         L = C.CDLL(LIB)
         L.sim_model_new.restype = C.c_void_p
         L.sim_model_new.argtypes = [C.c_int] + [D] * 9 + [C.c_int, I, D, I, C.c_int, C.c_int, C.c_int, C.c_int,
