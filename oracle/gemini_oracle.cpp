@@ -1534,6 +1534,161 @@ static int gfkColumn(const Ctx& c, double omre, double omim, double dll1, int jr
 }
 
 
+// getSysmatComplexSolid (solidOdeSystem.f90:78-111): the 4 x 4 system of the
+// displacement-stress vector (U, R, V, S), used only for the radial derivatives
+// of the direct Green functions of the multi-source driver
+static void sysmatSolid(const Env& e, double x, SysMat& S) {
+  double ro; cx z[8];
+  evalElcon(*e.m, *e.s, x, e.nl, ro, z);
+  cx zom = C(e.omre, e.omim);
+  double dll1 = e.dll1;
+  double rr = 1.0 / x;
+  double rr2 = rr * rr;
+  cx zrpc = rdiv(1.0, z[2]);
+  cx z1 = sub(sub(z[1], mul(mul(z[3], z[3]), zrpc)), z[5]);
+  cx z2 = mul(mulr(z[3], rr), zrpc);
+  cx zdlz1 = rmul(dll1, z1);
+  cx zomro = mulr(mul(zom, zom), ro);
+  S.a[1][1] = rmul(-2.0, z2);
+  S.a[1][2] = zrpc;
+  S.a[1][3] = rmul(dll1, z2);
+  S.a[1][4] = C(-0.0);
+  S.a[2][1] = add(neg(zomro), mulr(rmul(4.0, z1), rr2));
+  S.a[2][2] = subr(rmul(2.0, z2), 2.0 * rr);
+  S.a[2][3] = mul(C(-2.0 * rr2), zdlz1);
+  S.a[2][4] = C(dll1 * rr);
+  S.a[3][1] = C(-rr);
+  S.a[3][2] = C(-0.0);
+  S.a[3][3] = C(rr);
+  S.a[3][4] = rdiv(1.0, z[4]);
+  S.a[4][1] = mulr(rmul(-2.0, z1), rr2);
+  S.a[4][2] = neg(z2);
+  S.a[4][3] = add(neg(zomro), mulr(add(zdlz1, rmul(dll1 - 2.0, z[5])), rr2));
+  S.a[4][4] = C(-3.0 * rr);
+}
+
+// computeDsvDerivatives (dsvDerivatives.f90:38-68): gfd = A(r) gf for nj jump
+// columns of nvar components, A of the system `sys` (0 solid, 1 toroidal, 2
+// fluid) in the layer that holds r (getLayerIndexNodeEarthmodel); gf and gfd
+// are [comp][jump], 1-based
+static void dsvDerivatives(const Env& env0, int sys, double r, int nj, cx gf[5][5], cx gfd[5][5]) {
+  Env env = env0;
+  int nl, top;
+  getLayerIndex(*env.m, r, nl, top);
+  env.nl = nl;
+  SysMat A;
+  for (int i = 0; i < 6; ++i) for (int j = 0; j < 6; ++j) A.a[i][j] = C(0.0);
+  int nvar = 2;
+  if (sys == 0) { sysmatSolid(env, r, A); nvar = 4; }
+  else if (sys == 1) sysmatToroidal(env, r, A);
+  else sysmatFluid(env, r, A);
+  for (int i = 1; i <= nvar; ++i)
+    for (int j = 1; j <= nj; ++j) {
+      cx acc = C(0.0);
+      for (int k = 1; k <= nvar; ++k) acc = add(acc, mul(A.a[i][k], gf[k][j]));
+      gfd[i][j] = acc;
+    }
+}
+
+// getHorcompComplexFluid (fluidOdeSystem.f90:171-188): V = -R / (r om^2 ro) and
+// dV/dr = -(U - V) / r at a fluid node, for nj jump columns of (U, R)
+static void horcompFluid(const Env& env, double r, int nj, cx gf[5][5], cx v[5], cx vr[5]) {
+  int nl, top;
+  getLayerIndex(*env.m, r, nl, top);
+  double ro; cx z[8];
+  evalElcon(*env.m, *env.s, r, nl, ro, z);
+  cx zom = C(env.omre, env.omim);
+  cx zomro = mulr(mul(zom, zom), ro);
+  for (int j = 1; j <= nj; ++j) {
+    v[j] = neg(cdiv(gf[2][j], mulr(zomro, r)));
+    vr[j] = mulr(sub(gf[1][j], v[j]), -1.0 / r);
+  }
+}
+
+// One (omega, l) of the multi-source driver (computeGreenFKSpectraForASKI.f90:
+// 458-590): minors over the source node range, then for each source node js
+// (nsnod moment or force sources at jsnod[js], and a FORCE at the receiver jre
+// as source nsnod+1) the unit-jump Green functions of a jump at that node, the
+// source terms there, and at every node n the direct Green functions (U, V) and
+// (W) with their radial derivatives.  out: [js][n][nspmax] with
+// isp = U(j) V(j) dU(j) dV(j) W(j) dW(j), nsp(js) = 4 njsph + 2 njtor, the rest zero.
+static int askiColumn(const Ctx& c, double omre, double omim, double dll1, int nsnod, const int* jsnod, int jre,
+                      int istyp, double eps, int nspmax, std::vector<cx>& out, Stats& st) {
+  Env env; env.m = &c.m; env.nod = &c.nod; env.s = &c.s; env.omre = omre; env.omim = omim; env.dll1 = dll1; env.nl = 1;
+  const Nodes& N = c.nod;
+  const int nnod = N.nnod;
+  if (jre < 1 || jre > nnod) return ERR_NODEIDX;
+  int jsl = nnod + 1, jsr = 0;
+  bool jre_in = false;
+  for (int js = 0; js < nsnod; ++js) {
+    if (jsnod[js] < 1 || jsnod[js] > nnod) return ERR_NODEIDX;
+    jsl = std::min(jsl, jsnod[js]);
+    jsr = std::max(jsr, jsnod[js]);
+    if (jsnod[js] == jre) jre_in = true;
+  }
+  if (!jre_in) return ERR_NODEIDX;          // the receiver node must also be a source node (:287-290)
+  out.assign((size_t)(nsnod + 1) * nnod * nspmax, C(0.0));
+  ToroidalMinors T; SpheroidalMinors S;
+  int rc = toroidalMinors(env, jsl, jsr, eps, T, st, nullptr);
+  if (rc != OK) return rc;
+  rc = spheroidalMinors(env, jsl, jsr, eps, S, st, nullptr);
+  if (rc != OK) return rc;
+  for (int js = 1; js <= nsnod + 1; ++js) {
+    const int styp = js == nsnod + 1 ? 0 : istyp;
+    const int jsh = js == nsnod + 1 ? jre : jsnod[js - 1];
+    if (N.state[jsh] == 1) return ERR_UNSUPPORTED;   // a source in the ocean
+    const int njsph = styp == 1 ? 4 : 2, njtor = styp == 1 ? 2 : 1;
+    Column col;
+    rc = unitJumpGreen(env, T, S, jsh, eps, col, st);
+    if (rc != OK) return rc;
+    cx zs[5][5], zt[3][3];
+    sourceTerms(env, styp, N.rnod[jsh], N.state[jsh], N.layer[jsh], zs, zt);
+    for (int n = 1; n <= nnod; ++n) {
+      cx gfs[5][5], gfd[5][5], gft[5][5], gfdt[5][5];
+      for (int i = 0; i < 5; ++i)
+        for (int j = 0; j < 5; ++j) gfs[i][j] = gfd[i][j] = gft[i][j] = gfdt[i][j] = C(0.0);
+      auto U = [&](int i, int k) { return col.ujs[(n - 1) * 16 + (k - 1) * 4 + (i - 1)]; };
+      auto W = [&](int i, int k) { return col.ujt[(n - 1) * 4 + (k - 1) * 2 + (i - 1)]; };
+      if (N.state[n] == 1) {                 // receiver node in the ocean: (U, R), then V and dV/dr
+        for (int i = 1; i <= 2; ++i)
+          for (int j = 1; j <= njsph; ++j) {
+            cx acc = C(0.0);
+            for (int k = 1; k <= 4; ++k) acc = add(acc, mul(U(i, k), zs[k][j]));
+            gfs[i][j] = acc;
+          }
+        dsvDerivatives(env, 2, N.rnod[n], njsph, gfs, gfd);
+        cx v[5], vr[5];
+        horcompFluid(env, N.rnod[n], njsph, gfs, v, vr);
+        for (int j = 1; j <= njsph; ++j) { gfs[3][j] = v[j]; gfd[3][j] = vr[j]; }
+      } else {
+        for (int i = 1; i <= 4; ++i)
+          for (int j = 1; j <= njsph; ++j) {
+            cx acc = C(0.0);
+            for (int k = 1; k <= 4; ++k) acc = add(acc, mul(U(i, k), zs[k][j]));
+            gfs[i][j] = acc;
+          }
+        for (int i = 1; i <= 2; ++i)
+          for (int j = 1; j <= njtor; ++j) {
+            cx acc = C(0.0);
+            for (int k = 1; k <= 2; ++k) acc = add(acc, mul(W(i, k), zt[k][j]));
+            gft[i][j] = acc;
+          }
+        dsvDerivatives(env, 0, N.rnod[n], njsph, gfs, gfd);
+        dsvDerivatives(env, 1, N.rnod[n], njtor, gft, gfdt);
+      }
+      cx* o = &out[((size_t)(js - 1) * nnod + (n - 1)) * nspmax];
+      int isp = 0;
+      for (int i = 1; i <= 4; i += 2)
+        for (int j = 1; j <= njsph; ++j) o[isp++] = gfs[i][j];
+      for (int i = 1; i <= 4; i += 2)
+        for (int j = 1; j <= njsph; ++j) o[isp++] = gfd[i][j];
+      for (int j = 1; j <= njtor; ++j) o[isp++] = gft[1][j];
+      for (int j = 1; j <= njtor; ++j) o[isp++] = gfdt[1][j];
+    }
+  }
+  return OK;
+}
+
 }  // namespace orc
 
 using namespace orc;
@@ -1780,6 +1935,68 @@ int orc_gfk_wavenumbers(void* h, double omre, double omim, double rearth, int iw
 }
 
 // Engine test: integrate y' = A y (A constant, nvar <= 5, row-major complex) from x1 to x2.
+// All degrees l = lmin .. lmin+nl-1 of one frequency through the multi-source
+// driver (computeGreenFKSpectraForASKI.f90:458-590).  gf: complex
+// [nsnod+1][nnod][nspmax][nl] (Fortran gfre(nwn, nspmax, nnod, nsnod+1)),
+// nspmax = 20 (moment) or 10 (force), zero beyond each source's nsp.
+int orc_aski_degrees(void* h, double omre, double omim, double rearth, int lmin, int nl, int nsnod,
+                     const int* jsnod, int jre, int istyp, double eps, int nthreads, double* gf, int* status,
+                     int64_t* stats) {
+  Ctx* c = (Ctx*)h;
+  const int nnod = c->nod.nnod;
+  const int nspmax = istyp == 1 ? 20 : 10;
+  Stats total;
+#ifdef _OPENMP
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel num_threads(nthreads)
+#endif
+  {
+    Stats st;
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+    for (int il = 0; il < nl; ++il) {
+      const double dll1 = instanceDll1(lmin + il, 0.0, rearth);
+      std::vector<cx> col;
+      const int rc = askiColumn(*c, omre, omim, dll1, nsnod, jsnod, jre, istyp, eps, nspmax, col, st);
+      status[il] = rc;
+      for (size_t b = 0; b < (size_t)(nsnod + 1) * nnod * nspmax; ++b) {
+        const size_t o = b * nl + il;
+        const cx v = rc == OK ? col[b] : C(0.0);
+        gf[2 * o] = v.re; gf[2 * o + 1] = v.im;
+      }
+    }
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+    total.merge(st);
+  }
+  if (stats) for (int i = 0; i < 10; ++i) stats[i] += total.v[i];
+  int bad = 0;
+  for (int il = 0; il < nl; ++il) if (status[il] != OK) bad = status[il];
+  return bad;
+}
+
+// system matrix A(r) of the solid (0), toroidal (1) or fluid (2) system in the
+// layer holding r, complex [nvar][nvar] row-major (a check of the restatements)
+int orc_sysmat_at(void* h, int sys, double omre, double omim, double dll1, double r, double* a) {
+  Ctx* c = (Ctx*)h;
+  Env env; env.m = &c->m; env.nod = &c->nod; env.s = &c->s; env.omre = omre; env.omim = omim; env.dll1 = dll1;
+  int nl, top;
+  getLayerIndex(c->m, r, nl, top);
+  env.nl = nl;
+  SysMat A;
+  for (int i = 0; i < 6; ++i) for (int j = 0; j < 6; ++j) A.a[i][j] = C(0.0);
+  int nv = 2;
+  if (sys == 0) { sysmatSolid(env, r, A); nv = 4; }
+  else if (sys == 1) sysmatToroidal(env, r, A);
+  else if (sys == 2) sysmatFluid(env, r, A);
+  else { sysmatSolidt(env, r, A); nv = 4; }
+  for (int i = 1; i <= nv; ++i)
+    for (int j = 1; j <= nv; ++j) { a[2 * ((i - 1) * nv + j - 1)] = A.a[i][j].re; a[2 * ((i - 1) * nv + j - 1) + 1] = A.a[i][j].im; }
+  return OK;
+}
+
 int orc_propagate_const(int nvar, const double* A, double x1, double x2, double htry, double eps, const double* y0,
                         double* y1, int64_t* stats) {
   ConstSys cs; cs.nvar = nvar;

@@ -66,6 +66,9 @@ def lib():
                                             C.c_int, C.c_int, C.c_double, C.c_int, I, L]
         _lib.orc_gfk_wavenumbers.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                                              C.c_double, C.c_int, C.c_int, C.c_double, C.c_int, D, I, L]
+        _lib.orc_aski_degrees.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                          C.c_int, I, C.c_int, C.c_int, C.c_double, C.c_int, D, I, L]
+        _lib.orc_sysmat_at.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, D]
         _lib.orc_minors_wavenumbers.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                                                 C.c_double, C.c_int, C.c_int, C.c_double, C.c_int, I, L]
     return _lib
@@ -216,6 +219,26 @@ class Oracle:
         rc = lib().orc_gfk_degrees(self.h, omre, omim, self.nem.rearth, lmin, nl, jre, istyp, eps, nthreads,
                                    _c(gf), status.ctypes.data_as(I), self.stats.ctypes.data_as(L))
         return rc, gf, status
+
+    def aski_degrees(self, omre, omim, lmin, nl, jsnod, jre, istyp, eps, nthreads=1):
+        """Multi-source columns for l = lmin..lmin+nl-1 at the current frequency:
+        gf[js, n, isp, il] for the len(jsnod) sources and the force at jre (last)."""
+        n = self.nodes.nnod
+        nspmax = 20 if istyp == 1 else 10
+        jsn = np.ascontiguousarray(jsnod, dtype=np.int32)
+        gf = np.zeros((len(jsn) + 1, n, nspmax, nl), np.complex128)
+        status = np.zeros(nl, np.int32)
+        rc = lib().orc_aski_degrees(self.h, omre, omim, self.nem.rearth, lmin, nl, len(jsn), jsn.ctypes.data_as(I),
+                                    jre, istyp, eps, nthreads, _c(gf), status.ctypes.data_as(I),
+                                    self.stats.ctypes.data_as(L))
+        return rc, gf, status
+
+    def sysmat_at(self, sys, omre, omim, dll1, r):
+        """System matrix at r: sys 0 solid, 1 toroidal, 2 fluid, 3 solidt (adjoint)."""
+        nv = 4 if sys in (0, 3) else 2
+        a = np.zeros((nv, nv), np.complex128)
+        lib().orc_sysmat_at(self.h, sys, omre, omim, dll1, r, _c(a))
+        return a
 
 
 def propagate_const(A, x1, x2, htry, eps, y0):
