@@ -337,7 +337,7 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
                double eps, int attmode, int jsl, int jsr, int jre, int istyp, bool full, int flags,
                double* u_tor, double* d_tor, double* dettor, double* u_sph, double* d_sph, double* ulg, double* dlg,
                double* detsph, double* gf, float* gfre, float* gfim, int* status, long long* stats,
-               const GmNoAlloc* na = nullptr) {
+               const GmNoAlloc* na = nullptr, int nsnod = -1, const int* jsnod = nullptr) {
   if (!c) return GM_ERR_ARG;
   if (na && nfreq > GM_NOALLOC_MAXF)
     return gm_fail(c, GM_ERR_ARG, "%d frequencies: the batch without allocation takes at most %d", nfreq,
@@ -358,6 +358,16 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
     if (m.hascore && !(m.rnod[jre] > m.roc))
       return gm_fail(c, GM_ERR_UNSUPPORTED, "receiver node below the core-mantle boundary is not supported");
   }
+  // multi-source batch: every source node takes the receiver's place as the jump node of the adjoints
+  const bool aski = nsnod >= 0;
+  const int nsrc = aski ? nsnod + 1 : 1;
+  if (aski)
+    for (int js = 0; js < nsnod; ++js) {
+      const int jsh = jsnod[js];
+      if (m.state[jsh] != 0) return gm_fail(c, GM_ERR_UNSUPPORTED, "source node %d lies in a fluid layer", jsh);
+      if (m.hascore && !(m.rnod[jsh] > m.roc))
+        return gm_fail(c, GM_ERR_UNSUPPORTED, "source node %d below the core-mantle boundary is not supported", jsh);
+    }
   // nodes in a fluid layer or the inner core take the adjoint's end values, inner-core nodes a fifth job
   bool core_nodes = false, ic_nodes = false;
   for (int k = 1; k <= nnod; ++k) {
@@ -399,7 +409,9 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
       }
   }
   const size_t n = P.ninst;
-  const int nsp = istyp == 1 ? 10 : 5;
+  // spectra per node: U, V, W (moment 10, force 5); multi-source: U, V, dU/dr, dV/dr, W, dW/dr (20, 10)
+  const int nsp = aski ? (istyp == 1 ? 20 : 10) : (istyp == 1 ? 10 : 5);
+  const size_t nout = (size_t)nsrc * nsp * nnod * n;
   HIPCHK(c, hipSetDevice(c->device));
   // device buffer layout
   size_t off = 0;
@@ -419,12 +431,12 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   size_t o_ujs = full ? take(16 * 16 * nnod * n + 16) : 0;
   size_t o_adje = (full && core_nodes) ? take(16 * 8 * n + 16) : 0;
   bool dev = (flags & GM_FLAG_DEVICE_PTRS) != 0;
-  size_t o_gf = (full && (gf && !dev)) ? take(16 * nsp * nnod * n + 16) : 0;
-  size_t o_gfre = (full && (gfre && !dev)) ? take(4 * nsp * nnod * n + 4) : 0;
-  size_t o_gfim = (full && (gfim && !dev)) ? take(4 * nsp * nnod * n + 4) : 0;
+  size_t o_gf = (full && (gf && !dev)) ? take(16 * nout + 16) : 0;
+  size_t o_gfre = (full && (gfre && !dev)) ? take(4 * nout + 4) : 0;
+  size_t o_gfim = (full && (gfim && !dev)) ? take(4 * nout + 4) : 0;
   size_t o_hist = take(sizeof(unsigned long long) * 6 * 8);
   size_t o_uq = take(sizeof(int) * (nunits + 1)), o_ust = take(sizeof(int) * (nunits + 1));
-  size_t o_ue = take(sizeof(int) * (nunits + 1)), o_uc = take(sizeof(int) * GM_NCOUNTERS);
+  size_t o_ue = take(sizeof(int) * (nunits + 1)), o_uc = take(sizeof(int) * GM_NCOUNTERS * nsrc);
   const int ctl_mode = c->log_max > 0 ? 1 : (c->prop_enabled == 1 ? 2 : 0);
   // per-instance policy: the proposer network's (enable 2) or a fixed target try count
   const bool pol_net = ctl_mode == 0 && c->prop_enabled == 2;
@@ -466,6 +478,12 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
     HIPCHK(c, hipMemcpyAsync(base + o_ue, P.ue.data(), sizeof(int) * nunits, hipMemcpyHostToDevice, s));
     std::vector<unsigned long long> zero(48, 0ULL);
     HIPCHK(c, hipMemcpyAsync(base + o_hist, zero.data(), sizeof(unsigned long long) * 48, hipMemcpyHostToDevice, s));
+    // the work counters of the sources after the first (k_setup clears the first block)
+    if (nsrc > 1) {
+      std::vector<int> zc((size_t)GM_NCOUNTERS * (nsrc - 1), 0);
+      HIPCHK(c, hipMemcpyAsync(base + o_uc + sizeof(int) * GM_NCOUNTERS, zc.data(), sizeof(int) * zc.size(),
+                               hipMemcpyHostToDevice, s));
+    }
   }
   GmBatch B;
   memset(&B, 0, sizeof B);
@@ -518,7 +536,7 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   // the kernel stages as one hipGraph (plain control, no policy): the uploads above and the copies
   // below stay on the stream; every stage clears what it accumulates with kernels of its own, so a
   // replay needs no memset node
-  const bool use_graph = c->graph && ctl_mode == 0 && !pol && !na;
+  const bool use_graph = c->graph && ctl_mode == 0 && !pol && !na && !aski;
   if (use_graph) {
     struct Shape { int nfreq, attmode, pgrid, waves, variant, pers, full; size_t lds; const void* dM; const void* dF;
                    const void* dT; } sh;
@@ -569,7 +587,20 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
     HIPCHK(c, hipEventRecord(c->ev[3], s));
     launch(pers ? 7 : 3, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
     HIPCHK(c, hipEventRecord(c->ev[4], s));
-    if (full) {
+    if (full && aski) {
+      // per source: the adjoints of a jump at its node, then the direct Green functions and their
+      // derivatives into its output block (the last source is a FORCE at the receiver node)
+      for (int js = 0; js < nsrc; ++js) {
+        GmBatch b = B;
+        b.jre = js < nsnod ? jsnod[js] : jre;
+        b.unit_counter = B.unit_counter + (size_t)GM_NCOUNTERS * js;
+        b.aski_js = js;
+        b.aski_styp = js < nsnod ? istyp : 0;
+        b.aski_nspmax = nsp;
+        launch(4, dM, b, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
+        launch(8, dM, b, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
+      }
+    } else if (full) {
       launch(4, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
       launch(5, dM, B, nfreq, dF, attmode, dT, s, pgrid, lds, c->waves);
     }
@@ -591,9 +622,9 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   HIPCHK(c, cp(dlg, o_dlg, 8 * nnod * n));
   HIPCHK(c, cp(detsph, o_dets, 16 * nnod * n));
   if (full && !dev) {
-    if (gf) HIPCHK(c, hipMemcpyAsync(gf, base + o_gf, 16 * nsp * nnod * n, hipMemcpyDeviceToHost, s));
-    if (gfre) HIPCHK(c, hipMemcpyAsync(gfre, base + o_gfre, 4 * nsp * nnod * n, hipMemcpyDeviceToHost, s));
-    if (gfim) HIPCHK(c, hipMemcpyAsync(gfim, base + o_gfim, 4 * nsp * nnod * n, hipMemcpyDeviceToHost, s));
+    if (gf) HIPCHK(c, hipMemcpyAsync(gf, base + o_gf, 16 * nout, hipMemcpyDeviceToHost, s));
+    if (gfre) HIPCHK(c, hipMemcpyAsync(gfre, base + o_gfre, 4 * nout, hipMemcpyDeviceToHost, s));
+    if (gfim) HIPCHK(c, hipMemcpyAsync(gfim, base + o_gfim, 4 * nout, hipMemcpyDeviceToHost, s));
   }
   int first = 0;
   std::vector<int> hstatus;
@@ -668,6 +699,29 @@ extern "C" int gm_gfk_batch_wn(gm_ctx* c, int nfreq, const double* freqs, const 
   if (!(dwn > 0.0)) return gm_fail(c, GM_ERR_ARG, "dwn must be positive");
   return run(c, nfreq, freqs, iwnmin, nwn, dwn, sigma, eps, attmode, jre, jre, jre, istyp, true, flags, nullptr,
              nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, gf, gfre, gfim, status, stats);
+}
+
+extern "C" int gm_aski_batch(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const int* nl, double sigma,
+                             double eps, int attmode, int nsnod, const int* jsnod, int jre, int istyp, int flags,
+                             double* gf, float* gfre, float* gfim, int* status, long long* stats) {
+  if (!c) return GM_ERR_ARG;
+  if (!c->has_model || !c->has_nodes) return gm_fail(c, GM_ERR_ARG, "model and nodes must be set first");
+  const int nnod = c->hmodel.nnod;
+  if (nsnod < 1 || !jsnod) return gm_fail(c, GM_ERR_ARG, "at least one source node is required");
+  if (jre < 1 || jre > nnod) return gm_fail(c, GM_ERR_NODEIDX, "Invalid index for receiver node");
+  int jsl = nnod + 1, jsr = 0;
+  bool jre_in = false;
+  for (int js = 0; js < nsnod; ++js) {
+    if (jsnod[js] < 1 || jsnod[js] > nnod) return gm_fail(c, GM_ERR_NODEIDX, "Invalid index for source node");
+    jsl = std::min(jsl, jsnod[js]);
+    jsr = std::max(jsr, jsnod[js]);
+    if (jsnod[js] == jre) jre_in = true;
+  }
+  // computeGreenFKSpectraForASKI.f90:287-290
+  if (!jre_in) return gm_fail(c, GM_ERR_ARG, "receiver node must also be a source node");
+  return run(c, nfreq, freqs, lmin, nl, 0.0, sigma, eps, attmode, jsl, jsr, jre, istyp, true, flags, nullptr,
+             nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, gf, gfre, gfim, status, stats, nullptr,
+             nsnod, jsnod);
 }
 
 // elasticConstantsExternalRadialNodes (externalRadialNodes.f90:422-433) for

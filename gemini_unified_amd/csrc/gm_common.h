@@ -143,6 +143,9 @@ struct GmBatch {
   // targets 6) and the first step of the spheroidal up pass in metres (<= 0:
   // the start value's lambda/4); read by the toroidal and spheroidal kernels
   const float* __restrict__ policy;
+  // multi-source assembly (k_aski_assemble): output block of this source, its
+  // source type and the spectra per node of the output layout
+  int aski_js, aski_styp, aski_nspmax;
 };
 #define GM_LOGREC 20
 // work counters (GmBatch::unit_counter): persistent toroidal / spheroidal

@@ -236,6 +236,33 @@ class GreenFKEngine:
             check(self.h, rc)
         return dict(gf=gf, gfre=gfre, gfim=gfim, status=status, stats=st)
 
+    def aski(self, freqs, lmin, nl, sigma, eps, jsnod, jre, istyp=1, attmode="ATTENUATION_AND_DISPERSION",
+             want_fp32=True, raise_on_error=True, want_gf=True):
+        """Multi-source columns of computeGreenFKSpectraForASKI (gm_aski_batch): for the sources at the
+        1-based nodes jsnod (MOMENT istyp = 1 or FORCE 0) and a FORCE at the receiver node jre (last),
+        gf[js, node, isp, k] with isp = U(j) V(j) dU/dr(j) dV/dr(j) W(j) dW/dr(j), 20 (moment) or 10
+        (force) spectra per node, zero beyond a source's own count."""
+        freqs, lmin, nl, nk = self._plan(freqs, lmin, nl)
+        nn = self.nodes.nnod
+        js = np.ascontiguousarray(jsnod, dtype=np.int32)
+        nsrc, nsp = js.size + 1, (20 if istyp == 1 else 10)
+        gf = np.zeros((nsrc, nn, nsp, nk), np.complex128) if want_gf else None
+        gfre = np.zeros((nsrc, nn, nsp, nk), np.float32) if want_fp32 else None
+        gfim = np.zeros((nsrc, nn, nsp, nk), np.float32) if want_fp32 else None
+        status = np.zeros(nk, np.int32)
+        st = np.zeros(10, np.int64)
+        am = ATTMODES[attmode] if isinstance(attmode, str) else int(attmode)
+        rc = _lib.lib().gm_aski_batch(self.h, freqs.size, freqs.ctypes.data_as(D), lmin.ctypes.data_as(I),
+                                      nl.ctypes.data_as(I), float(sigma), float(eps), am, int(js.size),
+                                      js.ctypes.data_as(I), int(jre), int(istyp), 0, _dp(gf),
+                                      None if gfre is None else gfre.ctypes.data_as(F),
+                                      None if gfim is None else gfim.ctypes.data_as(F), status.ctypes.data_as(I),
+                                      st.ctypes.data_as(LL))
+        self.stats += st
+        if raise_on_error:
+            check(self.h, rc)
+        return dict(gf=gf, gfre=gfre, gfim=gfim, status=status, stats=st)
+
     def node_properties(self, f=None, attmode="ELASTIC"):
         """elasticConstantsExternalRadialNodes (externalRadialNodes.f90:422-433): density ro[nnod],
         complex moduli elcon[nnod, 7] (A, C, F, L, N, kappa, mu) and inverse Q qinv[nnod, 2] at the

@@ -179,6 +179,27 @@ int gm_gfk_batch_wn(gm_ctx* ctx, int nfreq, const double* freqs, const int* iwnm
                     double sigma, double eps, int attmode, int jre, int istyp, int flags, double* gf, float* gfre,
                     float* gfim, int* status, long long* stats);
 
+/* Multi-source Green-FK columns with radial derivatives: the degree loop body
+ * of computeGreenFKSpectraForASKI (f90/computeGreenFKSpectraForASKI.f90:
+ * 458-590).  Minors over the source node range min(jsnod)..max(jsnod)
+ * (computeToroidalMinors / computeSpheroidalMinors with jsl < jsr); then for
+ * each of the nsnod + 1 sources (nsnod MOMENT (istyp = 1) or FORCE (0) sources
+ * at the 1-based nodes jsnod[js], and a FORCE at the receiver node jre, which
+ * must be one of jsnod) the unit-jump Green functions of a jump at that node
+ * (computeUnitJumpGreenFunctions, f90/unitJumpGreenFunctions.f90:62), the
+ * source terms there (oneNodeSourceTerms, f90/sourceTerms.f90:59), and at
+ * every node the direct Green functions U, V and W with their radial
+ * derivatives (computeDsvDerivatives, f90/dsvDerivatives.f90:38; at fluid
+ * nodes V and dV/dr from getHorcompComplexFluid, f90/fluidOdeSystem.f90:171).
+ * Outputs [nsnod+1][nnod][nspmax][k] (the reference's gfre(nwn, nspmax, nnod,
+ * nsnod+1)), nspmax = 20 for istyp = 1 and 10 for istyp = 0; per node
+ * isp = U(j) V(j) dU/dr(j) dV/dr(j) W(j) dW/dr(j) with njsph = 4 / 2 and
+ * njtor = 2 / 1 jumps (moment / force), zero beyond a source's
+ * 4 njsph + 2 njtor.  Source nodes must lie in the solid mantle. */
+int gm_aski_batch(gm_ctx* ctx, int nfreq, const double* freqs, const int* lmin, const int* nl, double sigma,
+                  double eps, int attmode, int nsnod, const int* jsnod, int jre, int istyp, int flags, double* gf,
+                  float* gfre, float* gfim, int* status, long long* stats);
+
 /* Kernel time of the last batch call in milliseconds (HIP events on the
  * context stream), per stage: 0 tables, 1 setup, 2 toroidal, 3 spheroidal,
  * 4 adjoint+assembly, 5 total. */

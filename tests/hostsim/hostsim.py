@@ -42,6 +42,7 @@ def lib():
         L.sim_proposer_new.argtypes = [F, C.c_float, C.c_int, C.c_int]
         L.sim_proposer_free.argtypes = [C.c_void_p]
         L.sim_set_adjoint_pair.argtypes = [C.c_int]
+        L.sim_set_sources.argtypes = [C.c_int, I]
         for v in VARIANTS:
             fn = getattr(L, "sim_run_" + v)
             fn.argtypes = [C.c_void_p, C.c_int, D, I, I, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
@@ -68,6 +69,7 @@ class HostSim:
                                  k[10].ctypes.data_as(D), i[2].ctypes.data_as(I), i[3].ctypes.data_as(I))
         assert self.m
         self.prop, self.mlp = None, None
+        self._sources = None
         self.policy_net, self.order_target = False, 6
 
     def __del__(self):
@@ -104,10 +106,15 @@ class HostSim:
         nl = np.ascontiguousarray(np.broadcast_to(nl, (nf,)), dtype=np.int32)
         n, nn = int(nl.sum()), self.nodes.nnod
         nsp = 10 if istyp == 1 else 5
+        nsrc = 1
+        if self._sources is not None:
+            nsp, nsrc = (20 if istyp == 1 else 10), len(self._sources) + 1
         o = dict(u_tor=np.zeros((nn, 2, n), complex), d_tor=np.zeros((nn, 2, n), complex),
                  det_tor=np.zeros((nn, n), complex), u_sph=np.zeros((nn, 6, n), complex),
                  d_sph=np.zeros((nn, 6, n), complex), ulg=np.zeros((nn, n)), dlg=np.zeros((nn, n)),
-                 det_sph=np.zeros((nn, n), complex), gf=np.zeros((nn, nsp, n), complex) if full else None,
+                 det_sph=np.zeros((nn, n), complex),
+                 gf=(np.zeros((nn, nsp, n), complex) if nsrc == 1 else np.zeros((nsrc, nn, nsp, n), complex))
+                 if full else None,
                  status=np.zeros(n, np.int32), hist=np.zeros(48, np.uint64))
         log = np.zeros((max(logmax, 1), 20), np.float32)
         nlog = C.c_int(0)
@@ -131,6 +138,17 @@ class HostSim:
 
     def gfk(self, freqs, lmin, nl, sigma, eps, jre, istyp=1, dwn=0.0, learned=False):
         return self._run(freqs, lmin, nl, sigma, eps, jre, jre, jre, istyp, True, 2 if learned else 0, dwn)
+
+    def aski(self, freqs, lmin, nl, sigma, eps, jsnod, jre, istyp=1):
+        """Multi-source columns (gm_aski_batch): gf[js, node, isp, instance]."""
+        js = np.ascontiguousarray(jsnod, dtype=np.int32)
+        self._sources = js
+        lib().sim_set_sources(len(js), js.ctypes.data_as(I))
+        try:
+            return self._run(freqs, lmin, nl, sigma, eps, int(js.min()), int(js.max()), jre, istyp, True, 0)
+        finally:
+            self._sources = None
+            lib().sim_set_sources(-1, js.ctypes.data_as(I))
 
     def steplog(self, freqs, lmin, nl, sigma, eps, jsl, jsr, maxrec, dwn=0.0):
         return self._run(freqs, lmin, nl, sigma, eps, jsl, jsr, jsr, 1, False, 1, dwn, logmax=maxrec)

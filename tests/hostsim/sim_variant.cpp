@@ -6,6 +6,8 @@
 #include <vector>
 #include "gm_common.h"
 extern "C" int g_sim_adj_pair;   // sim_api.cpp: GM_OPT_ADJOINT_PAIR of the CPU build
+extern "C" int g_sim_nsnod;      // sim_api.cpp: source nodes of a multi-source run (-1: none)
+extern "C" int g_sim_jsnod[];
 #define GM_NS SIM_NS
 namespace SIM_NS {
 static constexpr bool kFast = SIM_FAST != 0;

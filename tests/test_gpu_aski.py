@@ -1,0 +1,74 @@
+"""GPU parity of the multi-source batch (gm_aski_batch, the degree loop of
+computeGreenFKSpectraForASKI.f90:458-590) against the CPU oracle: sources at three
+mantle depths and a force at the receiver, nodes in the inner core, the fluid outer
+core and the mantle."""
+import numpy as np
+import pytest
+
+import oracle
+from gemini_unified_amd import MC_PI, ExternalRadialNodes, GreenFKEngine, global_model
+from gemini_unified_amd._lib import GeminiFKError
+
+pytestmark = pytest.mark.gpu
+
+SIGMA = 5.0 / 2048.0
+CASES = [(40 / 2048.0, 0, 40), (10 / 2048.0, 1, 20), (90 / 2048.0, 250, 30)]
+
+
+def rel_l2(a, b):
+    k = a.shape[-1]
+    num = np.linalg.norm((a - b).reshape(-1, k), axis=0)
+    den = np.linalg.norm(b.reshape(-1, k), axis=0)
+    return np.where(den > 0, num / np.where(den > 0, den, 1), num)
+
+
+@pytest.fixture(scope="module")
+def case():
+    nem = global_model(1, 200)
+    prn = np.array([600e3, 2000e3, 3000e3, 5000e3, nem.rearth - 300e3, nem.rearth - 40e3, nem.rearth - 15e3,
+                    nem.rearth])
+    nodes = ExternalRadialNodes.split_at_discos(nem, prn, nem.rearth)
+    jre = nodes.node_index(nem.rearth)
+    jsnod = [nodes.node_index(nem.rearth - d) for d in (15e3, 300e3, 40e3)] + [jre]
+    return nem, nodes, jre, jsnod
+
+
+@pytest.mark.parametrize("istyp", [1, 0])
+@pytest.mark.parametrize("f,lmin,nl", CASES)
+def test_aski_matches_oracle(case, istyp, f, lmin, nl):
+    nem, nodes, jre, jsnod = case
+    print(f"aski istyp={istyp} f={f:.5f} l={lmin}..{lmin + nl - 1}", flush=True)
+    orc = oracle.Oracle(nem, nodes)
+    orc.set_frequency(f, 3)
+    orc.stats[:] = 0
+    rc, ref, _ = orc.aski_degrees(2 * MC_PI * f, -SIGMA, lmin, nl, jsnod, jre, istyp, 1e-6)
+    assert rc == 0
+    ex = GreenFKEngine(nem, nodes, variant="exact")
+    r = ex.aski([f], lmin, nl, SIGMA, 1e-6, jsnod, jre, istyp)
+    assert (r["status"] == 0).all()
+    # exact variant: bit for bit, work counters included; FP32 outputs are the casts
+    assert np.array_equal(r["gf"], ref)
+    np.testing.assert_array_equal(r["stats"], orc.stats)
+    np.testing.assert_array_equal(r["gfre"], ref.real.astype(np.float32))
+    np.testing.assert_array_equal(r["gfim"], ref.imag.astype(np.float32))
+    ex.close()
+    fa = GreenFKEngine(nem, nodes, variant="fast")
+    g = fa.aski([f], lmin, nl, SIGMA, 1e-6, jsnod, jre, istyp)["gf"]
+    err = rel_l2(g, ref).max()
+    print(f"  fast variant max rel-L2 {err:.2e}")
+    assert err < 1e-8
+    fa.close()
+
+
+def test_aski_rejects_bad_sources(case):
+    nem, nodes, jre, jsnod = case
+    eng = GreenFKEngine(nem, nodes, variant="fast")
+    # the receiver node must be one of the sources (computeGreenFKSpectraForASKI.f90:287-290)
+    with pytest.raises(GeminiFKError):
+        eng.aski([0.02], 0, 4, SIGMA, 1e-6, jsnod[:-1], jre)
+    # a source in the fluid outer core is not supported
+    fluid = int(np.flatnonzero(nodes.state == 1)[0]) + 1
+    with pytest.raises(GeminiFKError) as ei:
+        eng.aski([0.02], 0, 4, SIGMA, 1e-6, [fluid, jre], jre)
+    assert ei.value.code == 8
+    eng.close()

@@ -290,3 +290,32 @@ def test_adjoint_pairs_keep_the_error(case):
             assert res[1]["stats"][0] < res[0]["stats"][0]
     finally:
         hostsim.lib().sim_set_adjoint_pair(0)
+
+
+@pytest.mark.parametrize("istyp", [1, 0])
+def test_kernel_source_multi_source_matches_oracle(istyp):
+    """Multi-source batch (gm_aski_batch) from the kernel source: the adjoints of a jump at each
+    source node in turn and k_aski_assemble's direct Green functions with their radial derivatives,
+    bit for bit with the oracle (exact variant) and work counters included; nodes in the inner core,
+    the fluid outer core and the mantle, sources at three mantle depths and the receiver."""
+    nem = global_model(1, 200)
+    prn = np.array([600e3, 2000e3, 3000e3, 5000e3, nem.rearth - 300e3, nem.rearth - 40e3, nem.rearth - 15e3,
+                    nem.rearth])
+    nodes = ExternalRadialNodes.split_at_discos(nem, prn, nem.rearth)
+    jre = nodes.node_index(nem.rearth)
+    jsnod = [nodes.node_index(nem.rearth - d) for d in (15e3, 300e3, 40e3)] + [jre]
+    for variant in ("exact", "fast"):
+        for f, lmin, nl in [(40 / 2048.0, 0, 10), (10 / 2048.0, 1, 5), (60 / 2048.0, 120, 5)]:
+            r = HostSim(nem, nodes, variant).aski([f], lmin, nl, 5 / 2048.0, 1e-6, jsnod, jre, istyp)
+            orc = oracle.Oracle(nem, nodes)
+            orc.set_frequency(f, 3)
+            orc.stats[:] = 0
+            rc, ref, st = orc.aski_degrees(2 * MC_PI * f, -5 / 2048.0, lmin, nl, jsnod, jre, istyp, 1e-6)
+            assert rc == 0 and np.all(r["status"] == 0)
+            g = r["gf"]
+            assert g.shape == ref.shape
+            if variant == "exact":
+                assert np.array_equal(g, ref)
+                np.testing.assert_array_equal(r["stats"], orc.stats)
+            else:
+                assert _rel_l2(g, ref).max() < 1e-12
