@@ -291,6 +291,39 @@ def legendre(lmax, theta):
     return out
 
 
+def strain_spectra(istyp, nf1, nf2, nwn, gr, gi, theta, phi, re, fomt, elcon=None, tapfrac=0.2, nthreads=1):
+    """Strain spectra of one node (spectra with radial derivatives, nsp 20 moment / 10 force):
+    dict hdsum [nrcv, nf, nsum], zsp [nrcv, 3, nf2], strain [nrcv, 3, 3, nf2], stress (with elcon)."""
+    theta = np.ascontiguousarray(np.atleast_1d(theta), dtype=np.float64)
+    phi = np.ascontiguousarray(np.broadcast_to(phi, theta.shape), dtype=np.float64)
+    nrcv, nf, nsum = theta.size, nf2 - nf1 + 1, 42 if istyp == 1 else 21
+    gr = np.ascontiguousarray(gr, dtype=np.float32)
+    gi = np.ascontiguousarray(gi, dtype=np.float32)
+    nkf = gr.shape[1]
+    nw = np.ascontiguousarray(nwn, dtype=np.int32)
+    out = dict(hdsum=np.zeros((nrcv, nf, nsum), np.complex128), zsp=np.zeros((nrcv, 3, nf2), np.complex128),
+               strain=np.zeros((nrcv, 3, 3, nf2), np.complex128),
+               stress=np.zeros((nrcv, 3, 3, nf2), np.complex128) if elcon is not None else None)
+    el = None if elcon is None else np.ascontiguousarray(elcon, dtype=np.complex128)
+    rc = lib().orc_strain_spectra(int(istyp), int(nf1), int(nf2), nw.ctypes.data_as(I), C.c_long(nkf),
+                                  gr.ctypes.data_as(C.POINTER(C.c_float)), gi.ctypes.data_as(C.POINTER(C.c_float)),
+                                  nrcv, _d(theta), _d(phi), C.c_double(re), _d(fomt),
+                                  None if el is None else _c(el), C.c_double(tapfrac), _c(out["hdsum"]),
+                                  _c(out["zsp"]), _c(out["strain"]),
+                                  None if el is None else _c(out["stress"]), int(nthreads))
+    assert rc == 0
+    return out
+
+
+def spherical_strain(u, r, delta):
+    """sphericalStrainSingleFrequencyDisplacement: u[4, 3] complex (u, d/dr u, 1/r d/dtheta u,
+    1/(r sin theta) d/dphi u) -> strain [3, 3] complex."""
+    u = np.ascontiguousarray(u, dtype=np.complex128)
+    e = np.zeros((3, 3), np.complex128)
+    lib().orc_spherical_strain(_c(u), C.c_double(r), C.c_double(delta), _c(e))
+    return e
+
+
 def taper(lmax, tapfrac=0.2):
     w = np.zeros(lmax + 1)
     lib().orc_taper(int(lmax), C.c_double(tapfrac), _d(w))

@@ -183,6 +183,193 @@ void force_displacement(double phi, const double* f, const cd* w, cd* zsp) {
   zsp[2] = (w[5] + w[6]) * b;
 }
 
+// ---------------------------------------------------------------------------
+// Strain spectra (spectra with radial derivatives: derivflag = 1, the files of
+// the multi-source driver).  Restates momentHarmonicDegreeSums /
+// forceHarmonicDegreeSums for every kint (f90/harmonicDegreeSums.f90:95-351),
+// getIndexArrayHarmonicDegreeSums (:375-420), the sum -> (component, jump)
+// maps (:34-46) with positionDataGreenFKSpectra (f90/greenFKSpectra.f90:
+// 204-212), the loop of sphericalHarmonicsDisplacementSpectra with strainsp
+// (f90/displacementSpectra.f90:49-106, 200-228) and the routines of
+// f90/singleFrequencyDisplacement.f90:35-278.
+// ---------------------------------------------------------------------------
+// the reference's (component, jump) of each sum: moment kint 1..46, force 1..23
+static const int kMomentMap[46][2] = {
+    {1, 1}, {1, 2}, {1, 3}, {1, 4}, {3, 1}, {3, 2}, {3, 3}, {3, 4}, {7, 1}, {7, 2}, {7, 1}, {7, 2}, {3, 3}, {3, 4},
+    {5, 1}, {5, 2}, {5, 3}, {5, 4}, {6, 1}, {6, 2}, {6, 3}, {6, 4}, {9, 1}, {9, 2}, {9, 1}, {9, 2}, {6, 3}, {6, 4},
+    {1, 1}, {1, 2}, {1, 3}, {1, 4}, {3, 1}, {3, 2}, {3, 3}, {3, 4}, {7, 1}, {7, 2}, {7, 1}, {7, 2}, {3, 3}, {3, 4},
+    {2, 1}, {2, 2}, {2, 3}, {2, 4}};
+static const int kForceMap[23][2] = {
+    {1, 1}, {1, 2}, {3, 1}, {3, 2}, {7, 1}, {7, 1}, {3, 2}, {5, 1}, {5, 2}, {6, 1}, {6, 2}, {9, 1}, {9, 1}, {6, 2},
+    {1, 1}, {1, 2}, {3, 1}, {3, 2}, {7, 1}, {7, 1}, {3, 2}, {2, 1}, {2, 2}};
+// Legendre rows: 0 P_l0, 1 P_l1, 2 P_l2, 3 dP_l0, 4 dP_l1, 5 dP_l2, 6 P_l1/sin, 7 P_l2/sin,
+// 8 d2P_l0, 9 d2P_l1, 10 d2P_l2, 11 d(P_l1/sin), 12 d(P_l2/sin); -1: the sum is not in the case table (0)
+struct KintDef { int basis, lmin; double fac; bool over_ll1; };
+static KintDef momentKint(int k) {
+  switch (k) {
+    case 1: case 2: case 15: case 16: return {0, 0, 1.0, false};
+    case 3: case 17: return {1, 1, 2.0, true};
+    case 4: case 18: return {2, 2, 2.0, true};
+    case 5: case 19: case 29: case 6: case 20: case 30: return {3, 1, 1.0, false};
+    case 7: case 21: case 31: return {4, 1, 2.0, true};
+    case 8: case 22: case 32: return {5, 2, 2.0, true};
+    case 9: case 23: return {6, 1, 2.0, true};
+    case 10: case 24: return {7, 2, 4.0, true};
+    case 11: case 25: return {4, 1, 2.0, true};
+    case 12: case 26: return {5, 2, -2.0, true};
+    case 13: case 27: return {6, 1, 2.0, true};
+    case 14: case 28: return {7, 2, -4.0, true};
+    case 33: case 34: return {8, 1, 1.0, false};
+    case 35: return {9, 1, 2.0, true};
+    case 36: return {10, 2, 2.0, true};
+    case 37: return {11, 1, 2.0, true};
+    case 38: return {12, 2, 4.0, true};
+    case 39: return {9, 1, 2.0, true};
+    case 40: return {10, 2, -2.0, true};
+    case 41: return {11, 1, 2.0, true};
+    case 42: return {12, 2, -4.0, true};
+    default: return {-1, 0, 0.0, false};
+  }
+}
+static KintDef forceKint(int k) {
+  switch (k) {
+    case 1: case 8: return {0, 0, 1.0, false};
+    case 2: case 9: return {1, 1, -2.0, true};
+    case 3: case 10: case 15: return {3, 1, 1.0, false};
+    case 4: case 11: case 16: return {4, 1, -2.0, true};
+    case 5: case 12: return {6, 1, -2.0, true};
+    case 6: case 13: return {4, 1, 2.0, true};
+    case 7: case 14: return {6, 1, 2.0, true};
+    case 17: return {8, 1, 1.0, false};
+    case 18: return {9, 1, -2.0, true};
+    case 19: return {11, 1, -2.0, true};
+    case 20: return {9, 1, 2.0, true};
+    case 21: return {11, 1, 2.0, true};
+    default: return {-1, 0, 0.0, false};
+  }
+}
+// 0-based spectrum offset of (component isp, jump jsp) for dsvmask (U, V, UP, VP, W, WP)
+// = (1,0,1,0,1,1,1,0,1) (greenFKSpectra.f90:109-111 with derivflag = 1, dsvstep = 2)
+static int positionData(int isp, int jsp, int njsph, int njtor) {
+  static const int mask[10] = {0, 1, 0, 1, 0, 1, 1, 1, 0, 1};
+  int off = 0;
+  if (isp <= 6) {
+    for (int k = 1; k < isp; ++k) off += mask[k] * njsph;
+  } else {
+    for (int k = 1; k <= 6; ++k) off += mask[k] * njsph;
+    for (int k = 7; k < isp; ++k) off += mask[k] * njtor;
+  }
+  return off + jsp - 1;
+}
+
+// All degree sums of one receiver (kint 1..42 moment, 1..21 force: derivatives
+// without stresses, getIndexArrayHarmonicDegreeSums with dsvmask(2) = 0,
+// dsvmask(5) = 1); hdsum: [nf][nsum] complex
+void strain_sums(int istyp, int nf1, int nf2, const int* nwn, long nkf, const float* gr, const float* gi,
+                 double theta, double tapfrac, cd* hdsum) {
+  const int nsum = istyp == 1 ? 42 : 21, njsph = istyp == 1 ? 4 : 2, njtor = istyp == 1 ? 2 : 1;
+  int lmax = 0;
+  for (int jf = nf1; jf <= nf2; ++jf) lmax = std::max(lmax, nwn[jf - 1] - 1);
+  const int L1 = lmax + 1;
+  std::vector<double> plm(3 * L1), dpdt(3 * L1), pdst(3 * L1), d2(3 * L1), ds(3 * L1), w(L1);
+  legendre(lmax, theta, plm.data(), dpdt.data(), pdst.data(), d2.data(), ds.data());
+  const double* rows[13] = {&plm[0], &plm[L1], &plm[2 * L1], &dpdt[0], &dpdt[L1], &dpdt[2 * L1], &pdst[L1],
+                            &pdst[2 * L1], &d2[0], &d2[L1], &d2[2 * L1], &ds[L1], &ds[2 * L1]};
+  const double fourpi = 4.0 * kPi;
+  long off = 0;
+  for (int jf = nf1; jf <= nf2; ++jf) {
+    const int n = nwn[jf - 1];
+    const int lm = n - 1;
+    taper(lm, tapfrac, w.data());
+    for (int k = 1; k <= nsum; ++k) {
+      const KintDef d = istyp == 1 ? momentKint(k) : forceKint(k);
+      const int* cj = istyp == 1 ? kMomentMap[k - 1] : kForceMap[k - 1];
+      const int isp = positionData(cj[0], cj[1], njsph, njtor);
+      cd z = 0.0;
+      if (d.basis >= 0)
+        for (int l = d.lmin; l <= lm; ++l) {
+          double b = rows[d.basis][l];
+          if (d.over_ll1) b = b * d.fac / double(l * (l + 1));
+          const cd g(gr[(long)isp * nkf + off + l], gi[(long)isp * nkf + off + l]);
+          z += b * g * (2.0 * l + 1.0) / fourpi * w[l];
+        }
+      hdsum[(long)(jf - nf1) * nsum + k - 1] = z;
+    }
+    off += n;
+  }
+}
+
+// displacement and its derivatives from the sums: d = 0 the displacement
+// (moment/forceSingleFrequencyDisplacement), 1 d/dr (RadDeriv), 2 1/r d/dtheta
+// (ThetaDeriv), 3 1/(r sin theta) d/dphi (PhiDeriv); f90/singleFrequencyDisplacement.f90:35-194
+void strain_parts(int istyp, double re, double delta, double phi, const double* fm, const cd* w, cd u[4][3]) {
+  const double sp = std::sin(phi), cp = std::cos(phi);
+  if (istyp == 1) {
+    const double s2 = std::sin(2. * phi), c2 = std::cos(2. * phi);
+    const double a = fm[3] * cp + fm[4] * sp, b = (fm[2] - fm[1]) * c2 - 2. * fm[5] * s2;
+    const double c = fm[4] * cp - fm[3] * sp, d = (fm[2] - fm[1]) * s2 + 2. * fm[5] * c2;
+    for (int part = 0; part < 3; ++part) {
+      const cd* h = w + 14 * part;
+      u[part][0] = h[0] * fm[0] + h[1] * (fm[1] + fm[2]) + h[2] * a + h[3] * b;
+      u[part][1] = h[4] * fm[0] + h[5] * (fm[1] + fm[2]) + (h[6] + h[8]) * a + (h[7] + h[9]) * b;
+      u[part][2] = (h[10] + h[12]) * c + (h[11] + h[13]) * d;
+    }
+    // kint 1..14, then 15..28 (d/dr), 29..42 (d/dtheta) in the loop above: parts 0, 1, 2
+    for (int c3 = 0; c3 < 3; ++c3) u[2][c3] = u[2][c3] / re;
+    const double rst = re * std::sin(delta);
+    const double ap = -fm[3] * sp + fm[4] * cp, bp = -2. * (fm[2] - fm[1]) * s2 - 4. * fm[5] * c2;
+    const double cq = -fm[4] * sp - fm[3] * cp, dq = +2. * (fm[2] - fm[1]) * c2 - 4. * fm[5] * s2;
+    u[3][0] = w[2] * ap + w[3] * bp;
+    u[3][1] = (w[6] + w[8]) * ap + (w[7] + w[9]) * bp;
+    u[3][2] = (w[10] + w[12]) * cq + (w[11] + w[13]) * dq;
+    for (int c3 = 0; c3 < 3; ++c3) u[3][c3] = u[3][c3] / rst;
+  } else {
+    const double a = fm[1] * cp + fm[2] * sp, b = fm[1] * sp - fm[2] * cp;
+    for (int part = 0; part < 3; ++part) {
+      const cd* h = w + 7 * part;
+      u[part][0] = h[0] * fm[0] + h[1] * a;
+      u[part][1] = h[2] * fm[0] + (h[3] + h[4]) * a;
+      u[part][2] = (h[5] + h[6]) * b;
+    }
+    for (int c3 = 0; c3 < 3; ++c3) u[2][c3] = u[2][c3] / re;
+    const double rst = re * std::sin(delta);
+    const double ap = -fm[1] * sp + fm[2] * cp, bp = fm[1] * cp + fm[2] * sp;
+    u[3][0] = w[1] * ap;
+    u[3][1] = (w[3] + w[4]) * ap;
+    u[3][2] = (w[5] + w[6]) * bp;
+    for (int c3 = 0; c3 < 3; ++c3) u[3][c3] = u[3][c3] / rst;
+  }
+}
+
+// sphericalStrainSingleFrequencyDisplacement (:223-241): strain[3][3] from
+// u, d/dr u, 1/r d/dtheta u, 1/(r sin) d/dphi u
+void spherical_strain(const cd u[4][3], double r, double delta, cd e[3][3]) {
+  const double rtandel = r * std::tan(delta);
+  const cd *z = u[0], *zr = u[1], *zt = u[2], *zf = u[3];
+  e[0][0] = zr[0];
+  e[1][1] = zt[1] + z[0] / r;
+  e[2][2] = zf[2] + z[0] / r + z[1] / rtandel;
+  e[0][1] = 0.5 * (zt[0] + zr[1] - z[1] / r);
+  e[0][2] = 0.5 * (zf[0] + zr[2] - z[2] / r);
+  e[1][2] = 0.5 * (zf[1] + zt[2] - z[2] / rtandel);
+  e[2][1] = e[1][2];
+  e[2][0] = e[0][2];
+  e[1][0] = e[0][1];
+}
+
+// sphericalStressSingleFrequencyDisplacement (:253-266) with elcon = (A, C, F, L, N, ...)
+void spherical_stress(const cd e[3][3], const cd* el, cd t[3][3]) {
+  t[0][0] = el[1] * e[0][0] + el[2] * (e[1][1] + e[2][2]);
+  t[1][1] = el[2] * e[0][0] + el[0] * (e[1][1] + e[2][2]) - 2.0 * el[4] * e[2][2];
+  t[2][2] = el[2] * e[0][0] + el[0] * (e[1][1] + e[2][2]) - 2.0 * el[4] * e[1][1];
+  t[0][1] = 2.0 * el[3] * e[0][1];
+  t[0][2] = 2.0 * el[3] * e[0][2];
+  t[1][2] = 2.0 * el[4] * e[1][2];
+  t[1][0] = t[0][1];
+  t[2][0] = t[0][2];
+  t[2][1] = t[1][2];
+}
+
 // In-place radix-2 complex FFT, e^{+i...} for is > 0, no normalisation.
 // Restates doubleFastFourierTransform (f90/fourierTransform.f90:118-192):
 // decimation in frequency with bit-reversed reordering at the end.
@@ -462,6 +649,55 @@ int orc_bessel_sums_src(int istyp, int nf1, int nf2, const int* nwn, long nkf, c
 int orc_bessel_sums(int nf1, int nf2, const int* nwn, long nkf, const float* gr, const float* gi, double dwn,
                     double re, int nrcv, const double* x, double tapfrac, double* wnint, int nthreads) {
   return orc_bessel_sums_src(1, nf1, nf2, nwn, nkf, gr, gi, dwn, re, nrcv, x, tapfrac, wnint, nthreads);
+}
+
+// Strain (and stress) spectra of one node for nrcv receivers at epicentral
+// distance theta[r] and azimuth phi[r], receiver radius re: istyp 1 (nsp 20:
+// U, V, UP, VP x 4 jumps, W, WP x 2) or 0 (nsp 10).  Outputs (any may be NULL):
+// hdsum [nrcv][nf][nsum] complex, zsp [nrcv][3][nf2], strain [nrcv][3][3][nf2]
+// and, with elcon (7 complex moduli at the receiver), stress [nrcv][3][3][nf2]
+// (the reference's zsp(nf2, 3) and strainsp(nf2, 3, 3) per receiver).
+int orc_strain_spectra(int istyp, int nf1, int nf2, const int* nwn, long nkf, const float* gr, const float* gi,
+                       int nrcv, const double* theta, const double* phi, double re, const double* fomt,
+                       const double* elcon, double tapfrac, double* hdsum, double* zsp, double* strain,
+                       double* stress, int nthreads) {
+  const int nf = nf2 - nf1 + 1, nsum = istyp == 1 ? 42 : 21;
+  const syn::cd* el = reinterpret_cast<const syn::cd*>(elcon);
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads > 0 ? nthreads : 1)
+  for (int r = 0; r < nrcv; ++r) {
+    std::vector<syn::cd> h((size_t)nf * nsum);
+    syn::strain_sums(istyp, nf1, nf2, nwn, nkf, gr, gi, theta[r], tapfrac, h.data());
+    if (hdsum) std::memcpy(hdsum + (size_t)r * nf * nsum * 2, h.data(), sizeof(syn::cd) * nf * nsum);
+    for (int jf = 1; jf <= nf2; ++jf) {
+      syn::cd u[4][3] = {}, e[3][3] = {}, t[3][3] = {};
+      if (jf >= nf1) {
+        syn::strain_parts(istyp, re, theta[r], phi[r], fomt, &h[(size_t)(jf - nf1) * nsum], u);
+        syn::spherical_strain(u, re, theta[r], e);
+        if (el) syn::spherical_stress(e, el, t);
+      }
+      for (int a = 0; a < 3; ++a) {
+        if (zsp) {
+          const size_t o = ((size_t)r * 3 + a) * nf2 + jf - 1;
+          zsp[2 * o] = u[0][a].real(); zsp[2 * o + 1] = u[0][a].imag();
+        }
+        for (int b = 0; b < 3; ++b) {
+          const size_t o = (((size_t)r * 3 + a) * 3 + b) * nf2 + jf - 1;
+          if (strain) { strain[2 * o] = e[a][b].real(); strain[2 * o + 1] = e[a][b].imag(); }
+          if (stress && el) { stress[2 * o] = t[a][b].real(); stress[2 * o + 1] = t[a][b].imag(); }
+        }
+      }
+    }
+  }
+  return 0;
+}
+
+// pure strain combination for checks: u[4][3] complex -> e[3][3] complex
+int orc_spherical_strain(const double* u, double r, double delta, double* e) {
+  syn::cd uu[4][3], ee[3][3];
+  for (int p = 0; p < 4; ++p) for (int c = 0; c < 3; ++c) uu[p][c] = syn::cd(u[2 * (3 * p + c)], u[2 * (3 * p + c) + 1]);
+  syn::spherical_strain(uu, r, delta, ee);
+  for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) { e[2 * (3 * a + b)] = ee[a][b].real(); e[2 * (3 * a + b) + 1] = ee[a][b].imag(); }
+  return 0;
 }
 
 // hfil: complex [nf2]; filspecs = (high-pass order, corner, low-pass order, corner)
