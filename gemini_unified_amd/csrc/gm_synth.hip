@@ -162,6 +162,9 @@ __global__ void __launch_bounds__(64) k_legendre_seed(int R, int Rp, int lmax, c
 
 // mode 0: basis rows for the contraction; the degrees are cut into segments of
 // seglen (blockIdx.z), each continuing the recurrence from its seeds (above).
+// mode 2: the same rows of the theta derivatives (dP_lm, d2P_lm/dtheta2,
+// d(P_lm/sin)/dtheta in place of P_lm, dP_lm, P_lm/sin), the basis of the
+// theta-derivative sums of the strain (harmonicDegreeSums.f90:316-351).
 // mode 1: FP64 outputs of computeLegendreFunctions, one segment.
 __global__ void __launch_bounds__(64) k_legendre(int R, int Rp, int lmax, int Lp, const double* __restrict__ theta,
                                                  __bf16* __restrict__ Bs, double* __restrict__ o_plm,
@@ -170,7 +173,7 @@ __global__ void __launch_bounds__(64) k_legendre(int R, int Rp, int lmax, int Lp
                                                  int nseg, int seglen, const double* __restrict__ seeds) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const int m = blockIdx.y;
-  if (r >= (mode == 0 ? Rp : R)) return;
+  if (r >= (mode == 1 ? R : Rp)) return;
   const size_t plane = (size_t)Rp * Lp;
   const int L1 = lmax + 1;
   const bool pad = r >= R;
@@ -247,15 +250,17 @@ __global__ void __launch_bounds__(64) k_legendre(int R, int Rp, int lmax, int Lp
       double b[3];
       const double w = (2.0 * l + 1.0) / fourpi;
       const double g = l >= 1 ? 2.0 / ll1 : 0.0;
+      const bool dv = mode == 2;
+      const double q0 = dv ? d : p, q1 = dv ? d2 : d, q2 = dv ? ds : sv;
       if (m == 0) {
-        b[0] = p * w;
-        b[1] = l >= 1 ? d * w : 0.0;
+        b[0] = q0 * w;
+        b[1] = l >= 1 ? q1 * w : 0.0;
         b[2] = 0.0;
       } else {
         const bool on = l >= m;
-        b[0] = on ? p * g * w : 0.0;
-        b[1] = on ? d * g * w : 0.0;
-        b[2] = on ? sv * g * w : 0.0;
+        b[0] = on ? q0 * g * w : 0.0;
+        b[1] = on ? q1 * g * w : 0.0;
+        b[2] = on ? q2 * g * w : 0.0;
       }
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
@@ -298,9 +303,13 @@ __device__ __forceinline__ double taperWn(int l, int n, double dwn, double tapfr
 // 2f and 2f + 1 of every spectrum plane are written, zero beyond the
 // frequency's degrees and in the padding columns (f >= nf).  Grid (Lp / 256
 // rounded up, Np / 2).
+// Plane s of Gs takes spectrum smap.s[s] of the node's spectra (the identity for
+// files with U, V, W; the displacement or the radial-derivative spectra of files
+// with derivatives).
+struct SpecMap { int s[10]; };
 __global__ void k_gprep(int nsp, int nf, int Np, int Lp, const int* __restrict__ nwn, const long long* __restrict__ koff,
                         long long nkf, const float* __restrict__ gr, const float* __restrict__ gi, double tapfrac,
-                        double dwn, int istyp, __bf16* __restrict__ Gs) {
+                        double dwn, int istyp, SpecMap smap, __bf16* __restrict__ Gs) {
   const int l = blockIdx.x * blockDim.x + threadIdx.x;
   const int f = blockIdx.y;
   if (l >= Lp) return;
@@ -329,8 +338,9 @@ __global__ void k_gprep(int nsp, int nf, int Np, int Lp, const int* __restrict__
   }
   for (int s = 0; s < nsp; ++s) {
     const bool on = l < n && (regional ? l >= 1 : l >= c_lmin[istyp][s]);
-    const double vr = on ? (double)gr[(long long)s * nkf + k] * w : 0.0;
-    const double vi = on ? (double)gi[(long long)s * nkf + k] * w : 0.0;
+    const long long so = (long long)smap.s[s] * nkf + k;
+    const double vr = on ? (double)gr[so] * w : 0.0;
+    const double vi = on ? (double)gi[so] * w : 0.0;
     const size_t o = (size_t)s * per_s + (size_t)(2 * f) * Lp + l;
     __bf16 a, b, c;
     split3(vr, a, b, c);
@@ -631,6 +641,105 @@ __global__ void k_wnfix(int R, int nf, int Rp, int Np, const int* __restrict__ n
 // event and station filters (computeSyntheticSeismograms.f90:361-371; evf:
 // complex [nf2], stf: complex [R][3][nf2], NULL: unity), velocity (:376-378),
 // ZNE rotation (axesRotation.f90:560-568).
+// Strain and stress spectra of one (r, f), from the 3 x nsum sums H[part][j]
+// (part 0: the displacement sums, 1: on the radial-derivative spectra, 2: on
+// the theta-derivative basis): displacement u and its derivatives d/dr u,
+// 1/r d/dtheta u, 1/(r sin theta) d/dphi u (moment/force Single, RadDeriv,
+// ThetaDeriv and PhiDeriv FrequencyDisplacement, singleFrequencyDisplacement.f90:
+// 35-194), the spherical strain (:223-241) and, with elcon (A, C, F, L, N at the
+// receiver), the stress (:253-266).  Z [R][3][nf2], E and T [R][3][3][nf2]
+// complex, zero below nf1.
+__global__ void k_strain(int R, int nf1, int nf2, int Rp, int Np, int nsum, const double* __restrict__ H,
+                         const double* __restrict__ theta, const double* __restrict__ phi, double re,
+                         const double* __restrict__ fm, const double* __restrict__ elcon, int istyp,
+                         double* __restrict__ Z, double* __restrict__ E, double* __restrict__ T) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= R * nf2) return;
+  const int r = id / nf2, jf = id % nf2 + 1;
+  cd u[4][3], e[3][3], t[3][3];
+  for (int a = 0; a < 3; ++a) {
+    for (int p = 0; p < 4; ++p) u[p][a] = cd{0.0, 0.0};
+    for (int b = 0; b < 3; ++b) { e[a][b] = cd{0.0, 0.0}; t[a][b] = cd{0.0, 0.0}; }
+  }
+  if (jf >= nf1) {
+    const int f = jf - nf1;
+    auto h = [&](int part, int j) {
+      const size_t o = (((size_t)part * nsum + j) * Rp + r) * Np + 2 * f;
+      return cd{H[o], H[o + 1]};
+    };
+    const double ph = phi[r], del = theta[r];
+    const double sp = sin(ph), cp = cos(ph);
+    const double rst = re * sin(del);
+    if (istyp) {
+      const double s2 = sin(2.0 * ph), c2 = cos(2.0 * ph);
+      const double a = fm[3] * cp + fm[4] * sp, b = (fm[2] - fm[1]) * c2 - 2.0 * fm[5] * s2;
+      const double c = fm[4] * cp - fm[3] * sp, d = (fm[2] - fm[1]) * s2 + 2.0 * fm[5] * c2;
+      for (int part = 0; part < 3; ++part) {
+        u[part][0] = cadd(cadd(cadd(crs(h(part, 0), fm[0]), crs(h(part, 1), fm[1] + fm[2])), crs(h(part, 2), a)),
+                          crs(h(part, 3), b));
+        u[part][1] = cadd(cadd(cadd(crs(h(part, 4), fm[0]), crs(h(part, 5), fm[1] + fm[2])),
+                               crs(cadd(h(part, 6), h(part, 8)), a)), crs(cadd(h(part, 7), h(part, 9)), b));
+        u[part][2] = cadd(crs(cadd(h(part, 10), h(part, 12)), c), crs(cadd(h(part, 11), h(part, 13)), d));
+      }
+      const double ap = -fm[3] * sp + fm[4] * cp, bp = -2.0 * (fm[2] - fm[1]) * s2 - 4.0 * fm[5] * c2;
+      const double cq = -fm[4] * sp - fm[3] * cp, dq = 2.0 * (fm[2] - fm[1]) * c2 - 4.0 * fm[5] * s2;
+      u[3][0] = cadd(crs(h(0, 2), ap), crs(h(0, 3), bp));
+      u[3][1] = cadd(crs(cadd(h(0, 6), h(0, 8)), ap), crs(cadd(h(0, 7), h(0, 9)), bp));
+      u[3][2] = cadd(crs(cadd(h(0, 10), h(0, 12)), cq), crs(cadd(h(0, 11), h(0, 13)), dq));
+    } else {
+      const double a = fm[1] * cp + fm[2] * sp, b = fm[1] * sp - fm[2] * cp;
+      for (int part = 0; part < 3; ++part) {
+        u[part][0] = cadd(crs(h(part, 0), fm[0]), crs(h(part, 1), a));
+        u[part][1] = cadd(crs(h(part, 2), fm[0]), crs(cadd(h(part, 3), h(part, 4)), a));
+        u[part][2] = crs(cadd(h(part, 5), h(part, 6)), b);
+      }
+      const double ap = -fm[1] * sp + fm[2] * cp, bp = fm[1] * cp + fm[2] * sp;
+      u[3][0] = crs(h(0, 1), ap);
+      u[3][1] = crs(cadd(h(0, 3), h(0, 4)), ap);
+      u[3][2] = crs(cadd(h(0, 5), h(0, 6)), bp);
+    }
+    for (int a = 0; a < 3; ++a) {
+      u[2][a] = cd{u[2][a].re / re, u[2][a].im / re};
+      u[3][a] = cd{u[3][a].re / rst, u[3][a].im / rst};
+    }
+    const double rtandel = re * tan(del);
+    const cd *z = u[0], *zr = u[1], *zt = u[2], *zf = u[3];
+    auto dv = [](cd x, double y) { return cd{x.re / y, x.im / y}; };
+    e[0][0] = zr[0];
+    e[1][1] = cadd(zt[1], dv(z[0], re));
+    e[2][2] = cadd(cadd(zf[2], dv(z[0], re)), dv(z[1], rtandel));
+    e[0][1] = crs(csub(cadd(zt[0], zr[1]), dv(z[1], re)), 0.5);
+    e[0][2] = crs(csub(cadd(zf[0], zr[2]), dv(z[2], re)), 0.5);
+    e[1][2] = crs(csub(cadd(zf[1], zt[2]), dv(z[2], rtandel)), 0.5);
+    e[2][1] = e[1][2];
+    e[2][0] = e[0][2];
+    e[1][0] = e[0][1];
+    if (elcon) {
+      cd el[5];
+      for (int k = 0; k < 5; ++k) el[k] = cd{elcon[2 * k], elcon[2 * k + 1]};
+      const cd tr2 = cadd(e[1][1], e[2][2]);
+      t[0][0] = cadd(cmul(el[1], e[0][0]), cmul(el[2], tr2));
+      t[1][1] = csub(cadd(cmul(el[2], e[0][0]), cmul(el[0], tr2)), cmul(crs(el[4], 2.0), e[2][2]));
+      t[2][2] = csub(cadd(cmul(el[2], e[0][0]), cmul(el[0], tr2)), cmul(crs(el[4], 2.0), e[1][1]));
+      t[0][1] = cmul(crs(el[3], 2.0), e[0][1]);
+      t[0][2] = cmul(crs(el[3], 2.0), e[0][2]);
+      t[1][2] = cmul(crs(el[4], 2.0), e[1][2]);
+      t[1][0] = t[0][1];
+      t[2][0] = t[0][2];
+      t[2][1] = t[1][2];
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    const size_t oz = (((size_t)r * 3 + a) * nf2 + jf - 1) * 2;
+    Z[oz] = u[0][a].re; Z[oz + 1] = u[0][a].im;
+    for (int b = 0; b < 3; ++b) {
+      const size_t o = ((((size_t)r * 3 + a) * 3 + b) * nf2 + jf - 1) * 2;
+      E[o] = e[a][b].re; E[o + 1] = e[a][b].im;
+      if (T) { T[o] = t[a][b].re; T[o + 1] = t[a][b].im; }
+    }
+  }
+}
+
 __global__ void k_combine(int R, int nf1, int nf2, int Rp, int Np, const double* __restrict__ H,
                           const double* __restrict__ phi, const double* __restrict__ propdir,
                           const double* __restrict__ tred, const double* __restrict__ evf,
@@ -982,10 +1091,12 @@ int run_sums(gm_ctx* c, const SynPlan& P, const double* theta, const float* gr, 
                        (const double*)seeds);
   }
   HIPCHK(c, hipEventRecord(c->sev[1], s));
+  SpecMap ident;
+  for (int k = 0; k < 10; ++k) ident.s[k] = k;
   hipLaunchKernelGGL(k_gprep, dim3((unsigned)((P.Lp + 255) / 256), (unsigned)(P.Np / 2)), dim3(256), 0, s, P.nsp, P.nf,
                      P.Np, P.Lp,
                      (const int*)(base + o.nwn), (const long long*)(base + o.koff), P.nkf, dgr, dgi, 0.2, dwn,
-                     P.istyp, (__bf16*)(base + o.Gs));
+                     P.istyp, ident, (__bf16*)(base + o.Gs));
   HIPCHK(c, hipEventRecord(c->sev[2], s));
   int nsplit, kper;
   contractSplits(P, nsplit, kper);
@@ -1177,6 +1288,134 @@ static int synth_impl(gm_ctx* c, int nf1, int nf2, double df, double sigma, cons
   (void)hipEventElapsedTime(&ms, c->sev[0], c->sev[5]);
   c->stiming[5] = ms;
   return GM_OK;
+}
+
+// Strain spectra: three contractions of the 14 (moment) or 7 (force) sum
+// pattern over the node's spectra with radial derivatives (the displacement
+// spectra, the derivative spectra, and the displacement spectra against the
+// theta-derivative basis give kint 1..14, 15..28 and 29..42 / 1..7, 8..14 and
+// 15..21), then k_strain.
+static int strain_impl(gm_ctx* c, int nf1, int nf2, const int* nwn, int nsp, long long nkf, const float* gfkspr,
+                       const float* gfkspi, int nrcv, const double* theta, const double* phi, double re, int istyp,
+                       const double* fomt, const double* elcon, int flags, double* hdsum, double* zsp,
+                       double* strainsp, double* stresssp) {
+  if (!c) return GM_ERR_ARG;
+  if (istyp != 0 && istyp != 1) return gm_fail(c, GM_ERR_ARG, "istyp must be 0 (FORCE) or 1 (MOMENT)");
+  // numberGreenFKSpectra with derivflag = 1 (greenFKSpectra.f90:109-111, 216-220): U, V, UP, VP x njsph, W, WP x njtor
+  if (nsp != (istyp ? 20 : 10))
+    return gm_fail(c, GM_ERR_UNSUPPORTED, "strain spectra need nsp = %d (U, V, dU/dr, dV/dr, W, dW/dr), got %d",
+                   istyp ? 20 : 10, nsp);
+  if (!gfkspr || !gfkspi || !theta || !phi || !fomt || !strainsp)
+    return gm_fail(c, GM_ERR_ARG, "spectra, theta, phi, fomt and strainsp are required");
+  if (!(re > 0.0)) return gm_fail(c, GM_ERR_ARG, "receiver radius must be positive");
+  SynPlan P;
+  int rc = plan_sums(c, nf1, nf2, nwn, istyp ? 10 : 5, nkf, nrcv, P);   // 10 / 5 planes per contraction
+  if (rc) return rc;
+  const bool dev = (flags & GM_FLAG_DEVICE_PTRS) != 0;
+  HIPCHK(c, hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  const size_t plane_h = (size_t)P.nsum * P.Rp * P.Np;
+  int nsplit, kper;
+  contractSplits(P, nsplit, kper);
+  int nseg, seglen;
+  legendreSegments(P.Lp, nseg, seglen);
+  Carve cv;
+  const size_t o_th = cv.take(sizeof(double) * P.R), o_ph = cv.take(sizeof(double) * P.R);
+  const size_t o_seeds = cv.take(sizeof(double) * 4 * 3 * (size_t)P.Rp * nseg);
+  const size_t o_nwn = cv.take(sizeof(int) * P.nf), o_koff = cv.take(sizeof(long long) * P.nf);
+  const size_t nbs = sizeof(__bf16) * kPieces * kRows * (size_t)P.Rp * P.Lp;
+  const size_t ngs = sizeof(__bf16) * kPieces * (size_t)P.nsp * P.Np * P.Lp;
+  const size_t o_B0 = cv.take(nbs), o_B1 = cv.take(nbs), o_G0 = cv.take(ngs), o_G1 = cv.take(ngs);
+  const size_t o_H = cv.take(sizeof(double) * 3 * plane_h);
+  const size_t o_Hp = nsplit > 1 ? cv.take(sizeof(double) * nsplit * plane_h) : 0;
+  const size_t o_gr = dev ? 0 : cv.take(sizeof(float) * nsp * (size_t)nkf);
+  const size_t o_gi = dev ? 0 : cv.take(sizeof(float) * nsp * (size_t)nkf);
+  const size_t o_fm = cv.take(sizeof(double) * 6), o_el = cv.take(sizeof(double) * 14);
+  const size_t nz = 2 * 3 * (size_t)P.R * nf2, ne = 2 * 9 * (size_t)P.R * nf2;
+  const size_t o_Z = cv.take(sizeof(double) * nz), o_E = cv.take(sizeof(double) * ne);
+  const size_t o_T = elcon ? cv.take(sizeof(double) * ne) : 0;
+  if ((rc = ensure_sbuf(c, cv.off))) return rc;
+  char* base = (char*)c->sbuf;
+  HIPCHK(c, hipMemcpyAsync(base + o_th, theta, sizeof(double) * P.R, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(base + o_ph, phi, sizeof(double) * P.R, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(base + o_nwn, P.nwnf.data(), sizeof(int) * P.nf, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(base + o_koff, P.koff.data(), sizeof(long long) * P.nf, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(base + o_fm, fomt, sizeof(double) * (istyp ? 6 : 3), hipMemcpyHostToDevice, s));
+  if (elcon) HIPCHK(c, hipMemcpyAsync(base + o_el, elcon, sizeof(double) * 14, hipMemcpyHostToDevice, s));
+  const float *dgr = gfkspr, *dgi = gfkspi;
+  if (!dev) {
+    HIPCHK(c, hipMemcpyAsync(base + o_gr, gfkspr, sizeof(float) * nsp * (size_t)nkf, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(base + o_gi, gfkspi, sizeof(float) * nsp * (size_t)nkf, hipMemcpyHostToDevice, s));
+    dgr = (const float*)(base + o_gr);
+    dgi = (const float*)(base + o_gi);
+  }
+  const double* dth = (const double*)(base + o_th);
+  HIPCHK(c, hipEventRecord(c->sev[0], s));
+  double* seeds = (double*)(base + o_seeds);
+  if (nseg > 1)
+    hipLaunchKernelGGL(k_legendre_seed, dim3((P.Rp + 63) / 64, 3, nseg - 1), dim3(64), 0, s, P.R, P.Rp, P.L1 - 1, dth,
+                       nseg, seglen, seeds);
+  for (int mode = 0; mode <= 2; mode += 2)
+    hipLaunchKernelGGL(k_legendre, dim3((P.Rp + 63) / 64, 3, nseg), dim3(64), 0, s, P.R, P.Rp, P.L1 - 1, P.Lp, dth,
+                       (__bf16*)(base + (mode ? o_B1 : o_B0)), nullptr, nullptr, nullptr, nullptr, nullptr, mode, nseg,
+                       seglen, (const double*)seeds);
+  HIPCHK(c, hipEventRecord(c->sev[1], s));
+  // spectrum offsets (positionDataGreenFKSpectra with dsvmask U, V, UP, VP, W, WP)
+  SpecMap mdisp, mder;
+  if (istyp) {
+    const int a[10] = {0, 1, 2, 3, 4, 5, 6, 7, 16, 17}, b[10] = {8, 9, 10, 11, 12, 13, 14, 15, 18, 19};
+    for (int k = 0; k < 10; ++k) { mdisp.s[k] = a[k]; mder.s[k] = b[k]; }
+  } else {
+    const int a[10] = {0, 1, 2, 3, 8, 0, 0, 0, 0, 0}, b[10] = {4, 5, 6, 7, 9, 0, 0, 0, 0, 0};
+    for (int k = 0; k < 10; ++k) { mdisp.s[k] = a[k]; mder.s[k] = b[k]; }
+  }
+  for (int g = 0; g < 2; ++g)
+    hipLaunchKernelGGL(k_gprep, dim3((unsigned)((P.Lp + 255) / 256), (unsigned)(P.Np / 2)), dim3(256), 0, s, P.nsp,
+                       P.nf, P.Np, P.Lp, (const int*)(base + o_nwn), (const long long*)(base + o_koff), (long long)nkf,
+                       dgr, dgi, 0.2, 0.0, istyp, g ? mder : mdisp, (__bf16*)(base + (g ? o_G1 : o_G0)));
+  HIPCHK(c, hipEventRecord(c->sev[2], s));
+  // part 0: displacement spectra x basis; 1: derivative spectra x basis; 2: displacement spectra x theta-derivative basis
+  const size_t pb[3] = {o_B0, o_B0, o_B1}, pg[3] = {o_G0, o_G1, o_G0};
+  for (int part = 0; part < 3; ++part) {
+    double* Hpart = (double*)(base + o_H) + part * plane_h;
+    hipLaunchKernelGGL(k_contract, dim3((P.Np / TN) * (P.Rp / TM) * P.nsum * nsplit), dim3(256), 0, s, P.Rp, P.Np,
+                       P.Lp, P.nsp, (const __bf16*)(base + pb[part]), (const __bf16*)(base + pg[part]),
+                       nsplit > 1 ? (double*)(base + o_Hp) : Hpart, 0, istyp, nsplit, kper);
+    if (nsplit > 1)
+      hipLaunchKernelGGL(k_sumsplits, dim3((unsigned)((plane_h + 255) / 256)), dim3(256), 0, s, plane_h, nsplit,
+                         (const double*)(base + o_Hp), Hpart);
+  }
+  HIPCHK(c, hipEventRecord(c->sev[3], s));
+  hipLaunchKernelGGL(k_strain, dim3((unsigned)(((size_t)P.R * nf2 + 255) / 256)), dim3(256), 0, s, P.R, nf1, nf2,
+                     P.Rp, P.Np, P.nsum, (const double*)(base + o_H), dth, (const double*)(base + o_ph), re,
+                     (const double*)(base + o_fm), elcon ? (const double*)(base + o_el) : nullptr, istyp,
+                     (double*)(base + o_Z), (double*)(base + o_E), elcon ? (double*)(base + o_T) : nullptr);
+  HIPCHK(c, hipEventRecord(c->sev[4], s));
+  HIPCHK(c, hipGetLastError());
+  const hipMemcpyKind kind = dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  if (hdsum)          // H[part][j][Rp][Np] -> hdsum[part * nsum + j][R][nf] complex
+    for (int j = 0; j < 3 * P.nsum; ++j)
+      HIPCHK(c, hipMemcpy2DAsync(hdsum + (size_t)j * P.R * P.nf * 2, sizeof(double) * 2 * P.nf,
+                                 base + o_H + sizeof(double) * (size_t)j * P.Rp * P.Np, sizeof(double) * P.Np,
+                                 sizeof(double) * 2 * P.nf, P.R, kind, s));
+  if (zsp) HIPCHK(c, hipMemcpyAsync(zsp, base + o_Z, sizeof(double) * nz, kind, s));
+  HIPCHK(c, hipMemcpyAsync(strainsp, base + o_E, sizeof(double) * ne, kind, s));
+  if (elcon && stresssp) HIPCHK(c, hipMemcpyAsync(stresssp, base + o_T, sizeof(double) * ne, kind, s));
+  HIPCHK(c, hipEventRecord(c->sev[5], s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  float ms;
+  for (int i = 0; i < 5; ++i) { (void)hipEventElapsedTime(&ms, c->sev[i], c->sev[i + 1]); c->stiming[i] = ms; }
+  (void)hipEventElapsedTime(&ms, c->sev[0], c->sev[5]);
+  c->stiming[5] = ms;
+  return GM_OK;
+}
+
+extern "C" int gm_strain_spectra(gm_ctx* c, int nf1, int nf2, const int* nwn, int nsp, long long nkf,
+                                 const float* gfkspr, const float* gfkspi, int nrcv, const double* theta,
+                                 const double* phi, double re, int istyp, const double* fomt, const double* elcon,
+                                 int flags, double* hdsum, double* zsp, double* strainsp, double* stresssp) {
+  return strain_impl(c, nf1, nf2, nwn, nsp, nkf, gfkspr, gfkspi, nrcv, theta, phi, re, istyp, fomt, elcon, flags,
+                     hdsum, zsp, strainsp, stresssp);
 }
 
 extern "C" int gm_moment_sums(gm_ctx* c, int nf1, int nf2, const int* nwn, int nsp, long long nkf, const float* gfkspr,

@@ -212,6 +212,28 @@ module geminiFKc
          type(c_ptr), value :: status
          type(c_ptr), value :: stats
       end function gm_gfk_batch_wn
+      integer(c_int) function gm_aski_batch(ctx, nfreq, freqs, lmin, nl, sigma, eps, attmode, nsnod, jsnod, &
+          jre, istyp, flags, gf, gfre, gfim, status, stats) bind(C, name='gm_aski_batch')
+         import
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: nfreq
+         type(c_ptr), value :: freqs
+         type(c_ptr), value :: lmin
+         type(c_ptr), value :: nl
+         real(c_double), value :: sigma
+         real(c_double), value :: eps
+         integer(c_int), value :: attmode
+         integer(c_int), value :: nsnod
+         type(c_ptr), value :: jsnod
+         integer(c_int), value :: jre
+         integer(c_int), value :: istyp
+         integer(c_int), value :: flags
+         type(c_ptr), value :: gf
+         type(c_ptr), value :: gfre
+         type(c_ptr), value :: gfim
+         type(c_ptr), value :: status
+         type(c_ptr), value :: stats
+      end function gm_aski_batch
       integer(c_int) function gm_last_timing(ctx, ms6) bind(C, name='gm_last_timing')
          import
          type(c_ptr), value :: ctx
@@ -367,6 +389,31 @@ module geminiFKc
          type(c_ptr), value :: ts
          type(c_ptr), value :: zsp
       end function gm_synthetics
+      integer(c_int) function gm_strain_spectra(ctx, nf1, nf2, nwn, nsp, nkf, gfkspr, gfkspi, nrcv, theta, &
+          phi, re, istyp, fomt, elcon, flags, hdsum, zsp, strainsp, stresssp) bind(C, &
+          name='gm_strain_spectra')
+         import
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: nf1
+         integer(c_int), value :: nf2
+         type(c_ptr), value :: nwn
+         integer(c_int), value :: nsp
+         integer(c_long_long), value :: nkf
+         type(c_ptr), value :: gfkspr
+         type(c_ptr), value :: gfkspi
+         integer(c_int), value :: nrcv
+         type(c_ptr), value :: theta
+         type(c_ptr), value :: phi
+         real(c_double), value :: re
+         integer(c_int), value :: istyp
+         type(c_ptr), value :: fomt
+         type(c_ptr), value :: elcon
+         integer(c_int), value :: flags
+         type(c_ptr), value :: hdsum
+         type(c_ptr), value :: zsp
+         type(c_ptr), value :: strainsp
+         type(c_ptr), value :: stresssp
+      end function gm_strain_spectra
       integer(c_int) function gm_bessel_sums(ctx, nf1, nf2, nwn, nsp, nkf, gfkspr, gfkspi, dwn, rearth, &
           nrcv, x, flags, wnint) bind(C, name='gm_bessel_sums')
          import

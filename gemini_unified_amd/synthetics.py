@@ -129,6 +129,33 @@ class SyntheticsEngine:
         gr, gi: [5, nkf] (U1, U2, V1, V2, W1)."""
         return self._sums(_lib.lib().gm_force_sums, FORCE_SUMS, nf1, nf2, nwn, gr, gi, theta)
 
+    def strain_spectra(self, nf1, nf2, nwn, gr, gi, theta, phi, re, fomt, istyp=1, elcon=None, want_sums=False):
+        """Strain (and, with elcon = 7 complex moduli at the receiver, stress) spectra from spectra with radial
+        derivatives (gr, gi: [20, nkf] moment or [10, nkf] force, U V dU dV W dW): dict strain and stress
+        [nrcv, 3, 3, nf2], zsp [nrcv, 3, nf2] (components r, theta, phi), hdsum [42 or 21, nrcv, nf]
+        (sphericalHarmonicsDisplacementSpectra with strainsp, displacementSpectra.f90:49-106)."""
+        nwn = np.ascontiguousarray(nwn, dtype=np.int32)
+        gr, gi = self._spectra(gr, gi)
+        theta = np.ascontiguousarray(np.atleast_1d(theta), dtype=np.float64)
+        phi = np.ascontiguousarray(np.broadcast_to(phi, theta.shape), dtype=np.float64)
+        fm = np.ascontiguousarray(fomt, dtype=np.float64)
+        if fm.size != (6 if istyp == 1 else 3):
+            raise ValueError("fomt needs 6 moment-tensor (istyp 1) or 3 force (istyp 0) components")
+        R, nf = theta.size, nf2 - nf1 + 1
+        out = dict(zsp=np.zeros((R, 3, nf2), np.complex128), strain=np.zeros((R, 3, 3, nf2), np.complex128),
+                   stress=np.zeros((R, 3, 3, nf2), np.complex128) if elcon is not None else None,
+                   hdsum=np.zeros((42 if istyp == 1 else 21, R, nf), np.complex128) if want_sums else None)
+        el = None if elcon is None else np.ascontiguousarray(elcon, dtype=np.complex128)
+        vp = lambda a: None if a is None else a.ctypes.data
+        check(self.h, _lib.lib().gm_strain_spectra(self.h, int(nf1), int(nf2), nwn.ctypes.data_as(I), gr.shape[0],
+                                                   gr.shape[1], gr.ctypes.data, gi.ctypes.data, R,
+                                                   theta.ctypes.data_as(D), phi.ctypes.data_as(D), float(re),
+                                                   int(istyp), fm.ctypes.data_as(D),
+                                                   None if el is None else el.ctypes.data_as(D), 0,
+                                                   vp(out["hdsum"]), vp(out["zsp"]), vp(out["strain"]),
+                                                   vp(out["stress"])))
+        return out
+
     def bessel_sums(self, nf1, nf2, nwn, gr, gi, dwn, rearth, x):
         """14 moment wavenumber integrals, complex [14, nrcv, nf2-nf1+1] (momentWavenumberIntegrals);
         x = epicentral distances in metres, wavenumbers k = (j-1) dwn."""

@@ -310,6 +310,31 @@ int gm_synthetics(gm_ctx* ctx, int nf1, int nf2, double df, double sigma, const 
                   int istyp, const double* fomt, int velocity, int nsamp, double dt, int flags, float* ts,
                   double* zsp);
 
+/* Strain (and stress) spectra of one source node for receivers at epicentral
+ * distances theta[r] and azimuths phi[r] (radians) at radius re (m), from
+ * spectra with radial derivatives (the multi-source driver's files,
+ * derivflag = 1: nsp = 20 for istyp = 1 (U, V, dU/dr, dV/dr x 4 jumps, W,
+ * dW/dr x 2) or 10 for istyp = 0 (x 2 and x 1), layout as gm_moment_sums).
+ * Replaces sphericalHarmonicsDisplacementSpectra with its strainsp argument
+ * (f90/displacementSpectra.f90:49-106, 200-228): momentHarmonicDegreeSums /
+ * forceHarmonicDegreeSums for kint 1..42 / 1..21 (f90/harmonicDegreeSums.f90:
+ * 95-351; with getIndexArrayHarmonicDegreeSums, :375-420, the stress sums
+ * kint 43..46 / 22..23 are not requested by such files), the displacement and
+ * its r, theta and phi derivatives and sphericalStrainSingleFrequencyDisplacement
+ * (f90/singleFrequencyDisplacement.f90:35-241); with elcon (7 complex moduli
+ * A, C, F, L, N, kappa, mu at the receiver; NULL: no stress),
+ * sphericalStressSingleFrequencyDisplacement (:253-266).  The three degree-sum
+ * contractions run on the matrix cores as in gm_moment_sums.
+ * Outputs: hdsum (optional) complex [42 or 21][nrcv][nf2-nf1+1] (kint order);
+ * zsp (optional) complex [nrcv][3][nf2]; strainsp and stresssp (optional,
+ * with elcon) complex [nrcv][3][3][nf2] (the reference's zsp(nf2, 3) and
+ * strainsp(nf2, 3, 3) per receiver), components r, theta, phi, zero below nf1.
+ * GM_FLAG_DEVICE_PTRS: spectra and outputs are device pointers. */
+int gm_strain_spectra(gm_ctx* ctx, int nf1, int nf2, const int* nwn, int nsp, long long nkf, const float* gfkspr,
+                      const float* gfkspi, int nrcv, const double* theta, const double* phi, double re, int istyp,
+                      const double* fomt, const double* elcon, int flags, double* hdsum, double* zsp, double* strainsp,
+                      double* stresssp);
+
 /* Regional (Bessel) branch: the 14 moment wavenumber integrals (kint 1..14) of
  * one source node at epicentral distances x[r] in metres, for spectra sampled
  * at k = (j-1) dwn, j = 1..nwn[jf-1] (j = 1 is k = 0 and is not summed).

@@ -118,6 +118,25 @@ static inline cx cdiv(cx a, cx b) {                                      // Smit
 }
 static inline cx rdiv(double a, cx b) { return cdiv(C(a, 0.0), b); }     // real / complex
 static inline double cabs_(cx a) { return std::hypot(a.re, a.im); }
+// 10^x of the minors renormalisation (10**lgn, spheroidalMinors.f90:535-552).
+// gfortran evaluates it with the C library's pow; here it is computed from
+// basic operations only (Cody-Waite reduction x = k log10(2) + r, k the nearest
+// integer of x log2(10), log10(2) as a 32-bit head and a tail, the Taylor series
+// of exp(r ln 10) to degree 14 by Horner, exact scaling by 2^k), within a few
+// ulp of 10^x, so that every IEEE platform gives the same bits.
+static double exp10Det(double x) {
+  if (!(std::fabs(x) < 4000.0)) return x != x ? x : (x > 0.0 ? HUGE_VAL : 0.0);
+  static const double c[14] = {1.0, 2.302585092994046, 2.650949055239199, 2.034678592293476, 1.171255148912267,
+                               0.5393829291955814, 0.2069958486968681, 0.06808936507443707, 0.019597694626478524,
+                               0.00501392883377544, 0.0011544997789984348, 0.00024166672554424694,
+                               4.6371516642572196e-05, 8.213412535439387e-06};
+  static const double c13 = 1.3508629476223687e-06;
+  const double k = std::floor(x * 3.321928094887362 + 0.5);
+  const double r = (x - k * 0.3010299955494702) - k * 1.1451100898021838e-10;
+  double p = c13;
+  for (int n = 13; n >= 0; --n) p = p * r + c[n];
+  return std::ldexp(p, (int)k);
+}
 static inline cx csqrt_(cx a) {
   std::complex<double> z(a.re, a.im);
   std::complex<double> s = std::sqrt(z);
@@ -1187,11 +1206,11 @@ static int spheroidalMinors(Env env, int jsl, int jsr, double eps, SpheroidalMin
     for (int n = 1; n <= nnod; ++n) {
       if (up) {
         double lgn = S.ulg[n] - lgnormend;
-        double f = std::pow(10.0, lgn);
+        double f = exp10Det(lgn);
         for (int i = 1; i <= 6; ++i) S.U(i, n) = mulr(S.U(i, n), f);
       } else {
         double lgn = S.dlg[n] - lgnormend;
-        double f = std::pow(10.0, lgn);
+        double f = exp10Det(lgn);
         for (int i = 1; i <= 6; ++i) S.D(i, n) = mulr(S.D(i, n), f);
       }
     }
@@ -2027,6 +2046,8 @@ int orc_num_threads(void) {
 }  // extern "C"
 
 extern "C" {
+double orc_exp10(double x) { return exp10Det(x); }
+
 // reciprocity relations for tests: bgin (4,4) column-major complex -> bgout (4,4)
 int orc_reciprocity_sph(double elp1, double re, double r, int state_node, const double* bgin, double* bgout) {
   std::vector<orc::cx> in(16);

@@ -68,6 +68,8 @@ def lib():
                                              C.c_double, C.c_int, C.c_int, C.c_double, C.c_int, D, I, L]
         _lib.orc_aski_degrees.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                                           C.c_int, I, C.c_int, C.c_int, C.c_double, C.c_int, D, I, L]
+        _lib.orc_exp10.restype = C.c_double
+        _lib.orc_exp10.argtypes = [C.c_double]
         _lib.orc_sysmat_at.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, D]
         _lib.orc_minors_wavenumbers.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                                                 C.c_double, C.c_int, C.c_int, C.c_double, C.c_int, I, L]
@@ -260,6 +262,11 @@ def zetl(zx2, el):
     out = np.zeros(2)
     rc = lib().orc_zetl(zx2.real, zx2.imag, el, out.ctypes.data_as(D))
     return rc, complex(out[0], out[1])
+
+
+def exp10(x):
+    """the oracle's 10^x of the minors renormalisation"""
+    return lib().orc_exp10(float(x))
 
 
 def num_threads():

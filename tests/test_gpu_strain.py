@@ -1,0 +1,71 @@
+"""GPU parity of the strain and stress spectra (gm_strain_spectra: the 42 moment / 21 force
+degree sums in three matrix-core contractions, displacement derivatives, spherical strain,
+VTI stress) against the CPU oracle, on seeded random spectra with radial derivatives."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "oracle"))
+import oracle  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+RE = 6371000.0
+
+
+@pytest.fixture(scope="module")
+def syn():
+    from gemini_unified_amd.synthetics import SyntheticsEngine
+    e = SyntheticsEngine(0)
+    yield e
+    e.close()
+
+
+def _case(istyp, seed):
+    rng = np.random.default_rng(seed)
+    nf1, nf2 = 2, 11
+    nwn = np.zeros(nf2, np.int32)
+    nwn[nf1 - 1:] = np.linspace(120, 330, nf2 - nf1 + 1).astype(np.int32)
+    nkf = int(nwn.sum())
+    l = np.concatenate([np.arange(n) for n in nwn[nf1 - 1:]])
+    env = np.exp(-l / 80.0) + 0.05
+    nsp = 20 if istyp == 1 else 10
+    gr = (rng.normal(size=(nsp, nkf)) * env).astype(np.float32)
+    gi = (rng.normal(size=(nsp, nkf)) * env).astype(np.float32)
+    theta = np.sort(rng.uniform(0.05, 3.09, size=45))
+    phi = rng.uniform(0, 2 * np.pi, size=45)
+    fomt = rng.normal(size=6 if istyp == 1 else 3)
+    return nf1, nf2, nwn, gr, gi, theta, phi, fomt
+
+
+@pytest.mark.parametrize("istyp", [1, 0])
+def test_strain_spectra_match_oracle(syn, istyp):
+    nf1, nf2, nwn, gr, gi, theta, phi, fomt = _case(istyp, 3 + istyp)
+    el = np.array([2.1e11 + 1e8j, 1.9e11 + 2e8j, 0.8e11 + 1e7j, 0.6e11 + 3e8j, 0.7e11 + 2e8j, 1.3e11, 0.65e11])
+    print(f"strain istyp={istyp}: {theta.size} receivers, {nf2 - nf1 + 1} frequencies", flush=True)
+    g = syn.strain_spectra(nf1, nf2, nwn, gr, gi, theta, phi, RE, fomt, istyp=istyp, elcon=el, want_sums=True)
+    o = oracle.strain_spectra(istyp, nf1, nf2, nwn, gr, gi, theta, phi, RE, fomt, elcon=el)
+    # sums: error relative to the largest sum of each (kint, receiver), as for the displacement sums
+    h = o["hdsum"].transpose(2, 0, 1)
+    scale = np.abs(h).max(axis=2, keepdims=True) + 1e-300
+    err = np.abs(g["hdsum"] - h) / scale
+    print(f"  sums max rel {err.max():.2e}", flush=True)
+    assert err.max() < 1e-5
+    for key in ("zsp", "strain", "stress"):
+        a, b = g[key], o[key]
+        sc = np.abs(b).max(axis=-1, keepdims=True) + 1e-300
+        e = (np.abs(a - b) / sc).max()
+        print(f"  {key} max rel {e:.2e}", flush=True)
+        assert e < 1e-5
+        assert np.all(a[..., :nf1 - 1] == 0)
+
+
+def test_strain_spectra_errors(syn):
+    nf1, nf2, nwn, gr, gi, theta, phi, fomt = _case(1, 5)
+    from gemini_unified_amd._lib import GeminiFKError
+    with pytest.raises(GeminiFKError):          # displacement-only spectra have no radial derivatives
+        syn.strain_spectra(nf1, nf2, nwn, gr[:10], gi[:10], theta, phi, RE, fomt, istyp=1)
+    with pytest.raises(GeminiFKError):
+        syn.strain_spectra(nf1, nf2, nwn, gr, gi, theta, phi, -1.0, fomt, istyp=1)

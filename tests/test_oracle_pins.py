@@ -459,3 +459,20 @@ def test_start_radius_decay_criterion(c1_setup):
         r -= dr
         q += dr * np.sqrt(F(r))
     assert abs(r - rs) < 1e-6 * nem.rearth
+
+
+def test_exp10_of_the_renormalisation_is_within_two_ulp():
+    """10^x of the minors renormalisation from basic operations (the same algorithm in the kernels,
+    so that the device and the host give the same bits), against 10^x to 40 digits."""
+    from decimal import Decimal, getcontext
+    getcontext().prec = 40
+    rng = np.random.default_rng(7)
+    xs = np.concatenate([rng.uniform(-300, 300, 3000), rng.uniform(-1, 1, 1000), [0.0, 1.0, -1.0, 22.0, -0.5]])
+    worst = 0.0
+    for x in xs:
+        got = oracle.exp10(x)
+        ref = Decimal(10) ** Decimal(float(x))
+        err = abs(Decimal(got) - ref) / (ref * Decimal(np.spacing(float(ref)) / float(ref)))
+        worst = max(worst, float(err))
+    assert worst <= 2.0, worst
+    assert oracle.exp10(0.0) == 1.0
