@@ -575,7 +575,8 @@ __global__ void k_bessel_basis(int R, int Rp, int Lp, const double* __restrict__
 template <int ISTYP>
 __global__ void k_wnfix(int R, int nf, int Rp, int Np, const int* __restrict__ nwn, const long long* __restrict__ koff,
                         long long nkf, const float* __restrict__ gr, const float* __restrict__ gi,
-                        const double* __restrict__ xr, double dwn, double re, double tapfrac, double* __restrict__ H) {
+                        const double* __restrict__ xr, double dwn, double re, double tapfrac, SpecMap smap,
+                        double* __restrict__ H) {
   const int id = blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= R * nf) return;
   const int r = id / nf, f = id % nf;
@@ -588,7 +589,7 @@ __global__ void k_wnfix(int R, int nf, int Rp, int Np, const int* __restrict__ n
     wnRows(k, g.x, re, b0, b1, -b0 + 2.0 / (k * xx) * b1, w);
   };
   auto spec = [&](int s, int j) -> cd {      // tapered spectrum s at wavenumber index j
-    const long long k = (long long)s * nkf + koff[f] + (j - 1);
+    const long long k = (long long)smap.s[s] * nkf + koff[f] + (j - 1);
     const double t = taperWn(j - 1, n, dwn, tapfrac);
     return cd{(double)gr[k] * t, (double)gi[k] * t};
   };
@@ -982,23 +983,42 @@ namespace {
 struct SynPlan {
   int R, Rp, L1, Lp, nf, Np, nsp;
   int istyp, nsum;       // source type from nsp (0 force: 5 spectra, 7 sums; 1 moment: 10 spectra, 14 sums)
+  int nsp_in;            // spectra per node of the input (nsp, or 20 / 10 with stresses or derivatives)
+  SpecMap smap;          // input spectrum of each of the nsp planes (U, V, W)
   long long nkf;
   std::vector<int> nwnf;
   std::vector<long long> koff;
 };
 
-int plan_sums(gm_ctx* c, int nf1, int nf2, const int* nwn, int nsp, long long nkf, int nrcv, SynPlan& P) {
+int plan_sums(gm_ctx* c, int nf1, int nf2, const int* nwn, int nsp, long long nkf, int nrcv, SynPlan& P,
+              int flags = 0) {
   if (nf1 < 1 || nf2 < nf1) return gm_fail(c, GM_ERR_ARG, "invalid frequency index range %d..%d", nf1, nf2);
   if (!nwn) return gm_fail(c, GM_ERR_ARG, "nwn is required");
-  // dsvmask (U, V, W): numberGreenFKSpectra (greenFKSpectra.f90:216-220) is
-  // 2 x 4 + 2 = 10 for a moment source and 2 x 2 + 1 = 5 for a single force
-  if (nsp != 10 && nsp != 5)
+  const bool stress = (flags & GM_FLAG_STRESS_SPECTRA) != 0, deriv = (flags & GM_FLAG_DERIV_SPECTRA) != 0;
+  if (stress && deriv) return gm_fail(c, GM_ERR_ARG, "a file has either stresses or derivatives");
+  // numberGreenFKSpectra (greenFKSpectra.f90:216-220): dsvmask (U, V, W) gives 2 x 4 + 2 = 10 for a
+  // moment source and 2 x 2 + 1 = 5 for a single force; with stresses or derivatives, 20 and 10
+  if (stress || deriv) {
+    if (nsp != 20 && nsp != 10)
+      return gm_fail(c, GM_ERR_ARG, "spectra with %s need nsp = 20 (moment) or 10 (force), got %d",
+                     stress ? "stresses" : "derivatives", nsp);
+  } else if (nsp != 10 && nsp != 5) {
     return gm_fail(c, GM_ERR_UNSUPPORTED,
                    "the sums need nsp = 10 (moment: U, V x 4 jumps, W x 2) or 5 (force: U, V x 2 jumps, W x 1), got %d", nsp);
+  }
   if (nrcv < 1) return gm_fail(c, GM_ERR_ARG, "nrcv must be positive");
   P.nf = nf2 - nf1 + 1;
-  P.nsp = nsp;
-  P.istyp = nsp == 10 ? 1 : 0;
+  P.nsp_in = nsp;
+  P.istyp = (stress || deriv) ? (nsp == 20 ? 1 : 0) : (nsp == 10 ? 1 : 0);
+  P.nsp = P.istyp ? 10 : 5;
+  // positionDataGreenFKSpectra of U(j), V(j), W(j): njsph = 4 / 2, njtor = 2 / 1
+  {
+    const int nj = P.istyp ? 4 : 2, nt = P.istyp ? 2 : 1;
+    const int vOff = (stress ? 2 : 1) * nj, wOff = 4 * nj * (stress || deriv ? 1 : 0) + (stress || deriv ? 0 : 2 * nj);
+    for (int k = 0; k < 10; ++k) P.smap.s[k] = 0;
+    for (int j = 0; j < nj; ++j) { P.smap.s[j] = j; P.smap.s[nj + j] = vOff + j; }
+    for (int j = 0; j < nt; ++j) P.smap.s[2 * nj + j] = wOff + j;
+  }
   P.nsum = P.istyp ? kSums : kForceSums;
   P.nwnf.resize(P.nf);
   P.koff.resize(P.nf);
@@ -1067,8 +1087,8 @@ int run_sums(gm_ctx* c, const SynPlan& P, const double* theta, const float* gr, 
   HIPCHK(c, hipMemcpyAsync(base + o.koff, P.koff.data(), sizeof(long long) * P.nf, hipMemcpyHostToDevice, s));
   const float *dgr = gr, *dgi = gi;
   if (!dev_in) {
-    HIPCHK(c, hipMemcpyAsync(base + o.gr, gr, sizeof(float) * P.nsp * P.nkf, hipMemcpyHostToDevice, s));
-    HIPCHK(c, hipMemcpyAsync(base + o.gi, gi, sizeof(float) * P.nsp * P.nkf, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(base + o.gr, gr, sizeof(float) * P.nsp_in * P.nkf, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(base + o.gi, gi, sizeof(float) * P.nsp_in * P.nkf, hipMemcpyHostToDevice, s));
     dgr = (const float*)(base + o.gr);
     dgi = (const float*)(base + o.gi);
   }
@@ -1091,12 +1111,10 @@ int run_sums(gm_ctx* c, const SynPlan& P, const double* theta, const float* gr, 
                        (const double*)seeds);
   }
   HIPCHK(c, hipEventRecord(c->sev[1], s));
-  SpecMap ident;
-  for (int k = 0; k < 10; ++k) ident.s[k] = k;
   hipLaunchKernelGGL(k_gprep, dim3((unsigned)((P.Lp + 255) / 256), (unsigned)(P.Np / 2)), dim3(256), 0, s, P.nsp, P.nf,
                      P.Np, P.Lp,
                      (const int*)(base + o.nwn), (const long long*)(base + o.koff), P.nkf, dgr, dgi, 0.2, dwn,
-                     P.istyp, ident, (__bf16*)(base + o.Gs));
+                     P.istyp, P.smap, (__bf16*)(base + o.Gs));
   HIPCHK(c, hipEventRecord(c->sev[2], s));
   int nsplit, kper;
   contractSplits(P, nsplit, kper);
@@ -1112,7 +1130,7 @@ int run_sums(gm_ctx* c, const SynPlan& P, const double* theta, const float* gr, 
   if (regional)
     hipLaunchKernelGGL(P.istyp ? k_wnfix<1> : k_wnfix<0>, dim3((P.R * P.nf + 255) / 256), dim3(256), 0, s, P.R, P.nf,
                        P.Rp, P.Np, (const int*)(base + o.nwn), (const long long*)(base + o.koff), P.nkf, dgr, dgi, dth,
-                       dwn, re, 0.2, (double*)(base + o.H));
+                       dwn, re, 0.2, P.smap, (double*)(base + o.H));
   HIPCHK(c, hipEventRecord(c->sev[3], s));
   HIPCHK(c, hipGetLastError());
   return GM_OK;
@@ -1132,8 +1150,8 @@ SumBufs carve_sums(const SynPlan& P, bool dev_in, Carve& cv) {
   int nsplit, kper;
   contractSplits(P, nsplit, kper);
   o.Hp = nsplit > 1 ? cv.take(sizeof(double) * nsplit * P.nsum * (size_t)P.Rp * P.Np) : 0;
-  o.gr = dev_in ? 0 : cv.take(sizeof(float) * P.nsp * P.nkf);
-  o.gi = dev_in ? 0 : cv.take(sizeof(float) * P.nsp * P.nkf);
+  o.gr = dev_in ? 0 : cv.take(sizeof(float) * P.nsp_in * P.nkf);
+  o.gi = dev_in ? 0 : cv.take(sizeof(float) * P.nsp_in * P.nkf);
   return o;
 }
 
@@ -1172,10 +1190,11 @@ static int sums_impl(gm_ctx* c, int istyp, int nf1, int nf2, const int* nwn, int
                      double dwn, double re) {
   if (!c) return GM_ERR_ARG;
   SynPlan P;
-  int rc = plan_sums(c, nf1, nf2, nwn, nsp, nkf, nrcv, P);
+  int rc = plan_sums(c, nf1, nf2, nwn, nsp, nkf, nrcv, P, flags);
   if (rc) return rc;
   if (P.istyp != istyp)
-    return gm_fail(c, GM_ERR_ARG, istyp ? "moment sums need nsp = 10 spectra" : "force sums need nsp = 5 spectra");
+    return gm_fail(c, GM_ERR_ARG, istyp ? "moment sums need a moment source's spectra (nsp = 10, or 20 with stresses or derivatives)"
+                                        : "force sums need a force's spectra (nsp = 5, or 10 with stresses or derivatives)");
   if (!gfkspr || !gfkspi || !theta || !hdsum) return gm_fail(c, GM_ERR_ARG, "spectra, theta and hdsum are required");
   const bool dev = (flags & GM_FLAG_DEVICE_PTRS) != 0;
   HIPCHK(c, hipSetDevice(c->device));
@@ -1207,11 +1226,10 @@ static int synth_impl(gm_ctx* c, int nf1, int nf2, double df, double sigma, cons
   if (!c) return GM_ERR_ARG;
   if (istyp != 0 && istyp != 1) return gm_fail(c, GM_ERR_ARG, "istyp must be 0 (FORCE) or 1 (MOMENT)");
   SynPlan P;
-  int rc = plan_sums(c, nf1, nf2, nwn, nsp, nkf, nrcv, P);
+  int rc = plan_sums(c, nf1, nf2, nwn, nsp, nkf, nrcv, P, flags);
   if (rc) return rc;
   if (P.istyp != istyp)
-    return gm_fail(c, GM_ERR_UNSUPPORTED, "istyp %d needs nsp = %d spectra (U, V, W without stresses), got %d", istyp,
-                   istyp ? 10 : 5, nsp);
+    return gm_fail(c, GM_ERR_UNSUPPORTED, "istyp %d does not match nsp = %d spectra of this layout", istyp, nsp);
   if (!gfkspr || !gfkspi || !theta || !phi || !mt || !ts)
     return gm_fail(c, GM_ERR_ARG, "spectra, theta, phi, fomt and ts are required");
   if (!(df > 0.0) || !(dt > 0.0)) return gm_fail(c, GM_ERR_ARG, "df and dt must be positive");

@@ -25,6 +25,10 @@ FORCE_SUMS = 7
 NSP = {0: 5, 1: 10}
 
 
+# spectrum layouts of a GFK file and their flags (GM_FLAG_STRESS_SPECTRA, GM_FLAG_DERIV_SPECTRA)
+LAYOUT_FLAGS = {"uvw": 0, "stress": 2, "deriv": 4}
+
+
 class SyntheticsEngine:
     def __init__(self, device: int = 0):
         self.device = device
@@ -101,13 +105,13 @@ class SyntheticsEngine:
             raise ValueError("gr, gi must be [nsp, nkf] arrays of one shape")
         return gr, gi
 
-    def _sums(self, fn, nsum, nf1, nf2, nwn, gr, gi, theta):
+    def _sums(self, fn, nsum, nf1, nf2, nwn, gr, gi, theta, layout="uvw"):
         nwn = np.ascontiguousarray(nwn, dtype=np.int32)
         gr, gi = self._spectra(gr, gi)
         theta = np.ascontiguousarray(np.atleast_1d(theta), dtype=np.float64)
         out = np.zeros((nsum, theta.size, nf2 - nf1 + 1), np.complex128)
         check(self.h, fn(self.h, int(nf1), int(nf2), nwn.ctypes.data_as(I), gr.shape[0], gr.shape[1], gr.ctypes.data,
-                         gi.ctypes.data, theta.size, theta.ctypes.data_as(D), 0, out.ctypes.data))
+                         gi.ctypes.data, theta.size, theta.ctypes.data_as(D), LAYOUT_FLAGS[layout], out.ctypes.data))
         return out
 
     def _wn_sums(self, fn, nsum, nf1, nf2, nwn, gr, gi, dwn, rearth, x):
@@ -119,10 +123,10 @@ class SyntheticsEngine:
                          gi.ctypes.data, float(dwn), float(rearth), x.size, x.ctypes.data_as(D), 0, out.ctypes.data))
         return out
 
-    def moment_sums(self, nf1, nf2, nwn, gr, gi, theta):
+    def moment_sums(self, nf1, nf2, nwn, gr, gi, theta, layout="uvw"):
         """14 moment harmonic degree sums, complex [14, nrcv, nf2-nf1+1] (momentHarmonicDegreeSums);
-        gr, gi: [10, nkf]."""
-        return self._sums(_lib.lib().gm_moment_sums, MOMENT_SUMS, nf1, nf2, nwn, gr, gi, theta)
+        gr, gi: [10, nkf] (layout "uvw") or [20, nkf] ("stress", "deriv")."""
+        return self._sums(_lib.lib().gm_moment_sums, MOMENT_SUMS, nf1, nf2, nwn, gr, gi, theta, layout)
 
     def force_sums(self, nf1, nf2, nwn, gr, gi, theta):
         """7 single-force harmonic degree sums, complex [7, nrcv, nf2-nf1+1] (forceHarmonicDegreeSums);
@@ -167,7 +171,7 @@ class SyntheticsEngine:
 
     def synthetics(self, nf1, nf2, df, sigma, nwn, gr, gi, theta, phi, mt, nsamp, dt, propdir=None, tred=None,
                    velocity=False, want_spectra=False, device_ptrs=None, dwn=None, rearth=None, evfilter=None,
-                   stafilter=None, istyp=None):
+                   stafilter=None, istyp=None, layout="uvw"):
         """Seismograms [nrcv, 3, nsamp] FP32 (and displacement spectra [nrcv, 3, nf2] if asked).
 
         Source type istyp (the GFK file's): 1, moment tensor, nsp = 10, mt = (Mrr, Mtt, Mff, Mrt, Mrf, Mtf);
@@ -179,14 +183,17 @@ class SyntheticsEngine:
         theta = epicentral distances in metres, wavenumber integrals with k = (j-1) dwn.
         device_ptrs = (gr_ptr, gi_ptr, nsp, nkf, ts_ptr): spectra and output already in HBM
         (integers, e.g. torch tensors' data_ptr()); gr, gi are then ignored and ts is written in place.
+        layout: the file's spectra per node: "uvw" (U, V, W), "stress" (U, R, V, S, W, T: STRESSFLAG
+        files) or "deriv" (U, V, dU/dr, dV/dr, W, dW/dr: the multi-source driver's files).
         """
+        lflag = LAYOUT_FLAGS[layout]
         nwn = np.ascontiguousarray(nwn, dtype=np.int32)
         theta = np.ascontiguousarray(np.atleast_1d(theta), dtype=np.float64)
         phi = np.ascontiguousarray(np.atleast_1d(phi), dtype=np.float64)
         mt = np.ascontiguousarray(mt, dtype=np.float64)
         nsp_in = device_ptrs[2] if device_ptrs is not None else np.shape(gr)[0]
         if istyp is None:
-            istyp = 0 if int(nsp_in) == NSP[0] else 1
+            istyp = (0 if int(nsp_in) == NSP[0] else 1) if lflag == 0 else (1 if int(nsp_in) == 20 else 0)
         if phi.size != theta.size or mt.size != (6 if istyp == 1 else 3):
             raise ValueError("phi must match theta and mt must hold 6 (moment) or 3 (force) entries")
         ev = None if evfilter is None else np.ascontiguousarray(evfilter, dtype=np.complex128)
@@ -201,14 +208,14 @@ class SyntheticsEngine:
         zsp = np.zeros((theta.size, 3, nf2), np.complex128) if want_spectra else None
         if device_ptrs is None:
             gr, gi = self._spectra(gr, gi)
-            nsp, nkf, pgr, pgi, flags = gr.shape[0], gr.shape[1], gr.ctypes.data, gi.ctypes.data, 0
+            nsp, nkf, pgr, pgi, flags = gr.shape[0], gr.shape[1], gr.ctypes.data, gi.ctypes.data, lflag
             ts = np.zeros((theta.size, 3, nsamp), np.float32)
             pts = ts.ctypes.data
         else:
             if want_spectra:
                 raise ValueError("want_spectra needs host outputs")
             pgr, pgi, nsp, nkf, pts = device_ptrs
-            ts, flags = None, 1
+            ts, flags = None, 1 | lflag
         tail = (theta.size, theta.ctypes.data_as(D), phi.ctypes.data_as(D), None if pd is None else pd.ctypes.data_as(D),
                 None if tr is None else tr.ctypes.data_as(D), None if ev is None else ev.ctypes.data_as(D),
                 None if st is None else st.ctypes.data_as(D), int(istyp), mt.ctypes.data_as(D), int(bool(velocity)),

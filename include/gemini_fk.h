@@ -51,6 +51,14 @@ typedef struct gm_ctx gm_ctx;
 
 /* flags of the batch calls */
 #define GM_FLAG_DEVICE_PTRS 1  /* output pointers are device (HBM) pointers */
+/* Spectrum layout of the synthetics calls' input (dsvmask of the GFK file, greenFKSpectra.f90:
+ * 107-111, 204-212).  Neither flag: U, V (x njsph jumps), W (x njtor): nsp = 10 (moment) or 5
+ * (force).  GM_FLAG_STRESS_SPECTRA: U, R, V, S and W, T (files written with STRESSFLAG,
+ * dsvstep = 1): nsp = 20 or 10.  GM_FLAG_DERIV_SPECTRA: U, V, dU/dr, dV/dr and W, dW/dr (the
+ * multi-source driver's files, derivflag = 1): nsp = 20 or 10.  The displacement sums read U, V
+ * and W wherever the layout puts them. */
+#define GM_FLAG_STRESS_SPECTRA 2
+#define GM_FLAG_DERIV_SPECTRA 4
 #define GM_FILTER_BUTTERWORTH_BANDPASS 1  /* event filter type 'BUTTERWORTH_BANDPASS' (f90/eventFilter.f90:56) */
 
 /* options (gm_set_option) */
@@ -262,7 +270,8 @@ int gm_legendre_batch(gm_ctx* ctx, int nrcv, const double* theta, int lmax, doub
 /* The 14 moment-tensor harmonic degree sums (kint 1..14) of one source node
  * for receivers at epicentral distances theta[r] (radians) and frequency
  * indices jf = nf1..nf2 (Fortran numbering, f = (jf-1) df) with nwn[jf-1]
- * degrees each.  gfkspr/gfkspi: FP32 [nsp = 10][nkf], k over the frequencies
+ * degrees each.  gfkspr/gfkspi: FP32 [nsp = 10][nkf] (or 20 with
+ * GM_FLAG_STRESS_SPECTRA or GM_FLAG_DERIV_SPECTRA), k over the frequencies
  * nf1..nf2 with degrees fastest (readDataNodeGreenFKSpectra's layout,
  * f90/greenFKSpectra.f90:171-212).  hdsum: complex [14][nrcv][nf2-nf1+1].
  * Replaces momentHarmonicDegreeSums with taperHarmonicDegreeSums
@@ -276,7 +285,8 @@ int gm_moment_sums(gm_ctx* ctx, int nf1, int nf2, const int* nwn, int nsp, long 
 
 /* The 7 single-force harmonic degree sums (kint 1..7, displacement) of one
  * source node, arguments as for gm_moment_sums with nsp = 5 spectra (U, V x 2
- * jumps, W x 1: the FK spectra of a FORCE source, istyp = 0).
+ * jumps, W x 1: the FK spectra of a FORCE source, istyp = 0; 10 with the
+ * layout flags).
  * hdsum: complex [7][nrcv][nf2-nf1+1].  Replaces forceHarmonicDegreeSums with
  * taperHarmonicDegreeSums (f90/harmonicDegreeSums.f90:95-167, 356-369) in the
  * frequency loop of sphericalHarmonicsDisplacementSpectra
@@ -289,8 +299,9 @@ int gm_force_sums(gm_ctx* ctx, int nf1, int nf2, const int* nwn, int nsp, long l
  * f90/displacementSpectra.f90:200-228) by the GFK file's source type istyp:
  * istyp = 1, moment source, nsp = 10, fomt = Mrr, Mtt, Mff, Mrt, Mrf, Mtf
  * (f90/singleFrequencyDisplacement.f90:108-125); istyp = 0, single force,
- * nsp = 5, fomt = Fr, Ft, Ff (:35-46).  Spectra with stresses (STRESSFLAG:
- * nsp = 20 or 10 with dsvmask U, R, V, S, W, T) are not supported.  Then time shift by tred[r] (NULL: none),
+ * nsp = 5, fomt = Fr, Ft, Ff (:35-46); with GM_FLAG_STRESS_SPECTRA or
+ * GM_FLAG_DERIV_SPECTRA in flags, nsp = 20 or 10 and U, V, W are read where that
+ * layout puts them.  Then time shift by tred[r] (NULL: none),
  * event and station filters (evfilter: complex [nf2], stafilter: complex
  * [nrcv][3][nf2] for the components r, theta, phi, i.e. the reference's
  * stafilter(nf2, 3, nstat); NULL: unity; f90/computeSyntheticSeismograms.f90:

@@ -69,3 +69,29 @@ def test_strain_spectra_errors(syn):
         syn.strain_spectra(nf1, nf2, nwn, gr[:10], gi[:10], theta, phi, RE, fomt, istyp=1)
     with pytest.raises(GeminiFKError):
         syn.strain_spectra(nf1, nf2, nwn, gr, gi, theta, phi, -1.0, fomt, istyp=1)
+
+
+@pytest.mark.parametrize("istyp", [1, 0])
+@pytest.mark.parametrize("layout", ["stress", "deriv"])
+def test_file_layouts_give_the_displacement_of_their_uvw_spectra(syn, istyp, layout):
+    """Files with stresses (STRESSFLAG: U, R, V, S, W, T) or radial derivatives (U, V, dU, dV, W, dW):
+    the seismograms, global and regional, are those of the U, V, W spectra alone, bit for bit
+    (positionDataGreenFKSpectra, greenFKSpectra.f90:204-212, with each file's dsvmask)."""
+    nf1, nf2, nwn, gr, gi, theta, phi, fomt = _case(istyp, 9)
+    nj, nt = (4, 2) if istyp == 1 else (2, 1)
+    if layout == "stress":
+        uvw = np.concatenate([np.arange(nj), 2 * nj + np.arange(nj), 4 * nj + np.arange(nt)])
+    else:
+        uvw = np.concatenate([np.arange(2 * nj), 4 * nj + np.arange(nt)])
+    df, sigma = 1 / 2048.0, 5 / 2048.0
+    kw = dict(istyp=istyp, want_spectra=True)
+    ts20, z20 = syn.synthetics(nf1, nf2, df, sigma, nwn, gr, gi, theta, phi, fomt, 16, 1.0 / (16 * df),
+                               layout=layout, **kw)
+    ts10, z10 = syn.synthetics(nf1, nf2, df, sigma, nwn, gr[uvw], gi[uvw], theta, phi, fomt, 16, 1.0 / (16 * df), **kw)
+    assert np.array_equal(z20, z10) and np.array_equal(ts20, ts10)
+    x = theta * 1.0e5
+    kwr = dict(dwn=2 * np.pi / 8.0e6, rearth=6371000.0, **kw)
+    tr20, zr20 = syn.synthetics(nf1, nf2, df, sigma, nwn, gr, gi, x, phi, fomt, 16, 1.0 / (16 * df), layout=layout,
+                                **kwr)
+    tr10, zr10 = syn.synthetics(nf1, nf2, df, sigma, nwn, gr[uvw], gi[uvw], x, phi, fomt, 16, 1.0 / (16 * df), **kwr)
+    assert np.array_equal(zr20, zr10) and np.array_equal(tr20, tr10)
