@@ -352,9 +352,10 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   if (const int rc = check_nodes(c)) return rc;
   if (full) {
     if (istyp != 0 && istyp != 1) return gm_fail(c, GM_ERR_ARG, "istyp must be 0 (FORCE) or 1 (MOMENT)");
-    // unitJumpGreenFunctions.f90:241-298 (receiver in the ocean) is not built; the receiver must lie in
-    // the solid mantle, the other nodes anywhere (inner core, fluid core, mantle, ocean)
-    if (m.state[jre] != 0) return gm_fail(c, GM_ERR_UNSUPPORTED, "receiver node in a fluid layer is not supported");
+    // the receiver lies in the solid mantle or the ocean (unitJumpGreenFunctions.f90:119-123, 242),
+    // the other nodes anywhere (inner core, fluid core, mantle, ocean)
+    if (m.state[jre] != 0 && m.layer[jre] != m.nlay)
+      return gm_fail(c, GM_ERR_UNSUPPORTED, "receiver node in the fluid core is not supported");
     if (m.hascore && !(m.rnod[jre] > m.roc))
       return gm_fail(c, GM_ERR_UNSUPPORTED, "receiver node below the core-mantle boundary is not supported");
   }
@@ -409,8 +410,10 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
       }
   }
   const size_t n = P.ninst;
-  // spectra per node: U, V, W (moment 10, force 5); multi-source: U, V, dU/dr, dV/dr, W, dW/dr (20, 10)
-  const int nsp = aski ? (istyp == 1 ? 20 : 10) : (istyp == 1 ? 10 : 5);
+  // spectra per node: U, V, W (moment 10, force 5), U, V for a receiver in the ocean (8, 4);
+  // multi-source: U, V, dU/dr, dV/dr, W, dW/dr (20, 10)
+  const bool ocean = full && m.state[jre] == 1;
+  const int nsp = aski ? (istyp == 1 ? 20 : 10) : (istyp == 1 ? 10 : 5) - (ocean ? (istyp == 1 ? 2 : 1) : 0);
   const size_t nout = (size_t)nsrc * nsp * nnod * n;
   HIPCHK(c, hipSetDevice(c->device));
   // device buffer layout
@@ -501,7 +504,8 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   B.dlg = (double*)(base + o_dlg); B.dets = (double*)(base + o_dets);
   B.ujs = full ? (double*)(base + o_ujs) : nullptr;
   B.adjend = (full && core_nodes) ? (double*)(base + o_adje) : nullptr;
-  B.adj_jumps = c->adj_pair ? 2 : 4;
+  // adjoint jobs per instance: one per jump (two with GM_OPT_ADJOINT_PAIR), one for a receiver in the ocean
+  B.adj_jumps = ocean ? 1 : (c->adj_pair ? 2 : 4);
   B.adj_jobs = B.adj_jumps + (ic_nodes ? 1 : 0);
   if (full) {
     B.gf = gf ? (dev ? gf : (double*)(base + o_gf)) : nullptr;

@@ -305,7 +305,7 @@ __device__ __forceinline__ double taperWn(int l, int n, double dwn, double tapfr
 // rounded up, Np / 2).
 // Plane s of Gs takes spectrum smap.s[s] of the node's spectra (the identity for
 // files with U, V, W; the displacement or the radial-derivative spectra of files
-// with derivatives).
+// with derivatives; -1 for the absent W of a receiver in the ocean).
 struct SpecMap { int s[10]; };
 __global__ void k_gprep(int nsp, int nf, int Np, int Lp, const int* __restrict__ nwn, const long long* __restrict__ koff,
                         long long nkf, const float* __restrict__ gr, const float* __restrict__ gi, double tapfrac,
@@ -337,8 +337,9 @@ __global__ void k_gprep(int nsp, int nf, int Np, int Lp, const int* __restrict__
     }
   }
   for (int s = 0; s < nsp; ++s) {
-    const bool on = l < n && (regional ? l >= 1 : l >= c_lmin[istyp][s]);
-    const long long so = (long long)smap.s[s] * nkf + k;
+    // an absent spectrum (smap.s[s] < 0: W of a receiver in the ocean) is zero
+    const bool on = smap.s[s] >= 0 && l < n && (regional ? l >= 1 : l >= c_lmin[istyp][s]);
+    const long long so = on ? (long long)smap.s[s] * nkf + k : 0;
     const double vr = on ? (double)gr[so] * w : 0.0;
     const double vi = on ? (double)gi[so] * w : 0.0;
     const size_t o = (size_t)s * per_s + (size_t)(2 * f) * Lp + l;
@@ -588,7 +589,8 @@ __global__ void k_wnfix(int R, int nf, int Rp, int Np, const int* __restrict__ n
     const double b0 = j0(k * xx), b1 = j1(k * xx);
     wnRows(k, g.x, re, b0, b1, -b0 + 2.0 / (k * xx) * b1, w);
   };
-  auto spec = [&](int s, int j) -> cd {      // tapered spectrum s at wavenumber index j
+  auto spec = [&](int s, int j) -> cd {      // tapered spectrum s at wavenumber index j (absent: zero)
+    if (smap.s[s] < 0) return cd{0.0, 0.0};
     const long long k = (long long)smap.s[s] * nkf + koff[f] + (j - 1);
     const double t = taperWn(j - 1, n, dwn, tapfrac);
     return cd{(double)gr[k] * t, (double)gi[k] * t};
@@ -995,10 +997,18 @@ int plan_sums(gm_ctx* c, int nf1, int nf2, const int* nwn, int nsp, long long nk
   if (nf1 < 1 || nf2 < nf1) return gm_fail(c, GM_ERR_ARG, "invalid frequency index range %d..%d", nf1, nf2);
   if (!nwn) return gm_fail(c, GM_ERR_ARG, "nwn is required");
   const bool stress = (flags & GM_FLAG_STRESS_SPECTRA) != 0, deriv = (flags & GM_FLAG_DERIV_SPECTRA) != 0;
+  const bool ocean = (flags & GM_FLAG_OCEAN_SPECTRA) != 0;
   if (stress && deriv) return gm_fail(c, GM_ERR_ARG, "a file has either stresses or derivatives");
+  if (ocean && deriv) return gm_fail(c, GM_ERR_ARG, "files with derivatives have no receiver in the ocean");
   // numberGreenFKSpectra (greenFKSpectra.f90:216-220): dsvmask (U, V, W) gives 2 x 4 + 2 = 10 for a
-  // moment source and 2 x 2 + 1 = 5 for a single force; with stresses or derivatives, 20 and 10
-  if (stress || deriv) {
+  // moment source and 2 x 2 + 1 = 5 for a single force; with stresses or derivatives, 20 and 10;
+  // a receiver in the ocean (ncsph = 2, nctor = 0: no S, no W, :129-133) 8 and 4, with stresses 12 and 6
+  if (ocean) {
+    const int nm = stress ? 12 : 8;
+    if (nsp != nm && nsp != nm / 2)
+      return gm_fail(c, GM_ERR_ARG, "spectra of a receiver in the ocean%s need nsp = %d (moment) or %d (force), got %d",
+                     stress ? " with stresses" : "", nm, nm / 2, nsp);
+  } else if (stress || deriv) {
     if (nsp != 20 && nsp != 10)
       return gm_fail(c, GM_ERR_ARG, "spectra with %s need nsp = 20 (moment) or 10 (force), got %d",
                      stress ? "stresses" : "derivatives", nsp);
@@ -1009,7 +1019,7 @@ int plan_sums(gm_ctx* c, int nf1, int nf2, const int* nwn, int nsp, long long nk
   if (nrcv < 1) return gm_fail(c, GM_ERR_ARG, "nrcv must be positive");
   P.nf = nf2 - nf1 + 1;
   P.nsp_in = nsp;
-  P.istyp = (stress || deriv) ? (nsp == 20 ? 1 : 0) : (nsp == 10 ? 1 : 0);
+  P.istyp = ocean ? (nsp == (stress ? 12 : 8) ? 1 : 0) : (stress || deriv) ? (nsp == 20 ? 1 : 0) : (nsp == 10 ? 1 : 0);
   P.nsp = P.istyp ? 10 : 5;
   // positionDataGreenFKSpectra of U(j), V(j), W(j): njsph = 4 / 2, njtor = 2 / 1
   {
@@ -1017,7 +1027,7 @@ int plan_sums(gm_ctx* c, int nf1, int nf2, const int* nwn, int nsp, long long nk
     const int vOff = (stress ? 2 : 1) * nj, wOff = 4 * nj * (stress || deriv ? 1 : 0) + (stress || deriv ? 0 : 2 * nj);
     for (int k = 0; k < 10; ++k) P.smap.s[k] = 0;
     for (int j = 0; j < nj; ++j) { P.smap.s[j] = j; P.smap.s[nj + j] = vOff + j; }
-    for (int j = 0; j < nt; ++j) P.smap.s[2 * nj + j] = wOff + j;
+    for (int j = 0; j < nt; ++j) P.smap.s[2 * nj + j] = ocean ? -1 : wOff + j;
   }
   P.nsum = P.istyp ? kSums : kForceSums;
   P.nwnf.resize(P.nf);

@@ -23,6 +23,7 @@ module geminiFKc
    integer(c_int), parameter :: GM_FLAG_DEVICE_PTRS = 1
    integer(c_int), parameter :: GM_FLAG_STRESS_SPECTRA = 2
    integer(c_int), parameter :: GM_FLAG_DERIV_SPECTRA = 4
+   integer(c_int), parameter :: GM_FLAG_OCEAN_SPECTRA = 8
    integer(c_int), parameter :: GM_FILTER_BUTTERWORTH_BANDPASS = 1
    integer(c_int), parameter :: GM_OPT_VARIANT = 1
    integer(c_int), parameter :: GM_OPT_PERSISTENT = 2

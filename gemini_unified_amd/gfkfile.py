@@ -329,6 +329,18 @@ class GreenFKSpectra:
         """numberGreenFKSpectra (greenFKSpectra.f90:210-215)."""
         return int(self.dsvmask[:6].sum()) * self.njsph + int(self.dsvmask[6:].sum()) * self.njtor
 
+    @property
+    def layout(self) -> str:
+        """the synthetics engine's name of this file's spectrum layout (dsvmask, greenFKSpectra.f90:
+        107-133): "uvw", "stress" (STRESSFLAG), "deriv" (radial derivatives), "ocean" or
+        "ocean_stress" (a receiver in the ocean, no W)"""
+        m = self.dsvmask
+        if self.nctor == 0:
+            return "ocean_stress" if m[1] else "ocean"
+        if m[4]:
+            return "deriv"
+        return "stress" if m[1] else "uvw"
+
     def position(self, isp: int, jsp: int) -> int:
         """positionDataGreenFKSpectra (greenFKSpectra.f90:197-205): 1-based spectrum index of
         component isp (1..9) and jump jsp."""
@@ -365,7 +377,8 @@ def write_green_fk_spectra_for_synthetics(engine, path: str, grid: FrequencyGrid
                                           freq_block: int = 64):
     """Computes the Green-FK spectra of every frequency of grid on the GPU and writes the GFK file
     as computeGreenFKSpectraForSynthetics does (computeGreenFKSpectraForSynthetics.f90:236-345,
-    500-519), receiver node jre in a solid layer.  Global mode: degrees l = 0 .. nwn-1; regional
+    500-519), receiver node jre in a solid layer or the ocean (ncsph = 2, nctor = 0: U and V of the
+    spheroidal jumps, :293-300).  Global mode: degrees l = 0 .. nwn-1; regional
     mode (global_ = 0): wavenumbers k = (iwn-1) dwn, iwn = 1 .. nwn, with k = 0 written as zero.
     Returns the identity string."""
     nodes = engine.nodes
@@ -378,7 +391,7 @@ def write_green_fk_spectra_for_synthetics(engine, path: str, grid: FrequencyGrid
     if global_ == 0 and not dwn > 0.0:
         raise ValueError("regional mode needs the wavenumber spacing dwn")
     njsph, njtor = (4, 2) if istyp == 1 else (2, 1)
-    ncsph, nctor = 4, 2                    # receiver in a solid layer
+    ncsph, nctor = (2, 0) if nodes.state[jre - 1] == 1 else (4, 2)   # receiver in the ocean or a solid
     nsp = writer_number_of_spectra(2, ncsph, njsph, njtor)
     identity = fdate() if identity is None else identity
     ro, elcon, qinv = engine.node_properties(nem.fref, "ELASTIC")

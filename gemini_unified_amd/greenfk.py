@@ -207,12 +207,22 @@ class GreenFKEngine:
             check(self.h, rc)
         return out
 
+    def nsp(self, jre, istyp=1):
+        """spectra per node of the columns for a receiver at node jre: U, V (x 4 / 2 jumps) and W
+        (x 2 / 1), 10 or 5; 8 or 4 for a receiver in the ocean, which has no W
+        (computeGreenFKSpectraForSynthetics.f90:293-313)"""
+        njsph, njtor = (4, 2) if istyp == 1 else (2, 1)
+        return 2 * njsph + (0 if self.nodes.state[int(jre) - 1] == 1 else njtor)
+
     def gfk(self, freqs, lmin, nl, sigma, eps, jre, istyp=1, attmode="ATTENUATION_AND_DISPERSION",
             want_fp32=True, raise_on_error=True, dwn=None, want_gf=True):
-        """Green-FK columns; gf[node, isp, k] complex128 (and FP32 re/im as written to the file)."""
+        """Green-FK columns; gf[node, isp, k] complex128 (and FP32 re/im as written to the file),
+        with nsp = self.nsp(jre, istyp) spectra per node."""
         freqs, lmin, nl, nk = self._plan(freqs, lmin, nl)
         nn = self.nodes.nnod
-        nsp = 10 if istyp == 1 else 5
+        if not 1 <= int(jre) <= nn:
+            raise _lib.GeminiFKError(3, f"receiver node {jre} outside 1..{nn}")
+        nsp = self.nsp(jre, istyp)
         gf = np.zeros((nn, nsp, nk), np.complex128) if want_gf else None
         gfre = np.zeros((nn, nsp, nk), np.float32) if want_fp32 else None
         gfim = np.zeros((nn, nsp, nk), np.float32) if want_fp32 else None
@@ -281,7 +291,7 @@ class GreenFKEngine:
         """The frequency loop of computeGreenFKSpectraForSynthetics: returns gfre, gfim with the
         file layout (nkf, nsp, nnod) in Fortran order, i.e. C arrays [nnod][nsp][nkf]."""
         nn = self.nodes.nnod
-        nsp = 10 if istyp == 1 else 5
+        nsp = self.nsp(jre, istyp)
         gfre = np.zeros((nn, nsp, grid.nkf), np.float32)
         gfim = np.zeros((nn, nsp, grid.nkf), np.float32)
         freqs = grid.freqs

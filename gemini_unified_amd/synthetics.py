@@ -25,8 +25,12 @@ FORCE_SUMS = 7
 NSP = {0: 5, 1: 10}
 
 
-# spectrum layouts of a GFK file and their flags (GM_FLAG_STRESS_SPECTRA, GM_FLAG_DERIV_SPECTRA)
-LAYOUT_FLAGS = {"uvw": 0, "stress": 2, "deriv": 4}
+# spectrum layouts of a GFK file and their flags (GM_FLAG_STRESS_SPECTRA, GM_FLAG_DERIV_SPECTRA,
+# GM_FLAG_OCEAN_SPECTRA): "ocean" and "ocean_stress" are the files of a receiver in the ocean
+# (U, V or U, R, V; no W)
+LAYOUT_FLAGS = {"uvw": 0, "stress": 2, "deriv": 4, "ocean": 8, "ocean_stress": 10}
+# spectra per node of a moment source in each layout (half of it for a single force)
+LAYOUT_NSP = {"uvw": 10, "stress": 20, "deriv": 20, "ocean": 8, "ocean_stress": 12}
 
 
 class SyntheticsEngine:
@@ -125,7 +129,7 @@ class SyntheticsEngine:
 
     def moment_sums(self, nf1, nf2, nwn, gr, gi, theta, layout="uvw"):
         """14 moment harmonic degree sums, complex [14, nrcv, nf2-nf1+1] (momentHarmonicDegreeSums);
-        gr, gi: [10, nkf] (layout "uvw") or [20, nkf] ("stress", "deriv")."""
+        gr, gi: [LAYOUT_NSP[layout], nkf], e.g. [10, nkf] (layout "uvw") or [20, nkf] ("stress", "deriv")."""
         return self._sums(_lib.lib().gm_moment_sums, MOMENT_SUMS, nf1, nf2, nwn, gr, gi, theta, layout)
 
     def force_sums(self, nf1, nf2, nwn, gr, gi, theta):
@@ -184,7 +188,8 @@ class SyntheticsEngine:
         device_ptrs = (gr_ptr, gi_ptr, nsp, nkf, ts_ptr): spectra and output already in HBM
         (integers, e.g. torch tensors' data_ptr()); gr, gi are then ignored and ts is written in place.
         layout: the file's spectra per node: "uvw" (U, V, W), "stress" (U, R, V, S, W, T: STRESSFLAG
-        files) or "deriv" (U, V, dU/dr, dV/dr, W, dW/dr: the multi-source driver's files).
+        files), "deriv" (U, V, dU/dr, dV/dr, W, dW/dr: the multi-source driver's files), "ocean"
+        (U, V: a receiver in the ocean) or "ocean_stress" (U, R, V).
         """
         lflag = LAYOUT_FLAGS[layout]
         nwn = np.ascontiguousarray(nwn, dtype=np.int32)
@@ -193,7 +198,7 @@ class SyntheticsEngine:
         mt = np.ascontiguousarray(mt, dtype=np.float64)
         nsp_in = device_ptrs[2] if device_ptrs is not None else np.shape(gr)[0]
         if istyp is None:
-            istyp = (0 if int(nsp_in) == NSP[0] else 1) if lflag == 0 else (1 if int(nsp_in) == 20 else 0)
+            istyp = 0 if int(nsp_in) == LAYOUT_NSP[layout] // 2 else 1
         if phi.size != theta.size or mt.size != (6 if istyp == 1 else 3):
             raise ValueError("phi must match theta and mt must hold 6 (moment) or 3 (force) entries")
         ev = None if evfilter is None else np.ascontiguousarray(evfilter, dtype=np.complex128)

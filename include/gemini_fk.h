@@ -55,10 +55,13 @@ typedef struct gm_ctx gm_ctx;
  * 107-111, 204-212).  Neither flag: U, V (x njsph jumps), W (x njtor): nsp = 10 (moment) or 5
  * (force).  GM_FLAG_STRESS_SPECTRA: U, R, V, S and W, T (files written with STRESSFLAG,
  * dsvstep = 1): nsp = 20 or 10.  GM_FLAG_DERIV_SPECTRA: U, V, dU/dr, dV/dr and W, dW/dr (the
- * multi-source driver's files, derivflag = 1): nsp = 20 or 10.  The displacement sums read U, V
- * and W wherever the layout puts them. */
+ * multi-source driver's files, derivflag = 1): nsp = 20 or 10.  GM_FLAG_OCEAN_SPECTRA: a
+ * receiver in the ocean (ncsph = 2, nctor = 0, greenFKSpectra.f90:129-133): U, V, no W: nsp = 8
+ * or 4; with GM_FLAG_STRESS_SPECTRA U, R, V: nsp = 12 or 6.  The displacement sums read U, V
+ * and W wherever the layout puts them, an absent W as zero. */
 #define GM_FLAG_STRESS_SPECTRA 2
 #define GM_FLAG_DERIV_SPECTRA 4
+#define GM_FLAG_OCEAN_SPECTRA 8
 #define GM_FILTER_BUTTERWORTH_BANDPASS 1  /* event filter type 'BUTTERWORTH_BANDPASS' (f90/eventFilter.f90:56) */
 
 /* options (gm_set_option) */
@@ -153,7 +156,11 @@ int gm_minors_batch(gm_ctx* ctx, int nfreq, const double* freqs, const int* lmin
  * spheroidalReciprocity/toroidalReciprocity (f90/reciprocity.f90:82,169) and
  * oneNodeSourceTerms (f90/sourceTerms.f90:59).
  * gf: complex FP64 [nnod][nsp][k]; gfre/gfim: FP32 [nnod][nsp][k] (the file's
- * values); nsp = 10 (moment) or 5 (force).  Any output may be NULL. */
+ * values); nsp = 10 (moment) or 5 (force): U, V (x njsph jumps), W (x njtor).
+ * A receiver in the ocean (a fluid node of the top layer, :293-300): nsp = 8 or
+ * 4, U and V = -R / (r om^2 rho) of the spheroidal jumps, r and rho taken at the
+ * node as the reference does (:465); a receiver in the fluid core returns
+ * GM_ERR_UNSUPPORTED.  Any output may be NULL. */
 int gm_gfk_batch(gm_ctx* ctx, int nfreq, const double* freqs, const int* lmin, const int* nl, double sigma,
                  double eps, int attmode, int jre, int istyp, int flags, double* gf, float* gfre, float* gfim,
                  int* status, long long* stats);

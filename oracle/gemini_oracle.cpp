@@ -1298,13 +1298,16 @@ static int unitJumpGreen(const Env& env, ToroidalMinors& T, SpheroidalMinors& S,
   double ruc = ruc_(M), roc = M.roc, ric = M.ric;
   col.ujs.assign(16 * nnod, C(0.0));
   col.ujt.assign(4 * nnod, C(0.0));
-  if (N.state[jn] == 1) return ERR_UNSUPPORTED;   // source/receiver node in the ocean
+  // a jump node in a fluid layer: the ocean (unitJumpGreenFunctions.f90:119-123, 242) only
+  const bool ocean = N.state[jn] == 1;
+  if (ocean && N.layer[jn] != M.nlay) return ERR_UNSUPPORTED;
   auto US = [&](int i, int j, int n) -> cx& { return col.ujs[(n - 1) * 16 + (j - 1) * 4 + (i - 1)]; };
   auto UT = [&](int i, int j, int n) -> cx& { return col.ujt[(n - 1) * 4 + (j - 1) * 2 + (i - 1)]; };
   double ujv[5][5] = {{0}};
   for (int i = 1; i <= 4; ++i) ujv[i][i] = 1.0;
-  // toroidal
-  if (cabs_(T.det[jn]) > TINYD) {
+  // toroidal (none for a jump in the ocean: njt = 0)
+  if (ocean) {
+  } else if (cabs_(T.det[jn]) > TINYD) {
     double rstart = startRadius(env, true);
     for (int j = 1; j <= 2; ++j) {
       cx d = cdiv(sub(rmul(ujv[2][j], T.D(1, jn)), rmul(ujv[1][j], T.D(2, jn))), T.det[jn]);
@@ -1341,6 +1344,52 @@ static int unitJumpGreen(const Env& env, ToroidalMinors& T, SpheroidalMinors& S,
     }
   }
   cx mtn[5][5], men[5][5];
+  if (ocean) {
+    // jump in the ocean (:242-298): the mantle propagator P_4m(r) from the ocean
+    // bottom down to the start radius, the CMB or the lowest node, once for both
+    // jumps; e D(r) at and above the jump, d U(r) in the ocean below it, d u_mtil
+    // P_4m in the mantle, d P_4m(roc)_3 U(r) in the outer core and
+    // d P_4m(roc)_3 u_mtil P_4m^ic in the inner core
+    std::vector<cx> adjma(4 * nnod, C(0.0));
+    cx adjma_roc3 = C(0.0);
+    const double ra = ruc, re = std::fmax(std::fmax(rstart, roc), N.rnod[1]);
+    if (re < ra) {
+      cx ys[6] = {C(0), C(0), C(0), C(0), C(1.0), C(0)};
+      int rc = spheroidalAdjoint(env, ys, ra, re, eps, adjma, st);
+      if (rc != OK) return rc;
+      adjma_roc3 = ys[3];
+    }
+    for (int j = 1; j <= 2; ++j) {
+      cx d = cdiv(sub(rmul(ujv[2][j], S.D(1, jn)), rmul(ujv[1][j], S.D(2, jn))), S.det[jn]);
+      cx e = cdiv(sub(rmul(ujv[2][j], S.U(1, jn)), rmul(ujv[1][j], S.U(2, jn))), S.det[jn]);
+      for (int n = jn; n <= nnod; ++n) { US(1, j, n) = mul(e, S.D(1, n)); US(2, j, n) = mul(e, S.D(2, n)); }
+      for (int n = jn - 1; n >= 1; --n) {
+        if (N.rnod[n] < rstart) continue;
+        if (N.state[n] == 1) {
+          if (N.rnod[n] > ruc) {
+            US(1, j, n) = mul(d, S.U(1, n)); US(2, j, n) = mul(d, S.U(2, n));
+          } else if (N.rnod[n] < roc) {
+            const cx s = mul(d, adjma_roc3);
+            US(1, j, n) = mul(s, S.U(1, n)); US(2, j, n) = mul(s, S.U(2, n));
+          }
+        } else {
+          const std::vector<cx>* a = nullptr;
+          cx s = d;
+          if (N.rnod[n] > roc) a = &adjma;
+          else if (N.rnod[n] < ric) { a = &adjic; s = mul(d, adjma_roc3); }
+          if (a) {
+            mtilFromMinors(S.u, n, mtn, men);
+            for (int i = 1; i <= 4; ++i) {
+              cx acc = C(0.0);
+              for (int k = 1; k <= 4; ++k) acc = add(acc, mul(mtn[i][k], (*a)[(n - 1) * 4 + (k - 1)]));
+              US(i, j, n) = mul(s, acc);
+            }
+          }
+        }
+      }
+    }
+    return OK;
+  }
   for (int j = 1; j <= 4; ++j) {
     // adjoint above the source: f(+)(r_s) = -s_i/det * u_meps(:,i)
     cx ys[6];
@@ -1493,14 +1542,21 @@ struct Ctx {
   int attmode = 3;
 };
 
-// full column for one (omega, l): computeGreenFKSpectraForSynthetics.f90:413-494
+static void horcompFluid(const Env& env, double r, int nj, cx gf[5][5], cx v[5], cx vr[5]);
+
+// full column for one (omega, l): computeGreenFKSpectraForSynthetics.f90:413-494.
+// For a receiver in the ocean only rows 1-2 of the reciprocal Green functions
+// enter, and spheroidalReciprocity's rows 1-2 for a source in the ocean
+// (reciprocity.f90:99-103, 119-128) are those for a source in the solid.
 static int gfkColumn(const Ctx& c, double omre, double omim, double dll1, int jre, int istyp, double eps,
                      Column& col, Stats& st, ToroidalMinors* Tout = nullptr, SpheroidalMinors* Sout = nullptr) {
   Env env; env.m = &c.m; env.nod = &c.nod; env.s = &c.s; env.omre = omre; env.omim = omim; env.dll1 = dll1; env.nl = 1;
   const Nodes& N = c.nod;
   int nnod = N.nnod;
   if (jre < 1 || jre > nnod) return ERR_NODEIDX;
-  if (N.state[jre] == 1) return ERR_UNSUPPORTED;     // receiver in the ocean
+  // a receiver in a fluid layer: the ocean only (ncsph = 2, nctor = 0, :293-300)
+  const bool ocean = N.state[jre] == 1;
+  if (ocean && N.layer[jre] != c.m.nlay) return ERR_UNSUPPORTED;
   ToroidalMinors T; SpheroidalMinors S;
   int rc = toroidalMinors(env, jre, jre, eps, T, st, nullptr);
   if (rc != OK) return rc;
@@ -1509,8 +1565,9 @@ static int gfkColumn(const Ctx& c, double omre, double omim, double dll1, int jr
   rc = unitJumpGreen(env, T, S, jre, eps, col, st);
   if (rc != OK) return rc;
   int njsph = istyp == 1 ? 4 : 2, njtor = istyp == 1 ? 2 : 1;
-  int ncsph = 4, nctor = 2, dsvstep = 2;
-  col.nsp = 2 * njsph + 1 * njtor;
+  const int ncsph = ocean ? 2 : 4, nctor = ocean ? 0 : 2, dsvstep = 2;
+  // U and V of the spheroidal jumps (V from (U, R) for a receiver in the ocean), W of the toroidal ones
+  col.nsp = 2 * njsph + (ocean ? 0 : njtor);
   col.gf.assign(col.nsp * nnod, C(0.0));
   double re = N.rnod[jre];
   for (int n = 1; n <= nnod; ++n) {
@@ -1544,6 +1601,13 @@ static int gfkColumn(const Ctx& c, double omre, double omim, double dll1, int jr
     int isp = 0;
     for (int i = 1; i <= ncsph; i += dsvstep)
       for (int j = 1; j <= njsph; ++j) { isp++; col.gf[(n - 1) * col.nsp + (isp - 1)] = gfs[i][j]; }
+    if (ocean) {
+      // V = -R / (r om^2 ro) of the receiver's fluid, evaluated by the reference at the
+      // node radius rnod(n) (:465, getHorcompComplexFluid)
+      cx v[5], vr[5];
+      horcompFluid(env, N.rnod[n], njsph, gfs, v, vr);
+      for (int j = 1; j <= njsph; ++j) { isp++; col.gf[(n - 1) * col.nsp + (isp - 1)] = v[j]; }
+    }
     for (int i = 1; i <= nctor; i += dsvstep)
       for (int j = 1; j <= njtor; ++j) { isp++; col.gf[(n - 1) * col.nsp + (isp - 1)] = gft[i][j]; }
   }
@@ -1907,7 +1971,9 @@ static int gfkInstances(void* h, double omre, double omim, double rearth, int lm
                         int istyp, double eps, int nthreads, double* gf, int* status, int64_t* stats) {
   Ctx* c = (Ctx*)h;
   int nnod = c->nod.nnod;
-  int nsp = istyp == 1 ? 10 : 5;
+  if (jre < 1 || jre > nnod) return ERR_NODEIDX;
+  const int njsph = istyp == 1 ? 4 : 2, njtor = istyp == 1 ? 2 : 1;
+  const int nsp = 2 * njsph + (c->nod.state[jre] == 1 ? 0 : njtor);
   Stats total;
 #ifdef _OPENMP
   if (nthreads < 1) nthreads = 1;

@@ -175,9 +175,15 @@ class Oracle:
                                           self.stats.ctypes.data_as(L))
         return rc, adj, yend
 
+    def nsp(self, jre, istyp):
+        """spectra per node: U, V of the spheroidal jumps and W of the toroidal ones; no W for a
+        receiver in the ocean (computeGreenFKSpectraForSynthetics.f90:293-313)"""
+        njsph, njtor = (4, 2) if istyp == 1 else (2, 1)
+        return 2 * njsph + (0 if self.nodes.state[jre - 1] == 1 else njtor)
+
     def gfk_column(self, omre, omim, dll1, jre, istyp, eps):
         n = self.nodes.nnod
-        nsp = 10 if istyp == 1 else 5
+        nsp = self.nsp(jre, istyp)
         gf = np.zeros((n, nsp), np.complex128)
         ujs = np.zeros((n, 4, 4), np.complex128)
         ujt = np.zeros((n, 2, 2), np.complex128)
@@ -196,7 +202,7 @@ class Oracle:
     def gfk_wavenumbers(self, omre, omim, iwnmin, nwn, dwn, jre, istyp, eps, nthreads=1):
         """Regional-mode columns for wavenumber indices iwnmin..iwnmin+nwn-1 (k = (iwn-1) dwn); gf[n, isp, i]."""
         n = self.nodes.nnod
-        nsp = 10 if istyp == 1 else 5
+        nsp = self.nsp(jre, istyp)
         gf = np.zeros((n, nsp, nwn), np.complex128)
         status = np.zeros(nwn, np.int32)
         rc = lib().orc_gfk_wavenumbers(self.h, C.c_double(omre), C.c_double(omim), C.c_double(self.nem.rearth),
@@ -215,7 +221,7 @@ class Oracle:
     def gfk_degrees(self, omre, omim, lmin, nl, jre, istyp, eps, nthreads=1):
         """Columns for l = lmin..lmin+nl-1 at the current frequency; gf[n, isp, il]."""
         n = self.nodes.nnod
-        nsp = 10 if istyp == 1 else 5
+        nsp = self.nsp(jre, istyp)
         gf = np.zeros((n, nsp, nl), np.complex128)
         status = np.zeros(nl, np.int32)
         rc = lib().orc_gfk_degrees(self.h, omre, omim, self.nem.rearth, lmin, nl, jre, istyp, eps, nthreads,

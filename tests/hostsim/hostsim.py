@@ -105,7 +105,8 @@ class HostSim:
         lmin = np.ascontiguousarray(np.broadcast_to(lmin, (nf,)), dtype=np.int32)
         nl = np.ascontiguousarray(np.broadcast_to(nl, (nf,)), dtype=np.int32)
         n, nn = int(nl.sum()), self.nodes.nnod
-        nsp = 10 if istyp == 1 else 5
+        nsp = (10 if istyp == 1 else 5) - (((2 if istyp == 1 else 1) if self.nodes.state[jre - 1] == 1 else 0)
+                                           if full else 0)
         nsrc = 1
         if self._sources is not None:
             nsp, nsrc = (20 if istyp == 1 else 10), len(self._sources) + 1

@@ -250,3 +250,28 @@ def test_errors(tmp_path):
     with pytest.raises(gfkfile.GfkFileError):
         g.read_meta("sourceNodeRadii")
     g.close()
+
+
+@pytest.mark.parametrize("dsvstep,layout,nsp", [(2, "ocean", 8), (1, "ocean_stress", 12)])
+def test_ocean_receiver_layout(tmp_path, dsvstep, layout, nsp):
+    """A receiver in the ocean: dataSpecificIntegers (istyp, ncsph = 2, nctor = 0, njsph, njtor) masks
+    S and the toroidal components (greenFKSpectra.f90:129-133): U, V (8 spectra of a moment source)
+    or, with STRESSFLAG, U, R, V (12); the synthetics read the file in the matching layout."""
+    nwn = np.array([0, 4, 5], np.int32)
+    path = str(tmp_path / "gfk_ocean.MOMENT")
+    nem = global_model(1, 200)
+    nodes = ExternalRadialNodes.from_blocks(nem, [2], [30000.0])
+    with gfkfile.GfkFileWriter(path) as w:
+        w.write_identity("id")
+        w.write_meta("m", "ELASTIC", (6371e3, 6370e3, 0.1, 0.02, 0.0, 1.0), (2, 3, 9, 5, dsvstep, 0, 1, 1), nwn)
+        w.create_spectra((1, 2, 0, 4, 2), nsp, nodes.nnod, 2, 3, nwn)
+    g = gfkfile.GreenFKSpectra(path)
+    for p in ("integers", "dataSpecificIntegers"):
+        g.read_meta(p)
+    assert g.number_of_spectra() == nsp and g.layout == layout
+    assert gfkfile.number_of_spectra((2, 3, 9, 5, dsvstep, 0, 1, 1), (1, 2, 0, 4, 2)) == nsp
+    # V after U (and R): positionDataGreenFKSpectra of V, jump 1
+    assert g.position(3, 1) == (5 if dsvstep == 2 else 9)
+    g.close()
+    # the writer's count of a receiver in the ocean without STRESSFLAG
+    assert gfkfile.writer_number_of_spectra(2, 2, 4, 2) == 8

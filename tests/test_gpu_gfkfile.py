@@ -107,3 +107,53 @@ def test_regional_file_matches_oracle(tmp_path):
         off += n
     g.close()
     eng.close()
+
+
+def test_ocean_receiver_file_matches_oracle(tmp_path):
+    """A receiver in the ocean: dataSpecificIntegers (istyp, 2, 0, njsph, njtor), 8 spectra per node
+    (U, V), the file's values the oracle's columns rounded; the synthetics read it in its layout."""
+    import sys
+    sys.path.insert(0, os.path.dirname(__file__))
+    from ocean_model import ocean_model
+    from test_oracle_ocean import ocean_receivers
+    from gemini_unified_amd.synthetics import SyntheticsEngine
+    nem = ocean_model()
+    nodes = ocean_receivers(nem)
+    jre = 8
+    tlen = 2048.0
+    grid = FrequencyGrid(df=1 / tlen, nf1=10, nf2=13, sigma=5 / tlen, nwn=np.array([10, 11, 11, 12]))
+    eng = GreenFKEngine(nem, nodes, variant="exact")
+    path = str(tmp_path / "gfk_ocean.MOMENT")
+    print("ocean-receiver GFK file, 4 frequencies", flush=True)
+    gfkfile.write_green_fk_spectra_for_synthetics(eng, path, grid, jre, 1e-6, istyp=1, freq_block=2)
+    eng.close()
+    g = gfkfile.GreenFKSpectra.open(path)
+    assert (g.ncsph, g.nctor) == (2, 0) and g.number_of_spectra() == 8 and g.layout == "ocean"
+    orc = oracle.Oracle(nem, nodes)
+    ref = {}
+    for i, f in enumerate(grid.freqs):
+        orc.set_frequency(f, 3)
+        rc, gf, _ = orc.gfk_degrees(2.0 * MC_PI * f, -grid.sigma, 0, int(grid.nwn[i]), jre, 1, 1e-6)
+        assert rc == 0
+        ref[i] = gf
+    for jr in (1, 3, 6, 10):
+        gr, gi = g.read_data_node(jr)
+        want = np.concatenate([ref[i][jr - 1] for i in range(grid.freqs.size)], axis=1)
+        np.testing.assert_array_equal(gr, want.real.astype(np.float32))
+        np.testing.assert_array_equal(gi, want.imag.astype(np.float32))
+    # synthetics of a source at node 3 (mantle) in the file's layout: those of U, V with W = 0
+    gr, gi = g.read_data_node(3)
+    g.close()
+    nwn = np.zeros(grid.nf2, np.int32)
+    nwn[grid.nf1 - 1:] = grid.nwn
+    theta, phi, mt = np.array([0.3, 1.1]), np.array([0.2, 2.0]), np.array([1.0, -0.5, -0.5, 0.3, 0.1, -0.2])
+    z = np.zeros((10, gr.shape[1]), np.float32)
+    zr, zi = z.copy(), z.copy()
+    zr[:8], zi[:8] = gr, gi
+    syn = SyntheticsEngine(0)
+    kw = dict(istyp=1, want_spectra=True)
+    to, so = syn.synthetics(grid.nf1, grid.nf2, grid.df, grid.sigma, nwn, gr, gi, theta, phi, mt, 32, tlen / 32,
+                            layout=g.layout, **kw)
+    tu, su = syn.synthetics(grid.nf1, grid.nf2, grid.df, grid.sigma, nwn, zr, zi, theta, phi, mt, 32, tlen / 32, **kw)
+    syn.close()
+    assert np.array_equal(so, su) and np.array_equal(to, tu) and np.abs(so).max() > 0

@@ -281,6 +281,55 @@ def test_ocean_nodes_match_oracle(f, lmin, nl):
                 np.testing.assert_array_equal(r["gfre"], ref.real.astype(np.float32))
 
 
+@pytest.mark.parametrize("f,lmin,nl", [(10 / TLEN, 0, 12), (40 / TLEN, 25, 10)])
+def test_receiver_in_the_ocean_matches_oracle(f, lmin, nl):
+    """Receivers in the ocean, 1 mm above the ocean bottom and 1 km above it
+    (unitJumpGreenFunctions.f90:242-298): nodes in the inner core, the outer core, the mantle and
+    the ocean; U and V of the spheroidal jumps (8 or 4 spectra per node)."""
+    from ocean_model import ocean_model
+    from test_oracle_ocean import ocean_receivers
+    nem = ocean_model()
+    nodes = ocean_receivers(nem)
+    orc = oracle.Oracle(nem, nodes)
+    orc.set_frequency(f, 3)
+    for jre in (6, 8):
+        for istyp in (1, 0):
+            orc.stats[:] = 0
+            rc, ref, status = orc.gfk_degrees(2.0 * MC_PI * f, -SIGMA, lmin, nl, jre, istyp, EPS)
+            # at l = 11 of 0.0049 Hz the start radius (778 km) lies between the inner-core node and
+            # the ICB: the inner-core adjoint passes no node, an error in the reference as well
+            # (spheroidalAdjoint.f90:129-132), and that column is zero
+            assert (status == 0).sum() >= nl - 1
+            for variant, tol in (("exact", 1e-12), ("fast", 1e-8)):
+                eng = GreenFKEngine(nem, nodes, variant=variant)
+                print(f"ocean receiver {jre} {variant} istyp={istyp} f={f:.5f} l={lmin}..{lmin + nl - 1}", flush=True)
+                r = eng.gfk([f], lmin, nl, SIGMA, EPS, jre, istyp, raise_on_error=False)
+                eng.close()
+                np.testing.assert_array_equal(r["status"], status)
+                assert r["gf"].shape == ref.shape == (nodes.nnod, 8 if istyp == 1 else 4, nl)
+                err = rel_l2(r["gf"], ref)
+                print(f"  max rel-L2 {err.max():.2e}", flush=True)
+                assert err.max() < tol
+                if variant == "exact":
+                    np.testing.assert_array_equal(r["gfre"], ref.real.astype(np.float32))
+                    np.testing.assert_array_equal(r["gfim"], ref.imag.astype(np.float32))
+                    if (status == 0).all():    # a failed column's jobs stop at different points
+                        np.testing.assert_array_equal(r["stats"], orc.stats)
+
+
+def test_receiver_in_the_fluid_core_is_rejected():
+    from ocean_model import ocean_model
+    from test_oracle_ocean import ocean_receivers
+    from gemini_unified_amd._lib import GeminiFKError
+    nem = ocean_model()
+    nodes = ocean_receivers(nem)
+    eng = GreenFKEngine(nem, nodes, variant="fast")
+    with pytest.raises(GeminiFKError) as ei:
+        eng.gfk([0.01], 0, 4, SIGMA, EPS, 2, 1)
+    assert ei.value.code == 8
+    eng.close()
+
+
 @pytest.mark.parametrize("f,lmin,nl", [(40 / TLEN, 1, 24), (90 / TLEN, 300, 16)])
 def test_adjoint_pairs_keep_the_error(f, lmin, nl):
     """GM_OPT_ADJOINT_PAIR on the GPU: the Green functions from two adjoint integrations per leg carry

@@ -95,3 +95,29 @@ def test_file_layouts_give_the_displacement_of_their_uvw_spectra(syn, istyp, lay
                                 **kwr)
     tr10, zr10 = syn.synthetics(nf1, nf2, df, sigma, nwn, gr[uvw], gi[uvw], x, phi, fomt, 16, 1.0 / (16 * df), **kwr)
     assert np.array_equal(zr20, zr10) and np.array_equal(tr20, tr10)
+
+
+@pytest.mark.parametrize("istyp", [1, 0])
+@pytest.mark.parametrize("layout", ["ocean", "ocean_stress"])
+def test_ocean_layouts_give_the_displacement_without_w(syn, istyp, layout):
+    """Files of a receiver in the ocean (ncsph = 2, nctor = 0: U, V or U, R, V; greenFKSpectra.f90:
+    129-133): the seismograms, global and regional, are those of U, V, W spectra with W = 0, bit for
+    bit (the absent W's sums are zero, displacementSpectra.f90:84-87)."""
+    nf1, nf2, nwn, gr, gi, theta, phi, fomt = _case(istyp, 9)
+    nj, nt = (4, 2) if istyp == 1 else (2, 1)
+    ocean = layout == "ocean"
+    sel = np.arange(2 * nj if ocean else 3 * nj)                   # U, V or U, R, V
+    vrows = (1 if ocean else 2) * nj + np.arange(nj)
+    g10r = np.zeros((2 * nj + nt, gr.shape[1]), np.float32)        # U, V, W = 0
+    g10i = np.zeros_like(g10r)
+    g10r[:nj], g10i[:nj] = gr[:nj], gi[:nj]
+    g10r[nj:2 * nj], g10i[nj:2 * nj] = gr[vrows], gi[vrows]
+    df, sigma = 1 / 2048.0, 5 / 2048.0
+    kw = dict(istyp=istyp, want_spectra=True)
+    for kx in (dict(), dict(dwn=2 * np.pi / 8.0e6, rearth=6371000.0)):
+        th = theta if not kx else theta * 1.0e5
+        to, zo = syn.synthetics(nf1, nf2, df, sigma, nwn, gr[sel], gi[sel], th, phi, fomt, 16, 1.0 / (16 * df),
+                                layout=layout, **kw, **kx)
+        tu, zu = syn.synthetics(nf1, nf2, df, sigma, nwn, g10r, g10i, th, phi, fomt, 16, 1.0 / (16 * df), **kw, **kx)
+        assert np.array_equal(zo, zu) and np.array_equal(to, tu)
+        assert np.abs(zo).max() > 0

@@ -260,6 +260,29 @@ def test_kernel_source_ocean_nodes_match_oracle():
             np.testing.assert_array_equal(r["stats"], orc.stats)
 
 
+def test_kernel_source_receiver_in_the_ocean_matches_oracle():
+    """Receivers in the ocean (unitJumpGreenFunctions.f90:242-298): the mantle propagator from the
+    ocean bottom, nodes in the inner core, the outer core, the mantle and the ocean; bit for bit with
+    the oracle, moment and force sources, with the same work counters."""
+    from ocean_model import ocean_model
+    from test_oracle_ocean import ocean_receivers
+    nem = ocean_model()
+    nodes = ocean_receivers(nem)
+    for jre in (6, 8):
+        for istyp in (1, 0):
+            for f, lmin, nl in [(10 / 2048.0, 0, 6), (40 / 2048.0, 25, 5)]:
+                r = HostSim(nem, nodes, "exact").gfk([f], lmin, nl, 5 / 2048.0, 1e-6, jre, istyp)
+                orc = oracle.Oracle(nem, nodes)
+                orc.set_frequency(f, 3)
+                orc.stats[:] = 0
+                rc, ref, st = orc.gfk_degrees(2 * MC_PI * f, -5 / 2048.0, lmin, nl, jre, istyp, 1e-6)
+                assert rc == 0 and np.all(r["status"] == 0)
+                assert r["gf"].shape == ref.shape == (nodes.nnod, 8 if istyp == 1 else 4, nl)
+                np.testing.assert_array_equal(r["gf"], ref)
+                np.testing.assert_array_equal(r["stats"], orc.stats)
+                assert np.abs(ref[0]).max() > 0 if lmin == 0 else True     # the inner-core node
+
+
 @pytest.mark.parametrize("case", ["mantle", "core"])
 def test_adjoint_pairs_keep_the_error(case):
     """Two adjoint integrations per leg (GM_OPT_ADJOINT_PAIR) in the kernel source: the four unit-jump
