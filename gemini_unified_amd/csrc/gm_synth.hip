@@ -51,7 +51,7 @@ constexpr int kRows = 8;       // basis rows
 constexpr int kSums = 14;      // moment-tensor degree sums (the larger set)
 constexpr int kForceSums = 7;  // single-force degree sums
 constexpr int kPieces = 3;     // bf16 pieces per value
-constexpr int TM = 128, TN = 64, TK = 32, LDSK = 40;   // tile sizes; LDS row pitch (bf16)
+constexpr int TM = 128, TN = 64, TK = 32, LDSK = 48;   // tile sizes; LDS row pitch (bf16)
 constexpr int kXcd = 8;        // XCDs of the MI355X the contraction's tile order is laid out for
 constexpr int kStrip = 8;      // row tiles per strip of that order
 constexpr int kMaxFFT = 8192;  // longest time series (ns/2 complex FP64 = 64 KB of LDS, 68 KB padded)
@@ -358,14 +358,22 @@ __global__ void k_gprep(int nsp, int nf, int Np, int Lp, const int* __restrict__
 // H[j][Rp][Np] (FP64) = c_fac[t][j] * sum over degrees of basis row c_row[t][j]
 // times spectrum c_spec[t][j] (the j-th sum of source type t = istyp); one
 // block per (degree split, sum, tile) of Np/TN x Rp/TM tiles, 256 threads: 4
-// waves of 64 x 32.  With nsplit > 1 split s covers chunks [s kper, (s+1) kper)
+// waves, wave w the rows [32 w, 32 w + 32) of the tile and all TN = 64 columns
+// (2 x 4 MFMA tiles of 16 x 16).  The basis fragments go from global memory
+// straight to registers, two chunks of TK degrees ahead: in the basis layout
+// [piece][row][Lp/32][Rp][32] the 16 rows x 32 degrees of a fragment are one
+// contiguous 1 KB block, so a wave loads it fully coalesced and no other wave
+// needs it.  Only the spectrum tile (64 columns x 32 degrees x 3 pieces) is
+// staged in LDS, one chunk ahead, shared by the four waves.  Each output tile takes the six
+// piece products in the same order, and folds its FP32 partial into the FP64
+// sum after every chunk.  With nsplit > 1 split s covers chunks [s kper, (s+1) kper)
 // of TK degrees and writes its partial sums to plane s of H ([nsplit][nsum][Rp][Np]),
 // which k_sumsplits adds in split order.
 __global__ void __launch_bounds__(256, 2) k_contract(int Rp, int Np, int Lp, int nsp, const __bf16* __restrict__ Bs,
                                                   const __bf16* __restrict__ Gs, double* __restrict__ H, int regional,
                                                   int istyp, int nsplit, int kper) {
-  __shared__ __attribute__((aligned(16))) __bf16 sA[kPieces][TM][LDSK];
-  __shared__ __attribute__((aligned(16))) __bf16 sB[kPieces][TN][LDSK];
+  // the spectrum tile, double-buffered by chunk parity: one barrier per chunk
+  __shared__ __attribute__((aligned(16))) __bf16 sB2[2][kPieces][TN][LDSK];
   // XCD-aware tile order.  The hardware deals blocks to the kXcd XCDs
   // round-robin by block index, so block b runs on XCD b % kXcd as the
   // (b / kXcd)-th of its XCD.  Each XCD is given one contiguous range of the
@@ -387,101 +395,112 @@ __global__ void __launch_bounds__(256, 2) k_contract(int Rp, int Np, int Lp, int
   const double fac = regional ? c_fac_wn[istyp][j] : c_fac[istyp][j];
   const int r0 = ty * TM, n0 = tx * TN;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int wr = (w >> 1) * 64, wc = (w & 1) * 32;
   const size_t planeA = (size_t)Rp * Lp, planeG = (size_t)Np * Lp;
-  // global -> register staging: A: 3 pieces x 128 rows x 4 groups of 8 degrees; B: 3 x 64 x 4.
-  // Written out element by element with literal indices: as loops (even with
-  // unroll pragmas) the staging registers were indexed at run time and the
-  // compiler kept them in scratch.
-  uint4 ra0, ra1, ra2, ra3, ra4, ra5, rb0, rb1, rb2;
-#define GM_A_SRC(i, k0)                                                                                        \
-  (Bs + ((size_t)((t + 256 * (i)) >> 9) * kRows + rowb) * planeA +                                           \
-   ((size_t)((k0) >> 5) * Rp + r0 + (((t + 256 * (i)) & 511) >> 2)) * 32 + 8 * ((t + 256 * (i)) & 3))
+  const int fr = lane & 15, fk = (lane >> 4) * 8;
+  // this lane's basis fragment of chunk k0, row fragment a, piece pc:
+  // abase + pc apiece + (k0 / 32) Rp 32 + 16 a 32
+  const __bf16* __restrict__ abase = Bs + (size_t)rowb * planeA + (size_t)(r0 + 32 * w + fr) * 32 + fk;
+  const size_t apiece = (size_t)kRows * planeA;
+  // global -> register staging of the spectrum tile (3 pieces x 64 columns x 4
+  // groups of 8 degrees) and the basis fragments, written out with literal
+  // indices: as loops the registers were indexed at run time and kept in scratch
+  uint4 rb0, rb1, rb2;
 #define GM_B_SRC(i, k0)                                                                                        \
   (Gs + ((size_t)((t + 256 * (i)) >> 8) * nsp + spec) * planeG +                                             \
    (size_t)(n0 + (((t + 256 * (i)) & 255) >> 2)) * Lp + (k0) + 8 * ((t + 256 * (i)) & 3))
-#define GM_A_DST(i) (&sA[(t + 256 * (i)) >> 9][((t + 256 * (i)) & 511) >> 2][8 * ((t + 256 * (i)) & 3)])
 #define GM_B_DST(i) (&sB[(t + 256 * (i)) >> 8][((t + 256 * (i)) & 255) >> 2][8 * ((t + 256 * (i)) & 3)])
-#define GM_GLOAD(k0)                                                                                           \
+#define GM_SB(k0) (sB2[(((k0) - kbeg) / TK) & 1])
+#define GM_BLOAD(k0)                                                                                           \
   do {                                                                                                         \
-    ra0 = *(const uint4*)GM_A_SRC(0, k0); ra1 = *(const uint4*)GM_A_SRC(1, k0);                                \
-    ra2 = *(const uint4*)GM_A_SRC(2, k0); ra3 = *(const uint4*)GM_A_SRC(3, k0);                                \
-    ra4 = *(const uint4*)GM_A_SRC(4, k0); ra5 = *(const uint4*)GM_A_SRC(5, k0);                                \
     rb0 = *(const uint4*)GM_B_SRC(0, k0); rb1 = *(const uint4*)GM_B_SRC(1, k0);                                \
     rb2 = *(const uint4*)GM_B_SRC(2, k0);                                                                      \
   } while (0)
 #define GM_LSTORE()                                                                                            \
+  do { *(uint4*)GM_B_DST(0) = rb0; *(uint4*)GM_B_DST(1) = rb1; *(uint4*)GM_B_DST(2) = rb2; } while (0)
+#define GM_A1(Q, a, pc, p) Q##a##pc = *(const bf16x8*)((p) + (pc) * apiece + (a) * 512)
+#define GM_ALOAD(Q, k0)                                                                                        \
   do {                                                                                                         \
-    *(uint4*)GM_A_DST(0) = ra0; *(uint4*)GM_A_DST(1) = ra1; *(uint4*)GM_A_DST(2) = ra2;                        \
-    *(uint4*)GM_A_DST(3) = ra3; *(uint4*)GM_A_DST(4) = ra4; *(uint4*)GM_A_DST(5) = ra5;                        \
-    *(uint4*)GM_B_DST(0) = rb0; *(uint4*)GM_B_DST(1) = rb1; *(uint4*)GM_B_DST(2) = rb2;                        \
+    const __bf16* p_ = abase + (size_t)((k0) >> 5) * Rp * 32;                                                  \
+    GM_A1(Q, 0, 0, p_); GM_A1(Q, 0, 1, p_); GM_A1(Q, 0, 2, p_);                                                \
+    GM_A1(Q, 1, 0, p_); GM_A1(Q, 1, 1, p_); GM_A1(Q, 1, 2, p_);                                                \
   } while (0)
-  floatx4 acc[4][2];
-  double sum[4][2][4];
+  // the six piece products of the output tiles (a, b), pieces 0 = high .. 2 = low,
+  // stage by stage over the eight tiles: eight independent accumulation chains
+  // in flight, each tile's products in the same order
+#define GM_MF(Q, a, bb, pa, pb)                                                                                \
+  acc[a][bb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Q##a##pa, bq##bb##pb, acc[a][bb], 0, 0, 0)
+#define GM_STAGE(Q, pa, pb)                                                                                    \
+  do {                                                                                                         \
+    GM_MF(Q, 0, 0, pa, pb); GM_MF(Q, 1, 0, pa, pb); GM_MF(Q, 0, 1, pa, pb); GM_MF(Q, 1, 1, pa, pb);             \
+    GM_MF(Q, 0, 2, pa, pb); GM_MF(Q, 1, 2, pa, pb); GM_MF(Q, 0, 3, pa, pb); GM_MF(Q, 1, 3, pa, pb);             \
+  } while (0)
+#define GM_BQ(bb)                                                                                              \
+  const bf16x8 bq##bb##0 = *(const bf16x8*)&sB[0][16 * (bb) + fr][fk];                                        \
+  const bf16x8 bq##bb##1 = *(const bf16x8*)&sB[1][16 * (bb) + fr][fk];                                        \
+  const bf16x8 bq##bb##2 = *(const bf16x8*)&sB[2][16 * (bb) + fr][fk]
+  // one chunk: spectrum tile to LDS, next chunk's loads issued, the 48 MFMAs
+  // on the current fragments Q, the fold into FP64
+#define GM_STEP(Q, N, k0)                                                                                      \
+  do {                                                                                                         \
+    __bf16(*__restrict__ sB)[TN][LDSK] = GM_SB(k0);                                                            \
+    GM_LSTORE();                                                                                               \
+    __syncthreads();                                                                                           \
+    if ((k0) + TK < kend) GM_BLOAD((k0) + TK);                                                                \
+    if ((k0) + 2 * TK < kend) GM_ALOAD(N, (k0) + 2 * TK);                                                       \
+    {                                                                                                          \
+      GM_BQ(0); GM_BQ(1); GM_BQ(2); GM_BQ(3);                                                                  \
+      GM_STAGE(Q, 2, 0); GM_STAGE(Q, 1, 1); GM_STAGE(Q, 0, 2);                                                 \
+      GM_STAGE(Q, 1, 0); GM_STAGE(Q, 0, 1); GM_STAGE(Q, 0, 0);                                                 \
+    }                                                                                                          \
+    _Pragma("unroll") for (int a_ = 0; a_ < 2; ++a_)                                                           \
+    _Pragma("unroll") for (int b_ = 0; b_ < 4; ++b_) {                                                         \
+      _Pragma("unroll") for (int e_ = 0; e_ < 4; ++e_) sum[a_][b_][e_] += (double)acc[a_][b_][e_];             \
+      acc[a_][b_] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};                                                           \
+    }                                                                                                          \
+  } while (0)
+  floatx4 acc[2][4];
+  double sum[2][4][4];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      acc[a][b] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int bb = 0; bb < 4; ++bb) {
+      acc[a][bb] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-      for (int e = 0; e < 4; ++e) sum[a][b][e] = 0.0;
+      for (int e = 0; e < 4; ++e) sum[a][bb][e] = 0.0;
     }
-  const int fr = lane & 15, fk = (lane >> 4) * 8;
-  GM_GLOAD(kbeg);
-  for (int k0 = kbeg; k0 < kend; k0 += TK) {
-    GM_LSTORE();
-    __syncthreads();
-    if (k0 + TK < kend) GM_GLOAD(k0 + TK);
-    bf16x8 bq[2][kPieces];
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-      for (int pc = 0; pc < kPieces; ++pc) bq[b][pc] = *(const bf16x8*)&sB[pc][wc + b * 16 + fr][fk];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      bf16x8 aq[kPieces];
-#pragma unroll
-      for (int pc = 0; pc < kPieces; ++pc) aq[pc] = *(const bf16x8*)&sA[pc][wr + a * 16 + fr][fk];
-#pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        floatx4 c = acc[a][b];
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aq[2], bq[b][0], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aq[1], bq[b][1], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aq[0], bq[b][2], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aq[1], bq[b][0], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aq[0], bq[b][1], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aq[0], bq[b][0], c, 0, 0, 0);
-        acc[a][b] = c;
-      }
-    }
-    // compensation: the FP32 partial of this chunk goes into the FP64 sum
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < 2; ++b) {
-#pragma unroll
-        for (int e = 0; e < 4; ++e) sum[a][b][e] += (double)acc[a][b][e];
-        acc[a][b] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-      }
-    __syncthreads();
+  bf16x8 qa00, qa01, qa02, qa10, qa11, qa12;
+  bf16x8 qb00, qb01, qb02, qb10, qb11, qb12;
+  bf16x8 qc00, qc01, qc02, qc10, qc11, qc12;
+  GM_BLOAD(kbeg);
+  GM_ALOAD(qa, kbeg);
+  if (kbeg + TK < kend) GM_ALOAD(qb, kbeg + TK);
+  for (int k0 = kbeg; k0 < kend; k0 += 3 * TK) {
+    GM_STEP(qa, qc, k0);
+    if (k0 + TK < kend) GM_STEP(qb, qa, k0 + TK);
+    if (k0 + 2 * TK < kend) GM_STEP(qc, qb, k0 + 2 * TK);
   }
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int bb = 0; bb < 4; ++bb)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int row = r0 + wr + a * 16 + (lane >> 4) * 4 + e;
-        const int col = n0 + wc + b * 16 + (lane & 15);
-        H[(((size_t)split * nsum + j) * Rp + row) * Np + col] = sum[a][b][e] * fac;
+        const int row = r0 + 32 * w + a * 16 + (lane >> 4) * 4 + e;
+        const int col = n0 + bb * 16 + (lane & 15);
+        H[(((size_t)split * nsum + j) * Rp + row) * Np + col] = sum[a][bb][e] * fac;
       }
 }
-#undef GM_GLOAD
+#undef GM_BLOAD
 #undef GM_LSTORE
-#undef GM_A_SRC
 #undef GM_B_SRC
-#undef GM_A_DST
 #undef GM_B_DST
+#undef GM_A1
+#undef GM_ALOAD
+#undef GM_MF
+#undef GM_STAGE
+#undef GM_BQ
+#undef GM_SB
+#undef GM_STEP
 
 // H[i] = sum over s < nsplit of Hp[s][i], in split order, i < n (nsplit > 1)
 __global__ void k_sumsplits(size_t n, int nsplit, const double* __restrict__ Hp, double* __restrict__ H) {
