@@ -365,7 +365,9 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   if (aski)
     for (int js = 0; js < nsnod; ++js) {
       const int jsh = jsnod[js];
-      if (m.state[jsh] != 0) return gm_fail(c, GM_ERR_UNSUPPORTED, "source node %d lies in a fluid layer", jsh);
+      // a source in the ocean is taken (computeGreenFKSpectraForASKI.f90:359-376), one in the fluid core not
+      if (m.state[jsh] != 0 && m.layer[jsh] != m.nlay)
+        return gm_fail(c, GM_ERR_UNSUPPORTED, "source node %d lies in the fluid core", jsh);
       if (m.hascore && !(m.rnod[jsh] > m.roc))
         return gm_fail(c, GM_ERR_UNSUPPORTED, "source node %d below the core-mantle boundary is not supported", jsh);
     }
@@ -597,6 +599,9 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
       for (int js = 0; js < nsrc; ++js) {
         GmBatch b = B;
         b.jre = js < nsnod ? jsnod[js] : jre;
+        // one adjoint job for a jump in the ocean, else one per jump (two with GM_OPT_ADJOINT_PAIR)
+        b.adj_jumps = m.state[b.jre] == 1 ? 1 : (c->adj_pair ? 2 : 4);
+        b.adj_jobs = b.adj_jumps + (ic_nodes ? 1 : 0);
         b.unit_counter = B.unit_counter + (size_t)GM_NCOUNTERS * js;
         b.aski_js = js;
         b.aski_styp = js < nsnod ? istyp : 0;

@@ -210,7 +210,9 @@ int gm_gfk_batch_wn(gm_ctx* ctx, int nfreq, const double* freqs, const int* iwnm
  * nsnod+1)), nspmax = 20 for istyp = 1 and 10 for istyp = 0; per node
  * isp = U(j) V(j) dU/dr(j) dV/dr(j) W(j) dW/dr(j) with njsph = 4 / 2 and
  * njtor = 2 / 1 jumps (moment / force), zero beyond a source's
- * 4 njsph + 2 njtor.  Source nodes must lie in the solid mantle. */
+ * 4 njsph + 2 njtor.  Source nodes lie in the solid mantle or in the ocean (a
+ * source there has two (U, R) jumps and no toroidal motion, :359-376: 8 spectra);
+ * a source in the fluid core or below the CMB returns GM_ERR_UNSUPPORTED. */
 int gm_aski_batch(gm_ctx* ctx, int nfreq, const double* freqs, const int* lmin, const int* nl, double sigma,
                   double eps, int attmode, int nsnod, const int* jsnod, int jre, int istyp, int flags, double* gf,
                   float* gfre, float* gfim, int* status, long long* stats);

@@ -1719,8 +1719,11 @@ static int askiColumn(const Ctx& c, double omre, double omim, double dll1, int n
   for (int js = 1; js <= nsnod + 1; ++js) {
     const int styp = js == nsnod + 1 ? 0 : istyp;
     const int jsh = js == nsnod + 1 ? jre : jsnod[js - 1];
-    if (N.state[jsh] == 1) return ERR_UNSUPPORTED;   // a source in the ocean
-    const int njsph = styp == 1 ? 4 : 2, njtor = styp == 1 ? 2 : 1;
+    // a source in the ocean (:359-376): two (U, R) jumps for a force or a moment, no toroidal
+    // motion; unitJumpGreen refuses a source in the fluid core
+    const bool osrc = N.state[jsh] == 1;
+    const int njsph = osrc ? 2 : (styp == 1 ? 4 : 2), njtor = osrc ? 0 : (styp == 1 ? 2 : 1);
+    const int nk = osrc ? 2 : 4;
     Column col;
     rc = unitJumpGreen(env, T, S, jsh, eps, col, st);
     if (rc != OK) return rc;
@@ -1736,7 +1739,7 @@ static int askiColumn(const Ctx& c, double omre, double omim, double dll1, int n
         for (int i = 1; i <= 2; ++i)
           for (int j = 1; j <= njsph; ++j) {
             cx acc = C(0.0);
-            for (int k = 1; k <= 4; ++k) acc = add(acc, mul(U(i, k), zs[k][j]));
+            for (int k = 1; k <= nk; ++k) acc = add(acc, mul(U(i, k), zs[k][j]));
             gfs[i][j] = acc;
           }
         dsvDerivatives(env, 2, N.rnod[n], njsph, gfs, gfd);
@@ -1747,7 +1750,7 @@ static int askiColumn(const Ctx& c, double omre, double omim, double dll1, int n
         for (int i = 1; i <= 4; ++i)
           for (int j = 1; j <= njsph; ++j) {
             cx acc = C(0.0);
-            for (int k = 1; k <= 4; ++k) acc = add(acc, mul(U(i, k), zs[k][j]));
+            for (int k = 1; k <= nk; ++k) acc = add(acc, mul(U(i, k), zs[k][j]));
             gfs[i][j] = acc;
           }
         for (int i = 1; i <= 2; ++i)

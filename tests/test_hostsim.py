@@ -342,3 +342,27 @@ def test_kernel_source_multi_source_matches_oracle(istyp):
                 np.testing.assert_array_equal(r["stats"], orc.stats)
             else:
                 assert _rel_l2(g, ref).max() < 1e-12
+
+
+@pytest.mark.parametrize("istyp", [1, 0])
+def test_kernel_source_ocean_sources_match_oracle(istyp):
+    """Multi-source batch with sources in the ocean (computeGreenFKSpectraForASKI.f90:359-376) and
+    a receiver 500 m below the ocean bottom, nodes in the inner core, the outer core, the mantle and
+    the ocean: bit for bit with the oracle, work counters included."""
+    from ocean_model import ocean_model
+    nem = ocean_model()
+    ruc = nem.rb[-2]
+    rn = np.array([600e3, 2000e3, 4000e3, ruc - 500.0, ruc + 1000.0, nem.rearth - 500.0])
+    la = np.array([nem.layer_index(r)[0] for r in rn], np.int32)
+    nodes = ExternalRadialNodes(rearth=nem.rearth, rnod=rn, state=nem.iflso[la - 1].astype(np.int32), layer=la)
+    jre, jsnod = 4, [3, 5, 6, 4]
+    for f, lmin, nl in [(10 / 2048.0, 0, 8), (40 / 2048.0, 30, 5)]:
+        r = HostSim(nem, nodes, "exact").aski([f], lmin, nl, 5 / 2048.0, 1e-6, jsnod, jre, istyp)
+        orc = oracle.Oracle(nem, nodes)
+        orc.set_frequency(f, 3)
+        orc.stats[:] = 0
+        rc, ref, st = orc.aski_degrees(2 * MC_PI * f, -5 / 2048.0, lmin, nl, jsnod, jre, istyp, 1e-6)
+        assert rc == 0 and np.all(r["status"] == 0)
+        assert np.array_equal(r["gf"], ref)
+        np.testing.assert_array_equal(r["stats"], orc.stats)
+        assert np.abs(ref[1, :, :8]).max() > 0 and not ref[1, :, 8:].any()    # an ocean source: 8 spectra

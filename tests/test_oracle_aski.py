@@ -146,3 +146,52 @@ def test_radial_derivatives_are_central_differences(f, l):
                 # at 0.025 Hz; a wrong sign, entry or term is an O(1) error
                 tol = 1e-5 * max(abs(dy), abs(y[1]) * 2 * np.pi * f / 3000.0) + 1e-300
                 assert abs(fd - dy) < tol, (js, fluid, iv, fd, dy)
+
+
+@pytest.mark.parametrize("f,l", [(20 / 2048.0, 0), (20 / 2048.0, 7), (50 / 2048.0, 33)])
+def test_ocean_sources_equal_the_reciprocal_ones(f, l):
+    """Sources in the ocean (computeGreenFKSpectraForASKI.f90:359-376, 533-542: two (U, R) jumps,
+    no toroidal motion): the direct Green functions of a moment or force source at an ocean node,
+    seen at every other source node, equal the synthetics driver's reciprocal ones for a receiver
+    at that node and a source at the ocean node (its fluid-node source terms).  At a receiver in
+    the ocean the synthetics driver takes V = -R / (r om^2 rho) at the source node's radius and
+    density; the direct V uses the receiver's, so the comparison scales by r rho."""
+    from ocean_model import ocean_model
+    nem = ocean_model()
+    ruc = nem.rb[-2]
+    rn = np.array([2000e3, 4000e3, ruc - 500.0, ruc + 1000.0, nem.rearth - 500.0])
+    la = np.array([nem.layer_index(r)[0] for r in rn], np.int32)
+    nodes = ExternalRadialNodes(rearth=nem.rearth, rnod=rn, state=nem.iflso[la - 1].astype(np.int32), layer=la)
+    jre = 3
+    jsnod = [2, 4, 5, 3]
+    orc = oracle.Oracle(nem, nodes)
+    orc.set_frequency(f, 3)
+    om, sig, eps = 2 * MC_PI * f, -5 / 2048.0, 1e-11
+    rc, gfa, st = orc.aski_degrees(om, sig, l, 1, jsnod, jre, 1, eps)
+    assert rc == 0
+    rho = [orc.eval_elcon(r, k)[0] for r, k in zip(nodes.rnod, nodes.layer)]
+    checked = 0
+    for jr in (2, 3, 4, 5):
+        ocean_r = nodes.state[jr - 1] == 1
+        for istyp in (1, 0):
+            rc, gfs, _ = orc.gfk_degrees(om, sig, l, 1, jr, istyp, eps)
+            assert rc == 0
+            njs = 4 if istyp == 1 else 2
+            srcs = [(js, jn) for js, jn in enumerate(jsnod)] if istyp == 1 else [(len(jsnod), jre)]
+            for js, jsh in srcs:
+                if jsh == jr:
+                    continue
+                nj = 2 if nodes.state[jsh - 1] == 1 else njs
+                a = gfa[js, jr - 1, :2 * nj, 0]
+                bu, bv = gfs[jsh - 1, :nj, 0], gfs[jsh - 1, njs:njs + nj, 0]
+                if ocean_r:
+                    bv = bv * (nodes.rnod[jsh - 1] * rho[jsh - 1]) / (nodes.rnod[jr - 1] * rho[jr - 1])
+                b = np.concatenate([bu, bv])
+                keep = np.arange(2 * nj) if l > 0 else np.arange(nj)
+                scale = np.abs(b[keep]).max()
+                assert scale > 0
+                assert np.abs(a[keep] - b[keep]).max() < 1e-7 * scale, (jr, js, a, b)
+                if nodes.state[jsh - 1] == 1:
+                    assert not gfa[js, :, 2 * 2 * nj:, 0].any()     # no toroidal motion, 8 spectra
+                checked += 1
+    assert checked == 4 * 3 + 3
