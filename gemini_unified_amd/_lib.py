@@ -21,7 +21,8 @@ EXPORTS = ["gm_ctx_create", "gm_ctx_destroy", "gm_set_option", "gm_last_error", 
            "gm_mlp_nparams", "gm_mlp_forward", "gm_mlp_grad", "gm_mlp_train", "gm_set_proposer",
            "gm_collect_steplog", "gm_minors_batch_wn", "gm_gfk_batch_wn", "gm_legendre_batch", "gm_moment_sums", "gm_synthetics", "gm_last_synth_timing",
            "gm_bessel_sums", "gm_synthetics_regional", "gm_force_sums", "gm_force_bessel_sums",
-           "gm_event_filter", "gm_travel_times", "gm_gfk_batch_noalloc", "gm_node_properties", "gm_aski_batch", "gm_strain_spectra"]
+           "gm_event_filter", "gm_travel_times", "gm_gfk_batch_noalloc", "gm_node_properties", "gm_aski_batch", "gm_strain_spectra",
+           "gm_strain_spectra_regional"]
 
 
 class GeminiFKError(RuntimeError):
@@ -76,6 +77,9 @@ def lib():
         L.gm_force_sums.argtypes = L.gm_moment_sums.argtypes
         L.gm_strain_spectra.argtypes = [VP, C.c_int, C.c_int, I, C.c_int, C.c_longlong, VP, VP, C.c_int, D, D,
                                         C.c_double, C.c_int, D, D, C.c_int, VP, VP, VP, VP]
+        L.gm_strain_spectra_regional.argtypes = [VP, C.c_int, C.c_int, I, C.c_int, C.c_longlong, VP, VP, C.c_int, D,
+                                                 D, C.c_double, C.c_double, C.c_double, C.c_int, D, D, C.c_int, VP,
+                                                 VP, VP, VP]
         L.gm_force_bessel_sums.argtypes = L.gm_bessel_sums.argtypes
         L.gm_synthetics_regional.argtypes = [VP, C.c_int, C.c_int, C.c_double, C.c_double, I, C.c_int, C.c_longlong,
                                              VP, VP, C.c_double, C.c_double, C.c_int, D, D, D, D, D, D, C.c_int, D,

@@ -532,6 +532,22 @@ __device__ __forceinline__ void wnRows(double k, double x, double re, double b0,
   w[7] = k * 4.0 * k * re * b2 / (k * x);                 // 10, -14
 }
 
+// R d/dx of the eight rows (the kernels of the x-derivative integrals, moment
+// kint 29..42 and force 15..21, momentWavenumberIntegrals :353-373): row 0 -> 3,
+// 1 -> 4, 2 -> 5 of wnRows, and five rows of their own; the same (row,
+// spectrum, factor) tables then give those integrals
+__device__ __forceinline__ void wnRowsDeriv(double k, double x, double re, double b0, double b1, double b2, double* w) {
+  const double kx = k * x, kr = k * re;
+  w[0] = k * (-k * re) * b1;                                          // kint 29, 30
+  w[1] = k * 2.0 * (b0 - b1 / kx);                                    // 31
+  w[2] = k * 2.0 * k * re * (b1 - 2.0 * b2 / kx);                     // 32
+  w[3] = -k * kr * kr * (b0 - b1 / kx);                               // 33, 34
+  w[4] = -k * 2.0 * kr * (b1 - b2 / kx);                              // 35, 39
+  w[5] = k * 2.0 * kr * kr * ((6.0 / (kx * kx) - 1.0) * b2 - b1 / kx);  // 36, -40
+  w[6] = -k * 2.0 * kr * b2 / kx;                                     // 37, 41
+  w[7] = k * 4.0 * kr * kr * (-3.0 * b2 / (kx * kx) + b1 / kx);        // 38, -42
+}
+
 // quadrature geometry of one receiver (momentWavenumberIntegrals, :333-346, 427)
 struct WnGeom { double x, xdk, cF, sF; int j3; };
 __device__ __forceinline__ WnGeom wnGeom(double xx, double dwn) {
@@ -551,8 +567,9 @@ __device__ __forceinline__ WnGeom wnGeom(double xx, double dwn) {
 // wavenumber lies beyond j3; k_wnfix corrects each frequency's last point.
 // One thread per (receiver, wavenumber); layout [piece][row][Lp/32][Rp][32] as
 // for the Legendre basis, with the thread index running in that order.
+// deriv = 1: the rows of the x-derivative integrals (wnRowsDeriv).
 __global__ void k_bessel_basis(int R, int Rp, int Lp, const double* __restrict__ xr, double dwn, double re,
-                               __bf16* __restrict__ Bs) {
+                               __bf16* __restrict__ Bs, int deriv) {
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (size_t)Rp * Lp) return;
   const unsigned rest = (unsigned)(idx >> 5);
@@ -570,7 +587,8 @@ __global__ void k_bessel_basis(int R, int Rp, int Lp, const double* __restrict__
     const double k = l * dwn;
     const double b0 = j0(k * xx), b1 = j1(k * xx);
     const double b2 = -b0 + 2.0 / (k * xx) * b1;
-    wnRows(k, g.x, re, b0, b1, b2, w);
+    if (deriv) wnRowsDeriv(k, g.x, re, b0, b1, b2, w);
+    else wnRows(k, g.x, re, b0, b1, b2, w);
 #pragma unroll
     for (int i = 0; i < kRows; ++i) w[i] *= q;
   }
@@ -591,22 +609,27 @@ __global__ void k_bessel_basis(int R, int Rp, int Lp, const double* __restrict__
 //    terms (G(n) Y-kernel(k_n) - G(j3) Y-kernel(k_j3)) sF;
 //  - n == j3: trapezoid only, the point carries dwn/2 instead of dwn/2 + cF;
 //  - n < j3: trapezoid only, the last point carries dwn/2 instead of dwn.
-// ISTYP: 0 single force (7 sums), 1 moment tensor (14 sums).
+// ISTYP: 0 single force (7 sums), 1 moment tensor (14 sums); deriv = 1: the
+// kernels of the x-derivative integrals (wnRowsDeriv).
 template <int ISTYP>
 __global__ void k_wnfix(int R, int nf, int Rp, int Np, const int* __restrict__ nwn, const long long* __restrict__ koff,
                         long long nkf, const float* __restrict__ gr, const float* __restrict__ gi,
                         const double* __restrict__ xr, double dwn, double re, double tapfrac, SpecMap smap,
-                        double* __restrict__ H) {
+                        double* __restrict__ H, int deriv) {
   const int id = blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= R * nf) return;
   const int r = id / nf, f = id % nf;
   const double xx = xr[r];
   const WnGeom g = wnGeom(xx, dwn);
   const int n = nwn[f];
+  auto rows = [&](double k, double b0, double b1, double b2, double* w) {
+    if (deriv) wnRowsDeriv(k, g.x, re, b0, b1, b2, w);
+    else wnRows(k, g.x, re, b0, b1, b2, w);
+  };
   auto rowsAt = [&](int j, double* w) {      // Bessel kernels at wavenumber index j
     const double k = (j - 1) * dwn;
     const double b0 = j0(k * xx), b1 = j1(k * xx);
-    wnRows(k, g.x, re, b0, b1, -b0 + 2.0 / (k * xx) * b1, w);
+    rows(k, b0, b1, -b0 + 2.0 / (k * xx) * b1, w);
   };
   auto spec = [&](int s, int j) -> cd {      // tapered spectrum s at wavenumber index j (absent: zero)
     if (smap.s[s] < 0) return cd{0.0, 0.0};
@@ -620,9 +643,9 @@ __global__ void k_wnfix(int R, int nf, int Rp, int Np, const int* __restrict__ n
   if (filon) {
     const double wn3 = (g.j3 - 1) * dwn, wne = (n - 1) * dwn;
     double n0 = y0(wn3 * g.x), n1 = y1(wn3 * g.x);
-    wnRows(wn3, g.x, re, n0, n1, -n0 + 2.0 / (wn3 * g.x) * n1, y3);
+    rows(wn3, n0, n1, -n0 + 2.0 / (wn3 * g.x) * n1, y3);
     n0 = y0(wne * g.x); n1 = y1(wne * g.x);
-    wnRows(wne, g.x, re, n0, n1, -n0 + 2.0 / (wne * g.x) * n1, yn);
+    rows(wne, n0, n1, -n0 + 2.0 / (wne * g.x) * n1, yn);
   } else if (n == g.j3) {
     rowsAt(g.j3, w3);
   }
@@ -1127,7 +1150,7 @@ int run_sums(gm_ctx* c, const SynPlan& P, const double* theta, const float* gr, 
   if (regional) {
     const size_t nb = (size_t)P.Rp * P.Lp;
     hipLaunchKernelGGL(k_bessel_basis, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, P.R, P.Rp, P.Lp, dth, dwn,
-                       re, (__bf16*)(base + o.Bs));
+                       re, (__bf16*)(base + o.Bs), 0);
   } else {
     int nseg, seglen;
     legendreSegments(P.Lp, nseg, seglen);
@@ -1159,7 +1182,7 @@ int run_sums(gm_ctx* c, const SynPlan& P, const double* theta, const float* gr, 
   if (regional)
     hipLaunchKernelGGL(P.istyp ? k_wnfix<1> : k_wnfix<0>, dim3((P.R * P.nf + 255) / 256), dim3(256), 0, s, P.R, P.nf,
                        P.Rp, P.Np, (const int*)(base + o.nwn), (const long long*)(base + o.koff), P.nkf, dgr, dgi, dth,
-                       dwn, re, 0.2, P.smap, (double*)(base + o.H));
+                       dwn, re, 0.2, P.smap, (double*)(base + o.H), 0);
   HIPCHK(c, hipEventRecord(c->sev[3], s));
   HIPCHK(c, hipGetLastError());
   return GM_OK;
@@ -1342,10 +1365,13 @@ static int synth_impl(gm_ctx* c, int nf1, int nf2, double df, double sigma, cons
 // spectra, the derivative spectra, and the displacement spectra against the
 // theta-derivative basis give kint 1..14, 15..28 and 29..42 / 1..7, 8..14 and
 // 15..21), then k_strain.
+// dwn > 0: the regional branch (besselDisplacementSpectra, displacementSpectra.f90:125-184):
+// theta holds the epicentral distances in metres, the sums are wavenumber integrals with the
+// kernels of wnRows / wnRowsDeriv, and the strain takes the angle theta / rearth
 static int strain_impl(gm_ctx* c, int nf1, int nf2, const int* nwn, int nsp, long long nkf, const float* gfkspr,
                        const float* gfkspi, int nrcv, const double* theta, const double* phi, double re, int istyp,
                        const double* fomt, const double* elcon, int flags, double* hdsum, double* zsp,
-                       double* strainsp, double* stresssp) {
+                       double* strainsp, double* stresssp, double dwn = 0.0, double rearth = 0.0) {
   if (!c) return GM_ERR_ARG;
   if (istyp != 0 && istyp != 1) return gm_fail(c, GM_ERR_ARG, "istyp must be 0 (FORCE) or 1 (MOMENT)");
   // numberGreenFKSpectra with derivflag = 1 (greenFKSpectra.f90:109-111, 216-220): U, V, UP, VP x njsph, W, WP x njtor
@@ -1355,6 +1381,11 @@ static int strain_impl(gm_ctx* c, int nf1, int nf2, const int* nwn, int nsp, lon
   if (!gfkspr || !gfkspi || !theta || !phi || !fomt || !strainsp)
     return gm_fail(c, GM_ERR_ARG, "spectra, theta, phi, fomt and strainsp are required");
   if (!(re > 0.0)) return gm_fail(c, GM_ERR_ARG, "receiver radius must be positive");
+  const bool regional = dwn > 0.0;
+  if (regional && !(rearth > 0.0)) return gm_fail(c, GM_ERR_ARG, "rearth must be positive");
+  if (regional)
+    for (int r = 0; r < nrcv; ++r)
+      if (!(theta[r] > 0.0)) return gm_fail(c, GM_ERR_ARG, "epicentral distance %d must be positive", r);
   SynPlan P;
   int rc = plan_sums(c, nf1, nf2, nwn, istyp ? 10 : 5, nkf, nrcv, P);   // 10 / 5 planes per contraction
   if (rc) return rc;
@@ -1368,6 +1399,7 @@ static int strain_impl(gm_ctx* c, int nf1, int nf2, const int* nwn, int nsp, lon
   legendreSegments(P.Lp, nseg, seglen);
   Carve cv;
   const size_t o_th = cv.take(sizeof(double) * P.R), o_ph = cv.take(sizeof(double) * P.R);
+  const size_t o_dl = regional ? cv.take(sizeof(double) * P.R) : o_th;     // the angles of the strain
   const size_t o_seeds = cv.take(sizeof(double) * 4 * 3 * (size_t)P.Rp * nseg);
   const size_t o_nwn = cv.take(sizeof(int) * P.nf), o_koff = cv.take(sizeof(long long) * P.nf);
   const size_t nbs = sizeof(__bf16) * kPieces * kRows * (size_t)P.Rp * P.Lp;
@@ -1385,6 +1417,12 @@ static int strain_impl(gm_ctx* c, int nf1, int nf2, const int* nwn, int nsp, lon
   char* base = (char*)c->sbuf;
   HIPCHK(c, hipMemcpyAsync(base + o_th, theta, sizeof(double) * P.R, hipMemcpyHostToDevice, s));
   HIPCHK(c, hipMemcpyAsync(base + o_ph, phi, sizeof(double) * P.R, hipMemcpyHostToDevice, s));
+  std::vector<double> delta;
+  if (regional) {
+    delta.resize(P.R);
+    for (int r = 0; r < P.R; ++r) delta[r] = theta[r] / rearth;
+    HIPCHK(c, hipMemcpyAsync(base + o_dl, delta.data(), sizeof(double) * P.R, hipMemcpyHostToDevice, s));
+  }
   HIPCHK(c, hipMemcpyAsync(base + o_nwn, P.nwnf.data(), sizeof(int) * P.nf, hipMemcpyHostToDevice, s));
   HIPCHK(c, hipMemcpyAsync(base + o_koff, P.koff.data(), sizeof(long long) * P.nf, hipMemcpyHostToDevice, s));
   HIPCHK(c, hipMemcpyAsync(base + o_fm, fomt, sizeof(double) * (istyp ? 6 : 3), hipMemcpyHostToDevice, s));
@@ -1399,13 +1437,21 @@ static int strain_impl(gm_ctx* c, int nf1, int nf2, const int* nwn, int nsp, lon
   const double* dth = (const double*)(base + o_th);
   HIPCHK(c, hipEventRecord(c->sev[0], s));
   double* seeds = (double*)(base + o_seeds);
-  if (nseg > 1)
-    hipLaunchKernelGGL(k_legendre_seed, dim3((P.Rp + 63) / 64, 3, nseg - 1), dim3(64), 0, s, P.R, P.Rp, P.L1 - 1, dth,
-                       nseg, seglen, seeds);
-  for (int mode = 0; mode <= 2; mode += 2)
-    hipLaunchKernelGGL(k_legendre, dim3((P.Rp + 63) / 64, 3, nseg), dim3(64), 0, s, P.R, P.Rp, P.L1 - 1, P.Lp, dth,
-                       (__bf16*)(base + (mode ? o_B1 : o_B0)), nullptr, nullptr, nullptr, nullptr, nullptr, mode, nseg,
-                       seglen, (const double*)seeds);
+  if (regional) {
+    // the wavenumber rows and their x-derivatives (R d/dx)
+    const size_t nb = (size_t)P.Rp * P.Lp;
+    for (int d = 0; d < 2; ++d)
+      hipLaunchKernelGGL(k_bessel_basis, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, P.R, P.Rp, P.Lp, dth,
+                         dwn, rearth, (__bf16*)(base + (d ? o_B1 : o_B0)), d);
+  } else {
+    if (nseg > 1)
+      hipLaunchKernelGGL(k_legendre_seed, dim3((P.Rp + 63) / 64, 3, nseg - 1), dim3(64), 0, s, P.R, P.Rp, P.L1 - 1,
+                         dth, nseg, seglen, seeds);
+    for (int mode = 0; mode <= 2; mode += 2)
+      hipLaunchKernelGGL(k_legendre, dim3((P.Rp + 63) / 64, 3, nseg), dim3(64), 0, s, P.R, P.Rp, P.L1 - 1, P.Lp, dth,
+                         (__bf16*)(base + (mode ? o_B1 : o_B0)), nullptr, nullptr, nullptr, nullptr, nullptr, mode,
+                         nseg, seglen, (const double*)seeds);
+  }
   HIPCHK(c, hipEventRecord(c->sev[1], s));
   // spectrum offsets (positionDataGreenFKSpectra with dsvmask U, V, UP, VP, W, WP)
   SpecMap mdisp, mder;
@@ -1419,7 +1465,7 @@ static int strain_impl(gm_ctx* c, int nf1, int nf2, const int* nwn, int nsp, lon
   for (int g = 0; g < 2; ++g)
     hipLaunchKernelGGL(k_gprep, dim3((unsigned)((P.Lp + 255) / 256), (unsigned)(P.Np / 2)), dim3(256), 0, s, P.nsp,
                        P.nf, P.Np, P.Lp, (const int*)(base + o_nwn), (const long long*)(base + o_koff), (long long)nkf,
-                       dgr, dgi, 0.2, 0.0, istyp, g ? mder : mdisp, (__bf16*)(base + (g ? o_G1 : o_G0)));
+                       dgr, dgi, 0.2, regional ? dwn : 0.0, istyp, g ? mder : mdisp, (__bf16*)(base + (g ? o_G1 : o_G0)));
   HIPCHK(c, hipEventRecord(c->sev[2], s));
   // part 0: displacement spectra x basis; 1: derivative spectra x basis; 2: displacement spectra x theta-derivative basis
   const size_t pb[3] = {o_B0, o_B0, o_B1}, pg[3] = {o_G0, o_G1, o_G0};
@@ -1427,14 +1473,19 @@ static int strain_impl(gm_ctx* c, int nf1, int nf2, const int* nwn, int nsp, lon
     double* Hpart = (double*)(base + o_H) + part * plane_h;
     hipLaunchKernelGGL(k_contract, dim3((P.Np / TN) * (P.Rp / TM) * P.nsum * nsplit), dim3(256), 0, s, P.Rp, P.Np,
                        P.Lp, P.nsp, (const __bf16*)(base + pb[part]), (const __bf16*)(base + pg[part]),
-                       nsplit > 1 ? (double*)(base + o_Hp) : Hpart, 0, istyp, nsplit, kper);
+                       nsplit > 1 ? (double*)(base + o_Hp) : Hpart, regional ? 1 : 0, istyp, nsplit, kper);
     if (nsplit > 1)
       hipLaunchKernelGGL(k_sumsplits, dim3((unsigned)((plane_h + 255) / 256)), dim3(256), 0, s, plane_h, nsplit,
                          (const double*)(base + o_Hp), Hpart);
+    if (regional)      // end points, Neumann terms and R^2 / (2 pi) of this part's integrals
+      hipLaunchKernelGGL(istyp ? k_wnfix<1> : k_wnfix<0>, dim3((P.R * P.nf + 255) / 256), dim3(256), 0, s, P.R, P.nf,
+                         P.Rp, P.Np, (const int*)(base + o_nwn), (const long long*)(base + o_koff), (long long)nkf,
+                         dgr, dgi, dth, dwn, rearth, 0.2, part == 1 ? mder : mdisp, Hpart, part == 2 ? 1 : 0);
   }
   HIPCHK(c, hipEventRecord(c->sev[3], s));
   hipLaunchKernelGGL(k_strain, dim3((unsigned)(((size_t)P.R * nf2 + 255) / 256)), dim3(256), 0, s, P.R, nf1, nf2,
-                     P.Rp, P.Np, P.nsum, (const double*)(base + o_H), dth, (const double*)(base + o_ph), re,
+                     P.Rp, P.Np, P.nsum, (const double*)(base + o_H), (const double*)(base + o_dl),
+                     (const double*)(base + o_ph), re,
                      (const double*)(base + o_fm), elcon ? (const double*)(base + o_el) : nullptr, istyp,
                      (double*)(base + o_Z), (double*)(base + o_E), elcon ? (double*)(base + o_T) : nullptr);
   HIPCHK(c, hipEventRecord(c->sev[4], s));
@@ -1463,6 +1514,18 @@ extern "C" int gm_strain_spectra(gm_ctx* c, int nf1, int nf2, const int* nwn, in
                                  int flags, double* hdsum, double* zsp, double* strainsp, double* stresssp) {
   return strain_impl(c, nf1, nf2, nwn, nsp, nkf, gfkspr, gfkspi, nrcv, theta, phi, re, istyp, fomt, elcon, flags,
                      hdsum, zsp, strainsp, stresssp);
+}
+
+extern "C" int gm_strain_spectra_regional(gm_ctx* c, int nf1, int nf2, const int* nwn, int nsp, long long nkf,
+                                          const float* gfkspr, const float* gfkspi, int nrcv, const double* x,
+                                          const double* phi, double re, double dwn, double rearth, int istyp,
+                                          const double* fomt, const double* elcon, int flags, double* hdsum,
+                                          double* zsp, double* strainsp, double* stresssp) {
+  if (!c) return GM_ERR_ARG;
+  if (!(dwn > 0.0)) return gm_fail(c, GM_ERR_ARG, "dwn must be positive");
+  if (!x) return gm_fail(c, GM_ERR_ARG, "x is required");
+  return strain_impl(c, nf1, nf2, nwn, nsp, nkf, gfkspr, gfkspi, nrcv, x, phi, re, istyp, fomt, elcon, flags, hdsum,
+                     zsp, strainsp, stresssp, dwn, rearth);
 }
 
 extern "C" int gm_moment_sums(gm_ctx* c, int nf1, int nf2, const int* nwn, int nsp, long long nkf, const float* gfkspr,

@@ -453,6 +453,33 @@ module geminiFKc
          integer(c_int), value :: flags
          type(c_ptr), value :: wnint
       end function gm_force_bessel_sums
+      integer(c_int) function gm_strain_spectra_regional(ctx, nf1, nf2, nwn, nsp, nkf, gfkspr, gfkspi, &
+          nrcv, x, phi, re, dwn, rearth, istyp, fomt, elcon, flags, hdsum, zsp, strainsp, stresssp) bind(C, &
+          name='gm_strain_spectra_regional')
+         import
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: nf1
+         integer(c_int), value :: nf2
+         type(c_ptr), value :: nwn
+         integer(c_int), value :: nsp
+         integer(c_long_long), value :: nkf
+         type(c_ptr), value :: gfkspr
+         type(c_ptr), value :: gfkspi
+         integer(c_int), value :: nrcv
+         type(c_ptr), value :: x
+         type(c_ptr), value :: phi
+         real(c_double), value :: re
+         real(c_double), value :: dwn
+         real(c_double), value :: rearth
+         integer(c_int), value :: istyp
+         type(c_ptr), value :: fomt
+         type(c_ptr), value :: elcon
+         integer(c_int), value :: flags
+         type(c_ptr), value :: hdsum
+         type(c_ptr), value :: zsp
+         type(c_ptr), value :: strainsp
+         type(c_ptr), value :: stresssp
+      end function gm_strain_spectra_regional
       integer(c_int) function gm_synthetics_regional(ctx, nf1, nf2, df, sigma, nwn, nsp, nkf, gfkspr, &
           gfkspi, dwn, rearth, nrcv, x, phi, propdir, tred, evfilter, stafilter, istyp, fomt, velocity, &
           nsamp, dt, flags, ts, zsp) bind(C, name='gm_synthetics_regional')

@@ -137,11 +137,14 @@ class SyntheticsEngine:
         gr, gi: [5, nkf] (U1, U2, V1, V2, W1)."""
         return self._sums(_lib.lib().gm_force_sums, FORCE_SUMS, nf1, nf2, nwn, gr, gi, theta)
 
-    def strain_spectra(self, nf1, nf2, nwn, gr, gi, theta, phi, re, fomt, istyp=1, elcon=None, want_sums=False):
+    def strain_spectra(self, nf1, nf2, nwn, gr, gi, theta, phi, re, fomt, istyp=1, elcon=None, want_sums=False,
+                       dwn=None, rearth=None):
         """Strain (and, with elcon = 7 complex moduli at the receiver, stress) spectra from spectra with radial
         derivatives (gr, gi: [20, nkf] moment or [10, nkf] force, U V dU dV W dW): dict strain and stress
         [nrcv, 3, 3, nf2], zsp [nrcv, 3, nf2] (components r, theta, phi), hdsum [42 or 21, nrcv, nf]
-        (sphericalHarmonicsDisplacementSpectra with strainsp, displacementSpectra.f90:49-106)."""
+        (sphericalHarmonicsDisplacementSpectra with strainsp, displacementSpectra.f90:49-106).
+        Regional branch (dwn given): theta = epicentral distances in metres, wavenumber integrals with
+        k = (j-1) dwn and rearth (besselDisplacementSpectra, :125-184)."""
         nwn = np.ascontiguousarray(nwn, dtype=np.int32)
         gr, gi = self._spectra(gr, gi)
         theta = np.ascontiguousarray(np.atleast_1d(theta), dtype=np.float64)
@@ -155,13 +158,21 @@ class SyntheticsEngine:
                    hdsum=np.zeros((42 if istyp == 1 else 21, R, nf), np.complex128) if want_sums else None)
         el = None if elcon is None else np.ascontiguousarray(elcon, dtype=np.complex128)
         vp = lambda a: None if a is None else a.ctypes.data
-        check(self.h, _lib.lib().gm_strain_spectra(self.h, int(nf1), int(nf2), nwn.ctypes.data_as(I), gr.shape[0],
-                                                   gr.shape[1], gr.ctypes.data, gi.ctypes.data, R,
-                                                   theta.ctypes.data_as(D), phi.ctypes.data_as(D), float(re),
-                                                   int(istyp), fm.ctypes.data_as(D),
-                                                   None if el is None else el.ctypes.data_as(D), 0,
-                                                   vp(out["hdsum"]), vp(out["zsp"]), vp(out["strain"]),
-                                                   vp(out["stress"])))
+        eld = None if el is None else el.ctypes.data_as(D)
+        if dwn:
+            rc = _lib.lib().gm_strain_spectra_regional(self.h, int(nf1), int(nf2), nwn.ctypes.data_as(I), gr.shape[0],
+                                                       gr.shape[1], gr.ctypes.data, gi.ctypes.data, R,
+                                                       theta.ctypes.data_as(D), phi.ctypes.data_as(D), float(re),
+                                                       float(dwn), float(rearth if rearth else re), int(istyp),
+                                                       fm.ctypes.data_as(D), eld, 0, vp(out["hdsum"]), vp(out["zsp"]),
+                                                       vp(out["strain"]), vp(out["stress"]))
+        else:
+            rc = _lib.lib().gm_strain_spectra(self.h, int(nf1), int(nf2), nwn.ctypes.data_as(I), gr.shape[0],
+                                              gr.shape[1], gr.ctypes.data, gi.ctypes.data, R,
+                                              theta.ctypes.data_as(D), phi.ctypes.data_as(D), float(re),
+                                              int(istyp), fm.ctypes.data_as(D), eld, 0, vp(out["hdsum"]),
+                                              vp(out["zsp"]), vp(out["strain"]), vp(out["stress"]))
+        check(self.h, rc)
         return out
 
     def bessel_sums(self, nf1, nf2, nwn, gr, gi, dwn, rearth, x):

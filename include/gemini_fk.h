@@ -379,6 +379,20 @@ int gm_force_bessel_sums(gm_ctx* ctx, int nf1, int nf2, const int* nwn, int nsp,
 
 /* gm_synthetics for the regional branch: receivers at epicentral distances
  * x[r] (metres) and azimuths phi[r]; otherwise as gm_synthetics. */
+/* gm_strain_spectra for the regional branch (besselDisplacementSpectra with
+ * strainsp, f90/displacementSpectra.f90:125-184): receivers at epicentral
+ * distances x[r] (metres, > 0) and azimuths phi[r] at radius re, wavenumbers
+ * k = (j-1) dwn.  momentWavenumberIntegrals / forceWavenumberIntegrals for
+ * kint 1..42 / 1..21 (f90/wavenumberIntegrals.f90:122-477): the wavenumber
+ * integrals of gm_bessel_sums on the displacement and on the radial-derivative
+ * spectra, and the x-derivative integrals (kernels R d/dx of those rows); then
+ * the strain of the epicentral angle x[r] / rearth as in gm_strain_spectra.
+ * Arguments and outputs otherwise as gm_strain_spectra. */
+int gm_strain_spectra_regional(gm_ctx* ctx, int nf1, int nf2, const int* nwn, int nsp, long long nkf,
+                               const float* gfkspr, const float* gfkspi, int nrcv, const double* x, const double* phi,
+                               double re, double dwn, double rearth, int istyp, const double* fomt, const double* elcon,
+                               int flags, double* hdsum, double* zsp, double* strainsp, double* stresssp);
+
 int gm_synthetics_regional(gm_ctx* ctx, int nf1, int nf2, double df, double sigma, const int* nwn, int nsp,
                            long long nkf, const float* gfkspr, const float* gfkspi, double dwn, double rearth,
                            int nrcv, const double* x, const double* phi, const double* propdir, const double* tred,

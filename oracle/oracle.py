@@ -304,9 +304,11 @@ def legendre(lmax, theta):
     return out
 
 
-def strain_spectra(istyp, nf1, nf2, nwn, gr, gi, theta, phi, re, fomt, elcon=None, tapfrac=0.2, nthreads=1):
+def strain_spectra(istyp, nf1, nf2, nwn, gr, gi, theta, phi, re, fomt, elcon=None, tapfrac=0.2, nthreads=1,
+                   dwn=0.0, rearth=0.0):
     """Strain spectra of one node (spectra with radial derivatives, nsp 20 moment / 10 force):
-    dict hdsum [nrcv, nf, nsum], zsp [nrcv, 3, nf2], strain [nrcv, 3, 3, nf2], stress (with elcon)."""
+    dict hdsum [nrcv, nf, nsum], zsp [nrcv, 3, nf2], strain [nrcv, 3, 3, nf2], stress (with elcon).
+    dwn > 0: the regional branch, wavenumber integrals at x = theta * rearth (theta in radians)."""
     theta = np.ascontiguousarray(np.atleast_1d(theta), dtype=np.float64)
     phi = np.ascontiguousarray(np.broadcast_to(phi, theta.shape), dtype=np.float64)
     nrcv, nf, nsum = theta.size, nf2 - nf1 + 1, 42 if istyp == 1 else 21
@@ -318,12 +320,12 @@ def strain_spectra(istyp, nf1, nf2, nwn, gr, gi, theta, phi, re, fomt, elcon=Non
                strain=np.zeros((nrcv, 3, 3, nf2), np.complex128),
                stress=np.zeros((nrcv, 3, 3, nf2), np.complex128) if elcon is not None else None)
     el = None if elcon is None else np.ascontiguousarray(elcon, dtype=np.complex128)
-    rc = lib().orc_strain_spectra(int(istyp), int(nf1), int(nf2), nw.ctypes.data_as(I), C.c_long(nkf),
-                                  gr.ctypes.data_as(C.POINTER(C.c_float)), gi.ctypes.data_as(C.POINTER(C.c_float)),
-                                  nrcv, _d(theta), _d(phi), C.c_double(re), _d(fomt),
-                                  None if el is None else _c(el), C.c_double(tapfrac), _c(out["hdsum"]),
-                                  _c(out["zsp"]), _c(out["strain"]),
-                                  None if el is None else _c(out["stress"]), int(nthreads))
+    rc = lib().orc_strain_spectra_wn(int(istyp), int(nf1), int(nf2), nw.ctypes.data_as(I), C.c_long(nkf),
+                                     gr.ctypes.data_as(C.POINTER(C.c_float)), gi.ctypes.data_as(C.POINTER(C.c_float)),
+                                     nrcv, _d(theta), _d(phi), C.c_double(re), _d(fomt),
+                                     None if el is None else _c(el), C.c_double(tapfrac), C.c_double(dwn),
+                                     C.c_double(rearth), _c(out["hdsum"]), _c(out["zsp"]), _c(out["strain"]),
+                                     None if el is None else _c(out["stress"]), int(nthreads))
     assert rc == 0
     return out
 

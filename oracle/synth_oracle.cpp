@@ -482,36 +482,54 @@ void taper_wn(int nwn, double dwn, double tapfrac, double* w) {
   }
 }
 
-// Kernel of moment wavenumber integral kint (1..14) at wavenumber k for Bessel
-// (or Neumann) values b0, b1, b2 of order 0..2 at k x.
-// Restates the case tables of momentWavenumberIntegrals, f90/wavenumberIntegrals.f90:353-373 (and :379-399).
+// Kernel of moment wavenumber integral kint (1..42) at wavenumber k for Bessel
+// (or Neumann) values b0, b1, b2 of order 0..2 at k x: kint 15..28 take the
+// kernels of 1..14 (on the radial-derivative spectra), 29..42 those of the
+// x-derivatives.  Restates the case tables of momentWavenumberIntegrals,
+// f90/wavenumberIntegrals.f90:353-373 (and :379-399); kint 43..46 (stresses) are not taken.
 static double wn_kernel(int kint, double k, double x, double re, double b0, double b1, double b2) {
+  const double kx = k * x, kr = k * re;
   switch (kint) {
-    case 1: case 2: return k * b0;
-    case 3: return 2.0 / re * b1;
-    case 4: return k * 2.0 * b2;
-    case 5: case 6: return k * (-k * re) * b1;
-    case 7: case 11: return k * 2.0 * (b0 - b1 / (k * x));
-    case 8: return k * 2.0 * k * re * (b1 - 2.0 * b2 / (k * x));
-    case 9: case 13: return k * 2.0 * b1 / (k * x);
-    case 10: return k * 4.0 * k * re * b2 / (k * x);
-    case 12: return -k * 2.0 * k * re * (b1 - 2.0 * b2 / (k * x));
-    case 14: return -k * 4.0 * k * re * b2 / (k * x);
+    case 1: case 2: case 15: case 16: return k * b0;
+    case 3: case 17: return 2.0 / re * b1;
+    case 4: case 18: return k * 2.0 * b2;
+    case 5: case 6: case 19: case 20: case 29: case 30: return k * (-k * re) * b1;
+    case 7: case 11: case 21: case 25: case 31: return k * 2.0 * (b0 - b1 / kx);
+    case 8: case 22: case 32: return k * 2.0 * k * re * (b1 - 2.0 * b2 / kx);
+    case 9: case 13: case 23: case 27: return k * 2.0 * b1 / kx;
+    case 10: case 24: return k * 4.0 * k * re * b2 / kx;
+    case 12: case 26: return -k * 2.0 * k * re * (b1 - 2.0 * b2 / kx);
+    case 14: case 28: return -k * 4.0 * k * re * b2 / kx;
+    case 33: case 34: return -k * kr * kr * (b0 - b1 / kx);
+    case 35: case 39: return -k * 2.0 * kr * (b1 - b2 / kx);
+    case 36: return k * 2.0 * kr * kr * ((6.0 / (kx * kx) - 1.0) * b2 - b1 / kx);
+    case 37: case 41: return -k * 2.0 * kr * b2 / kx;
+    case 38: return k * 4.0 * kr * kr * (-3.0 * b2 / (kx * kx) + b1 / kx);
+    case 40: return -k * 2.0 * kr * kr * ((6.0 / (kx * kx) - 1.0) * b2 - b1 / kx);
+    case 42: return -k * 4.0 * kr * kr * (-3.0 * b2 / (kx * kx) + b1 / kx);
   }
   return 0.0;
 }
 
-// Kernel of force wavenumber integral kint (1..7) at wavenumber k.
-// Restates the case tables of forceWavenumberIntegrals, f90/wavenumberIntegrals.f90:162-176 (and :183-197, :220-234).
-static double wn_kernel_force(int kint, double k, double x, double re, double b0, double b1, double) {
+// Kernel of force wavenumber integral kint (1..21) at wavenumber k: 8..14 take
+// the kernels of 1..7, 15..21 those of the x-derivatives.  Restates the case
+// tables of forceWavenumberIntegrals, f90/wavenumberIntegrals.f90:162-176 (and
+// :183-197, :220-234); kint 22, 23 (stresses) are not taken.
+static double wn_kernel_force(int kint, double k, double x, double re, double b0, double b1, double b2) {
+  const double kx = k * x, kr = k * re;
   switch (kint) {
-    case 1: return k * b0;
-    case 2: return -2.0 * b1 / re;
-    case 3: return -k * k * b1 * re;
-    case 4: return -2.0 * k * (b0 - b1 / (k * x));
-    case 5: return -2.0 * k * b1 / (k * x);
-    case 6: return 2.0 * k * (b0 - b1 / (k * x));
-    case 7: return 2.0 * k * b1 / (k * x);
+    case 1: case 8: return k * b0;
+    case 2: case 9: return -2.0 * b1 / re;
+    case 3: case 10: case 15: return -k * k * b1 * re;
+    case 4: case 11: case 16: return -2.0 * k * (b0 - b1 / kx);
+    case 5: case 12: return -2.0 * k * b1 / kx;
+    case 6: case 13: return 2.0 * k * (b0 - b1 / kx);
+    case 7: case 14: return 2.0 * k * b1 / kx;
+    case 17: return -k * kr * kr * (b0 - b1 / kx);
+    case 18: return -2.0 * k * kr * (-b1 + b2 / kx);
+    case 19: return 2.0 * k * kr * b2 / kx;
+    case 20: return 2.0 * k * kr * (-b1 + b2 / kx);
+    case 21: return -2.0 * k * kr * b2 / kx;
   }
   return 0.0;
 }
@@ -526,9 +544,18 @@ static double wn_kernel_force(int kint, double k, double x, double re, double b0
 // wnint: [nf][nsum] complex.
 static const int kSpecWn[14] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 8, 9, 6, 7};
 static const int kSpecWnForce[7] = {0, 1, 2, 3, 4, 4, 3};
+// strain = false: the displacement integrals kint 1..14 / 1..7 on U, V, W spectra;
+// strain = true: kint 1..42 / 1..21 on spectra with radial derivatives (U, V,
+// UP, VP, W, WP; positionDataGreenFKSpectra through the (component, jump) maps)
 void bessel_sums(int istyp, int nf1, int nf2, const int* nwn, long nkf, const float* gr, const float* gi, double dwn,
-                 double re, double xx, double tapfrac, cd* wnint) {
-  const int nsum = istyp == 1 ? 14 : 7;
+                 double re, double xx, double tapfrac, cd* wnint, bool strain = false) {
+  const int nsum = strain ? (istyp == 1 ? 42 : 21) : (istyp == 1 ? 14 : 7);
+  const int njsph = istyp == 1 ? 4 : 2, njtor = istyp == 1 ? 2 : 1;
+  auto specOf = [&](int kint) {
+    if (!strain) return istyp == 1 ? kSpecWn[kint - 1] : kSpecWnForce[kint - 1];
+    const int* cj = istyp == 1 ? kMomentMap[kint - 1] : kForceMap[kint - 1];
+    return positionData(cj[0], cj[1], njsph, njtor);
+  };
   auto kernel = [&](int kint, double k, double x, double b0, double b1, double b2) {
     return istyp == 1 ? wn_kernel(kint, k, x, re, b0, b1, b2) : wn_kernel_force(kint, k, x, re, b0, b1, b2);
   };
@@ -558,7 +585,7 @@ void bessel_sums(int istyp, int nf1, int nf2, const int* nwn, long nkf, const fl
     const int j33 = std::min(j3, n);
     const double wne = (n - 1) * dwn;
     for (int kint = 1; kint <= nsum; ++kint) {
-      const int s = istyp == 1 ? kSpecWn[kint - 1] : kSpecWnForce[kint - 1];
+      const int s = specOf(kint);
       auto G = [&](int j) { return cd(gr[(long)s * nkf + off + j - 1], gi[(long)s * nkf + off + j - 1]); };
       for (int j = 1; j <= n; ++j) kern[j - 1] = kernel(kint, (j - 1) * dwn, x, J0[j - 1], J1[j - 1], J2[j - 1]);
       cd z = 0.0;
@@ -657,16 +684,22 @@ int orc_bessel_sums(int nf1, int nf2, const int* nwn, long nkf, const float* gr,
 // hdsum [nrcv][nf][nsum] complex, zsp [nrcv][3][nf2], strain [nrcv][3][3][nf2]
 // and, with elcon (7 complex moduli at the receiver), stress [nrcv][3][3][nf2]
 // (the reference's zsp(nf2, 3) and strainsp(nf2, 3, 3) per receiver).
-int orc_strain_spectra(int istyp, int nf1, int nf2, const int* nwn, long nkf, const float* gr, const float* gi,
-                       int nrcv, const double* theta, const double* phi, double re, const double* fomt,
-                       const double* elcon, double tapfrac, double* hdsum, double* zsp, double* strain,
-                       double* stress, int nthreads) {
+// dwn > 0: the regional branch (besselDisplacementSpectra, f90/displacementSpectra.f90:
+// 125-184): wavenumber integrals kint 1..42 / 1..21 at x = theta rearth (theta
+// the epicentral angle, as the reference passes delta)
+int orc_strain_spectra_wn(int istyp, int nf1, int nf2, const int* nwn, long nkf, const float* gr, const float* gi,
+                          int nrcv, const double* theta, const double* phi, double re, const double* fomt,
+                          const double* elcon, double tapfrac, double dwn, double rearth, double* hdsum, double* zsp,
+                          double* strain, double* stress, int nthreads) {
   const int nf = nf2 - nf1 + 1, nsum = istyp == 1 ? 42 : 21;
   const syn::cd* el = reinterpret_cast<const syn::cd*>(elcon);
 #pragma omp parallel for schedule(dynamic) num_threads(nthreads > 0 ? nthreads : 1)
   for (int r = 0; r < nrcv; ++r) {
     std::vector<syn::cd> h((size_t)nf * nsum);
-    syn::strain_sums(istyp, nf1, nf2, nwn, nkf, gr, gi, theta[r], tapfrac, h.data());
+    if (dwn > 0.0)
+      syn::bessel_sums(istyp, nf1, nf2, nwn, nkf, gr, gi, dwn, rearth, theta[r] * rearth, tapfrac, h.data(), true);
+    else
+      syn::strain_sums(istyp, nf1, nf2, nwn, nkf, gr, gi, theta[r], tapfrac, h.data());
     if (hdsum) std::memcpy(hdsum + (size_t)r * nf * nsum * 2, h.data(), sizeof(syn::cd) * nf * nsum);
     for (int jf = 1; jf <= nf2; ++jf) {
       syn::cd u[4][3] = {}, e[3][3] = {}, t[3][3] = {};
@@ -689,6 +722,14 @@ int orc_strain_spectra(int istyp, int nf1, int nf2, const int* nwn, long nkf, co
     }
   }
   return 0;
+}
+
+int orc_strain_spectra(int istyp, int nf1, int nf2, const int* nwn, long nkf, const float* gr, const float* gi,
+                       int nrcv, const double* theta, const double* phi, double re, const double* fomt,
+                       const double* elcon, double tapfrac, double* hdsum, double* zsp, double* strain,
+                       double* stress, int nthreads) {
+  return orc_strain_spectra_wn(istyp, nf1, nf2, nwn, nkf, gr, gi, nrcv, theta, phi, re, fomt, elcon, tapfrac, 0.0, 0.0,
+                               hdsum, zsp, strain, stress, nthreads);
 }
 
 // pure strain combination for checks: u[4][3] complex -> e[3][3] complex

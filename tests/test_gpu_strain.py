@@ -121,3 +121,33 @@ def test_ocean_layouts_give_the_displacement_without_w(syn, istyp, layout):
         tu, zu = syn.synthetics(nf1, nf2, df, sigma, nwn, g10r, g10i, th, phi, fomt, 16, 1.0 / (16 * df), **kw, **kx)
         assert np.array_equal(zo, zu) and np.array_equal(to, tu)
         assert np.abs(zo).max() > 0
+
+
+@pytest.mark.parametrize("istyp", [1, 0])
+def test_regional_strain_spectra_match_oracle(syn, istyp):
+    """The regional branch (besselDisplacementSpectra with strainsp): wavenumber integrals kint 1..42 /
+    1..21 (displacement and radial-derivative spectra, and the x-derivative kernels) in three
+    matrix-core contractions with their end terms, then the strain and stress of the angle x / R."""
+    nf1, nf2, nwn, gr, gi, theta, phi, fomt = _case(istyp, 13 + istyp)
+    x = 1.0e3 + theta / np.pi * 2.0e5                 # 1 .. 200 km
+    dwn = 2 * np.pi / 8.0e6
+    el = np.array([2.1e11 + 1e8j, 1.9e11 + 2e8j, 0.8e11 + 1e7j, 0.6e11 + 3e8j, 0.7e11 + 2e8j, 1.3e11, 0.65e11])
+    print(f"regional strain istyp={istyp}: {x.size} receivers, {nf2 - nf1 + 1} frequencies", flush=True)
+    g = syn.strain_spectra(nf1, nf2, nwn, gr, gi, x, phi, RE, fomt, istyp=istyp, elcon=el, want_sums=True,
+                           dwn=dwn, rearth=RE)
+    o = oracle.strain_spectra(istyp, nf1, nf2, nwn, gr, gi, x / RE, phi, RE, fomt, elcon=el, dwn=dwn, rearth=RE)
+    h = o["hdsum"].transpose(2, 0, 1)
+    scale = np.abs(h).max(axis=2, keepdims=True) + 1e-300
+    err = np.abs(g["hdsum"] - h) / scale
+    print(f"  integrals max rel {err.max():.2e}", flush=True)
+    assert err.max() < 1e-5
+    for key in ("zsp", "strain", "stress"):
+        a, b = g[key], o[key]
+        sc = np.abs(b).max(axis=-1, keepdims=True) + 1e-300
+        e = (np.abs(a - b) / sc).max()
+        print(f"  {key} max rel {e:.2e}", flush=True)
+        assert e < 1e-5
+        assert np.all(a[..., :nf1 - 1] == 0)
+    from gemini_unified_amd._lib import GeminiFKError
+    with pytest.raises(GeminiFKError):            # distances must be positive
+        syn.strain_spectra(nf1, nf2, nwn, gr, gi, -x, phi, RE, fomt, istyp=istyp, dwn=dwn, rearth=RE)

@@ -127,3 +127,56 @@ def test_stress_is_hooke_law_of_a_vti_medium():
         ref = np.array([[sv[2], sv[4], sv[3]], [sv[4], sv[0], sv[5]], [sv[3], sv[5], sv[1]]])
         assert np.abs(t - ref).max() < 1e-13 * np.abs(ref).max()
         assert np.all(t == t.T) and np.all(e == e.T)
+
+
+# ---- regional branch: wavenumber integrals kint 1..42 / 1..21 (wavenumberIntegrals.f90:122-477) ----
+
+def _smooth_spectra(istyp, dwn, kmax=0.04):
+    """spectra exp(-a_s k) with a rate a_s of their own per spectrum s (a spectrum read from the wrong
+    position gives another function), in the units the integrals' R^2 / 2 pi expects"""
+    nsp = 20 if istyp == 1 else 10
+    n = int(round(kmax / dwn))
+    k = np.arange(n) * dwn
+    a = 800.0 + 150.0 * np.arange(nsp)
+    c = (1.0 + 0.1 * np.arange(nsp)) * 1e20 * 2 * np.pi / RE ** 2
+    gr = (np.exp(-a[:, None] * k) * c[:, None]).astype(np.float32)
+    gi = (np.exp(-1.3 * a[:, None] * k) * 0.5 * c[:, None]).astype(np.float32)
+    return np.array([n], np.int32), gr, gi
+
+
+@pytest.mark.parametrize("istyp", [1, 0])
+def test_regional_displacement_and_radial_integrals(istyp):
+    """kint 1..14 / 1..7 are the displacement integrals of the U, V, W spectra and 15..28 / 8..14
+    the same integrals of the radial-derivative spectra."""
+    dwn = 1e-5
+    nwn, gr, gi = _smooth_spectra(istyp, dwn)
+    U, V, UP, VP, W, WP = layout(istyp)
+    x = np.array([3000.0, 20000.0])
+    nd = 14 if istyp == 1 else 7
+    h = oracle.strain_spectra(istyp, 1, 1, nwn, gr, gi, x / RE, 0.3, RE, FOMT[istyp], dwn=dwn, rearth=RE)["hdsum"]
+    disp, deriv = np.concatenate([U, V, W]), np.concatenate([UP, VP, WP])
+    ref = oracle.bessel_sums(1, 1, nwn, gr[disp], gi[disp], dwn, RE, x, istyp=istyp)
+    np.testing.assert_allclose(h[..., :nd], ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+    refd = oracle.bessel_sums(1, 1, nwn, gr[deriv], gi[deriv], dwn, RE, x, istyp=istyp)
+    np.testing.assert_allclose(h[..., nd:2 * nd], refd, rtol=1e-12, atol=1e-12 * np.abs(refd).max())
+
+
+@pytest.mark.parametrize("istyp", [1, 0])
+def test_regional_x_derivative_integrals_converge_to_derivatives(istyp):
+    """kint 29..42 (moment) and 15..21 (force) are R d/dx of kint 1..14 / 1..7: the difference
+    between their quadrature and the central difference in x of the displacement quadrature falls
+    at third order in dwn (a wrong kernel, factor or spectrum would leave an O(1) difference)."""
+    nd = 14 if istyp == 1 else 7
+    x, hx = 5000.0, 0.5
+    errs = []
+    for dwn in (1e-5, 5e-6, 2.5e-6):
+        nwn, gr, gi = _smooth_spectra(istyp, dwn)
+        f = lambda xx: oracle.strain_spectra(istyp, 1, 1, nwn, gr, gi, np.array([xx / RE]), 0.3, RE, FOMT[istyp],
+                                             dwn=dwn, rearth=RE)["hdsum"][0, 0]
+        h0, hp, hm = f(x), f(x + hx), f(x - hx)
+        fd = RE * (hp[:nd] - hm[:nd]) / (2 * hx)
+        errs.append(np.abs(h0[2 * nd:3 * nd] - fd) / np.abs(fd).max())
+    assert errs[2].max() < 2e-3, errs[2]
+    big = errs[0] > 1e-8
+    assert big.sum() >= nd // 2
+    assert np.all(errs[1][big] < errs[0][big] / 4) and np.all(errs[2][big] < errs[1][big] / 4), errs
