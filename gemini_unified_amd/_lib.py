@@ -22,7 +22,7 @@ EXPORTS = ["gm_ctx_create", "gm_ctx_destroy", "gm_set_option", "gm_last_error", 
            "gm_collect_steplog", "gm_minors_batch_wn", "gm_gfk_batch_wn", "gm_legendre_batch", "gm_moment_sums", "gm_synthetics", "gm_last_synth_timing",
            "gm_bessel_sums", "gm_synthetics_regional", "gm_force_sums", "gm_force_bessel_sums",
            "gm_event_filter", "gm_travel_times", "gm_gfk_batch_noalloc", "gm_node_properties", "gm_aski_batch", "gm_strain_spectra",
-           "gm_strain_spectra_regional"]
+           "gm_strain_spectra_regional", "gm_ujgf_batch", "gm_ujgf_batch_wn"]
 
 
 class GeminiFKError(RuntimeError):
@@ -65,6 +65,10 @@ def lib():
                                     C.c_int, C.c_int, D, F, F, I, LL]
         L.gm_gfk_batch_wn.argtypes = [VP, C.c_int, D, I, I, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                                       C.c_int, C.c_int, D, F, F, I, LL]
+        L.gm_ujgf_batch.argtypes = [VP, C.c_int, D, I, I, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, D, D,
+                                    I, LL]
+        L.gm_ujgf_batch_wn.argtypes = [VP, C.c_int, D, I, I, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                       C.c_int, D, D, I, LL]
         # array arguments of the synthetics entry points are void* so that device pointers (ints) pass too
         L.gm_legendre_batch.argtypes = [VP, C.c_int, D, C.c_int, D, D, D, D, D]
         L.gm_moment_sums.argtypes = [VP, C.c_int, C.c_int, I, C.c_int, C.c_longlong, VP, VP, C.c_int, D, C.c_int, VP]

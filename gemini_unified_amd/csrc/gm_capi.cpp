@@ -337,7 +337,8 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
                double eps, int attmode, int jsl, int jsr, int jre, int istyp, bool full, int flags,
                double* u_tor, double* d_tor, double* dettor, double* u_sph, double* d_sph, double* ulg, double* dlg,
                double* detsph, double* gf, float* gfre, float* gfim, int* status, long long* stats,
-               const GmNoAlloc* na = nullptr, int nsnod = -1, const int* jsnod = nullptr) {
+               const GmNoAlloc* na = nullptr, int nsnod = -1, const int* jsnod = nullptr,
+               double* ujs_out = nullptr, double* ujt_out = nullptr) {
   if (!c) return GM_ERR_ARG;
   if (na && nfreq > GM_NOALLOC_MAXF)
     return gm_fail(c, GM_ERR_ARG, "%d frequencies: the batch without allocation takes at most %d", nfreq,
@@ -439,6 +440,10 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   size_t o_gf = (full && (gf && !dev)) ? take(16 * nout + 16) : 0;
   size_t o_gfre = (full && (gfre && !dev)) ? take(4 * nout + 4) : 0;
   size_t o_gfim = (full && (gfim && !dev)) ? take(4 * nout + 4) : 0;
+  // unit-jump Green functions out (gm_ujgf_batch): [nnod][4][4][k] and [nnod][2][2][k] complex
+  const bool ujout = full && ujs_out && ujt_out;
+  size_t o_ujo = (ujout && !dev) ? take(16 * 16 * nnod * n + 16) : 0;
+  size_t o_ujto = (ujout && !dev) ? take(16 * 4 * nnod * n + 16) : 0;
   size_t o_hist = take(sizeof(unsigned long long) * 6 * 8);
   size_t o_uq = take(sizeof(int) * (nunits + 1)), o_ust = take(sizeof(int) * (nunits + 1));
   size_t o_ue = take(sizeof(int) * (nunits + 1)), o_uc = take(sizeof(int) * GM_NCOUNTERS * nsrc);
@@ -514,6 +519,10 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
     B.gfre = gfre ? (dev ? gfre : (float*)(base + o_gfre)) : nullptr;
     B.gfim = gfim ? (dev ? gfim : (float*)(base + o_gfim)) : nullptr;
     if ((B.gfre == nullptr) != (B.gfim == nullptr)) return gm_fail(c, GM_ERR_ARG, "gfre and gfim must both be given");
+    if (ujout) {
+      B.ujs_out = dev ? ujs_out : (double*)(base + o_ujo);
+      B.ujt_out = dev ? ujt_out : (double*)(base + o_ujto);
+    }
   }
   B.hist = (unsigned long long*)(base + o_hist);
   B.nunits = nunits;
@@ -622,6 +631,10 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
     if (dev) return hipMemcpyAsync(dst, base + o, bytes, hipMemcpyDeviceToDevice, s);
     return hipMemcpyAsync(dst, base + o, bytes, hipMemcpyDeviceToHost, s);
   };
+  if (ujout && !dev) {
+    HIPCHK(c, cp(ujs_out, o_ujo, 16 * 16 * nnod * n));
+    HIPCHK(c, cp(ujt_out, o_ujto, 16 * 4 * nnod * n));
+  }
   HIPCHK(c, cp(u_tor, o_ut, 16 * 2 * nnod * n));
   HIPCHK(c, cp(d_tor, o_dt, 16 * 2 * nnod * n));
   HIPCHK(c, cp(dettor, o_dett, 16 * nnod * n));
@@ -699,6 +712,27 @@ extern "C" int gm_gfk_batch(gm_ctx* c, int nfreq, const double* freqs, const int
                             float* gfim, int* status, long long* stats) {
   return run(c, nfreq, freqs, lmin, nl, 0.0, sigma, eps, attmode, jre, jre, jre, istyp, true, flags, nullptr, nullptr,
              nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, gf, gfre, gfim, status, stats);
+}
+
+extern "C" int gm_ujgf_batch(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const int* nl, double sigma,
+                             double eps, int attmode, int jre, int flags, double* ujs, double* ujt, int* status,
+                             long long* stats) {
+  if (!c) return GM_ERR_ARG;
+  if (!ujs || !ujt) return gm_fail(c, GM_ERR_ARG, "ujs and ujt are required");
+  return run(c, nfreq, freqs, lmin, nl, 0.0, sigma, eps, attmode, jre, jre, jre, 1, true, flags, nullptr, nullptr,
+             nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, status, stats, nullptr,
+             -1, nullptr, ujs, ujt);
+}
+
+extern "C" int gm_ujgf_batch_wn(gm_ctx* c, int nfreq, const double* freqs, const int* iwnmin, const int* nwn,
+                                double dwn, double sigma, double eps, int attmode, int jre, int flags, double* ujs,
+                                double* ujt, int* status, long long* stats) {
+  if (!c) return GM_ERR_ARG;
+  if (!(dwn > 0.0)) return gm_fail(c, GM_ERR_ARG, "dwn must be positive");
+  if (!ujs || !ujt) return gm_fail(c, GM_ERR_ARG, "ujs and ujt are required");
+  return run(c, nfreq, freqs, iwnmin, nwn, dwn, sigma, eps, attmode, jre, jre, jre, 1, true, flags, nullptr, nullptr,
+             nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, status, stats, nullptr,
+             -1, nullptr, ujs, ujt);
 }
 
 extern "C" int gm_gfk_batch_wn(gm_ctx* c, int nfreq, const double* freqs, const int* iwnmin, const int* nwn, double dwn,

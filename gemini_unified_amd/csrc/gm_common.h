@@ -146,6 +146,10 @@ struct GmBatch {
   // multi-source assembly (k_aski_assemble): output block of this source, its
   // source type and the spectra per node of the output layout
   int aski_js, aski_styp, aski_nspmax;
+  // unit-jump Green functions out instead of FK columns (k_assemble, gm_ujgf_batch):
+  // complex [nnod][jump 4][component 4][n] and [nnod][jump 2][component 2][n]
+  double* __restrict__ ujs_out;
+  double* __restrict__ ujt_out;
 };
 #define GM_LOGREC 20
 // work counters (GmBatch::unit_counter): persistent toroidal / spheroidal

@@ -215,6 +215,43 @@ module geminiFKc
          type(c_ptr), value :: status
          type(c_ptr), value :: stats
       end function gm_gfk_batch_wn
+      integer(c_int) function gm_ujgf_batch(ctx, nfreq, freqs, lmin, nl, sigma, eps, attmode, jre, flags, &
+          ujs, ujt, status, stats) bind(C, name='gm_ujgf_batch')
+         import
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: nfreq
+         type(c_ptr), value :: freqs
+         type(c_ptr), value :: lmin
+         type(c_ptr), value :: nl
+         real(c_double), value :: sigma
+         real(c_double), value :: eps
+         integer(c_int), value :: attmode
+         integer(c_int), value :: jre
+         integer(c_int), value :: flags
+         type(c_ptr), value :: ujs
+         type(c_ptr), value :: ujt
+         type(c_ptr), value :: status
+         type(c_ptr), value :: stats
+      end function gm_ujgf_batch
+      integer(c_int) function gm_ujgf_batch_wn(ctx, nfreq, freqs, iwnmin, nwn, dwn, sigma, eps, attmode, &
+          jre, flags, ujs, ujt, status, stats) bind(C, name='gm_ujgf_batch_wn')
+         import
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: nfreq
+         type(c_ptr), value :: freqs
+         type(c_ptr), value :: iwnmin
+         type(c_ptr), value :: nwn
+         real(c_double), value :: dwn
+         real(c_double), value :: sigma
+         real(c_double), value :: eps
+         integer(c_int), value :: attmode
+         integer(c_int), value :: jre
+         integer(c_int), value :: flags
+         type(c_ptr), value :: ujs
+         type(c_ptr), value :: ujt
+         type(c_ptr), value :: status
+         type(c_ptr), value :: stats
+      end function gm_ujgf_batch_wn
       integer(c_int) function gm_aski_batch(ctx, nfreq, freqs, lmin, nl, sigma, eps, attmode, nsnod, jsnod, &
           jre, istyp, flags, gf, gfre, gfim, status, stats) bind(C, name='gm_aski_batch')
          import

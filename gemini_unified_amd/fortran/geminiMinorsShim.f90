@@ -1,9 +1,11 @@
-! Drop-in replacements of the reference's toroidalMinors and spheroidalMinors
-! modules (f90/toroidalMinors.f90, f90/spheroidalMinors.f90): the same module,
+! Drop-in replacements of the reference's toroidalMinors, spheroidalMinors and
+! unitJumpGreenFunctions modules (f90/toroidalMinors.f90, f90/spheroidalMinors.f90,
+! f90/unitJumpGreenFunctions.f90): the same module,
 ! type and procedure names, the same arguments and the same components filled,
-! so that the degree loops that call computeToroidalMinors and
-! computeSpheroidalMinors per (omega, l) (computeGreenFKSpectraForSynthetics.f90:
-! 419-425, computeGreenFKSpectraForASKI.f90:479-485) compile and run unchanged.
+! so that the degree loops that call computeToroidalMinors,
+! computeSpheroidalMinors and computeUnitJumpGreenFunctions per (omega, l)
+! (computeGreenFKSpectraForSynthetics.f90:419-430, computeGreenFKSpectraForASKI.f90:
+! 479-511) compile and run unchanged.
 ! The minors of one (omega, l) are served from a batch that libgemini_fk.so
 ! computes on the GPU for a block of consecutive degrees (or wavenumbers) of
 ! the same frequency, source node range and accuracy: the first call of a
@@ -18,8 +20,8 @@ module geminiMinorsCache
    use geminiFKc
    implicit none
    private
-   public :: gpuMinorsInit, gpuMinorsFetch, gpuMinorsNnod, gpuMinorsError, gpuMinorsSolid
-   public :: c_ut, c_dt, c_dett, c_us, c_ds, c_ulg, c_dlg, c_dets, c_status
+   public :: gpuMinorsInit, gpuMinorsFetch, gpuMinorsNnod, gpuMinorsError, gpuMinorsSolid, gpuUjgfFetch
+   public :: c_ut, c_dt, c_dett, c_us, c_ds, c_ulg, c_dlg, c_dets, c_status, c_ujs, c_ujt, c_ustatus
    ! pi of the reference's omega = 2 pi f (mathConstants.f90:28, single precision)
    real(c_double), parameter :: pi32 = real(3.141592653589793, c_double)
    type(c_ptr), save :: ctx = c_null_ptr
@@ -34,6 +36,11 @@ module geminiMinorsCache
    complex(c_double_complex), allocatable, target, save :: c_us(:,:,:), c_ds(:,:,:), c_dets(:,:)
    real(c_double), allocatable, target, save :: c_ulg(:,:), c_dlg(:,:)
    integer(c_int), allocatable, target, save :: c_status(:)
+   ! the unit-jump Green functions of a block: C [nnod][jump][comp][k] seen as (k, comp, jump, nnod)
+   real(c_double), save :: u_omre = -1.0_c_double, u_omim = 0.0_c_double, u_eps = -1.0_c_double
+   integer, save :: u_jn = 0, u_first = 0, u_count = 0
+   complex(c_double_complex), allocatable, target, save :: c_ujs(:,:,:,:), c_ujt(:,:,:,:)
+   integer(c_int), allocatable, target, save :: c_ustatus(:)
    character(len=256), save :: last = ''
 contains
    ! the context (model and nodes set), the external nodes' states, earth radius,
@@ -50,7 +57,7 @@ contains
       allocate(nstate(nnod))
       nstate = the_state
       if (present(the_block)) block = max(1, the_block)
-      k_count = 0
+      k_count = 0; u_count = 0
    end subroutine gpuMinorsInit
 
    integer function gpuMinorsNnod() result(n)
@@ -79,11 +86,7 @@ contains
       if (.not. c_associated(ctx)) then
          last = 'gpuMinorsInit was not called'; return
       endif
-      if (dwn > 0.0_c_double) then
-         deg = nint(sqrt(dll1) / (dwn * rearth)) + 1              ! wavenumber index iwn
-      else
-         deg = nint((-1.0_c_double + sqrt(1.0_c_double + 4.0_c_double * dll1)) / 2.0_c_double)   ! degree l
-      endif
+      deg = degreeOf(dll1)
       if (k_count > 0 .and. omre == k_omre .and. omim == k_omim .and. eps == k_eps .and. jsl == k_jsl .and. &
           jsr == k_jsr .and. deg >= k_first .and. deg < k_first + k_count) then
          k = deg - k_first + 1
@@ -115,6 +118,57 @@ contains
       k_omre = omre; k_omim = omim; k_eps = eps; k_jsl = jsl; k_jsr = jsr; k_first = deg; k_count = block
       k = 1
    end function gpuMinorsFetch
+
+   ! degree (or wavenumber index) of dll1
+   integer function degreeOf(dll1) result(deg)
+      real(c_double), intent(in) :: dll1
+      if (dwn > 0.0_c_double) then
+         deg = nint(sqrt(dll1) / (dwn * rearth)) + 1
+      else
+         deg = nint((-1.0_c_double + sqrt(1.0_c_double + 4.0_c_double * dll1)) / 2.0_c_double)
+      endif
+   end function degreeOf
+
+   ! index k of (omre, omim, dll1) in the unit-jump Green function cache of a jump
+   ! at node jn, after filling it with gm_ujgf_batch if needed; 0 on an error
+   integer function gpuUjgfFetch(omre, omim, dll1, jn, eps) result(k)
+      real(c_double), intent(in) :: omre, omim, dll1, eps
+      integer, intent(in) :: jn
+      integer :: deg, rc
+      integer(c_int), target :: lmin(1), nl(1)
+      real(c_double), target :: freq(1)
+      integer(c_long_long), target :: stats(GM_NSTAT)
+      k = 0
+      if (.not. c_associated(ctx)) then
+         last = 'gpuMinorsInit was not called'; return
+      endif
+      deg = degreeOf(dll1)
+      if (u_count > 0 .and. omre == u_omre .and. omim == u_omim .and. eps == u_eps .and. jn == u_jn .and. &
+          deg >= u_first .and. deg < u_first + u_count) then
+         k = deg - u_first + 1
+         return
+      endif
+      u_count = 0
+      if (allocated(c_ujs)) deallocate(c_ujs, c_ujt, c_ustatus)
+      allocate(c_ujs(block, 4, 4, nnod), c_ujt(block, 2, 2, nnod), c_ustatus(block))
+      freq(1) = omre / (2.0_c_double * pi32)
+      lmin(1) = deg; nl(1) = block
+      stats = 0
+      if (dwn > 0.0_c_double) then
+         rc = gm_ujgf_batch_wn(ctx, 1_c_int, c_loc(freq), c_loc(lmin), c_loc(nl), dwn, -omim, eps, &
+                               int(attmode, c_int), int(jn, c_int), 0_c_int, c_loc(c_ujs), c_loc(c_ujt), &
+                               c_loc(c_ustatus), c_loc(stats))
+      else
+         rc = gm_ujgf_batch(ctx, 1_c_int, c_loc(freq), c_loc(lmin), c_loc(nl), -omim, eps, int(attmode, c_int), &
+                            int(jn, c_int), 0_c_int, c_loc(c_ujs), c_loc(c_ujt), c_loc(c_ustatus), c_loc(stats))
+      endif
+      if (rc /= GM_OK .and. all(c_ustatus == 0)) then
+         write(last, '(a,i0)') 'gm_ujgf_batch failed with status ', rc
+         return
+      endif
+      u_omre = omre; u_omim = omim; u_eps = eps; u_jn = jn; u_first = deg; u_count = block
+      k = 1
+   end function gpuUjgfFetch
 end module geminiMinorsCache
 
 module toroidalMinors
@@ -282,3 +336,72 @@ contains
       doneSpheroidalMinors = this%done
    end function doneSpheroidalMinors
 end module spheroidalMinors
+
+module unitJumpGreenFunctions
+   use geminiIntegrationEnvironment
+   use errorMessage
+   use toroidalMinors
+   use spheroidalMinors
+   use geminiMinorsCache
+   implicit none
+   interface dealloc; module procedure deallocUnitJumpGreenFunctions; end interface
+   type unit_jump_green_functions
+      double complex, dimension(:,:,:), allocatable :: ujgfsph         ! (4, njs, nnod)
+      double complex, dimension(:,:,:), allocatable :: ujgftor         ! (2, njt, nnod), none for a jump in the ocean
+   end type
+contains
+   ! computeUnitJumpGreenFunctions (unitJumpGreenFunctions.f90:62-380) from the GPU
+   ! cache: the batch integrates the minors and the spheroidal adjoints of its own,
+   ! toromin and spheromin only have to be done, as the reference requires
+   subroutine computeUnitJumpGreenFunctions(this, toromin, spheromin, jnods, gem_intenv, eps, secrecy, errmsg)
+      type (unit_jump_green_functions) :: this
+      type (toroidal_minors) :: toromin
+      type (spheroidal_minors) :: spheromin
+      integer :: jnods
+      type (gemini_integration_environment) :: gem_intenv
+      double precision :: eps
+      integer :: secrecy
+      type (error_message) :: errmsg
+      character(len=30) :: myname = 'computeUnitJumpGreenFunctions'
+      integer :: k, n, nn, j, njs, njt
+      if (.not. .done.toromin) then
+         call add(errmsg, 2, 'Toroidal minors not yet computed', myname); return
+      endif
+      if (.not. .done.spheromin) then
+         call add(errmsg, 2, 'Spheroidal minors not yet computed', myname); return
+      endif
+      nn = gpuMinorsNnod()
+      if (jnods < 1 .or. jnods > nn) then
+         call add(errmsg, 2, 'Invalid index for source node', myname); return
+      endif
+      k = gpuUjgfFetch(gem_intenv%omre, gem_intenv%omim, gem_intenv%dll1, jnods, eps)
+      if (k == 0) then
+         call add(errmsg, 2, trim(gpuMinorsError()), myname); return
+      endif
+      if (c_ustatus(k) /= 0) then
+         call add(errmsg, 2, 'Unit jump Green functions failed on the GPU', myname); return
+      endif
+      ! a jump in the ocean: (1,0) and (0,1) in (U, R), no toroidal jumps (:118-123)
+      njs = 4; njt = 2
+      if (.not. gpuMinorsSolid(jnods)) then
+         njs = 2; njt = 0
+      endif
+      call deallocUnitJumpGreenFunctions(this)
+      allocate(this%ujgfsph(4, njs, nn))
+      if (njt > 0) allocate(this%ujgftor(2, njt, nn))
+      do n = 1, nn
+         do j = 1, njs
+            this%ujgfsph(:, j, n) = c_ujs(k, :, j, n)
+         enddo
+         do j = 1, njt
+            this%ujgftor(:, j, n) = c_ujt(k, :, j, n)
+         enddo
+      enddo
+   end subroutine computeUnitJumpGreenFunctions
+
+   subroutine deallocUnitJumpGreenFunctions(this)
+      type (unit_jump_green_functions) :: this
+      if (allocated(this%ujgfsph)) deallocate(this%ujgfsph)
+      if (allocated(this%ujgftor)) deallocate(this%ujgftor)
+   end subroutine deallocUnitJumpGreenFunctions
+end module unitJumpGreenFunctions

@@ -246,6 +246,30 @@ class GreenFKEngine:
             check(self.h, rc)
         return dict(gf=gf, gfre=gfre, gfim=gfim, status=status, stats=st)
 
+    def ujgf(self, freqs, lmin, nl, sigma, eps, jre, attmode="ATTENUATION_AND_DISPERSION", raise_on_error=True,
+             dwn=None):
+        """Unit-jump Green functions of a jump at node jre (gm_ujgf_batch, computeUnitJumpGreenFunctions):
+        ujs[node, jump, comp, k] (4 x 4) and ujt[node, jump, comp, k] (2 x 2) complex128."""
+        freqs, lmin, nl, nk = self._plan(freqs, lmin, nl)
+        nn = self.nodes.nnod
+        ujs = np.zeros((nn, 4, 4, nk), np.complex128)
+        ujt = np.zeros((nn, 2, 2, nk), np.complex128)
+        status = np.zeros(nk, np.int32)
+        st = np.zeros(10, np.int64)
+        am = ATTMODES[attmode] if isinstance(attmode, str) else int(attmode)
+        if dwn:
+            rc = _lib.lib().gm_ujgf_batch_wn(self.h, freqs.size, freqs.ctypes.data_as(D), lmin.ctypes.data_as(I),
+                                             nl.ctypes.data_as(I), float(dwn), float(sigma), float(eps), am, int(jre),
+                                             0, _dp(ujs), _dp(ujt), status.ctypes.data_as(I), st.ctypes.data_as(LL))
+        else:
+            rc = _lib.lib().gm_ujgf_batch(self.h, freqs.size, freqs.ctypes.data_as(D), lmin.ctypes.data_as(I),
+                                          nl.ctypes.data_as(I), float(sigma), float(eps), am, int(jre), 0, _dp(ujs),
+                                          _dp(ujt), status.ctypes.data_as(I), st.ctypes.data_as(LL))
+        self.stats += st
+        if raise_on_error:
+            check(self.h, rc)
+        return dict(ujs=ujs, ujt=ujt, status=status, stats=st)
+
     def aski(self, freqs, lmin, nl, sigma, eps, jsnod, jre, istyp=1, attmode="ATTENUATION_AND_DISPERSION",
              want_fp32=True, raise_on_error=True, want_gf=True):
         """Multi-source columns of computeGreenFKSpectraForASKI (gm_aski_batch): for the sources at the

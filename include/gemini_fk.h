@@ -194,6 +194,23 @@ int gm_gfk_batch_wn(gm_ctx* ctx, int nfreq, const double* freqs, const int* iwnm
                     double sigma, double eps, int attmode, int jre, int istyp, int flags, double* gf, float* gfre,
                     float* gfim, int* status, long long* stats);
 
+/* The unit-jump Green functions of a jump at node jre themselves, for every
+ * instance: computeUnitJumpGreenFunctions (f90/unitJumpGreenFunctions.f90:62-380)
+ * after the minors, with the spheroidal adjoints.  Arguments as gm_gfk_batch
+ * (no source type); outputs ujs complex [nnod][jump 4][component 4][k]
+ * (ujgfsph(4, njs, nnod) per instance, njs = 4, or 2 for a jump in the ocean
+ * with jumps 3, 4 zero) and ujt complex [nnod][jump 2][component 2][k]
+ * (ujgftor, zero for a jump in the ocean), zero for an instance whose status
+ * is not 0.  The per-(omega, l) Fortran module unitJumpGreenFunctions serves
+ * the reference's calls from it. */
+int gm_ujgf_batch(gm_ctx* ctx, int nfreq, const double* freqs, const int* lmin, const int* nl, double sigma,
+                  double eps, int attmode, int jre, int flags, double* ujs, double* ujt, int* status,
+                  long long* stats);
+/* gm_ujgf_batch over wavenumber indices (regional mode, as gm_gfk_batch_wn). */
+int gm_ujgf_batch_wn(gm_ctx* ctx, int nfreq, const double* freqs, const int* iwnmin, const int* nwn, double dwn,
+                     double sigma, double eps, int attmode, int jre, int flags, double* ujs, double* ujt,
+                     int* status, long long* stats);
+
 /* Multi-source Green-FK columns with radial derivatives: the degree loop body
  * of computeGreenFKSpectraForASKI (f90/computeGreenFKSpectraForASKI.f90:
  * 458-590).  Minors over the source node range min(jsnod)..max(jsnod)

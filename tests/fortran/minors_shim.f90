@@ -7,7 +7,7 @@
 !   minors_shim minors <in> <out> model, nodes and one frequency from <in> (gfk_call's format);
 !                                 per degree l = 0..nwn-1 into <out> (stream): level, u_tor(2,nnod),
 !                                 d_tor, det_tor(jre), u_minors(6,nnod), d_minors, det_sph(jre),
-!                                 u_mtil(4,4,nnod), d_meps(4,4,nnod)
+!                                 u_mtil(4,4,nnod), d_meps(4,4,nnod), ujgfsph(4,4,nnod), ujgftor(2,2,nnod)
 program minors_shim
    use iso_c_binding
    use geminiFKc
@@ -17,12 +17,14 @@ program minors_shim
    use geminiMinorsCache
    use toroidalMinors
    use spheroidalMinors
+   use unitJumpGreenFunctions
    implicit none
    type(c_ptr) :: ctx
    type (gemini_integration_environment) :: env
    type (error_message) :: errmsg
    type (toroidal_minors) :: toromin
    type (spheroidal_minors) :: spheromin
+   type (unit_jump_green_functions) :: ujgf
    integer(c_int) :: rc, nk, nlay, nic, noc, fsflag, equidis, nnod, nwn, attmode, jre, istyp, i, l
    real(c_double) :: rearth, ric, roc, fref, f, sigma, eps, wn
    real(c_double), allocatable :: rk(:), ro(:), vpv(:), vph(:), vsv(:), vsh(:), eta(:), qkinv(:), qminv(:), rb(:)
@@ -80,12 +82,14 @@ program minors_shim
       env%dll1 = (wn * rearth)**2
       call computeToroidalMinors(toromin, jre, jre, env, eps, 0, errmsg)
       if (.level.errmsg /= 2) call computeSpheroidalMinors(spheromin, jre, jre, env, eps, 0, errmsg)
+      if (.level.errmsg /= 2) call computeUnitJumpGreenFunctions(ujgf, toromin, spheromin, jre, env, eps, 0, errmsg)
       write(11) .level.errmsg
       if (.level.errmsg == 2) exit
       write(11) toromin%u_tor, toromin%d_tor, toromin%det(jre), spheromin%u_minors, spheromin%d_minors, &
-                spheromin%det(jre), spheromin%u_mtil, spheromin%d_meps
+                spheromin%det(jre), spheromin%u_mtil, spheromin%d_meps, ujgf%ujgfsph, ujgf%ujgftor
       call dealloc(toromin)
       call dealloc(spheromin)
+      call dealloc(ujgf)
    enddo
    close(11)
    rc = gm_ctx_destroy(ctx)

@@ -120,9 +120,10 @@ def _model_input(nem, nodes, f, nwn, sigma, eps, jre):
 @pytest.mark.gpu
 @needs_flang
 def test_minors_shims_serve_the_degree_loop(tmp_path):
-    """The reference's degree loop calls computeToroidalMinors and computeSpheroidalMinors per degree;
-    the drop-in modules fill toromin%u_tor, d_tor, det and spheromin%u_minors, d_minors, det and the
-    minors' matrices from GPU batches of 16 degrees (exact variant), equal to the oracle's minors."""
+    """The reference's degree loop calls computeToroidalMinors, computeSpheroidalMinors and
+    computeUnitJumpGreenFunctions per degree; the drop-in modules fill toromin%u_tor, d_tor, det,
+    spheromin%u_minors, d_minors, det, the minors' matrices and ujgf%ujgfsph, ujgftor from GPU
+    batches of 16 degrees (exact variant), equal to the oracle's."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     from gemini_unified_amd import MC_PI, ExternalRadialNodes, dll1_of, global_model
@@ -159,6 +160,8 @@ def test_minors_shims_serve_the_degree_loop(tmp_path):
         dets = take(1, np.complex128)[0]
         mtil = take(16 * nn, np.complex128).reshape(nn, 4, 4).transpose(0, 2, 1)     # Fortran (4, 4, nnod)
         meps = take(16 * nn, np.complex128).reshape(nn, 4, 4).transpose(0, 2, 1)
+        ujs = take(16 * nn, np.complex128).reshape(nn, 4, 4)                         # Fortran (4, njs, nnod)
+        ujt = take(4 * nn, np.complex128).reshape(nn, 2, 2)
         om, dll1 = 2 * MC_PI * f, dll1_of(l, nem.rearth)
         rc, u, d, det = orc.toroidal_minors(om, -sigma, dll1, jre, jre, eps)
         assert rc == 0
@@ -180,4 +183,9 @@ def test_minors_shims_serve_the_degree_loop(tmp_path):
                     want_e[i, j], want_e[j, i] = vf * d[n, 5 - m], -vf * d[n, 5 - m]
             np.testing.assert_allclose(mtil[n], want_t, rtol=1e-12, atol=1e-12 * np.abs(u).max())
             np.testing.assert_allclose(meps[n], want_e, rtol=1e-12, atol=1e-12 * np.abs(d).max())
+        # computeUnitJumpGreenFunctions: ujgfsph(comp, jump, node), ujgftor, as the oracle's column
+        rc, gf, ref_s, ref_t, flag = orc.gfk_column(om, -sigma, dll1, jre, 1, eps)
+        assert rc == 0
+        np.testing.assert_allclose(ujs, ref_s, rtol=1e-11, atol=1e-11 * np.abs(ref_s).max())
+        np.testing.assert_allclose(ujt, ref_t, rtol=1e-11, atol=1e-11 * max(np.abs(ref_t).max(), 1e-300))
     assert pos == len(raw)

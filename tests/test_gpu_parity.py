@@ -351,3 +351,28 @@ def test_adjoint_pairs_keep_the_error(f, lmin, nl):
     print(f"adjoint pairs f={f:.5f}: error {e1.max():.2e} against {e0.max():.2e} with four adjoints", flush=True)
     assert e1.max() <= 1.1 * e0.max() + 1e-9
     assert rel_l2(r1["gf"], r0["gf"]).max() < 1e-4
+
+
+@pytest.mark.parametrize("jre", [5, 6, 8])
+def test_ujgf_batch_matches_oracle(jre):
+    """gm_ujgf_batch: the unit-jump Green functions themselves (computeUnitJumpGreenFunctions) for
+    jumps below the ocean bottom (5) and in the ocean (6, 8), every node, exact variant."""
+    from ocean_model import ocean_model
+    from test_oracle_ocean import ocean_receivers
+    nem = ocean_model()
+    nodes = ocean_receivers(nem)
+    f, lmin, nl = 20 / TLEN, 0, 10
+    orc = oracle.Oracle(nem, nodes)
+    orc.set_frequency(f, 3)
+    eng = GreenFKEngine(nem, nodes, variant="exact")
+    print(f"ujgf batch jre={jre} f={f:.5f} l={lmin}..{lmin + nl - 1}", flush=True)
+    r = eng.ujgf([f], lmin, nl, SIGMA, EPS, jre)
+    eng.close()
+    assert (r["status"] == 0).all()
+    for il in range(nl):
+        rc, gf, ujs, ujt, flag = orc.gfk_column(2 * MC_PI * f, -SIGMA, dll1_of(lmin + il, nem.rearth), jre, 1, EPS)
+        assert rc == 0
+        np.testing.assert_allclose(r["ujs"][..., il], ujs, rtol=1e-11, atol=1e-11 * np.abs(ujs).max())
+        np.testing.assert_allclose(r["ujt"][..., il], ujt, rtol=1e-11, atol=1e-11 * max(np.abs(ujt).max(), 1e-300))
+    if nodes.state[jre - 1] == 1:
+        assert not r["ujs"][:, 2:].any() and not r["ujt"].any()
