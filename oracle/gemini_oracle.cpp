@@ -1548,8 +1548,11 @@ static void horcompFluid(const Env& env, double r, int nj, cx gf[5][5], cx v[5],
 // For a receiver in the ocean only rows 1-2 of the reciprocal Green functions
 // enter, and spheroidalReciprocity's rows 1-2 for a source in the ocean
 // (reciprocity.f90:99-103, 119-128) are those for a source in the solid.
+// dsvstep 2: U, V (and W) per node; 1 (STRESSFLAG, :209-212): U, R, V, S and W, T, with the
+// spheroidal mask (1, 1, 1, 1) also for a receiver in the ocean, whose S slots stay zero (:313)
 static int gfkColumn(const Ctx& c, double omre, double omim, double dll1, int jre, int istyp, double eps,
-                     Column& col, Stats& st, ToroidalMinors* Tout = nullptr, SpheroidalMinors* Sout = nullptr) {
+                     Column& col, Stats& st, ToroidalMinors* Tout = nullptr, SpheroidalMinors* Sout = nullptr,
+                     int dsvstep = 2) {
   Env env; env.m = &c.m; env.nod = &c.nod; env.s = &c.s; env.omre = omre; env.omim = omim; env.dll1 = dll1; env.nl = 1;
   const Nodes& N = c.nod;
   int nnod = N.nnod;
@@ -1565,9 +1568,11 @@ static int gfkColumn(const Ctx& c, double omre, double omim, double dll1, int jr
   rc = unitJumpGreen(env, T, S, jre, eps, col, st);
   if (rc != OK) return rc;
   int njsph = istyp == 1 ? 4 : 2, njtor = istyp == 1 ? 2 : 1;
-  const int ncsph = ocean ? 2 : 4, nctor = ocean ? 0 : 2, dsvstep = 2;
-  // U and V of the spheroidal jumps (V from (U, R) for a receiver in the ocean), W of the toroidal ones
-  col.nsp = 2 * njsph + (ocean ? 0 : njtor);
+  const int ncsph = ocean ? 2 : 4, nctor = ocean ? 0 : 2;
+  // U and V of the spheroidal jumps (V from (U, R) for a receiver in the ocean), W of the toroidal
+  // ones; with stresses also R, S and T
+  const int nmask = dsvstep == 1 ? 4 : 2;
+  col.nsp = nmask * njsph + (ocean ? 0 : (nmask / 2) * njtor);
   col.gf.assign(col.nsp * nnod, C(0.0));
   double re = N.rnod[jre];
   for (int n = 1; n <= nnod; ++n) {
@@ -1971,12 +1976,13 @@ static double instanceDll1(int l, double dwn, double rearth) {
 }
 
 static int gfkInstances(void* h, double omre, double omim, double rearth, int lmin, int nl, double dwn, int jre,
-                        int istyp, double eps, int nthreads, double* gf, int* status, int64_t* stats) {
+                        int istyp, double eps, int nthreads, double* gf, int* status, int64_t* stats,
+                        int dsvstep = 2) {
   Ctx* c = (Ctx*)h;
   int nnod = c->nod.nnod;
   if (jre < 1 || jre > nnod) return ERR_NODEIDX;
-  const int njsph = istyp == 1 ? 4 : 2, njtor = istyp == 1 ? 2 : 1;
-  const int nsp = 2 * njsph + (c->nod.state[jre] == 1 ? 0 : njtor);
+  const int njsph = istyp == 1 ? 4 : 2, njtor = istyp == 1 ? 2 : 1, nmask = dsvstep == 1 ? 4 : 2;
+  const int nsp = nmask * njsph + (c->nod.state[jre] == 1 ? 0 : (nmask / 2) * njtor);
   Stats total;
 #ifdef _OPENMP
   if (nthreads < 1) nthreads = 1;
@@ -1991,7 +1997,7 @@ static int gfkInstances(void* h, double omre, double omim, double rearth, int lm
       int l = lmin + il;
       double dll1 = instanceDll1(l, dwn, rearth);
       Column col;
-      int rc = gfkColumn(*c, omre, omim, dll1, jre, istyp, eps, col, st);
+      int rc = gfkColumn(*c, omre, omim, dll1, jre, istyp, eps, col, st, nullptr, nullptr, dsvstep);
       status[il] = rc;
       for (int n = 0; n < nnod; ++n)
         for (int isp = 0; isp < nsp; ++isp) {
@@ -2014,6 +2020,12 @@ static int gfkInstances(void* h, double omre, double omim, double rearth, int lm
 int orc_gfk_degrees(void* h, double omre, double omim, double rearth, int lmin, int nl, int jre, int istyp,
                     double eps, int nthreads, double* gf, int* status, int64_t* stats) {
   return gfkInstances(h, omre, omim, rearth, lmin, nl, 0.0, jre, istyp, eps, nthreads, gf, status, stats);
+}
+
+// the same columns with stresses (STRESSFLAG: dsvstep = 1)
+int orc_gfk_degrees_stress(void* h, double omre, double omim, double rearth, int lmin, int nl, int jre, int istyp,
+                           double eps, int nthreads, double* gf, int* status, int64_t* stats) {
+  return gfkInstances(h, omre, omim, rearth, lmin, nl, 0.0, jre, istyp, eps, nthreads, gf, status, stats, 1);
 }
 
 // regional mode: 1-based wavenumber indices iwn = iwnmin .. iwnmin+nwn-1, k = (iwn-1) dwn

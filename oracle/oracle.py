@@ -58,6 +58,8 @@ def lib():
                                         C.c_double, D, D, D, I, L]
         _lib.orc_gfk_degrees.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                                          C.c_int, C.c_int, C.c_double, C.c_int, D, I, L]
+        _lib.orc_gfk_degrees_stress.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int,
+                                                C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, D, I, L]
         _lib.orc_propagate_const.argtypes = [C.c_int, D, C.c_double, C.c_double, C.c_double, C.c_double, D, D, L]
         _lib.orc_num_threads.restype = C.c_int
         _lib.orc_reciprocity_sph.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, D, D]
@@ -226,6 +228,18 @@ class Oracle:
         status = np.zeros(nl, np.int32)
         rc = lib().orc_gfk_degrees(self.h, omre, omim, self.nem.rearth, lmin, nl, jre, istyp, eps, nthreads,
                                    _c(gf), status.ctypes.data_as(I), self.stats.ctypes.data_as(L))
+        return rc, gf, status
+
+    def gfk_degrees_stress(self, omre, omim, lmin, nl, jre, istyp, eps, nthreads=1):
+        """Columns with stresses (STRESSFLAG): gf[n, isp, il] with U, R, V, S (x njsph jumps), W, T (x njtor);
+        U, R, V and zero S slots for a receiver in the ocean."""
+        n = self.nodes.nnod
+        njsph, njtor = (4, 2) if istyp == 1 else (2, 1)
+        nsp = 4 * njsph + (0 if self.nodes.state[jre - 1] == 1 else 2 * njtor)
+        gf = np.zeros((n, nsp, nl), np.complex128)
+        status = np.zeros(nl, np.int32)
+        rc = lib().orc_gfk_degrees_stress(self.h, omre, omim, self.nem.rearth, lmin, nl, jre, istyp, eps, nthreads,
+                                          _c(gf), status.ctypes.data_as(I), self.stats.ctypes.data_as(L))
         return rc, gf, status
 
     def aski_degrees(self, omre, omim, lmin, nl, jsnod, jre, istyp, eps, nthreads=1):

@@ -195,3 +195,48 @@ def test_ocean_sources_equal_the_reciprocal_ones(f, l):
                     assert not gfa[js, :, 2 * 2 * nj:, 0].any()     # no toroidal motion, 8 spectra
                 checked += 1
     assert checked == 4 * 3 + 3
+
+
+@pytest.mark.parametrize("f,l", [(20 / 2048.0, 3), (50 / 2048.0, 33)])
+def test_stress_columns_follow_from_the_direct_derivatives(f, l):
+    """Columns with stresses (STRESSFLAG, computeGreenFKSpectraForSynthetics.f90:209-212, 468-489):
+    R, S and T of a receiver at node jr for a source at another source node equal those the solid
+    and toroidal systems give from the multi-source driver's direct U, V, W and their radial
+    derivatives there, U' = A11 U + A12 R + A13 V, V' = A31 U + A33 V + A34 S, W' = A W + B T;
+    U, V and W are the columns without stresses."""
+    nem, nodes, jre, jsnod = _aski_case()
+    orc = oracle.Oracle(nem, nodes)
+    orc.set_frequency(f, 3)
+    om, sig, eps = 2 * MC_PI * f, -5 / 2048.0, 1e-11
+    dll1 = dll1_of(l, nem.rearth)
+    rc, gfa, st = orc.aski_degrees(om, sig, l, 1, jsnod, jre, 1, eps)
+    assert rc == 0
+    checked = 0
+    for jr in jsnod:
+        r = nodes.rnod[jr - 1]
+        A = orc.sysmat_at(0, om, sig, dll1, r)
+        At = orc.sysmat_at(1, om, sig, dll1, r)
+        for istyp in (1, 0):
+            rc, gs, _ = orc.gfk_degrees_stress(om, sig, l, 1, jr, istyp, eps)
+            assert rc == 0
+            rc, g2, _ = orc.gfk_degrees(om, sig, l, 1, jr, istyp, eps)
+            njs, njt = (4, 2) if istyp == 1 else (2, 1)
+            assert gs.shape[1] == 4 * njs + 2 * njt
+            # U, V, W are those of the columns without stresses, bit for bit
+            sel = np.concatenate([np.arange(njs), 2 * njs + np.arange(njs), 4 * njs + np.arange(njt)])
+            np.testing.assert_array_equal(gs[:, sel], g2)
+            srcs = [(js, jn) for js, jn in enumerate(jsnod)] if istyp == 1 else [(len(jsnod), jre)]
+            for js, jsh in srcs:
+                if jsh == jr:
+                    continue
+                d = gfa[js, jr - 1, :, 0]
+                U, V, dU, dV = d[:njs], d[njs:2 * njs], d[2 * njs:3 * njs], d[3 * njs:4 * njs]
+                W, dW = d[4 * njs:4 * njs + njt], d[4 * njs + njt:4 * njs + 2 * njt]
+                R = (dU - A[0, 0] * U - A[0, 2] * V) / A[0, 1]
+                S = (dV - A[2, 0] * U - A[2, 2] * V) / A[2, 3]
+                T = (dW - At[0, 0] * W) / At[0, 1]
+                c = gs[jsh - 1, :, 0]
+                for got, want in ((c[njs:2 * njs], R), (c[3 * njs:4 * njs], S), (c[4 * njs + njt:], T)):
+                    assert np.abs(got - want).max() < 1e-7 * np.abs(want).max(), (jr, js, got, want)
+                checked += 1
+    assert checked == len(jsnod) * (len(jsnod) - 1) + len(jsnod) - 1
