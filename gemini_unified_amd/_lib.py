@@ -15,6 +15,12 @@ I = C.POINTER(C.c_int)
 LL = C.POINTER(C.c_longlong)
 VP = C.c_void_p
 
+# flags of the batch entry points (gemini_fk.h)
+GM_FLAG_DEVICE_PTRS = 1
+GM_FLAG_STRESS_SPECTRA = 2
+GM_FLAG_DERIV_SPECTRA = 4
+GM_FLAG_OCEAN_SPECTRA = 8
+
 # every symbol declared in include/gemini_fk.h
 EXPORTS = ["gm_ctx_create", "gm_ctx_destroy", "gm_set_option", "gm_last_error", "gm_model_set",
            "gm_nodes_set", "gm_minors_batch", "gm_gfk_batch", "gm_last_timing", "gm_last_system_stats",

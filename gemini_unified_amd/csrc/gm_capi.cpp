@@ -414,9 +414,12 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   }
   const size_t n = P.ninst;
   // spectra per node: U, V, W (moment 10, force 5), U, V for a receiver in the ocean (8, 4);
+  // with stresses U, R, V, S, W, T (20, 10; ocean U, R, V and zero S: 16, 8);
   // multi-source: U, V, dU/dr, dV/dr, W, dW/dr (20, 10)
   const bool ocean = full && m.state[jre] == 1;
-  const int nsp = aski ? (istyp == 1 ? 20 : 10) : (istyp == 1 ? 10 : 5) - (ocean ? (istyp == 1 ? 2 : 1) : 0);
+  const bool stress = full && !aski && (flags & GM_FLAG_STRESS_SPECTRA) != 0;
+  const int njsph = istyp == 1 ? 4 : 2, njtor = istyp == 1 ? 2 : 1, nmask = stress ? 4 : 2;
+  const int nsp = aski ? 2 * (2 * njsph + njtor) : nmask * njsph + (ocean ? 0 : (nmask / 2) * njtor);
   const size_t nout = (size_t)nsrc * nsp * nnod * n;
   HIPCHK(c, hipSetDevice(c->device));
   // device buffer layout
@@ -498,6 +501,7 @@ static int run(gm_ctx* c, int nfreq, const double* freqs, const int* lmin, const
   GmBatch B;
   memset(&B, 0, sizeof B);
   B.n = (int)n; B.nnod = nnod; B.jsl = jsl; B.jsr = jsr; B.jre = jre; B.istyp = istyp;
+  B.stress = stress ? 1 : 0;
   B.dwn = dwn;
   B.omim = -sigma; B.eps = eps;
   B.q = (const int*)(base + o_q); B.l = (const int*)(base + o_l); B.omre = (const double*)(base + o_omre);

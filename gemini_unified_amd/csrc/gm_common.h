@@ -89,6 +89,7 @@ __host__ __device__ inline size_t gmLdsBytes(int nk, int nlay) { return gmImageB
 struct GmBatch {
   int n;                           // number of (omega, l) instances
   int nnod, jsl, jsr, jre, istyp;
+  int stress;                      // 1: FK columns with stresses, U, R, V, S and W, T (STRESSFLAG)
   double omim, eps;
   const int* __restrict__ q;       // frequency slot per instance
   const int* __restrict__ l;       // harmonic degree (dwn == 0) or 1-based wavenumber index (dwn > 0)

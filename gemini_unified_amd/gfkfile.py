@@ -349,10 +349,11 @@ class GreenFKSpectra:
             return int(m[:isp - 1].sum()) * self.njsph + jsp
         return int(m[:6].sum()) * self.njsph + int(m[6:isp - 1].sum()) * self.njtor + jsp
 
-    def read_data_node(self, jr: int, k0: int = 0, nk: int | None = None):
+    def read_data_node(self, jr: int, k0: int = 0, nk: int | None = None, nsp: int | None = None):
         """readDataNodeGreenFKSpectra (greenFKSpectra.f90:146-167): gfkr, gfki [nsp, nkfmax] float32
-        of node jr (1-based); k0/nk select a wavenumber range."""
-        nsp = self.number_of_spectra()
+        of node jr (1-based); k0/nk select a wavenumber range.  nsp: the spectra per node where the
+        integers do not tell them (a file written with STRESSFLAG carries dsvstep = 2 there)."""
+        nsp = self.number_of_spectra() if nsp is None else int(nsp)
         nk = self.nkfmax - k0 if nk is None else nk
         gr, gi = np.zeros((nsp, nk), np.float32), np.zeros((nsp, nk), np.float32)
         _check(self.h, lib().gm_gfkfile_read_node(self.h, int(jr), nsp, int(k0), int(nk), gr.ctypes.data_as(F),
@@ -374,12 +375,15 @@ def write_green_fk_spectra_for_synthetics(engine, path: str, grid: FrequencyGrid
                                           istyp: int = 1, attmode: str = "ATTENUATION_AND_DISPERSION",
                                           earth_model_name: str = "", identity: str | None = None,
                                           global_: int = 1, dwn: float | None = None, nwnmax: int | None = None,
-                                          freq_block: int = 64):
+                                          freq_block: int = 64, stress: bool = False):
     """Computes the Green-FK spectra of every frequency of grid on the GPU and writes the GFK file
     as computeGreenFKSpectraForSynthetics does (computeGreenFKSpectraForSynthetics.f90:236-345,
     500-519), receiver node jre in a solid layer or the ocean (ncsph = 2, nctor = 0: U and V of the
     spheroidal jumps, :293-300).  Global mode: degrees l = 0 .. nwn-1; regional
     mode (global_ = 0): wavenumbers k = (iwn-1) dwn, iwn = 1 .. nwn, with k = 0 written as zero.
+    stress (STRESSFLAG, :209-212): the datasets hold U, R, V, S and W, T (20 or 10 spectra; 16 or
+    8 in the ocean), while the integers carry dsvstep = 2 as the reference's writer puts there
+    (:283-285), so a reader of such a file is given the stress layout by its caller.
     Returns the identity string."""
     nodes = engine.nodes
     nem = engine.nem
@@ -392,7 +396,7 @@ def write_green_fk_spectra_for_synthetics(engine, path: str, grid: FrequencyGrid
         raise ValueError("regional mode needs the wavenumber spacing dwn")
     njsph, njtor = (4, 2) if istyp == 1 else (2, 1)
     ncsph, nctor = (2, 0) if nodes.state[jre - 1] == 1 else (4, 2)   # receiver in the ocean or a solid
-    nsp = writer_number_of_spectra(2, ncsph, njsph, njtor)
+    nsp = writer_number_of_spectra(1 if stress else 2, ncsph, njsph, njtor)
     identity = fdate() if identity is None else identity
     ro, elcon, qinv = engine.node_properties(nem.fref, "ELASTIC")
     reals = (nem.rearth, nodes.rnod[jre - 1], grid.sigma, grid.df, dwn, nem.fref)
@@ -407,10 +411,11 @@ def write_green_fk_spectra_for_synthetics(engine, path: str, grid: FrequencyGrid
             fb = freqs[b:b + freq_block]
             nb = nwn[nf1 - 1 + b:nf1 - 1 + b + fb.size]
             if global_ == 1:
-                r = engine.gfk(fb, 0, nb, grid.sigma, eps, jre, istyp, attmode, want_gf=False)
+                r = engine.gfk(fb, 0, nb, grid.sigma, eps, jre, istyp, attmode, want_gf=False, stress=stress)
                 gre, gim = r["gfre"], r["gfim"]
             else:                          # iwn = 2 .. nwn per frequency, column iwn = 1 stays zero
-                r = engine.gfk(fb, 2, nb - 1, grid.sigma, eps, jre, istyp, attmode, dwn=dwn, want_gf=False)
+                r = engine.gfk(fb, 2, nb - 1, grid.sigma, eps, jre, istyp, attmode, dwn=dwn, want_gf=False,
+                               stress=stress)
                 cols = np.concatenate([o + 1 + np.arange(n - 1) for o, n in zip(np.cumsum(nb) - nb, nb)])
                 gre = np.zeros((nodes.nnod, nsp, int(nb.sum())), np.float32)
                 gim = np.zeros_like(gre)

@@ -207,22 +207,26 @@ class GreenFKEngine:
             check(self.h, rc)
         return out
 
-    def nsp(self, jre, istyp=1):
+    def nsp(self, jre, istyp=1, stress=False):
         """spectra per node of the columns for a receiver at node jre: U, V (x 4 / 2 jumps) and W
         (x 2 / 1), 10 or 5; 8 or 4 for a receiver in the ocean, which has no W
-        (computeGreenFKSpectraForSynthetics.f90:293-313)"""
+        (computeGreenFKSpectraForSynthetics.f90:293-313).  With stresses (STRESSFLAG, :209-212)
+        U, R, V, S and W, T: 20 or 10; 16 or 8 in the ocean (U, R, V and zero S)."""
         njsph, njtor = (4, 2) if istyp == 1 else (2, 1)
-        return 2 * njsph + (0 if self.nodes.state[int(jre) - 1] == 1 else njtor)
+        nmask = 4 if stress else 2
+        return nmask * njsph + (0 if self.nodes.state[int(jre) - 1] == 1 else nmask // 2 * njtor)
 
     def gfk(self, freqs, lmin, nl, sigma, eps, jre, istyp=1, attmode="ATTENUATION_AND_DISPERSION",
-            want_fp32=True, raise_on_error=True, dwn=None, want_gf=True):
+            want_fp32=True, raise_on_error=True, dwn=None, want_gf=True, stress=False):
         """Green-FK columns; gf[node, isp, k] complex128 (and FP32 re/im as written to the file),
-        with nsp = self.nsp(jre, istyp) spectra per node."""
+        with nsp = self.nsp(jre, istyp, stress) spectra per node; stress: the columns with
+        stresses (U, R, V, S, W, T) of STRESSFLAG."""
         freqs, lmin, nl, nk = self._plan(freqs, lmin, nl)
         nn = self.nodes.nnod
         if not 1 <= int(jre) <= nn:
             raise _lib.GeminiFKError(3, f"receiver node {jre} outside 1..{nn}")
-        nsp = self.nsp(jre, istyp)
+        nsp = self.nsp(jre, istyp, stress)
+        fl = _lib.GM_FLAG_STRESS_SPECTRA if stress else 0
         gf = np.zeros((nn, nsp, nk), np.complex128) if want_gf else None
         gfre = np.zeros((nn, nsp, nk), np.float32) if want_fp32 else None
         gfim = np.zeros((nn, nsp, nk), np.float32) if want_fp32 else None
@@ -234,11 +238,11 @@ class GreenFKEngine:
         if dwn:
             rc = _lib.lib().gm_gfk_batch_wn(self.h, freqs.size, freqs.ctypes.data_as(D), lmin.ctypes.data_as(I),
                                             nl.ctypes.data_as(I), float(dwn), float(sigma), float(eps), am, int(jre),
-                                            int(istyp), 0, _dp(gf), fre, fim, status.ctypes.data_as(I),
+                                            int(istyp), fl, _dp(gf), fre, fim, status.ctypes.data_as(I),
                                             st.ctypes.data_as(LL))
         else:
             rc = _lib.lib().gm_gfk_batch(self.h, freqs.size, freqs.ctypes.data_as(D), lmin.ctypes.data_as(I),
-                                         nl.ctypes.data_as(I), float(sigma), float(eps), am, int(jre), int(istyp), 0,
+                                         nl.ctypes.data_as(I), float(sigma), float(eps), am, int(jre), int(istyp), fl,
                                          _dp(gf), fre, fim, status.ctypes.data_as(I),
                                          st.ctypes.data_as(LL))
         self.stats += st

@@ -160,7 +160,11 @@ int gm_minors_batch(gm_ctx* ctx, int nfreq, const double* freqs, const int* lmin
  * A receiver in the ocean (a fluid node of the top layer, :293-300): nsp = 8 or
  * 4, U and V = -R / (r om^2 rho) of the spheroidal jumps, r and rho taken at the
  * node as the reference does (:465); a receiver in the fluid core returns
- * GM_ERR_UNSUPPORTED.  Any output may be NULL. */
+ * GM_ERR_UNSUPPORTED.  GM_FLAG_STRESS_SPECTRA in flags: the columns with
+ * stresses (STRESSFLAG, dsvstep = 1, :209-212), U, R, V, S (x njsph) and W, T
+ * (x njtor), nsp = 20 or 10; in the ocean U, R, V and njsph zero S slots,
+ * nsp = 16 or 8, of which a reader takes the first 12 or 6
+ * (greenFKSpectra.f90:107-133).  Any output may be NULL. */
 int gm_gfk_batch(gm_ctx* ctx, int nfreq, const double* freqs, const int* lmin, const int* nl, double sigma,
                  double eps, int attmode, int jre, int istyp, int flags, double* gf, float* gfre, float* gfim,
                  int* status, long long* stats);

@@ -42,6 +42,7 @@ def lib():
         L.sim_proposer_new.argtypes = [F, C.c_float, C.c_int, C.c_int]
         L.sim_proposer_free.argtypes = [C.c_void_p]
         L.sim_set_adjoint_pair.argtypes = [C.c_int]
+        L.sim_set_stress.argtypes = [C.c_int]
         L.sim_set_sources.argtypes = [C.c_int, I]
         for v in VARIANTS:
             fn = getattr(L, "sim_run_" + v)
@@ -105,8 +106,9 @@ class HostSim:
         lmin = np.ascontiguousarray(np.broadcast_to(lmin, (nf,)), dtype=np.int32)
         nl = np.ascontiguousarray(np.broadcast_to(nl, (nf,)), dtype=np.int32)
         n, nn = int(nl.sum()), self.nodes.nnod
-        nsp = (10 if istyp == 1 else 5) - (((2 if istyp == 1 else 1) if self.nodes.state[jre - 1] == 1 else 0)
-                                           if full else 0)
+        nmask = 4 if (full and self._stress) else 2
+        njs, njt = (4, 2) if istyp == 1 else (2, 1)
+        nsp = nmask * njs + (0 if (full and self.nodes.state[jre - 1] == 1) else nmask // 2 * njt)
         nsrc = 1
         if self._sources is not None:
             nsp, nsrc = (20 if istyp == 1 else 10), len(self._sources) + 1
@@ -121,6 +123,7 @@ class HostSim:
         nlog = C.c_int(0)
         o["policy"] = np.zeros((n, 2), np.float32)
         dp = lambda a: None if a is None else a.ctypes.data_as(D)
+        lib().sim_set_stress(1 if (full and self._stress and self._sources is None) else 0)
         fn = getattr(lib(), "sim_run_" + self.variant)
         rc = fn(self.m, nf, freqs.ctypes.data_as(D), lmin.ctypes.data_as(I), nl.ctypes.data_as(I), float(dwn),
                 float(sigma), float(eps), int(attmode), int(jsl), int(jsr), int(jre), int(istyp), int(full), int(mode),
@@ -137,8 +140,14 @@ class HostSim:
     def minors(self, freqs, lmin, nl, sigma, eps, jsl, jsr, dwn=0.0, learned=False):
         return self._run(freqs, lmin, nl, sigma, eps, jsl, jsr, jsr, 1, False, 2 if learned else 0, dwn)
 
-    def gfk(self, freqs, lmin, nl, sigma, eps, jre, istyp=1, dwn=0.0, learned=False):
-        return self._run(freqs, lmin, nl, sigma, eps, jre, jre, jre, istyp, True, 2 if learned else 0, dwn)
+    _stress = False
+
+    def gfk(self, freqs, lmin, nl, sigma, eps, jre, istyp=1, dwn=0.0, learned=False, stress=False):
+        self._stress = stress
+        try:
+            return self._run(freqs, lmin, nl, sigma, eps, jre, jre, jre, istyp, True, 2 if learned else 0, dwn)
+        finally:
+            self._stress = False
 
     def aski(self, freqs, lmin, nl, sigma, eps, jsnod, jre, istyp=1):
         """Multi-source columns (gm_aski_batch): gf[js, node, isp, instance]."""

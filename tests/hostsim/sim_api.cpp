@@ -58,6 +58,8 @@ extern "C" void sim_proposer_free(GmProposer* p) { delete p; }
 extern "C" {
 int g_sim_adj_pair = 0;
 void sim_set_adjoint_pair(int v) { g_sim_adj_pair = v; }
+int g_sim_stress = 0;
+void sim_set_stress(int v) { g_sim_stress = v; }
 // multi-source runs (gm_aski_batch): nsnod >= 1 source nodes, -1 for the receiver-only batch
 int g_sim_nsnod = -1;
 int g_sim_jsnod[GM_MAXN + 1];

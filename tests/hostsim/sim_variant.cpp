@@ -5,7 +5,8 @@
 #include <algorithm>
 #include <vector>
 #include "gm_common.h"
-extern "C" int g_sim_adj_pair;   // sim_api.cpp: GM_OPT_ADJOINT_PAIR of the CPU build
+extern "C" int g_sim_adj_pair;
+extern "C" int g_sim_stress;    // sim_api.cpp: GM_FLAG_STRESS_SPECTRA of the columns   // sim_api.cpp: GM_OPT_ADJOINT_PAIR of the CPU build
 extern "C" int g_sim_nsnod;      // sim_api.cpp: source nodes of a multi-source run (-1: none)
 extern "C" int g_sim_jsnod[];
 #define GM_NS SIM_NS

@@ -157,3 +157,40 @@ def test_ocean_receiver_file_matches_oracle(tmp_path):
     tu, su = syn.synthetics(grid.nf1, grid.nf2, grid.df, grid.sigma, nwn, zr, zi, theta, phi, mt, 32, tlen / 32, **kw)
     syn.close()
     assert np.array_equal(so, su) and np.array_equal(to, tu) and np.abs(so).max() > 0
+
+
+@pytest.mark.parametrize("jre", [4, 8])
+def test_stress_file_matches_oracle(tmp_path, jre):
+    """A file written with STRESSFLAG (U, R, V, S and W, T per node; in the ocean U, R, V and zero S):
+    the datasets hold the oracle's columns with stresses rounded, the integers carry dsvstep = 2 as
+    the reference's writer puts there (computeGreenFKSpectraForSynthetics.f90:283-285)."""
+    import sys
+    sys.path.insert(0, os.path.dirname(__file__))
+    from ocean_model import ocean_model
+    from test_oracle_ocean import ocean_receivers
+    nem = ocean_model()
+    nodes = ocean_receivers(nem)
+    tlen = 2048.0
+    grid = FrequencyGrid(df=1 / tlen, nf1=10, nf2=12, sigma=5 / tlen, nwn=np.array([10, 11, 12]))
+    eng = GreenFKEngine(nem, nodes, variant="exact")
+    path = str(tmp_path / "gfk_stress.MOMENT")
+    print(f"stress GFK file, receiver node {jre}", flush=True)
+    gfkfile.write_green_fk_spectra_for_synthetics(eng, path, grid, jre, 1e-6, istyp=1, freq_block=2, stress=True)
+    eng.close()
+    g = gfkfile.GreenFKSpectra.open(path)
+    ocean = jre == 8
+    nsp = 16 if ocean else 20
+    assert g.ncsph == (2 if ocean else 4) and int(g._integers[4]) == 2
+    orc = oracle.Oracle(nem, nodes)
+    ref = {}
+    for i, f in enumerate(grid.freqs):
+        orc.set_frequency(f, 3)
+        rc, gf, _ = orc.gfk_degrees_stress(2.0 * MC_PI * f, -grid.sigma, 0, int(grid.nwn[i]), jre, 1, 1e-6)
+        assert rc == 0
+        ref[i] = gf
+    for jr in (1, 3, 6, 10):
+        gr, gi = g.read_data_node(jr, nsp=nsp)
+        want = np.concatenate([ref[i][jr - 1] for i in range(grid.freqs.size)], axis=1)
+        np.testing.assert_array_equal(gr, want.real.astype(np.float32))
+        np.testing.assert_array_equal(gi, want.imag.astype(np.float32))
+    g.close()

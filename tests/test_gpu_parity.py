@@ -317,6 +317,43 @@ def test_receiver_in_the_ocean_matches_oracle(f, lmin, nl):
                         np.testing.assert_array_equal(r["stats"], orc.stats)
 
 
+@pytest.mark.parametrize("jre", [4, 8])
+def test_stress_columns_match_oracle(jre):
+    """Columns with stresses (GM_FLAG_STRESS_SPECTRA, STRESSFLAG: U, R, V, S and W, T;
+    computeGreenFKSpectraForSynthetics.f90:209-212, 468-489) for a receiver in the mantle and in
+    the ocean (U, R, V and zero S slots), against the oracle's; the U, V, W rows are those of the
+    columns without stresses, bit for bit."""
+    from ocean_model import ocean_model
+    from test_oracle_ocean import ocean_receivers
+    nem = ocean_model()
+    nodes = ocean_receivers(nem)
+    f, lmin, nl = 30 / 2048.0, 3, 24
+    orc = oracle.Oracle(nem, nodes)
+    orc.set_frequency(f, 3)
+    for istyp in (1, 0):
+        rc, ref, status = orc.gfk_degrees_stress(2.0 * MC_PI * f, -SIGMA, lmin, nl, jre, istyp, EPS)
+        assert (status == 0).all()
+        njs, njt = (4, 2) if istyp == 1 else (2, 1)
+        ocean = jre == 8
+        eng = GreenFKEngine(nem, nodes, variant="exact")
+        print(f"stress columns receiver {jre} istyp={istyp} f={f:.5f} l={lmin}..{lmin + nl - 1}", flush=True)
+        r = eng.gfk([f], lmin, nl, SIGMA, EPS, jre, istyp, stress=True)
+        r2 = eng.gfk([f], lmin, nl, SIGMA, EPS, jre, istyp)
+        eng.close()
+        assert r["gf"].shape == ref.shape == (nodes.nnod, 4 * njs + (0 if ocean else 2 * njt), nl)
+        err = rel_l2(r["gf"], ref)
+        print(f"  max rel-L2 {err.max():.2e}", flush=True)
+        assert err.max() < 1e-12
+        np.testing.assert_array_equal(r["gfre"], ref.real.astype(np.float32))
+        np.testing.assert_array_equal(r["gfim"], ref.imag.astype(np.float32))
+        if ocean:
+            uvw = np.concatenate([np.arange(njs), 2 * njs + np.arange(njs)])
+            assert not r["gf"][:, 3 * njs:].any()
+        else:
+            uvw = np.concatenate([np.arange(njs), 2 * njs + np.arange(njs), 4 * njs + np.arange(njt)])
+        np.testing.assert_array_equal(r["gf"][:, uvw], r2["gf"])
+
+
 def test_receiver_in_the_fluid_core_is_rejected():
     from ocean_model import ocean_model
     from test_oracle_ocean import ocean_receivers

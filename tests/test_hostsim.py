@@ -366,3 +366,26 @@ def test_kernel_source_ocean_sources_match_oracle(istyp):
         assert np.array_equal(r["gf"], ref)
         np.testing.assert_array_equal(r["stats"], orc.stats)
         assert np.abs(ref[1, :, :8]).max() > 0 and not ref[1, :, 8:].any()    # an ocean source: 8 spectra
+
+
+def test_kernel_source_stress_columns_match_oracle():
+    """Columns with stresses (STRESSFLAG: U, R, V, S and W, T; in the ocean U, R, V and zero S slots,
+    computeGreenFKSpectraForSynthetics.f90:209-212, 293-300, 468-489): bit for bit with the oracle,
+    solid and ocean receivers, moment and force sources."""
+    from ocean_model import ocean_model
+    from test_oracle_ocean import ocean_receivers
+    nem = ocean_model()
+    nodes = ocean_receivers(nem)
+    for jre in (4, 8):                     # mantle, ocean
+        for istyp in (1, 0):
+            f, lmin, nl = 30 / 2048.0, 3, 5
+            r = HostSim(nem, nodes, "exact").gfk([f], lmin, nl, 5 / 2048.0, 1e-6, jre, istyp, stress=True)
+            orc = oracle.Oracle(nem, nodes)
+            orc.set_frequency(f, 3)
+            rc, ref, st = orc.gfk_degrees_stress(2 * MC_PI * f, -5 / 2048.0, lmin, nl, jre, istyp, 1e-6)
+            assert rc == 0 and np.all(r["status"] == 0)
+            njs, njt = (4, 2) if istyp == 1 else (2, 1)
+            assert r["gf"].shape == ref.shape == (nodes.nnod, 4 * njs + (0 if jre == 8 else 2 * njt), nl)
+            np.testing.assert_array_equal(r["gf"], ref)
+            if jre == 8:
+                assert not ref[:, 3 * njs:].any() and np.abs(ref[:, njs:2 * njs]).max() > 0
