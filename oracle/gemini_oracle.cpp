@@ -2034,6 +2034,12 @@ int orc_gfk_wavenumbers(void* h, double omre, double omim, double rearth, int iw
   return gfkInstances(h, omre, omim, rearth, iwnmin, nwn, dwn, jre, istyp, eps, nthreads, gf, status, stats);
 }
 
+// the regional columns with stresses (STRESSFLAG: dsvstep = 1)
+int orc_gfk_wavenumbers_stress(void* h, double omre, double omim, double rearth, int iwnmin, int nwn, double dwn,
+                               int jre, int istyp, double eps, int nthreads, double* gf, int* status, int64_t* stats) {
+  return gfkInstances(h, omre, omim, rearth, iwnmin, nwn, dwn, jre, istyp, eps, nthreads, gf, status, stats, 1);
+}
+
 // Engine test: integrate y' = A y (A constant, nvar <= 5, row-major complex) from x1 to x2.
 // All degrees l = lmin .. lmin+nl-1 of one frequency through the multi-source
 // driver (computeGreenFKSpectraForASKI.f90:458-590).  gf: complex

@@ -68,6 +68,9 @@ def lib():
                                             C.c_int, C.c_int, C.c_double, C.c_int, I, L]
         _lib.orc_gfk_wavenumbers.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                                              C.c_double, C.c_int, C.c_int, C.c_double, C.c_int, D, I, L]
+        _lib.orc_gfk_wavenumbers_stress.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int,
+                                                    C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int, D,
+                                                    I, L]
         _lib.orc_aski_degrees.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                                           C.c_int, I, C.c_int, C.c_int, C.c_double, C.c_int, D, I, L]
         _lib.orc_exp10.restype = C.c_double
@@ -240,6 +243,19 @@ class Oracle:
         status = np.zeros(nl, np.int32)
         rc = lib().orc_gfk_degrees_stress(self.h, omre, omim, self.nem.rearth, lmin, nl, jre, istyp, eps, nthreads,
                                           _c(gf), status.ctypes.data_as(I), self.stats.ctypes.data_as(L))
+        return rc, gf, status
+
+    def gfk_wavenumbers_stress(self, omre, omim, iwnmin, nwn, dwn, jre, istyp, eps, nthreads=1):
+        """Regional-mode columns with stresses (STRESSFLAG), k = (iwn-1) dwn; layout of gfk_degrees_stress."""
+        n = self.nodes.nnod
+        njsph, njtor = (4, 2) if istyp == 1 else (2, 1)
+        nsp = 4 * njsph + (0 if self.nodes.state[jre - 1] == 1 else 2 * njtor)
+        gf = np.zeros((n, nsp, nwn), np.complex128)
+        status = np.zeros(nwn, np.int32)
+        rc = lib().orc_gfk_wavenumbers_stress(self.h, C.c_double(omre), C.c_double(omim),
+                                              C.c_double(self.nem.rearth), int(iwnmin), int(nwn), C.c_double(dwn),
+                                              int(jre), int(istyp), C.c_double(eps), int(nthreads), _c(gf),
+                                              status.ctypes.data_as(I), self.stats.ctypes.data_as(L))
         return rc, gf, status
 
     def aski_degrees(self, omre, omim, lmin, nl, jsnod, jre, istyp, eps, nthreads=1):

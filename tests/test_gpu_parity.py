@@ -413,3 +413,28 @@ def test_ujgf_batch_matches_oracle(jre):
         np.testing.assert_allclose(r["ujt"][..., il], ujt, rtol=1e-11, atol=1e-11 * max(np.abs(ujt).max(), 1e-300))
     if nodes.state[jre - 1] == 1:
         assert not r["ujs"][:, 2:].any() and not r["ujt"].any()
+
+
+def test_regional_stress_columns_match_oracle():
+    """Regional wavenumbers with stresses (gm_gfk_batch_wn with GM_FLAG_STRESS_SPECTRA) for a
+    buried receiver, moment and force sources, against the oracle's."""
+    from gemini_unified_amd import ExternalRadialNodes, regional_model
+    nem = regional_model(6)
+    nodes = ExternalRadialNodes.from_blocks(nem, [4], [5000.0])
+    jre = nodes.nnod - 2
+    dwn = 2 * np.pi / 8.0e6
+    f, sigma = 1.3, 5 / 102.4
+    orc = oracle.Oracle(nem, nodes)
+    orc.set_frequency(f, 3)
+    eng = GreenFKEngine(nem, nodes, variant="exact")
+    for istyp in (1, 0):
+        print(f"regional stress columns istyp={istyp} f={f} iwn 2..201", flush=True)
+        rc, ref, status = orc.gfk_wavenumbers_stress(2.0 * MC_PI * f, -sigma, 2, 200, dwn, jre, istyp, EPS)
+        assert rc == 0 and (status == 0).all()
+        r = eng.gfk([f], 2, 200, sigma, EPS, jre, istyp, dwn=dwn, stress=True)
+        assert r["gf"].shape == ref.shape
+        err = rel_l2(r["gf"], ref)
+        print(f"  max rel-L2 {err.max():.2e}", flush=True)
+        assert err.max() < 1e-12
+        np.testing.assert_array_equal(r["gfre"], ref.real.astype(np.float32))
+    eng.close()

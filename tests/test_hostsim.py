@@ -389,3 +389,25 @@ def test_kernel_source_stress_columns_match_oracle():
             np.testing.assert_array_equal(r["gf"], ref)
             if jre == 8:
                 assert not ref[:, 3 * njs:].any() and np.abs(ref[:, njs:2 * njs]).max() > 0
+
+
+def test_kernel_source_regional_stress_columns_match_oracle():
+    """Regional wavenumbers with stresses (STRESSFLAG) for a buried receiver: bit for bit with the
+    oracle; the U, V, W rows are the oracle's columns without stresses."""
+    nem = regional_model(6)
+    nodes = ExternalRadialNodes.from_blocks(nem, [4], [5000.0])
+    jre = nodes.nnod - 2
+    dwn = 2 * np.pi / 8.0e6
+    f = 1.3
+    for istyp in (1, 0):
+        r = HostSim(nem, nodes, "exact").gfk([f], 2, 30, 5 / 102.4, 1e-6, jre, istyp, dwn=dwn, stress=True)
+        orc = oracle.Oracle(nem, nodes)
+        orc.set_frequency(f, 3)
+        rc, ref, status = orc.gfk_wavenumbers_stress(2 * MC_PI * f, -5 / 102.4, 2, 30, dwn, jre, istyp, 1e-6)
+        assert rc == 0 and np.all(status == 0)
+        np.testing.assert_array_equal(r["gf"], ref)
+        rc, g2, _ = orc.gfk_wavenumbers(2 * MC_PI * f, -5 / 102.4, 2, 30, dwn, jre, istyp, 1e-6)
+        njs, njt = (4, 2) if istyp == 1 else (2, 1)
+        uvw = np.concatenate([np.arange(njs), 2 * njs + np.arange(njs), 4 * njs + np.arange(njt)])
+        np.testing.assert_array_equal(ref[:, uvw], g2)
+        assert np.abs(ref[:, njs:2 * njs]).max() > 0 and np.abs(ref[:, 4 * njs + njt:]).max() > 0
